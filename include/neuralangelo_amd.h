@@ -1,0 +1,305 @@
+/*
+ * neuralangelo_amd — C ABI of the MI355X (gfx950) hot-path library.
+ *
+ * Conventions (every entry point):
+ *   - arguments are raw device pointers, int64 element counts and POD structs;
+ *     no C++ types and no exceptions cross this boundary;
+ *   - buffers are owned by the caller; the library never allocates device
+ *     memory on these paths;
+ *   - work is enqueued on `stream` (a hipStream_t, e.g. torch's current stream)
+ *     and the call returns without synchronising;
+ *   - the return value is NA_OK (0) or a negative status; na_last_error()
+ *     returns a thread-local message describing the last failure.
+ *
+ * Each declaration names the reference interface it replaces
+ * (NVlabs/neuralangelo path:line, or the third-party call made there).
+ */
+#ifndef NEURALANGELO_AMD_H
+#define NEURALANGELO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NA_OK 0
+#define NA_ERR_ARGUMENT (-1)
+#define NA_ERR_HIP (-2)
+#define NA_ERR_UNSUPPORTED (-3)
+
+/* element types */
+#define NA_F32 0
+#define NA_BF16 1
+#define NA_F64 3
+
+#define NA_MAX_LEVELS 32
+
+typedef void* na_stream_t; /* hipStream_t */
+
+/* Host-computed multiresolution hash-grid level table.  Both the kernels and
+ * the host read this one table, so hash indices never depend on device
+ * transcendental functions.  offset[] and the per-level sizes are in table
+ * entries (one entry = n_features parameters). */
+typedef struct na_grid_desc {
+  int32_t n_levels;
+  int32_t n_features;        /* 1, 2, 4 or 8 */
+  int32_t log2_hashmap_size;
+  int32_t base_resolution;
+  float per_level_scale;
+  float log2_per_level_scale;
+  float scale[NA_MAX_LEVELS];           /* exp2(l*log2 s)*base - 1, float32 */
+  uint32_t resolution[NA_MAX_LEVELS];   /* ceil(scale)+1 */
+  uint32_t offset[NA_MAX_LEVELS + 1];   /* prefix of per-level entry counts */
+} na_grid_desc;
+
+/* library */
+const char* na_last_error(void);
+int na_version(void);
+int na_device_synchronize(void);
+
+/* ---------------------------------------------------------------------------
+ * Hash-grid encoding (replaces tcnn.Encoding(3, {"otype": "HashGrid", ...})
+ * constructed at projects/neuralangelo/utils/modules.py:42-50 and called at
+ * projects/neuralangelo/utils/modules.py:84-86).
+ * ------------------------------------------------------------------------- */
+
+/* Fill *desc (host only, no GPU).  Returns the total number of parameters
+ * (entries * n_features) or a negative status. */
+int64_t na_hashgrid_layout(int32_t n_levels, int32_t n_features,
+                           int32_t log2_hashmap_size, int32_t base_resolution,
+                           float per_level_scale, na_grid_desc* desc);
+
+/* Debug/parity entry: per point, level and corner, the table entry index
+ * (offset included) and the trilinear weight.  idx: int32 [n, L, 8],
+ * w: f32 [n, L, 8]. */
+int na_hashgrid_corners(const na_grid_desc* desc, const float* x, int64_t n,
+                        int32_t* idx, float* w, na_stream_t stream);
+
+/* Forward.  x: f32 [n,3] in the unit cube (tcnn input convention).
+ * table: [n_params] of table_dtype (NA_F32 | NA_BF16).
+ * out: [n, L*F] of out_dtype (NA_F32 | NA_BF16), level-major.
+ * Levels >= active_levels are written as zero (coarse-to-fine mask of
+ * projects/neuralangelo/utils/modules.py:91-93,109-113). */
+int na_hashgrid_fwd(const na_grid_desc* desc, const float* x, int64_t n,
+                    const void* table, int32_t table_dtype, void* out,
+                    int32_t out_dtype, int32_t active_levels,
+                    na_stream_t stream);
+
+/* Backward w.r.t. the table: grad_table[(offset_l+idx)*F+f] +=
+ * w * dy[p, l*F+f] (fp32 atomics).  dy: [n, L*F] of dy_dtype. */
+int na_hashgrid_bwd(const na_grid_desc* desc, const float* x, int64_t n,
+                    const void* dy, int32_t dy_dtype, float* grad_table,
+                    int32_t active_levels, na_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Dense layers of the SDF and colour MLPs (replace torch.nn.Linear +
+ * activation at projects/neuralangelo/utils/mlp.py:55-69 and
+ * projects/nerf/utils/nerf_util.py:189-196; weight-normed weights
+ * W = g * v / ||v||_row as torch.nn.utils.weight_norm, mlp.py:42-43).
+ * ------------------------------------------------------------------------- */
+#define NA_ACT_NONE 0
+#define NA_ACT_RELU 1
+#define NA_ACT_SOFTPLUS 2 /* log1p(exp(beta*x))/beta, linear where beta*x > 20 */
+#define NA_ACT_SIGMOID 3
+
+#define NA_EPI_STORE 0  /* C[m,n] = value (c_dtype) */
+#define NA_EPI_ATOMIC 1 /* C[m,n] += value (fp32 atomics; needed when split_k > 1) */
+
+/* C[m,n] = epilogue( sum_k A(m,k) * B(n,k) ),  with
+ *   A(m,k) = trans_a ? A[k*lda + m] : A[m*lda + k]
+ *   B(n,k) = trans_b ? B[k*ldb + n] : B[n*ldb + k]
+ * epilogue, in order: v += bias[n]; v = act(v); v += r1_a[m]*r1_b[n];
+ *   v *= act'(dact_y[m*ld_dact+n]) (derivative expressed through the
+ *   activation's output y); colsum[n] += v (atomic); store/accumulate v.
+ * A and B share in_dtype (NA_F32: fp32 MFMA, NA_BF16: bf16 MFMA); the
+ * accumulation is fp32.  K and the leading dimensions must be multiples of 8
+ * (bf16) or 4 (fp32) and the pointers 16-byte aligned. */
+typedef struct na_gemm_args {
+  int64_t M, N, K;
+  const void* A;
+  int64_t lda;
+  int32_t trans_a;
+  const void* B;
+  int64_t ldb;
+  int32_t trans_b;
+  int32_t in_dtype;
+  int32_t epilogue;
+  void* C;
+  int64_t ldc;
+  int32_t c_dtype;
+  const float* bias;
+  int32_t act;
+  float act_beta;
+  const void* dact_y;
+  int64_t ld_dact;
+  int32_t dact_dtype;
+  int32_t dact;
+  const float* r1_a;
+  const float* r1_b;
+  float* colsum;
+  int32_t split_k;
+} na_gemm_args;
+
+int na_gemm(const na_gemm_args* args, na_stream_t stream);
+
+/* Narrow output layer (N <= 4): out[m,n] = act(sum_k X[m*ldx+k] W[n*K+k] + b[n]),
+ * X of x_dtype, W/b fp32, out fp32 [M, N].  Used for the SDF head
+ * (mlp.py:50-52,66) and the colour output layer + sigmoid (modules.py:219). */
+int na_linear_narrow_fwd(const void* X, int64_t ldx, int32_t x_dtype, int64_t M, int64_t K,
+                         const float* W, const float* b, int32_t N, int32_t act,
+                         float* out, na_stream_t stream);
+
+/* Backward of the narrow layer.  dpre [M,N] fp32 is the gradient w.r.t. the
+ * pre-activation.  dW[n,k] += sum_m dpre[m,n] X[m,k]; db[n] += sum_m dpre[m,n];
+ * if dX != NULL: dX[m,k] = (add_dx ? dX[m,k] : 0) + sum_n dpre[m,n] W[n,k], then
+ * multiplied by act'(Y[m*ldy+k]) when dact != NA_ACT_NONE (Y of y_dtype); dX
+ * is written as dx_dtype. */
+int na_linear_narrow_bwd(const void* X, int64_t ldx, int32_t x_dtype, int64_t M, int64_t K,
+                         const float* W, int32_t N, const float* dpre, float* dW, float* db,
+                         void* dX, int64_t lddx, int32_t dx_dtype, int32_t add_dx,
+                         const void* Y, int64_t ldy, int32_t y_dtype, int32_t dact,
+                         float act_beta, float* dx_colsum, na_stream_t stream);
+
+/* out[m, n] = dy[m*ld_dy + n] * act'(y[m*ld_y + n]) for n < N (derivative through
+ * the activation output y, as na_gemm's dact); colsum[n] += out[m,n] when
+ * colsum != NULL.  Converts an output gradient into a pre-activation gradient. */
+int na_act_grad(const void* dy, int64_t ld_dy, int32_t dy_dtype, const void* y, int64_t ld_y,
+                int32_t y_dtype, int64_t M, int64_t N, int32_t act, float act_beta, void* out,
+                int64_t ld_out, int32_t out_dtype, float* colsum, na_stream_t stream);
+
+/* Weight normalisation W = g * v / ||v||_row (v [N,K], g [N]); optionally also
+ * writes a bf16 copy Wb.  Backward: dg = sum_k dW*v/||v||,
+ * dv = g/||v|| * (dW - dg * v/||v||). */
+int na_weight_norm_fwd(const float* v, const float* g, int64_t N, int64_t K, float* W,
+                       void* Wlow, int32_t low_dtype, int64_t ld_low, na_stream_t stream);
+int na_weight_norm_bwd(const float* v, const float* g, const float* dW, int64_t N, int64_t K,
+                       float* dv, float* dg, na_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * SDF network input: [enc(L*F) | x(3) | zero pad] per point, row stride ld.
+ * Fuses the range normalisation of projects/neuralangelo/utils/modules.py:82-83,
+ * the encoder (modules.py:84-86), the coarse-to-fine mask (:91-93) and the
+ * concatenation with the raw points (:94); the encoder block comes first so
+ * its gradient slice is aligned (the first-layer weight columns are permuted
+ * to match).  Backward reads the encoder block of dY with the same stride.
+ * ------------------------------------------------------------------------- */
+int na_sdf_input_fwd(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
+                     float vmax, const void* table, int32_t table_dtype, int32_t active_levels,
+                     void* out, int32_t out_dtype, int64_t ld, na_stream_t stream);
+int na_sdf_input_bwd(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
+                     float vmax, const void* dy, int32_t dy_dtype, int64_t ld,
+                     float* grad_table, int32_t active_levels, na_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Per-ray operators.  R rays, N samples per ray, row-major [R, N(, K)].
+ * ------------------------------------------------------------------------- */
+
+/* Unit-sphere bounds and coarse samples (projects/neuralangelo/model.py:207-213,
+ * projects/nerf/utils/nerf_util.py:199-205 and :20-38):
+ * near/far from |c + t v| = 1, near = relu(near); rays that miss get
+ * outside = 1, near = 1, far = 1.2; dists[r,i] = (i + u[r,i]) / N * (far-near) + near
+ * with u = rands (stratified) or 0.5 when rands == NULL. */
+int na_ray_bounds_sample(const float* center, const float* ray_unit, const float* rands,
+                         int64_t R, int32_t N, float* near, float* far, uint8_t* outside,
+                         float* dists, na_stream_t stream);
+
+/* Inverse-CDF resampling (projects/nerf/utils/nerf_util.py:41-68):
+ * pdf = w / max(sum|w|, 1e-12); cdf = [0, serial fp32 prefix of pdf];
+ * u_j = linspace midpoints; idx = searchsorted(cdf, u, right=True); lerp.
+ * bins [R, N], weights [R, N-1] -> out [R, Nf]. */
+int na_sample_pdf(const float* bins, const float* weights, int64_t R, int32_t N, int32_t Nf,
+                  float* out, na_stream_t stream);
+
+/* One round of hierarchical sampling (projects/neuralangelo/model.py:239-247,250-267):
+ * NeuS alphas of the N-1 intervals from the sorted samples' SDFs (robust cos,
+ * inv_s), compositing weights, sample_pdf of Nf new dists, and a stable merge
+ * (ties keep coarse samples first) into merged [R, N+Nf].  pos_coarse [R,N] and
+ * pos_fine [R,Nf] receive each sample's position in the merged row. */
+int na_hier_resample(const float* dists, const float* sdfs, int64_t R, int32_t N, float inv_s,
+                     int32_t Nf, float* merged, float* fine, int32_t* pos_coarse,
+                     int32_t* pos_fine, na_stream_t stream);
+
+/* out[r, pos[r,i]] = src[r,i] for both lists (SDF gather after the merge). */
+int na_scatter_merge(const float* src_a, const int32_t* pos_a, int32_t Na, const float* src_b,
+                     const int32_t* pos_b, int32_t Nb, int64_t R, float* out,
+                     na_stream_t stream);
+
+/* NeuS SDF -> alpha (projects/neuralangelo/model.py:275-298).  sdfs [R,N]
+ * (rays with outside[r] != 0 use outside_val), grads [R,N,3], dists [R,N],
+ * far [R] (NULL: 1e10), s_var: device scalar, inv_s = exp(s_var). */
+int na_neus_alpha_fwd(const float* ray_unit, const float* sdfs, const float* grads,
+                      const float* dists, const float* far, const uint8_t* outside,
+                      float outside_val, const float* s_var, float progress, float anneal_end,
+                      int64_t R, int32_t N, float* alpha, na_stream_t stream);
+/* Backward: dsdf [R,N], dgrad [R,N,3] written; ds_var (device scalar) += . */
+int na_neus_alpha_bwd(const float* ray_unit, const float* sdfs, const float* grads,
+                      const float* dists, const float* far, const uint8_t* outside,
+                      float outside_val, const float* s_var, float progress, float anneal_end,
+                      int64_t R, int32_t N, const float* dalpha, float* dsdf, float* dgrad,
+                      float* ds_var, na_stream_t stream);
+
+/* Alpha compositing + composite (projects/nerf/utils/render.py:87-112):
+ * T_i = exp(sum_{j<i} log1p(-alpha_j)), w_i = alpha_i T_i, out[r,k] = sum_i w_i q[r,i,k].
+ * weights [R,N] always written; q [R,N,K] / out [R,K] optional (K may be 0). */
+int na_composite_fwd(const float* alpha, const float* q, int64_t R, int32_t N, int32_t K,
+                     float* weights, float* out, na_stream_t stream);
+/* Backward with upstream dout [R,K] (optional) and dweights [R,N] (optional):
+ * dalpha [R,N] written (exact at alpha = 1, no division), dq [R,N,K] written if non-NULL. */
+int na_composite_bwd(const float* alpha, const float* q, const float* weights, int64_t R,
+                     int32_t N, int32_t K, const float* dout, const float* dweights,
+                     float* dalpha, float* dq, na_stream_t stream);
+
+/* composite with given weights (projects/nerf/utils/render.py:102-112):
+ * out[r,k] = sum_i q[r,i,k] w[r,i]; backward dq = w dout, dw[r,i] = sum_k q dout. */
+int na_weighted_sum_fwd(const float* q, const float* w, int64_t R, int32_t N, int32_t K,
+                        float* out, na_stream_t stream);
+int na_weighted_sum_bwd(const float* q, const float* w, int64_t R, int32_t N, int32_t K,
+                        const float* dout, float* dq, float* dw, na_stream_t stream);
+
+/* 4-tap numerical gradient (projects/neuralangelo/utils/modules.py:157-175).
+ * taps: writes x + eps*k_i for k = (1,-1,-1), (-1,-1,1), (-1,1,-1), (1,1,1) as
+ * [4, n, 3] (eps = float32 of normal_eps/sqrt(3)).  combine: s_taps [4, n],
+ * sdf [n] (centre SDF after the outside fill), grad [n,3] = sum k_i s_i / (4 eps),
+ * hess [n,3] = ((sum s_i)/2 - 2 sdf)/eps_sq / 3 (only if hess != NULL), with
+ * eps_sq = float32 of the float64 square, as the reference evaluates eps ** 2. */
+int na_taps_points(const float* x, int64_t n, float eps, float* out, na_stream_t stream);
+int na_taps_combine_fwd(const float* s_taps, const float* sdf, int64_t n, float eps,
+                        float eps_sq, float* grad, float* hess, na_stream_t stream);
+int na_taps_combine_bwd(int64_t n, float eps, float eps_sq, const float* dgrad,
+                        const float* dhess, float* ds_taps, float* dsdf, int32_t add_dsdf,
+                        na_stream_t stream);
+
+/* Colour-MLP input (projects/neuralangelo/utils/modules.py:209-220, SH degree 3
+ * from projects/neuralangelo/utils/spherical_harmonics.py:47-71).  Row layout
+ * [feat(col0) | x(3) | SH16(ray) | normalize(grad)(3) | zero pad up to ld]; this
+ * call writes columns [col0, ld) (the SDF feature layer writes [0, col0) in
+ * place).  ray_unit is per ray ([n / n_per_ray, 3]).  normalize != 0: `grad`
+ * holds raw SDF gradients and the normal block is F.normalize(grad) (eps 1e-12,
+ * projects/neuralangelo/model.py:167); normalize == 0: `grad` already holds the
+ * normals.  Backward: d(grad) from the normal block (through the normalisation
+ * when normalize != 0), added to dgrad if add != 0. */
+int na_rgb_input_fwd(const float* x, const float* ray_unit, int64_t n_per_ray,
+                     const float* grad, int32_t normalize, int64_t n, int32_t col0, void* out,
+                     int32_t out_dtype, int64_t ld, na_stream_t stream);
+int na_rgb_input_bwd(const float* grad, int32_t normalize, int64_t n, int32_t col0,
+                     const void* dy, int32_t dy_dtype, int64_t ld, float* dgrad, int32_t add,
+                     na_stream_t stream);
+
+/* Losses (projects/neuralangelo/utils/misc.py:74-89, trainer.py:43):
+ * eikonal = mean((|g|-1)^2 * !outside), curvature = mean(|sum hess| * !outside)
+ * with NaN/Inf -> 0; both over all n = R * n_per_ray samples.  sums[0] += sum of
+ * eikonal terms, sums[1] += sum of curvature terms.  Backward writes the
+ * per-sample gradients scaled by w[0] / n (eikonal) and w[1] / n (curvature);
+ * w is a device array (the upstream gradients of the two means). */
+int na_sdf_losses_fwd(const float* grad, const float* hess, const uint8_t* outside,
+                      int64_t n, int64_t n_per_ray, float* sums, na_stream_t stream);
+int na_sdf_losses_bwd(const float* grad, const float* hess, const uint8_t* outside,
+                      int64_t n, int64_t n_per_ray, const float* w, float* dgrad,
+                      int32_t add_dgrad, float* dhess, na_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEURALANGELO_AMD_H */

@@ -1,0 +1,173 @@
+"""ctypes binding of ``libneuralangelo_amd.so`` (the C ABI in include/neuralangelo_amd.h).
+
+The product path has no fallback: if the library is missing or a call fails,
+a ``RuntimeError`` carrying the library's own message is raised.
+"""
+import ctypes
+import os
+
+import torch
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libneuralangelo_amd.so")
+
+NA_OK = 0
+NA_ERR_ARGUMENT = -1
+NA_ERR_HIP = -2
+NA_ERR_UNSUPPORTED = -3
+
+NA_F32 = 0
+NA_BF16 = 1
+NA_F64 = 3
+NA_MAX_LEVELS = 32
+
+NA_ACT_NONE = 0
+NA_ACT_RELU = 1
+NA_ACT_SOFTPLUS = 2
+NA_ACT_SIGMOID = 3
+NA_EPI_STORE = 0
+NA_EPI_ATOMIC = 1
+
+_DTYPE_CODE = {torch.float32: NA_F32, torch.bfloat16: NA_BF16, torch.float64: NA_F64}
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_levels", ctypes.c_int32),
+        ("n_features", ctypes.c_int32),
+        ("log2_hashmap_size", ctypes.c_int32),
+        ("base_resolution", ctypes.c_int32),
+        ("per_level_scale", ctypes.c_float),
+        ("log2_per_level_scale", ctypes.c_float),
+        ("scale", ctypes.c_float * NA_MAX_LEVELS),
+        ("resolution", ctypes.c_uint32 * NA_MAX_LEVELS),
+        ("offset", ctypes.c_uint32 * (NA_MAX_LEVELS + 1)),
+    ]
+
+
+class GemmArgs(ctypes.Structure):
+    _fields_ = [
+        ("M", ctypes.c_int64), ("N", ctypes.c_int64), ("K", ctypes.c_int64),
+        ("A", ctypes.c_void_p), ("lda", ctypes.c_int64), ("trans_a", ctypes.c_int32),
+        ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("trans_b", ctypes.c_int32),
+        ("in_dtype", ctypes.c_int32), ("epilogue", ctypes.c_int32),
+        ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64), ("c_dtype", ctypes.c_int32),
+        ("bias", ctypes.c_void_p), ("act", ctypes.c_int32), ("act_beta", ctypes.c_float),
+        ("dact_y", ctypes.c_void_p), ("ld_dact", ctypes.c_int64), ("dact_dtype", ctypes.c_int32),
+        ("dact", ctypes.c_int32),
+        ("r1_a", ctypes.c_void_p), ("r1_b", ctypes.c_void_p), ("colsum", ctypes.c_void_p),
+        ("split_k", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_F32 = ctypes.c_float
+_F64 = ctypes.c_double
+_PGRID = ctypes.POINTER(GridDesc)
+_PGEMM = ctypes.POINTER(GemmArgs)
+
+# name -> (restype, argtypes); mirrors include/neuralangelo_amd.h one to one
+_SIGNATURES = {
+    "na_last_error": (ctypes.c_char_p, []),
+    "na_version": (ctypes.c_int, []),
+    "na_device_synchronize": (ctypes.c_int, []),
+    "na_hashgrid_layout": (_I64, [_I32, _I32, _I32, _I32, _F32, _PGRID]),
+    "na_hashgrid_corners": (ctypes.c_int, [_PGRID, _P, _I64, _P, _P, _P]),
+    "na_hashgrid_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _I32, _P]),
+    "na_hashgrid_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _P]),
+    "na_gemm": (ctypes.c_int, [_PGEMM, _P]),
+    "na_linear_narrow_fwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P, _I32, _I32, _P, _P]),
+    "na_linear_narrow_bwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _I32, _P, _P, _P, _P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _F32, _P, _P]),
+    "na_act_grad": (ctypes.c_int, [_P, _I64, _I32, _P, _I64, _I32, _I64, _I64, _I32, _F32, _P, _I64, _I32, _P, _P]),
+    "na_weight_norm_fwd": (ctypes.c_int, [_P, _P, _I64, _I64, _P, _P, _I32, _I64, _P]),
+    "na_weight_norm_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P, _P]),
+    "na_sdf_input_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _P, _I32, _I32, _P, _I32, _I64, _P]),
+    "na_sdf_input_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _P, _I32, _I64, _P, _I32, _P]),
+    "na_ray_bounds_sample": (ctypes.c_int, [_P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P]),
+    "na_sample_pdf": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
+    "na_hier_resample": (ctypes.c_int, [_P, _P, _I64, _I32, _F32, _I32, _P, _P, _P, _P, _P]),
+    "na_scatter_merge": (ctypes.c_int, [_P, _P, _I32, _P, _P, _I32, _I64, _P, _P]),
+    "na_neus_alpha_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F32, _P, _F32, _F32, _I64, _I32, _P, _P]),
+    "na_neus_alpha_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F32, _P, _F32, _F32, _I64, _I32, _P, _P, _P, _P, _P]),
+    "na_composite_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _P]),
+    "na_composite_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P, _P]),
+    "na_weighted_sum_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
+    "na_weighted_sum_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _P, _P]),
+    "na_taps_points": (ctypes.c_int, [_P, _I64, _F32, _P, _P]),
+    "na_taps_combine_fwd": (ctypes.c_int, [_P, _P, _I64, _F32, _F32, _P, _P, _P]),
+    "na_taps_combine_bwd": (ctypes.c_int, [_I64, _F32, _F32, _P, _P, _P, _P, _I32, _P]),
+    "na_rgb_input_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _I64, _I32, _P, _I32, _I64, _P]),
+    "na_rgb_input_bwd": (ctypes.c_int, [_P, _I32, _I64, _I32, _P, _I32, _I64, _P, _I32, _P]),
+    "na_sdf_losses_fwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P]),
+    "na_sdf_losses_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P, _I32, _P, _P]),
+}
+
+_lib = None
+
+
+def exported_symbols():
+    """Names the Python side binds (every function declared in the public header)."""
+    return list(_SIGNATURES)
+
+
+def load():
+    """Load the shared library (once).  Raises RuntimeError if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"neuralangelo_amd: HIP library not built ({LIB_PATH} missing); "
+            "run `python -m neuralangelo_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error():
+    return load().na_last_error().decode(errors="replace")
+
+
+def check(status, what):
+    if status < 0:
+        raise RuntimeError(f"{what} failed (status {status}): {last_error()}")
+    return status
+
+
+def call(name, *args):
+    """Call a C-ABI entry point and raise on a negative status."""
+    fn = getattr(load(), name)
+    return check(fn(*args), name)
+
+
+def dtype_code(dtype):
+    try:
+        return _DTYPE_CODE[dtype]
+    except KeyError:
+        raise RuntimeError(f"neuralangelo_amd: unsupported dtype {dtype}") from None
+
+
+def ptr(t):
+    """Raw device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_of(t=None):
+    """hipStream_t of torch's current stream on the tensor's device."""
+    dev = t.device if t is not None else torch.device("cuda", torch.cuda.current_device())
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def require_cuda(*tensors):
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise RuntimeError("neuralangelo_amd: tensors must live on the GPU "
+                               f"(got device {t.device}); the HIP path has no CPU fallback")

@@ -1,0 +1,71 @@
+// Library-wide C ABI: error reporting, version, and host-side layout logic.
+#include <math.h>
+
+#include <string>
+
+#include "common.h"
+
+namespace na {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace na
+
+extern "C" const char* na_last_error(void) { return na::g_last_error.c_str(); }
+
+extern "C" int na_version(void) { return 1; }
+
+extern "C" int na_device_synchronize(void) {
+  NA_CHECK_HIP(hipDeviceSynchronize());
+  return NA_OK;
+}
+
+// Hash-grid level table (tiny-cuda-nn GridEncoding constructor semantics; the
+// reference passes per_level_scale from projects/neuralangelo/utils/modules.py:41):
+//   log2s   = log2f(per_level_scale)                       (float32)
+//   scale_l = exp2f(l * log2s) * base_resolution - 1       (float32, each op rounded)
+//   res_l   = ceil(scale_l) + 1
+//   size_l  = min(next_multiple_of_8(min(res_l^3, 2^31-1)), 2^log2_hashmap_size)
+// The float32 steps are written as separate statements and compiled with
+// -ffp-contract=off so no FMA fuses them.
+#pragma clang fp contract(off)
+extern "C" int64_t na_hashgrid_layout(int32_t n_levels, int32_t n_features,
+                                      int32_t log2_hashmap_size, int32_t base_resolution,
+                                      float per_level_scale, na_grid_desc* desc) {
+  NA_CHECK_ARG(desc != nullptr, "desc is null");
+  NA_CHECK_ARG(n_levels >= 1 && n_levels <= NA_MAX_LEVELS, "n_levels out of range [1, 32]");
+  NA_CHECK_ARG(n_features == 1 || n_features == 2 || n_features == 4 || n_features == 8,
+               "n_features must be 1, 2, 4 or 8");
+  NA_CHECK_ARG(log2_hashmap_size >= 3 && log2_hashmap_size <= 28,
+               "log2_hashmap_size out of range [3, 28]");
+  NA_CHECK_ARG(base_resolution >= 1, "base_resolution must be >= 1");
+  NA_CHECK_ARG(per_level_scale >= 1.0f, "per_level_scale must be >= 1");
+  *desc = na_grid_desc{};
+  desc->n_levels = n_levels;
+  desc->n_features = n_features;
+  desc->log2_hashmap_size = log2_hashmap_size;
+  desc->base_resolution = base_resolution;
+  desc->per_level_scale = per_level_scale;
+  volatile float log2s = log2f(per_level_scale);
+  desc->log2_per_level_scale = log2s;
+  const uint64_t hash_size = 1ull << log2_hashmap_size;
+  uint64_t offset = 0;
+  for (int l = 0; l < n_levels; ++l) {
+    volatile float a = (float)l * log2s;
+    volatile float e = exp2f(a);
+    volatile float b = e * (float)base_resolution;
+    float scale = b - 1.0f;
+    uint32_t res = (uint32_t)ceilf(scale) + 1u;
+    uint64_t r = res;
+    uint64_t params = r * r * r;
+    if (params > 0x7fffffffull) params = 0x7fffffffull;
+    params = (params + 7ull) / 8ull * 8ull;
+    if (params > hash_size) params = hash_size;
+    desc->scale[l] = scale;
+    desc->resolution[l] = res;
+    desc->offset[l] = (uint32_t)offset;
+    offset += params;
+    NA_CHECK_ARG(offset < 0xffffffffull, "table too large for 32-bit offsets");
+  }
+  desc->offset[n_levels] = (uint32_t)offset;
+  return (int64_t)offset * n_features;
+}

@@ -1,0 +1,593 @@
+// Dense layers of the SDF / colour MLPs on CDNA4 matrix cores.
+//
+// One GEMM core, C[m,n] = sum_k A(m,k) B(n,k), serves the three products of a
+// layer (forward Y = act(X W^T + b), backward-data dX = dPre W, backward-weight
+// dW = dPre^T X) through operand-layout flags and a fused epilogue:
+//   bias + activation (forward), rank-1 add + activation derivative (backward
+//   chaining into the previous layer), column sums (bias gradients) and fp32
+//   split-K atomics (weight gradients, reduced over millions of points).
+// Tiles: 128x128x32 per 256-thread workgroup, each of the 4 waves owns a 64x64
+// sub-tile computed as 2x2 MFMA 32x32 blocks (bf16: v_mfma_f32_32x32x16_bf16,
+// fp32: v_mfma_f32_32x32x2_f32), operands staged through LDS.
+//
+// Narrow layers (N <= 4: SDF head, colour output) use a wave-per-row kernel,
+// and weight normalisation (torch.nn.utils.weight_norm, dim=0) has its own
+// small kernels.
+#include "common.h"
+
+namespace na {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(16))) float f32x16_t;
+
+constexpr int GBM = 128, GBN = 128, GBK = 32, GNT = 256;
+
+struct EpiParams {
+  int epilogue;
+  void* C;
+  int64_t ldc;
+  int c_dtype;
+  const float* bias;
+  int act;
+  float beta;
+  const void* dact_y;
+  int64_t ld_dact;
+  int dact_dtype;
+  int dact;
+  const float* r1_a;
+  const float* r1_b;
+  float* colsum;
+};
+
+struct GemmParams {
+  int64_t M, N, K;
+  const void* A;
+  int64_t lda;
+  const void* B;
+  int64_t ldb;
+  int64_t k_chunk;
+  EpiParams e;
+};
+
+__device__ inline float act_forward(int act, float v, float beta) {
+  if (act == NA_ACT_RELU) return v > 0.0f ? v : 0.0f;
+  if (act == NA_ACT_SOFTPLUS) {
+    float z = v * beta;
+    return z > 20.0f ? v : log1pf(expf(z)) / beta;
+  }
+  if (act == NA_ACT_SIGMOID) return 1.0f / (1.0f + expf(-v));
+  return v;
+}
+
+// Derivative of the activation expressed through its output y.
+//   relu: [y > 0];  softplus: sigmoid(beta*x) = -expm1(-beta*y) (1 in the linear
+//   region beta*x > 20);  sigmoid: y(1-y).
+__device__ inline float act_deriv_from_y(int act, float y, float beta) {
+  if (act == NA_ACT_RELU) return y > 0.0f ? 1.0f : 0.0f;
+  if (act == NA_ACT_SOFTPLUS) {
+    float z = y * beta;
+    return z > 20.0f ? 1.0f : -expm1f(-z);
+  }
+  if (act == NA_ACT_SIGMOID) return y * (1.0f - y);
+  return 1.0f;
+}
+
+__device__ inline float load_as_f32(const void* p, int dtype, int64_t i) {
+  return dtype == NA_BF16 ? bf16_to_f32(((const bf16_t*)p)[i]) : ((const float*)p)[i];
+}
+
+__device__ inline float epilogue_value(const EpiParams& e, int64_t m, int64_t n, float v) {
+  if (e.bias) v += e.bias[n];
+  if (e.act) v = act_forward(e.act, v, e.beta);
+  if (e.r1_a) v += e.r1_a[m] * e.r1_b[n];
+  if (e.dact) v *= act_deriv_from_y(e.dact, load_as_f32(e.dact_y, e.dact_dtype, m * e.ld_dact + n), e.beta);
+  return v;
+}
+
+__device__ inline void epilogue_store(const EpiParams& e, int64_t m, int64_t n, float v) {
+  int64_t off = m * e.ldc + n;
+  if (e.epilogue == NA_EPI_ATOMIC) {
+    atomic_add_f32((float*)e.C + off, v);
+  } else if (e.c_dtype == NA_BF16) {
+    ((bf16_t*)e.C)[off] = f32_to_bf16(v);
+  } else {
+    ((float*)e.C)[off] = v;
+  }
+}
+
+// Stage a [ROWS x GBK] operand tile into LDS laid out [row][k] (k contiguous).
+//   TRANS = false: element (r,k) at X[(r0+r)*ld + k0+k]   (k contiguous in memory)
+//   TRANS = true : element (r,k) at X[(k0+k)*ld + r0+r]   (r contiguous in memory)
+template <typename T, bool TRANS, int LDK>
+__device__ inline void stage_tile(T* __restrict__ s, const T* __restrict__ X, int64_t ld,
+                                  int64_t r0, int64_t rmax, int64_t k0, int64_t kend, int tid) {
+  constexpr int VEC = 16 / (int)sizeof(T);             // elements per 16-byte vector
+  constexpr int NVEC = GBM * GBK / VEC / GNT;          // vectors per thread
+#pragma unroll
+  for (int j = 0; j < NVEC; ++j) {
+    int v = tid * NVEC + j;
+    T vals[VEC];
+    if (!TRANS) {
+      int r = v / (GBK / VEC);
+      int k = (v % (GBK / VEC)) * VEC;
+      int64_t gr = r0 + r, gk = k0 + k;
+      if (gr < rmax && gk < kend) {
+        uint4 q = *reinterpret_cast<const uint4*>(X + gr * ld + gk);
+        *reinterpret_cast<uint4*>(vals) = q;
+      } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) vals[e] = T(0);
+      }
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) s[r * LDK + k + e] = vals[e];
+    } else {
+      int k = v / (GBM / VEC);
+      int r = (v % (GBM / VEC)) * VEC;
+      int64_t gr = r0 + r, gk = k0 + k;
+      if (gk < kend && gr + VEC <= rmax) {
+        uint4 q = *reinterpret_cast<const uint4*>(X + gk * ld + gr);
+        *reinterpret_cast<uint4*>(vals) = q;
+      } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+          vals[e] = (gk < kend && gr + e < rmax) ? X[gk * ld + gr + e] : T(0);
+      }
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) s[(r + e) * LDK + k] = vals[e];
+    }
+  }
+}
+
+template <typename T> struct MfmaTraits;
+template <> struct MfmaTraits<bf16_t> {
+  static constexpr int LDK = GBK + 8;
+  __device__ static inline void step(const bf16_t* As, const bf16_t* Bs, int wm, int wn, int lane,
+                                     f32x16_t acc[2][2]) {
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 16) {
+      bf16x8_t a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const bf16_t* pa = As + (wm * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5) * 8;
+        const bf16_t* pb = Bs + (wn * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5) * 8;
+        a[i] = *reinterpret_cast<const bf16x8_t*>(pa);
+        b[i] = *reinterpret_cast<const bf16x8_t*>(pb);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+};
+template <> struct MfmaTraits<float> {
+  static constexpr int LDK = GBK + 1;
+  __device__ static inline void step(const float* As, const float* Bs, int wm, int wn, int lane,
+                                     f32x16_t acc[2][2]) {
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 2) {
+      float a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[i] = As[(wm * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5)];
+        b[i] = Bs[(wn * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+};
+
+template <typename T, bool TA, bool TB>
+__global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
+  constexpr int LDK = MfmaTraits<T>::LDK;
+  __shared__ __attribute__((aligned(16))) T As[GBM * LDK];
+  __shared__ __attribute__((aligned(16))) T Bs[GBN * LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t m0 = (int64_t)blockIdx.x * GBM;
+  const int64_t n0 = (int64_t)blockIdx.y * GBN;
+  const int64_t kbeg = (int64_t)blockIdx.z * p.k_chunk;
+  const int64_t kend = kbeg + p.k_chunk < p.K ? kbeg + p.k_chunk : p.K;
+  const T* A = (const T*)p.A;
+  const T* B = (const T*)p.B;
+
+  f32x16_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  for (int64_t k0 = kbeg; k0 < kend; k0 += GBK) {
+    stage_tile<T, TA, LDK>(As, A, p.lda, m0, p.M, k0, kend, tid);
+    stage_tile<T, TB, LDK>(Bs, B, p.ldb, n0, p.N, k0, kend, tid);
+    __syncthreads();
+    MfmaTraits<T>::step(As, Bs, wm, wn, lane, acc);
+    __syncthreads();
+  }
+
+  const EpiParams& e = p.e;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t n = n0 + wn * 64 + j * 32 + (lane & 31);
+    float csum = 0.0f;
+    if (n < p.N) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          if (m < p.M) {
+            float v = epilogue_value(e, m, n, acc[i][j][r]);
+            csum += v;
+            epilogue_store(e, m, n, v);
+          }
+        }
+      }
+    }
+    if (e.colsum) {
+      csum += __shfl_xor(csum, 32);
+      if (lane < 32 && n < p.N) atomic_add_f32(e.colsum + n, csum);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// narrow layers (N <= 4)
+// ---------------------------------------------------------------------------
+constexpr int kNarrowMax = 4;
+
+template <typename TX>
+__global__ void __launch_bounds__(256)
+narrow_fwd_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
+                  const float* __restrict__ W, const float* __restrict__ b, int N, int act,
+                  float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t m = wave; m < M; m += nwaves) {
+    float s[kNarrowMax] = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t k = lane; k < K; k += 64) {
+      float x = scalar_io<TX>::load(X + m * ldx + k);
+#pragma unroll
+      for (int n = 0; n < kNarrowMax; ++n)
+        if (n < N) s[n] = fmaf(x, W[n * K + k], s[n]);
+    }
+#pragma unroll
+    for (int n = 0; n < kNarrowMax; ++n) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s[n] += __shfl_xor(s[n], o);
+    }
+    if (lane == 0) {
+      for (int n = 0; n < N; ++n) out[m * N + n] = act_forward(act, s[n] + b[n], 1.0f);
+    }
+  }
+}
+
+template <typename TX, typename TDX>
+__global__ void __launch_bounds__(256)
+narrow_bwd_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
+                  const float* __restrict__ W, int N, const float* __restrict__ dpre,
+                  float* __restrict__ dW, float* __restrict__ db, TDX* __restrict__ dX,
+                  int64_t lddx, int add_dx, const void* Y, int64_t ldy, int y_dtype, int dact,
+                  float beta, float* __restrict__ dx_colsum) {
+  // each lane owns k = lane + 64*j, j < KJ; dW partials stay in registers
+  constexpr int KJ = 8;  // K <= 512
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float gw[kNarrowMax][KJ];
+  float gb[kNarrowMax] = {0.f, 0.f, 0.f, 0.f};
+  float cs[KJ];
+#pragma unroll
+  for (int j = 0; j < KJ; ++j) cs[j] = 0.0f;
+#pragma unroll
+  for (int n = 0; n < kNarrowMax; ++n)
+#pragma unroll
+    for (int j = 0; j < KJ; ++j) gw[n][j] = 0.0f;
+  for (int64_t m = wave; m < M; m += nwaves) {
+    float d[kNarrowMax];
+#pragma unroll
+    for (int n = 0; n < kNarrowMax; ++n) d[n] = n < N ? dpre[m * N + n] : 0.0f;
+#pragma unroll
+    for (int n = 0; n < kNarrowMax; ++n) gb[n] += d[n];
+#pragma unroll
+    for (int j = 0; j < KJ; ++j) {
+      int64_t k = lane + 64 * j;
+      if (k < K) {
+        float x = scalar_io<TX>::load(X + m * ldx + k);
+        float dx = 0.0f;
+#pragma unroll
+        for (int n = 0; n < kNarrowMax; ++n) {
+          if (n < N) {
+            gw[n][j] = fmaf(d[n], x, gw[n][j]);
+            dx = fmaf(d[n], W[n * K + k], dx);
+          }
+        }
+        if (dX) {
+          if (add_dx) dx += scalar_io<TDX>::load(dX + m * lddx + k);
+          if (dact) dx *= act_deriv_from_y(dact, load_as_f32(Y, y_dtype, m * ldy + k), beta);
+          scalar_io<TDX>::store(dX + m * lddx + k, dx);
+          cs[j] += dx;
+        }
+      }
+    }
+  }
+  for (int n = 0; n < N; ++n) {
+#pragma unroll
+    for (int j = 0; j < KJ; ++j) {
+      int64_t k = lane + 64 * j;
+      if (k < K) atomic_add_f32(dW + n * K + k, gw[n][j]);
+    }
+    float s = gb[n];
+    if (lane == 0) atomic_add_f32(db + n, s);
+  }
+  if (dX && dx_colsum) {
+#pragma unroll
+    for (int j = 0; j < KJ; ++j) {
+      int64_t k = lane + 64 * j;
+      if (k < K) atomic_add_f32(dx_colsum + k, cs[j]);
+    }
+  }
+}
+
+// out = dy * act'(y) with optional column sums (bias gradient of the layer)
+template <typename TD, typename TY, typename TO>
+__global__ void __launch_bounds__(256)
+act_grad_kernel(const TD* __restrict__ dy, int64_t ld_dy, const TY* __restrict__ y, int64_t ld_y,
+                int64_t M, int64_t N, int act, float beta, TO* __restrict__ out, int64_t ld_out,
+                float* __restrict__ colsum) {
+  // block: 256 threads = 64 columns x 4 row lanes; grid.x over row chunks, grid.y over columns
+  const int64_t n = (int64_t)blockIdx.y * 64 + (threadIdx.x & 63);
+  const int rl = threadIdx.x >> 6;
+  const int64_t rows_per_block = 256;
+  const int64_t m0 = (int64_t)blockIdx.x * rows_per_block;
+  float cs = 0.0f;
+  if (n < N) {
+    for (int64_t m = m0 + rl; m < M && m < m0 + rows_per_block; m += 4) {
+      float v = scalar_io<TD>::load(dy + m * ld_dy + n) *
+                act_deriv_from_y(act, scalar_io<TY>::load(y + m * ld_y + n), beta);
+      scalar_io<TO>::store(out + m * ld_out + n, v);
+      cs += v;
+    }
+  }
+  if (colsum) {
+    __shared__ float red[256];
+    red[threadIdx.x] = cs;
+    __syncthreads();
+    if (rl == 0 && n < N) {
+      float s = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
+      atomic_add_f32(colsum + n, s);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weight normalisation (one block per output row)
+// ---------------------------------------------------------------------------
+__device__ inline float block_sum_256(float v, float* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  float s = 0.0f;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+  return s;
+}
+
+__global__ void __launch_bounds__(256)
+weight_norm_fwd_kernel(const float* __restrict__ v, const float* __restrict__ g, int64_t K,
+                       float* __restrict__ W, void* Wlow, int low_dtype, int64_t ld_low) {
+  __shared__ float red[4];
+  const int64_t n = blockIdx.x;
+  float ss = 0.0f;
+  for (int64_t k = threadIdx.x; k < K; k += blockDim.x) {
+    float x = v[n * K + k];
+    ss = fmaf(x, x, ss);
+  }
+  float norm = sqrtf(block_sum_256(ss, red));
+  float scale = g[n] / norm;
+  for (int64_t k = threadIdx.x; k < K; k += blockDim.x) {
+    float w = v[n * K + k] * scale;
+    W[n * K + k] = w;
+    if (Wlow) {
+      if (low_dtype == NA_BF16) ((bf16_t*)Wlow)[n * ld_low + k] = f32_to_bf16(w);
+      else ((float*)Wlow)[n * ld_low + k] = w;
+    }
+  }
+  if (Wlow) {  // zero the padding columns of the low-precision copy
+    for (int64_t k = K + threadIdx.x; k < ld_low; k += blockDim.x) {
+      if (low_dtype == NA_BF16) ((bf16_t*)Wlow)[n * ld_low + k] = 0;
+      else ((float*)Wlow)[n * ld_low + k] = 0.0f;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+weight_norm_bwd_kernel(const float* __restrict__ v, const float* __restrict__ g,
+                       const float* __restrict__ dW, int64_t K, float* __restrict__ dv,
+                       float* __restrict__ dg) {
+  __shared__ float red[4];
+  const int64_t n = blockIdx.x;
+  float ss = 0.0f, dot = 0.0f;
+  for (int64_t k = threadIdx.x; k < K; k += blockDim.x) {
+    float x = v[n * K + k];
+    ss = fmaf(x, x, ss);
+    dot = fmaf(x, dW[n * K + k], dot);
+  }
+  float norm = sqrtf(block_sum_256(ss, red));
+  float d = block_sum_256(dot, red);
+  float dgn = d / norm;
+  if (threadIdx.x == 0) dg[n] += dgn;
+  float s = g[n] / norm;
+  for (int64_t k = threadIdx.x; k < K; k += blockDim.x) {
+    dv[n * K + k] += s * (dW[n * K + k] - dgn * v[n * K + k] / norm);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host dispatch
+// ---------------------------------------------------------------------------
+template <typename T>
+static int launch_gemm(const na_gemm_args* a, const GemmParams& p, dim3 grid, hipStream_t s) {
+  if (!a->trans_a && !a->trans_b)
+    hipLaunchKernelGGL((gemm_kernel<T, false, false>), grid, dim3(GNT), 0, s, p);
+  else if (!a->trans_a && a->trans_b)
+    hipLaunchKernelGGL((gemm_kernel<T, false, true>), grid, dim3(GNT), 0, s, p);
+  else if (a->trans_a && a->trans_b)
+    hipLaunchKernelGGL((gemm_kernel<T, true, true>), grid, dim3(GNT), 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, true, false>), grid, dim3(GNT), 0, s, p);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+static inline int num_waves_for(int64_t rows) {
+  int64_t w = rows < 4096 ? rows : 4096;
+  return (int)(w < 1 ? 1 : w);
+}
+
+}  // namespace na
+
+using namespace na;
+
+extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
+  NA_CHECK_ARG(a != nullptr, "args is null");
+  NA_CHECK_ARG(a->M >= 0 && a->N >= 0 && a->K >= 0, "negative size");
+  if (a->M == 0 || a->N == 0) return NA_OK;
+  NA_CHECK_ARG(a->in_dtype == NA_F32 || a->in_dtype == NA_BF16, "in_dtype must be F32 or BF16");
+  NA_CHECK_ARG(a->A && a->B && a->C, "null operand");
+  const int vec = a->in_dtype == NA_BF16 ? 8 : 4;
+  NA_CHECK_ARG(a->lda % vec == 0 && a->ldb % vec == 0, "leading dimensions must be multiples of 8 (bf16) / 4 (fp32)");
+  NA_CHECK_ARG(((uintptr_t)a->A & 15) == 0 && ((uintptr_t)a->B & 15) == 0, "operands must be 16-byte aligned");
+  NA_CHECK_ARG(a->K % vec == 0, "K must be a multiple of 8 (bf16) / 4 (fp32)");
+  NA_CHECK_ARG(a->epilogue == NA_EPI_STORE || a->epilogue == NA_EPI_ATOMIC, "bad epilogue");
+  NA_CHECK_ARG(a->epilogue == NA_EPI_ATOMIC || a->c_dtype == NA_F32 || a->c_dtype == NA_BF16, "bad c_dtype");
+  NA_CHECK_ARG(a->epilogue == NA_EPI_STORE || a->c_dtype == NA_F32, "atomic epilogue needs fp32 C");
+  int split = a->split_k < 1 ? 1 : a->split_k;
+  NA_CHECK_ARG(split == 1 || a->epilogue == NA_EPI_ATOMIC, "split_k > 1 needs the atomic epilogue");
+  NA_CHECK_ARG(split == 1 || (!a->act && !a->dact && !a->bias && !a->r1_a),
+               "split_k > 1 allows no nonlinear epilogue terms");
+  NA_CHECK_ARG(!a->dact || a->dact_y, "dact needs dact_y");
+  NA_CHECK_ARG(!a->r1_a || a->r1_b, "r1_a needs r1_b");
+  NA_CHECK_ARG((a->M + GBM - 1) / GBM < 0x7fffffffLL, "M too large");
+  GemmParams p;
+  p.M = a->M; p.N = a->N; p.K = a->K;
+  p.A = a->A; p.lda = a->lda; p.B = a->B; p.ldb = a->ldb;
+  int64_t kc = ceil_div(ceil_div(a->K, split), GBK) * GBK;
+  if (kc < GBK) kc = GBK;
+  p.k_chunk = kc;
+  int64_t nsplit = a->K == 0 ? 1 : ceil_div(a->K, kc);
+  p.e.epilogue = a->epilogue; p.e.C = a->C; p.e.ldc = a->ldc; p.e.c_dtype = a->c_dtype;
+  p.e.bias = a->bias; p.e.act = a->act; p.e.beta = a->act_beta;
+  p.e.dact_y = a->dact_y; p.e.ld_dact = a->ld_dact; p.e.dact_dtype = a->dact_dtype; p.e.dact = a->dact;
+  p.e.r1_a = a->r1_a; p.e.r1_b = a->r1_b; p.e.colsum = a->colsum;
+  dim3 grid((unsigned)ceil_div(a->M, GBM), (unsigned)ceil_div(a->N, GBN), (unsigned)nsplit);
+  hipStream_t s = (hipStream_t)stream;
+  if (a->in_dtype == NA_BF16) return launch_gemm<bf16_t>(a, p, grid, s);
+  return launch_gemm<float>(a, p, grid, s);
+}
+
+extern "C" int na_linear_narrow_fwd(const void* X, int64_t ldx, int32_t x_dtype, int64_t M,
+                                    int64_t K, const float* W, const float* b, int32_t N,
+                                    int32_t act, float* out, na_stream_t stream) {
+  NA_CHECK_ARG(N >= 1 && N <= kNarrowMax, "N must be in [1, 4]");
+  NA_CHECK_ARG(M >= 0 && K >= 1, "bad sizes");
+  if (M == 0) return NA_OK;
+  NA_CHECK_ARG(X && W && b && out, "null pointer");
+  int waves = num_waves_for(M);
+  dim3 grid((unsigned)ceil_div(waves, 4));
+  hipStream_t s = (hipStream_t)stream;
+  if (x_dtype == NA_BF16)
+    hipLaunchKernelGGL(narrow_fwd_kernel<bf16_t>, grid, dim3(256), 0, s, (const bf16_t*)X, ldx, M,
+                       K, W, b, N, act, out);
+  else
+    hipLaunchKernelGGL(narrow_fwd_kernel<float>, grid, dim3(256), 0, s, (const float*)X, ldx, M,
+                       K, W, b, N, act, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_linear_narrow_bwd(const void* X, int64_t ldx, int32_t x_dtype, int64_t M,
+                                    int64_t K, const float* W, int32_t N, const float* dpre,
+                                    float* dW, float* db, void* dX, int64_t lddx, int32_t dx_dtype,
+                                    int32_t add_dx, const void* Y, int64_t ldy, int32_t y_dtype,
+                                    int32_t dact, float act_beta, float* dx_colsum,
+                                    na_stream_t stream) {
+  NA_CHECK_ARG(N >= 1 && N <= kNarrowMax, "N must be in [1, 4]");
+  NA_CHECK_ARG(K >= 1 && K <= 512, "K must be in [1, 512]");
+  NA_CHECK_ARG(M >= 0, "bad sizes");
+  if (M == 0) return NA_OK;
+  NA_CHECK_ARG(X && W && dpre && dW && db, "null pointer");
+  NA_CHECK_ARG(!dact || Y, "dact needs Y");
+  int waves = num_waves_for(M);
+  dim3 grid((unsigned)ceil_div(waves, 4));
+  hipStream_t s = (hipStream_t)stream;
+#define NA_NARROW_BWD(TX, TDX)                                                             \
+  hipLaunchKernelGGL((narrow_bwd_kernel<TX, TDX>), grid, dim3(256), 0, s, (const TX*)X, ldx, M, \
+                     K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, Y, ldy, y_dtype, dact,   \
+                     act_beta, dx_colsum)
+  if (x_dtype == NA_BF16 && dx_dtype == NA_BF16) NA_NARROW_BWD(bf16_t, bf16_t);
+  else if (x_dtype == NA_BF16) NA_NARROW_BWD(bf16_t, float);
+  else if (dx_dtype == NA_BF16) NA_NARROW_BWD(float, bf16_t);
+  else NA_NARROW_BWD(float, float);
+#undef NA_NARROW_BWD
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_act_grad(const void* dy, int64_t ld_dy, int32_t dy_dtype, const void* y,
+                           int64_t ld_y, int32_t y_dtype, int64_t M, int64_t N, int32_t act,
+                           float act_beta, void* out, int64_t ld_out, int32_t out_dtype,
+                           float* colsum, na_stream_t stream) {
+  NA_CHECK_ARG(M >= 0 && N >= 0, "bad sizes");
+  if (M == 0 || N == 0) return NA_OK;
+  NA_CHECK_ARG(dy && y && out, "null pointer");
+  dim3 grid((unsigned)ceil_div(M, 256), (unsigned)ceil_div(N, 64));
+  hipStream_t s = (hipStream_t)stream;
+#define NA_ACTG(TD, TY, TO)                                                                   \
+  hipLaunchKernelGGL((act_grad_kernel<TD, TY, TO>), grid, dim3(256), 0, s, (const TD*)dy, ld_dy, \
+                     (const TY*)y, ld_y, M, N, act, act_beta, (TO*)out, ld_out, colsum)
+  const bool d16 = dy_dtype == NA_BF16, y16 = y_dtype == NA_BF16, o16 = out_dtype == NA_BF16;
+  if (!d16 && !y16 && !o16) NA_ACTG(float, float, float);
+  else if (!d16 && !y16 && o16) NA_ACTG(float, float, bf16_t);
+  else if (!d16 && y16 && !o16) NA_ACTG(float, bf16_t, float);
+  else if (!d16 && y16 && o16) NA_ACTG(float, bf16_t, bf16_t);
+  else if (d16 && !y16 && !o16) NA_ACTG(bf16_t, float, float);
+  else if (d16 && !y16 && o16) NA_ACTG(bf16_t, float, bf16_t);
+  else if (d16 && y16 && !o16) NA_ACTG(bf16_t, bf16_t, float);
+  else NA_ACTG(bf16_t, bf16_t, bf16_t);
+#undef NA_ACTG
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_weight_norm_fwd(const float* v, const float* g, int64_t N, int64_t K, float* W,
+                                  void* Wlow, int32_t low_dtype, int64_t ld_low,
+                                  na_stream_t stream) {
+  NA_CHECK_ARG(N >= 1 && K >= 1, "bad sizes");
+  NA_CHECK_ARG(v && g && W, "null pointer");
+  NA_CHECK_ARG(!Wlow || ld_low >= K, "ld_low < K");
+  hipLaunchKernelGGL(weight_norm_fwd_kernel, dim3((unsigned)N), dim3(256), 0, (hipStream_t)stream,
+                     v, g, K, W, Wlow, low_dtype, ld_low);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_weight_norm_bwd(const float* v, const float* g, const float* dW, int64_t N,
+                                  int64_t K, float* dv, float* dg, na_stream_t stream) {
+  NA_CHECK_ARG(N >= 1 && K >= 1, "bad sizes");
+  NA_CHECK_ARG(v && g && dW && dv && dg, "null pointer");
+  hipLaunchKernelGGL(weight_norm_bwd_kernel, dim3((unsigned)N), dim3(256), 0, (hipStream_t)stream,
+                     v, g, dW, K, dv, dg);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}

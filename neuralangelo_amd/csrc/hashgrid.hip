@@ -1,0 +1,422 @@
+// Multiresolution hash-grid encoding: forward gather and backward scatter.
+//
+// Replaces tcnn.Encoding(3, {"otype": "HashGrid", ...}) as constructed at
+// projects/neuralangelo/utils/modules.py:42-50 and called at :84-86.
+// Algorithm (Instant-NGP, Mueller et al. 2022, as implemented by tiny-cuda-nn):
+//   per level l: pos = fmaf(scale_l, x_d, 0.5); cell = floor(pos); t = pos - cell
+//   dense level  (res^3 <= size_l): idx = (cx + cy*res + cz*res^2) mod size_l
+//   hashed level (res^3 >  size_l): idx = (cx ^ cy*2654435761 ^ cz*805459861) mod T
+//   out[l*F+f] = sum_{corner c} (prod_d t_d or 1-t_d) * table[(off_l+idx_c)*F+f]
+// corners are enumerated with bit d of c selecting +1 along dim d, weights are
+// multiplied in dim order 0,1,2 and accumulated with fmaf in corner order.
+//
+// Layout: table is AoS per entry ([entries][F]), levels concatenated; the
+// output is [n, L*F] row-major (level-major within a row), which is what the
+// SDF MLP consumes.  Forward: one thread per (point, level), blockIdx.y =
+// level so a workgroup gathers from one level's table (keeps that level's
+// working set in L2/MALL).  Backward: one thread per (point, feature) so the F
+// atomics of one corner are adjacent lanes of one instruction and coalesce
+// into a single L2 atomic request.
+#include "common.h"
+
+namespace na {
+
+struct LevelParams {
+  float scale;
+  uint32_t res;
+  uint32_t off;
+  uint32_t size;
+  bool hashed;
+};
+
+__device__ inline LevelParams level_params(const na_grid_desc& d, int l) {
+  LevelParams p;
+  p.scale = d.scale[l];
+  p.res = d.resolution[l];
+  p.off = d.offset[l];
+  p.size = d.offset[l + 1] - p.off;
+  uint64_t r = p.res;
+  p.hashed = r * r * r > (uint64_t)p.size;
+  return p;
+}
+
+__device__ inline void pos_fract(float x, float scale, uint32_t& cell, float& t) {
+  float pos = fmaf(scale, x, 0.5f);
+  float fl = floorf(pos);
+  cell = (uint32_t)(int)fl;
+  t = pos - fl;
+}
+
+__device__ inline uint32_t corner_index(uint32_t x, uint32_t y, uint32_t z,
+                                        const LevelParams& p) {
+  if (p.hashed) {
+    return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (p.size - 1u);
+  }
+  return (x + y * p.res + z * p.res * p.res) % p.size;
+}
+
+__device__ inline float corner_weight(int c, const float t[3]) {
+  float w = (c & 1) ? t[0] : 1.0f - t[0];
+  w *= (c & 2) ? t[1] : 1.0f - t[1];
+  w *= (c & 4) ? t[2] : 1.0f - t[2];
+  return w;
+}
+
+// Load F table values of one entry as floats (vectorised).
+template <int F, typename TT> struct EntryLoad;
+template <int F> struct EntryLoad<F, float> {
+  __device__ static inline void load(const float* __restrict__ p, float v[F]) {
+    if constexpr (F == 8) {
+      float4 a = *reinterpret_cast<const float4*>(p);
+      float4 b = *reinterpret_cast<const float4*>(p + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else if constexpr (F == 4) {
+      float4 a = *reinterpret_cast<const float4*>(p);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else if constexpr (F == 2) {
+      float2 a = *reinterpret_cast<const float2*>(p);
+      v[0] = a.x; v[1] = a.y;
+    } else {
+      v[0] = p[0];
+    }
+  }
+};
+template <int F> struct EntryLoad<F, bf16_t> {
+  __device__ static inline void load(const bf16_t* __restrict__ p, float v[F]) {
+    if constexpr (F == 8) {
+      uint4 a = *reinterpret_cast<const uint4*>(p);
+      uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        v[2 * k] = bf16_to_f32((bf16_t)(w[k] & 0xffffu));
+        v[2 * k + 1] = bf16_to_f32((bf16_t)(w[k] >> 16));
+      }
+    } else if constexpr (F == 4) {
+      uint2 a = *reinterpret_cast<const uint2*>(p);
+      v[0] = bf16_to_f32((bf16_t)(a.x & 0xffffu));
+      v[1] = bf16_to_f32((bf16_t)(a.x >> 16));
+      v[2] = bf16_to_f32((bf16_t)(a.y & 0xffffu));
+      v[3] = bf16_to_f32((bf16_t)(a.y >> 16));
+    } else if constexpr (F == 2) {
+      uint32_t a = *reinterpret_cast<const uint32_t*>(p);
+      v[0] = bf16_to_f32((bf16_t)(a & 0xffffu));
+      v[1] = bf16_to_f32((bf16_t)(a >> 16));
+    } else {
+      v[0] = bf16_to_f32(p[0]);
+    }
+  }
+};
+
+template <int F, typename TO> struct RowStore;
+template <int F> struct RowStore<F, float> {
+  __device__ static inline void store(float* __restrict__ p, const float v[F]) {
+    if constexpr (F == 8) {
+      *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+      *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    } else if constexpr (F == 4) {
+      *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (F == 2) {
+      *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else {
+      p[0] = v[0];
+    }
+  }
+};
+template <int F> struct RowStore<F, bf16_t> {
+  __device__ static inline void store(bf16_t* __restrict__ p, const float v[F]) {
+    uint32_t w[(F + 1) / 2];
+#pragma unroll
+    for (int k = 0; k < F / 2; ++k)
+      w[k] = (uint32_t)f32_to_bf16(v[2 * k]) | ((uint32_t)f32_to_bf16(v[2 * k + 1]) << 16);
+    if constexpr (F == 8) {
+      *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else if constexpr (F == 4) {
+      *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+    } else if constexpr (F == 2) {
+      *reinterpret_cast<uint32_t*>(p) = w[0];
+    } else {
+      p[0] = f32_to_bf16(v[0]);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock)
+hashgrid_corners_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
+                        int32_t* __restrict__ idx_out, float* __restrict__ w_out) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int l = blockIdx.y;
+  if (i >= n) return;
+  LevelParams p = level_params(d, l);
+  uint32_t cell[3];
+  float t[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) pos_fract(x[i * 3 + k], p.scale, cell[k], t[k]);
+  int64_t base = (i * d.n_levels + l) * 8;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                               cell[2] + ((c >> 2) & 1), p);
+    idx_out[base + c] = (int32_t)(p.off + id);
+    w_out[base + c] = corner_weight(c, t);
+  }
+}
+
+// Input point -> unit-cube coordinate: u = (x - vmin) / vrange
+// (projects/neuralangelo/utils/modules.py:82-83; vmin = 0, vrange = 1 is the
+// tcnn convention of inputs already in [0,1]).
+__device__ inline float normalize_coord(float x, float vmin, float vrange) {
+  return (x - vmin) / vrange;
+}
+
+template <int F, typename TT, typename TO>
+__global__ void __launch_bounds__(kBlock)
+hashgrid_fwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
+                    float vrange, const TT* __restrict__ table, TO* __restrict__ out, int64_t ld,
+                    int write_extras, int active_levels) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int l = blockIdx.y;
+  if (i >= n) return;
+  const int L = d.n_levels;
+  TO* o = out + i * ld + l * F;
+  if (write_extras && l == 0) {
+    // [enc(L*F) | x(3) | zero pad up to ld]
+    TO* e = out + i * ld + L * F;
+    for (int k = 0; k < 3; ++k) scalar_io<TO>::store(e + k, x[i * 3 + k]);
+    for (int64_t c = L * F + 3; c < ld; ++c) scalar_io<TO>::store(out + i * ld + c, 0.0f);
+  }
+  float acc[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) acc[f] = 0.0f;
+  if (l < active_levels) {
+    LevelParams p = level_params(d, l);
+    uint32_t cell[3];
+    float t[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      pos_fract(normalize_coord(x[i * 3 + k], vmin, vrange), p.scale, cell[k], t[k]);
+    const TT* tab = table + (size_t)p.off * F;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                                 cell[2] + ((c >> 2) & 1), p);
+      float w = corner_weight(c, t);
+      float v[F];
+      EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
+#pragma unroll
+      for (int f = 0; f < F; ++f) acc[f] = fmaf(w, v[f], acc[f]);
+    }
+  }
+  RowStore<F, TO>::store(o, acc);
+}
+
+template <int F, typename TD>
+__global__ void __launch_bounds__(kBlock)
+hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
+                    float vrange, const TD* __restrict__ dy, int64_t ld,
+                    float* __restrict__ grad) {
+  int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int l = blockIdx.y;
+  int64_t i = tid / F;
+  int f = (int)(tid % F);
+  if (i >= n) return;
+  const int L = d.n_levels;
+  (void)L;
+  float g = scalar_io<TD>::load(dy + i * ld + l * F + f);
+  if (g == 0.0f) return;  // adding +0 leaves the accumulator unchanged
+  LevelParams p = level_params(d, l);
+  uint32_t cell[3];
+  float t[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    pos_fract(normalize_coord(x[i * 3 + k], vmin, vrange), p.scale, cell[k], t[k]);
+  float* gtab = grad + (size_t)p.off * F + f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                               cell[2] + ((c >> 2) & 1), p);
+    float w = corner_weight(c, t);
+    atomic_add_f32(gtab + (size_t)id * F, w * g);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static int check_desc(const na_grid_desc* d) {
+  NA_CHECK_ARG(d != nullptr, "desc is null");
+  NA_CHECK_ARG(d->n_levels >= 1 && d->n_levels <= NA_MAX_LEVELS, "n_levels out of range");
+  NA_CHECK_ARG(d->n_features == 1 || d->n_features == 2 || d->n_features == 4 ||
+                   d->n_features == 8,
+               "n_features must be 1, 2, 4 or 8");
+  return NA_OK;
+}
+
+struct FwdLaunch {
+  const na_grid_desc* d;
+  const float* x;
+  int64_t n;
+  float vmin, vrange;
+  const void* table;
+  void* out;
+  int64_t ld;
+  int extras;
+  int active;
+  hipStream_t s;
+};
+
+template <int F, typename TT, typename TO>
+static int launch_fwd(const FwdLaunch& a) {
+  dim3 grid((unsigned)ceil_div(a.n, kBlock), a.d->n_levels);
+  hipLaunchKernelGGL((hashgrid_fwd_kernel<F, TT, TO>), grid, dim3(kBlock), 0, a.s, *a.d, a.x, a.n,
+                     a.vmin, a.vrange, (const TT*)a.table, (TO*)a.out, a.ld, a.extras, a.active);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+template <int F>
+static int dispatch_fwd(const FwdLaunch& a, int tdt, int odt) {
+  if (tdt == NA_F32 && odt == NA_F32) return launch_fwd<F, float, float>(a);
+  if (tdt == NA_F32 && odt == NA_BF16) return launch_fwd<F, float, bf16_t>(a);
+  if (tdt == NA_BF16 && odt == NA_F32) return launch_fwd<F, bf16_t, float>(a);
+  if (tdt == NA_BF16 && odt == NA_BF16) return launch_fwd<F, bf16_t, bf16_t>(a);
+  set_error("hash-grid forward: unsupported dtype combination");
+  return NA_ERR_UNSUPPORTED;
+}
+
+static int run_fwd(const FwdLaunch& a, int tdt, int odt) {
+  switch (a.d->n_features) {
+    case 1: return dispatch_fwd<1>(a, tdt, odt);
+    case 2: return dispatch_fwd<2>(a, tdt, odt);
+    case 4: return dispatch_fwd<4>(a, tdt, odt);
+    default: return dispatch_fwd<8>(a, tdt, odt);
+  }
+}
+
+struct BwdLaunch {
+  const na_grid_desc* d;
+  const float* x;
+  int64_t n;
+  float vmin, vrange;
+  const void* dy;
+  int64_t ld;
+  float* grad;
+  int active;
+  hipStream_t s;
+};
+
+template <int F, typename TD>
+static int launch_bwd(const BwdLaunch& a) {
+  if (a.active <= 0) return NA_OK;
+  dim3 grid((unsigned)ceil_div(a.n * F, kBlock), a.active);
+  hipLaunchKernelGGL((hashgrid_bwd_kernel<F, TD>), grid, dim3(kBlock), 0, a.s, *a.d, a.x, a.n,
+                     a.vmin, a.vrange, (const TD*)a.dy, a.ld, a.grad);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+template <int F>
+static int dispatch_bwd(const BwdLaunch& a, int ddt) {
+  if (ddt == NA_F32) return launch_bwd<F, float>(a);
+  if (ddt == NA_BF16) return launch_bwd<F, bf16_t>(a);
+  set_error("hash-grid backward: unsupported dy dtype");
+  return NA_ERR_UNSUPPORTED;
+}
+
+static int run_bwd(const BwdLaunch& a, int ddt) {
+  switch (a.d->n_features) {
+    case 1: return dispatch_bwd<1>(a, ddt);
+    case 2: return dispatch_bwd<2>(a, ddt);
+    case 4: return dispatch_bwd<4>(a, ddt);
+    default: return dispatch_bwd<8>(a, ddt);
+  }
+}
+
+}  // namespace na
+
+using namespace na;
+
+extern "C" int na_hashgrid_corners(const na_grid_desc* desc, const float* x, int64_t n,
+                                   int32_t* idx, float* w, na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(x && idx && w, "null pointer");
+  dim3 grid((unsigned)ceil_div(n, kBlock), desc->n_levels);
+  hipLaunchKernelGGL(hashgrid_corners_kernel, grid, dim3(kBlock), 0, (hipStream_t)stream,
+                     *desc, x, n, idx, w);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_hashgrid_fwd(const na_grid_desc* desc, const float* x, int64_t n,
+                               const void* table, int32_t table_dtype, void* out,
+                               int32_t out_dtype, int32_t active_levels, na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(x && table && out, "null pointer");
+  NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
+  FwdLaunch a{desc, x, n, 0.0f, 1.0f, table, out,
+              (int64_t)desc->n_levels * desc->n_features, 0, active_levels, (hipStream_t)stream};
+  return run_fwd(a, table_dtype, out_dtype);
+}
+
+extern "C" int na_hashgrid_bwd(const na_grid_desc* desc, const float* x, int64_t n,
+                               const void* dy, int32_t dy_dtype, float* grad_table,
+                               int32_t active_levels, na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(x && dy && grad_table, "null pointer");
+  NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
+  int active = active_levels < desc->n_levels ? active_levels : desc->n_levels;
+  BwdLaunch a{desc, x, n, 0.0f, 1.0f, dy, (int64_t)desc->n_levels * desc->n_features,
+              grad_table, active, (hipStream_t)stream};
+  return run_bwd(a, dy_dtype);
+}
+
+extern "C" int na_sdf_input_fwd(const na_grid_desc* desc, const float* points, int64_t n,
+                                float vmin, float vmax, const void* table, int32_t table_dtype,
+                                int32_t active_levels, void* out, int32_t out_dtype, int64_t ld,
+                                na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(points && table && out, "null pointer");
+  NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
+  const int64_t LF = (int64_t)desc->n_levels * desc->n_features;
+  NA_CHECK_ARG(ld >= LF + 3, "ld must be >= L*F + 3");
+  NA_CHECK_ARG(ld % (out_dtype == NA_BF16 ? 8 : 4) == 0, "ld must keep rows 16-byte aligned");
+  NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
+  FwdLaunch a{desc, points, n, vmin, vmax - vmin, table, out, ld, 1, active_levels,
+              (hipStream_t)stream};
+  return run_fwd(a, table_dtype, out_dtype);
+}
+
+extern "C" int na_sdf_input_bwd(const na_grid_desc* desc, const float* points, int64_t n,
+                                float vmin, float vmax, const void* dy, int32_t dy_dtype,
+                                int64_t ld, float* grad_table, int32_t active_levels,
+                                na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(points && dy && grad_table, "null pointer");
+  NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
+  NA_CHECK_ARG(ld >= (int64_t)desc->n_levels * desc->n_features, "ld too small");
+  NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
+  int active = active_levels < desc->n_levels ? active_levels : desc->n_levels;
+  BwdLaunch a{desc, points, n, vmin, vmax - vmin, dy, ld, grad_table, active,
+              (hipStream_t)stream};
+  return run_bwd(a, dy_dtype);
+}

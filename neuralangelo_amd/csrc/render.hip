@@ -1,0 +1,857 @@
+// Per-ray operators of the NeuS / Neuralangelo renderer.
+//
+// Everything here is memory- or latency-bound elementwise/scan work.  Scans
+// along a ray (CDF, transmittance, merge) run one ray per lane with the
+// ray's prefix data staged in LDS, in a fixed sequential order so that the
+// discrete outputs (searchsorted indices, merge order) are reproducible
+// bit-for-bit.  FMA contraction is disabled in this file so every float op
+// rounds exactly where the reference formula rounds.
+#include "common.h"
+
+#pragma clang fp contract(off)
+
+namespace na {
+
+constexpr int kRayBlock = 64;  // rays per workgroup for the per-ray scan kernels
+
+__device__ inline float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+// torch.min over two values: NaN propagates.
+__device__ inline float tmin(float a, float b) { return (a != a || a < b) ? a : b; }
+
+// ---------------------------------------------------------------------------
+// bounds + coarse sampling
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+ray_bounds_sample_kernel(const float* __restrict__ center, const float* __restrict__ ray,
+                         const float* __restrict__ rands, int64_t R, int N,
+                         float* __restrict__ near_out, float* __restrict__ far_out,
+                         uint8_t* __restrict__ outside_out, float* __restrict__ dists) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * N) return;
+  int64_t r = t / N;
+  int i = (int)(t % N);
+  float c0 = center[r * 3], c1 = center[r * 3 + 1], c2 = center[r * 3 + 2];
+  float v0 = ray[r * 3], v1 = ray[r * 3 + 1], v2 = ray[r * 3 + 2];
+  float ctc = c0 * c0 + c1 * c1 + c2 * c2;
+  float ctv = c0 * v0 + c1 * v1 + c2 * v2;
+  float disc = ctv * ctv - (ctc - 1.0f);
+  float sq = sqrtf(disc);
+  float nr = -ctv - sq;
+  float fr = -ctv + sq;
+  nr = (nr != nr) ? nr : (nr > 0.0f ? nr : 0.0f);  // relu_, NaN kept
+  bool out = nr != nr;
+  if (out) {
+    nr = 1.0f;
+    fr = 1.2f;
+  }
+  if (i == 0) {
+    near_out[r] = nr;
+    far_out[r] = fr;
+    outside_out[r] = out ? 1 : 0;
+  }
+  float u = rands ? rands[t] : 0.5f;
+  float a = u + (float)i;
+  float d = a / (float)N * (fr - nr) + nr;
+  dists[t] = d;
+}
+
+// ---------------------------------------------------------------------------
+// inverse-CDF sampling (shared by na_sample_pdf and na_hier_resample)
+// cdf: LDS row of N floats (cdf[0] = 0, cdf[i] = prefix of pdf[0..i-1])
+// ---------------------------------------------------------------------------
+__device__ inline float linspace_val(int idx, int steps) {
+  // torch.linspace(0, 1, steps) in float32
+  float step = (1.0f - 0.0f) / (float)(steps - 1);
+  int halfway = steps / 2;
+  if (idx < halfway) return 0.0f + step * (float)idx;
+  return 1.0f - step * (float)(steps - idx - 1);
+}
+
+__device__ inline void pdf_sample_row(const float* __restrict__ cdf, int stride, int N,
+                                      const float* __restrict__ bins, int Nf,
+                                      float* __restrict__ out) {
+  for (int j = 0; j < Nf; ++j) {
+    float u = 0.5f * (linspace_val(j, Nf + 1) + linspace_val(j + 1, Nf + 1));
+    // searchsorted right=True: first index with cdf > u
+    int lo = 0, hi = N;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (cdf[mid * stride] <= u) lo = mid + 1;
+      else hi = mid;
+    }
+    int idx = lo;
+    int low = idx - 1 > 0 ? idx - 1 : 0;
+    int high = idx < N - 1 ? idx : N - 1;
+    float cl = cdf[low * stride], ch = cdf[high * stride];
+    float dmin = bins[low], dmax = bins[high];
+    float t = (u - cl) / (ch - cl + 1e-8f);
+    out[j] = dmin + t * (dmax - dmin);
+  }
+}
+
+// cdf column layout in LDS: element i of lane l at s[i * kRayBlock + l]
+__global__ void __launch_bounds__(kRayBlock)
+sample_pdf_kernel(const float* __restrict__ bins, const float* __restrict__ weights, int64_t R,
+                  int N, int Nf, float* __restrict__ out) {
+  extern __shared__ float s_cdf[];
+  const int l = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * kRayBlock + l;
+  if (r >= R) return;
+  const float* w = weights + r * (N - 1);
+  float l1 = 0.0f;
+  for (int i = 0; i < N - 1; ++i) l1 = l1 + fabsf(w[i]);
+  float denom = l1 > 1e-12f ? l1 : 1e-12f;
+  float* cdf = s_cdf + l;
+  float acc = 0.0f;
+  cdf[0] = 0.0f;
+  for (int i = 0; i < N - 1; ++i) {
+    acc = acc + w[i] / denom;
+    cdf[(i + 1) * kRayBlock] = acc;
+  }
+  pdf_sample_row(cdf, kRayBlock, N, bins + r * N, Nf, out + r * Nf);
+}
+
+__global__ void __launch_bounds__(kRayBlock)
+hier_resample_kernel(const float* __restrict__ dists, const float* __restrict__ sdfs, int64_t R,
+                     int N, float inv_s, int Nf, float* __restrict__ merged,
+                     float* __restrict__ fine, int32_t* __restrict__ pos_coarse,
+                     int32_t* __restrict__ pos_fine) {
+  extern __shared__ float s_cdf[];
+  const int l = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * kRayBlock + l;
+  if (r >= R) return;
+  const float* d = dists + r * N;
+  const float* s = sdfs + r * N;
+  float* cdf = s_cdf + l;
+  // pass 1: interval alphas (model.py:251-264) and compositing weights
+  // (render.py:94-98); the weights are parked in the cdf slots 1..N-1.
+  float T = 1.0f;
+  float prev_cos = 0.0f;
+  float l1 = 0.0f;
+  for (int i = 0; i < N - 1; ++i) {
+    float sp = s[i], sn = s[i + 1];
+    float dp = d[i], dn = d[i + 1];
+    float mid = (sp + sn) * 0.5f;
+    float cosv = (sn - sp) / (dn - dp + 1e-5f);
+    float cr = tmin(prev_cos, cosv);
+    prev_cos = cosv;
+    float intv = dn - dp;
+    float est_prev = mid - cr * intv * 0.5f;
+    float est_next = mid + cr * intv * 0.5f;
+    float pc = sigmoidf_(est_prev * inv_s);
+    float nc = sigmoidf_(est_next * inv_s);
+    float a = (pc - nc) / (pc + 1e-5f);
+    a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
+    float w = a * T;
+    T = T * (1.0f - a);
+    cdf[(i + 1) * kRayBlock] = w;
+    l1 = l1 + fabsf(w);
+  }
+  // pass 2: pdf -> cdf in place (nerf_util.py:50-53)
+  float denom = l1 > 1e-12f ? l1 : 1e-12f;
+  float acc = 0.0f;
+  cdf[0] = 0.0f;
+  for (int i = 0; i < N - 1; ++i) {
+    acc = acc + cdf[(i + 1) * kRayBlock] / denom;
+    cdf[(i + 1) * kRayBlock] = acc;
+  }
+  float* f = fine + r * Nf;
+  pdf_sample_row(cdf, kRayBlock, N, d, Nf, f);
+  // stable merge of the sorted coarse row and the (sorted) fine row
+  float* m = merged + r * (N + Nf);
+  int i = 0, j = 0;
+  for (int k = 0; k < N + Nf; ++k) {
+    bool take_coarse = (j >= Nf) || (i < N && !(f[j] < d[i]));
+    if (take_coarse) {
+      m[k] = d[i];
+      pos_coarse[r * N + i] = k;
+      ++i;
+    } else {
+      m[k] = f[j];
+      pos_fine[r * Nf + j] = k;
+      ++j;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+scatter_merge_kernel(const float* __restrict__ a, const int32_t* __restrict__ pa, int Na,
+                     const float* __restrict__ b, const int32_t* __restrict__ pb, int Nb,
+                     int64_t R, float* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int M = Na + Nb;
+  if (t >= R * M) return;
+  int64_t r = t / M;
+  int i = (int)(t % M);
+  if (i < Na) out[r * M + pa[r * Na + i]] = a[r * Na + i];
+  else out[r * M + pb[r * Nb + (i - Na)]] = b[r * Nb + (i - Na)];
+}
+
+// ---------------------------------------------------------------------------
+// NeuS alpha
+// ---------------------------------------------------------------------------
+struct NeusIn {
+  const float* ray;
+  const float* sdfs;
+  const float* grads;
+  const float* dists;
+  const float* far;
+  const uint8_t* outside;
+  float outside_val;
+  const float* s_var;
+  float anneal_ratio;
+  int64_t R;
+  int N;
+};
+
+struct NeusTerms {
+  float sdf, cosv, intv, iter_cos, est_prev, est_next, inv_s, pc, nc, ratio;
+  bool outside;
+};
+
+__device__ inline NeusTerms neus_terms(const NeusIn& p, int64_t t) {
+  NeusTerms q;
+  int64_t r = t / p.N;
+  int i = (int)(t % p.N);
+  q.outside = p.outside && p.outside[r];
+  q.sdf = q.outside ? p.outside_val : p.sdfs[t];
+  float v0 = p.ray[r * 3], v1 = p.ray[r * 3 + 1], v2 = p.ray[r * 3 + 2];
+  float g0 = p.grads[t * 3], g1 = p.grads[t * 3 + 1], g2 = p.grads[t * 3 + 2];
+  q.cosv = v0 * g0 + v1 * g1 + v2 * g2;
+  float a = p.anneal_ratio;
+  float u1 = -q.cosv * 0.5f + 0.5f;
+  float u2 = -q.cosv;
+  float r1 = u1 > 0.0f ? u1 : 0.0f;
+  float r2 = u2 > 0.0f ? u2 : 0.0f;
+  q.iter_cos = -(r1 * (1.0f - a) + r2 * a);
+  float dnext = (i + 1 < p.N) ? p.dists[t + 1] : (p.far ? p.far[r] : 1e10f);
+  q.intv = dnext - p.dists[t];
+  q.est_prev = q.sdf - q.iter_cos * q.intv * 0.5f;
+  q.est_next = q.sdf + q.iter_cos * q.intv * 0.5f;
+  q.inv_s = expf(*p.s_var);
+  q.pc = sigmoidf_(q.est_prev * q.inv_s);
+  q.nc = sigmoidf_(q.est_next * q.inv_s);
+  q.ratio = (q.pc - q.nc) / (q.pc + 1e-5f);
+  return q;
+}
+
+__global__ void __launch_bounds__(256)
+neus_alpha_fwd_kernel(NeusIn p, float* __restrict__ alpha) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p.R * p.N) return;
+  NeusTerms q = neus_terms(p, t);
+  float a = q.ratio;
+  alpha[t] = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);  // clip_, NaN kept
+}
+
+__global__ void __launch_bounds__(256)
+neus_alpha_bwd_kernel(NeusIn p, const float* __restrict__ dalpha, float* __restrict__ dsdf,
+                      float* __restrict__ dgrad, float* __restrict__ ds_var) {
+  __shared__ float red[4];
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float dsv = 0.0f;
+  if (t < p.R * p.N) {
+    NeusTerms q = neus_terms(p, t);
+    float da = dalpha[t];
+    // clamp backward: pass-through where 0 <= ratio <= 1
+    float dr = (q.ratio >= 0.0f && q.ratio <= 1.0f) ? da : 0.0f;
+    float num = q.pc - q.nc, den = q.pc + 1e-5f;
+    float dnum = dr / den;
+    float dden = -dr * num / (den * den);
+    float dpc = dnum + dden;
+    float dnc = -dnum;
+    float dzp = dpc * q.pc * (1.0f - q.pc);  // d(est_prev * inv_s)
+    float dzn = dnc * q.nc * (1.0f - q.nc);
+    float dep = dzp * q.inv_s, den_ = dzn * q.inv_s;
+    float dinv = dzp * q.est_prev + dzn * q.est_next;
+    dsv = dinv * q.inv_s;
+    dsdf[t] = q.outside ? 0.0f : (dep + den_);
+    float diter = (-dep + den_) * q.intv * 0.5f;
+    float a = p.anneal_ratio;
+    float u1 = -q.cosv * 0.5f + 0.5f;
+    float u2 = -q.cosv;
+    float dcos = diter * (0.5f * (1.0f - a) * (u1 > 0.0f ? 1.0f : 0.0f) + a * (u2 > 0.0f ? 1.0f : 0.0f));
+    int64_t r = t / p.N;
+    dgrad[t * 3] = dcos * p.ray[r * 3];
+    dgrad[t * 3 + 1] = dcos * p.ray[r * 3 + 1];
+    dgrad[t * 3 + 2] = dcos * p.ray[r * 3 + 2];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) dsv += __shfl_xor(dsv, o);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = dsv;
+  __syncthreads();
+  if (threadIdx.x == 0) atomic_add_f32(ds_var, red[0] + red[1] + red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------
+// alpha compositing + composite
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kRayBlock)
+composite_fwd_kernel(const float* __restrict__ alpha, const float* __restrict__ q, int64_t R,
+                     int N, int K, float* __restrict__ weights, float* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  if (r >= R) return;
+  float logT = 0.0f;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < N; ++i) {
+    float a = alpha[r * N + i];
+    float T = expf(logT);
+    float w = a * T;
+    weights[r * N + i] = w;
+    for (int k = 0; k < K; ++k) acc[k] = acc[k] + q[(r * N + i) * K + k] * w;
+    logT = logT + log1pf(-a);
+  }
+  for (int k = 0; k < K; ++k) out[r * K + k] = acc[k];
+}
+
+__global__ void __launch_bounds__(kRayBlock)
+composite_bwd_kernel(const float* __restrict__ alpha, const float* __restrict__ q,
+                     const float* __restrict__ weights, int64_t R, int N, int K,
+                     const float* __restrict__ dout, const float* __restrict__ dweights,
+                     float* __restrict__ dalpha, float* __restrict__ dq) {
+  extern __shared__ float s_T[];
+  const int l = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * kRayBlock + l;
+  if (r >= R) return;
+  float* T = s_T + l;
+  float logT = 0.0f;
+  for (int i = 0; i < N; ++i) {
+    T[i * kRayBlock] = expf(logT);
+    logT = logT + log1pf(-alpha[r * N + i]);
+  }
+  float d[4] = {0.f, 0.f, 0.f, 0.f};
+  if (dout)
+    for (int k = 0; k < K; ++k) d[k] = dout[r * K + k];
+  // U_j = sum_{i>j} g_i a_i prod_{j<k<i}(1-a_k);  dalpha_j = T_j (g_j - U_j)
+  float U = 0.0f;
+  for (int j = N - 1; j >= 0; --j) {
+    float g = dweights ? dweights[r * N + j] : 0.0f;
+    for (int k = 0; k < K; ++k) g = g + d[k] * q[(r * N + j) * K + k];
+    if (dq) {
+      float w = weights[r * N + j];
+      for (int k = 0; k < K; ++k) dq[(r * N + j) * K + k] = w * d[k];
+    }
+    float a = alpha[r * N + j];
+    float Tj = T[j * kRayBlock];
+    dalpha[r * N + j] = Tj * (g - U);
+    U = g * a + (1.0f - a) * U;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+weighted_sum_fwd_kernel(const float* __restrict__ q, const float* __restrict__ w, int64_t R,
+                        int N, int K, float* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * K) return;
+  int64_t r = t / K;
+  int k = (int)(t % K);
+  float acc = 0.0f;
+  for (int i = 0; i < N; ++i) acc = acc + q[(r * N + i) * K + k] * w[r * N + i];
+  out[t] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+weighted_sum_bwd_kernel(const float* __restrict__ q, const float* __restrict__ w, int64_t R,
+                        int N, int K, const float* __restrict__ dout, float* __restrict__ dq,
+                        float* __restrict__ dw) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * N) return;
+  int64_t r = t / N;
+  float acc = 0.0f;
+  for (int k = 0; k < K; ++k) {
+    float d = dout[r * K + k];
+    if (dq) dq[t * K + k] = w[t] * d;
+    acc = acc + q[t * K + k] * d;
+  }
+  if (dw) dw[t] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// 4-tap numerical gradient
+// ---------------------------------------------------------------------------
+__constant__ float kTap[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+
+__global__ void __launch_bounds__(256)
+taps_points_kernel(const float* __restrict__ x, int64_t n, float eps, float* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 3) return;
+  int64_t p = t / 3;
+  int d = (int)(t % 3);
+  float xv = x[t];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out[(k * n + p) * 3 + d] = xv + kTap[k][d] * eps;
+}
+
+__global__ void __launch_bounds__(256)
+taps_combine_fwd_kernel(const float* __restrict__ s, const float* __restrict__ sdf, int64_t n,
+                        float eps, float eps_sq, float* __restrict__ grad,
+                        float* __restrict__ hess) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float s1 = s[p], s2 = s[n + p], s3 = s[2 * n + p], s4 = s[3 * n + p];
+  float den = 4.0f * eps;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float g = kTap[0][d] * s1 + kTap[1][d] * s2;
+    g = g + kTap[2][d] * s3;
+    g = g + kTap[3][d] * s4;
+    grad[p * 3 + d] = g / den;
+  }
+  if (hess) {
+    float sum = ((s1 + s2) + s3) + s4;
+    float h = (sum / 2.0f - 2.0f * sdf[p]) / eps_sq;
+    float h3 = h / 3.0f;
+    hess[p * 3] = h3;
+    hess[p * 3 + 1] = h3;
+    hess[p * 3 + 2] = h3;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+taps_combine_bwd_kernel(int64_t n, float eps, float eps_sq, const float* __restrict__ dgrad,
+                        const float* __restrict__ dhess, float* __restrict__ ds,
+                        float* __restrict__ dsdf, int add_dsdf) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float den = 4.0f * eps;
+  float g0 = dgrad ? dgrad[p * 3] / den : 0.0f;
+  float g1 = dgrad ? dgrad[p * 3 + 1] / den : 0.0f;
+  float g2 = dgrad ? dgrad[p * 3 + 2] / den : 0.0f;
+  float dh = 0.0f;
+  if (dhess) dh = ((dhess[p * 3] + dhess[p * 3 + 1]) + dhess[p * 3 + 2]) / 3.0f / eps_sq;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    ds[k * n + p] = kTap[k][0] * g0 + kTap[k][1] * g1 + kTap[k][2] * g2 + dh / 2.0f;
+  if (dsdf) {
+    float v = -2.0f * dh;
+    dsdf[p] = add_dsdf ? dsdf[p] + v : v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// colour-MLP input: [feat | x | SH16 | normal | pad]
+// ---------------------------------------------------------------------------
+// Real SH constants (closed forms: C0 = 1/(2 sqrt(pi)), C1 = sqrt(3)/(2 sqrt(pi)), ...)
+#define SH_C0 0.28209479177387814f
+#define SH_C1 0.4886025119029199f
+#define SH_C2_0 1.0925484305920792f
+#define SH_C2_1 -1.0925484305920792f
+#define SH_C2_2 0.31539156525252005f
+#define SH_C2_3 -1.0925484305920792f
+#define SH_C2_4 0.5462742152960396f
+#define SH_C3_0 -0.5900435899266435f
+#define SH_C3_1 2.890611442640554f
+#define SH_C3_2 -0.4570457994644658f
+#define SH_C3_3 0.3731763325901154f
+#define SH_C3_4 -0.4570457994644658f
+#define SH_C3_5 1.445305721320277f
+#define SH_C3_6 -0.5900435899266435f
+
+__device__ inline void sh3(float x, float y, float z, float v[16]) {
+  v[0] = SH_C0;
+  v[1] = -SH_C1 * y;
+  v[2] = SH_C1 * z;
+  v[3] = -SH_C1 * x;
+  float xx = x * x, yy = y * y, zz = z * z;
+  float xy = x * y, yz = y * z, xz = x * z;
+  v[4] = SH_C2_0 * xy;
+  v[5] = SH_C2_1 * yz;
+  v[6] = SH_C2_2 * (2.0f * zz - xx - yy);
+  v[7] = SH_C2_3 * xz;
+  v[8] = SH_C2_4 * (xx - yy);
+  v[9] = SH_C3_0 * y * (3.0f * xx - yy);
+  v[10] = SH_C3_1 * xy * z;
+  v[11] = SH_C3_2 * y * (4.0f * zz - xx - yy);
+  v[12] = SH_C3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+  v[13] = SH_C3_4 * x * (4.0f * zz - xx - yy);
+  v[14] = SH_C3_5 * z * (xx - yy);
+  v[15] = SH_C3_6 * x * (xx - 3.0f * yy);
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(256)
+rgb_input_fwd_kernel(const float* __restrict__ x, const float* __restrict__ ray,
+                     int64_t n_per_ray, const float* __restrict__ grad, int normalize, int64_t n,
+                     int col0, TO* __restrict__ out, int64_t ld) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int64_t r = p / n_per_ray;
+  float row[22];
+  row[0] = x[p * 3];
+  row[1] = x[p * 3 + 1];
+  row[2] = x[p * 3 + 2];
+  sh3(ray[r * 3], ray[r * 3 + 1], ray[r * 3 + 2], row + 3);
+  float g0 = grad[p * 3], g1 = grad[p * 3 + 1], g2 = grad[p * 3 + 2];
+  if (normalize) {
+    float nrm = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
+    float dn = nrm > 1e-12f ? nrm : 1e-12f;
+    g0 = g0 / dn;
+    g1 = g1 / dn;
+    g2 = g2 / dn;
+  }
+  row[19] = g0;
+  row[20] = g1;
+  row[21] = g2;
+  TO* o = out + p * ld + col0;
+  for (int c = 0; c < 22; ++c) scalar_io<TO>::store(o + c, row[c]);
+  for (int64_t c = col0 + 22; c < ld; ++c) scalar_io<TO>::store(out + p * ld + c, 0.0f);
+}
+
+template <typename TD>
+__global__ void __launch_bounds__(256)
+rgb_input_bwd_kernel(const float* __restrict__ grad, int normalize, int64_t n, int col0,
+                     const TD* __restrict__ dy, int64_t ld, float* __restrict__ dgrad, int add) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float g0 = grad[p * 3], g1 = grad[p * 3 + 1], g2 = grad[p * 3 + 2];
+  float nrm = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
+  const TD* d = dy + p * ld + col0 + 19;
+  float d0 = scalar_io<TD>::load(d), d1 = scalar_io<TD>::load(d + 1), d2 = scalar_io<TD>::load(d + 2);
+  float o0, o1, o2;
+  if (!normalize) {
+    o0 = d0;
+    o1 = d1;
+    o2 = d2;
+  } else if (nrm > 1e-12f) {
+    float n0 = g0 / nrm, n1 = g1 / nrm, n2 = g2 / nrm;
+    float dot = n0 * d0 + n1 * d1 + n2 * d2;
+    o0 = (d0 - n0 * dot) / nrm;
+    o1 = (d1 - n1 * dot) / nrm;
+    o2 = (d2 - n2 * dot) / nrm;
+  } else {
+    o0 = d0 / 1e-12f;
+    o1 = d1 / 1e-12f;
+    o2 = d2 / 1e-12f;
+  }
+  if (add) {
+    o0 += dgrad[p * 3];
+    o1 += dgrad[p * 3 + 1];
+    o2 += dgrad[p * 3 + 2];
+  }
+  dgrad[p * 3] = o0;
+  dgrad[p * 3 + 1] = o1;
+  dgrad[p * 3 + 2] = o2;
+}
+
+// ---------------------------------------------------------------------------
+// eikonal + curvature losses
+// ---------------------------------------------------------------------------
+__device__ inline bool finitef_(float v) { return v == v && fabsf(v) != INFINITY; }
+
+__global__ void __launch_bounds__(256)
+sdf_losses_fwd_kernel(const float* __restrict__ grad, const float* __restrict__ hess,
+                      const uint8_t* __restrict__ outside, int64_t n, int64_t n_per_ray,
+                      float* __restrict__ sums) {
+  __shared__ float red[2][4];
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float e = 0.0f, c = 0.0f;
+  if (p < n) {
+    float m = (outside && outside[p / n_per_ray]) ? 0.0f : 1.0f;
+    if (grad) {
+      float g0 = grad[p * 3], g1 = grad[p * 3 + 1], g2 = grad[p * 3 + 2];
+      float nrm = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
+      float ev = (nrm - 1.0f) * (nrm - 1.0f);
+      e = (finitef_(ev) ? ev : 0.0f) * m;
+    }
+    if (hess) {
+      float lap = fabsf((hess[p * 3] + hess[p * 3 + 1]) + hess[p * 3 + 2]);
+      c = (finitef_(lap) ? lap : 0.0f) * m;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    e += __shfl_xor(e, o);
+    c += __shfl_xor(c, o);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][w] = e;
+    red[1][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_f32(sums, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+    atomic_add_f32(sums + 1, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+sdf_losses_bwd_kernel(const float* __restrict__ grad, const float* __restrict__ hess,
+                      const uint8_t* __restrict__ outside, int64_t n, int64_t n_per_ray,
+                      const float* __restrict__ wts, float* __restrict__ dgrad, int add_dgrad,
+                      float* __restrict__ dhess) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float w_eik = wts[0], w_curv = wts[1];
+  float m = (outside && outside[p / n_per_ray]) ? 0.0f : 1.0f;
+  float inv_n = 1.0f / (float)n;
+  if (dgrad) {
+    float g0 = grad[p * 3], g1 = grad[p * 3 + 1], g2 = grad[p * 3 + 2];
+    float nrm = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
+    float ev = (nrm - 1.0f) * (nrm - 1.0f);
+    float s = (finitef_(ev) && nrm > 0.0f) ? w_eik * inv_n * m * 2.0f * (nrm - 1.0f) / nrm : 0.0f;
+    float o0 = s * g0, o1 = s * g1, o2 = s * g2;
+    if (add_dgrad) {
+      o0 += dgrad[p * 3];
+      o1 += dgrad[p * 3 + 1];
+      o2 += dgrad[p * 3 + 2];
+    }
+    dgrad[p * 3] = o0;
+    dgrad[p * 3 + 1] = o1;
+    dgrad[p * 3 + 2] = o2;
+  }
+  if (dhess) {
+    float lap = (hess[p * 3] + hess[p * 3 + 1]) + hess[p * 3 + 2];
+    float sg = lap > 0.0f ? 1.0f : (lap < 0.0f ? -1.0f : 0.0f);
+    float v = finitef_(fabsf(lap)) ? w_curv * inv_n * m * sg : 0.0f;
+    dhess[p * 3] = v;
+    dhess[p * 3 + 1] = v;
+    dhess[p * 3 + 2] = v;
+  }
+}
+
+static inline dim3 grid1d(int64_t n, int block = 256) { return dim3((unsigned)ceil_div(n, block)); }
+
+}  // namespace na
+
+using namespace na;
+
+extern "C" int na_ray_bounds_sample(const float* center, const float* ray_unit, const float* rands,
+                                    int64_t R, int32_t N, float* near, float* far,
+                                    uint8_t* outside, float* dists, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(center && ray_unit && near && far && outside && dists, "null pointer");
+  hipLaunchKernelGGL(ray_bounds_sample_kernel, grid1d(R * N), dim3(256), 0, (hipStream_t)stream,
+                     center, ray_unit, rands, R, N, near, far, outside, dists);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+static size_t cdf_lds_bytes(int N) { return (size_t)(N + 1) * kRayBlock * sizeof(float); }
+
+extern "C" int na_sample_pdf(const float* bins, const float* weights, int64_t R, int32_t N,
+                             int32_t Nf, float* out, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 2 && Nf >= 1, "bad sizes");
+  NA_CHECK_ARG(cdf_lds_bytes(N) <= 65536, "N too large (<= 255)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(bins && weights && out, "null pointer");
+  hipLaunchKernelGGL(sample_pdf_kernel, grid1d(R, kRayBlock), dim3(kRayBlock), cdf_lds_bytes(N),
+                     (hipStream_t)stream, bins, weights, R, N, Nf, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_hier_resample(const float* dists, const float* sdfs, int64_t R, int32_t N,
+                                float inv_s, int32_t Nf, float* merged, float* fine,
+                                int32_t* pos_coarse, int32_t* pos_fine, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 2 && Nf >= 1, "bad sizes");
+  NA_CHECK_ARG(cdf_lds_bytes(N) <= 65536, "N too large (<= 255)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(dists && sdfs && merged && fine && pos_coarse && pos_fine, "null pointer");
+  hipLaunchKernelGGL(hier_resample_kernel, grid1d(R, kRayBlock), dim3(kRayBlock),
+                     cdf_lds_bytes(N), (hipStream_t)stream, dists, sdfs, R, N, inv_s, Nf, merged,
+                     fine, pos_coarse, pos_fine);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_scatter_merge(const float* src_a, const int32_t* pos_a, int32_t Na,
+                                const float* src_b, const int32_t* pos_b, int32_t Nb, int64_t R,
+                                float* out, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && Na >= 0 && Nb >= 0, "bad sizes");
+  if (R == 0 || Na + Nb == 0) return NA_OK;
+  NA_CHECK_ARG(out && (Na == 0 || (src_a && pos_a)) && (Nb == 0 || (src_b && pos_b)), "null pointer");
+  hipLaunchKernelGGL(scatter_merge_kernel, grid1d(R * (Na + Nb)), dim3(256), 0, (hipStream_t)stream,
+                     src_a, pos_a, Na, src_b, pos_b, Nb, R, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+static NeusIn make_neus(const float* ray_unit, const float* sdfs, const float* grads,
+                        const float* dists, const float* far, const uint8_t* outside,
+                        float outside_val, const float* s_var, float progress, float anneal_end,
+                        int64_t R, int32_t N) {
+  NeusIn p;
+  p.ray = ray_unit; p.sdfs = sdfs; p.grads = grads; p.dists = dists; p.far = far;
+  p.outside = outside; p.outside_val = outside_val; p.s_var = s_var;
+  float ratio = progress / anneal_end;  // _get_iter_cos: min(progress / anneal_end, 1)
+  p.anneal_ratio = ratio < 1.0f ? ratio : 1.0f;
+  p.R = R; p.N = N;
+  return p;
+}
+
+extern "C" int na_neus_alpha_fwd(const float* ray_unit, const float* sdfs, const float* grads,
+                                 const float* dists, const float* far, const uint8_t* outside,
+                                 float outside_val, const float* s_var, float progress,
+                                 float anneal_end, int64_t R, int32_t N, float* alpha,
+                                 na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(ray_unit && sdfs && grads && dists && s_var && alpha, "null pointer");
+  NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
+  NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
+                       anneal_end, R, N);
+  hipLaunchKernelGGL(neus_alpha_fwd_kernel, grid1d(R * N), dim3(256), 0, (hipStream_t)stream, p, alpha);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_neus_alpha_bwd(const float* ray_unit, const float* sdfs, const float* grads,
+                                 const float* dists, const float* far, const uint8_t* outside,
+                                 float outside_val, const float* s_var, float progress,
+                                 float anneal_end, int64_t R, int32_t N, const float* dalpha,
+                                 float* dsdf, float* dgrad, float* ds_var, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(ray_unit && sdfs && grads && dists && s_var && dalpha && dsdf && dgrad && ds_var,
+               "null pointer");
+  NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
+  NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
+                       anneal_end, R, N);
+  hipLaunchKernelGGL(neus_alpha_bwd_kernel, grid1d(R * N), dim3(256), 0, (hipStream_t)stream, p,
+                     dalpha, dsdf, dgrad, ds_var);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_composite_fwd(const float* alpha, const float* q, int64_t R, int32_t N,
+                                int32_t K, float* weights, float* out, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1 && K >= 0 && K <= 4, "bad sizes (K <= 4)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(alpha && weights && (K == 0 || (q && out)), "null pointer");
+  hipLaunchKernelGGL(composite_fwd_kernel, grid1d(R, kRayBlock), dim3(kRayBlock), 0,
+                     (hipStream_t)stream, alpha, q, R, N, K, weights, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_composite_bwd(const float* alpha, const float* q, const float* weights,
+                                int64_t R, int32_t N, int32_t K, const float* dout,
+                                const float* dweights, float* dalpha, float* dq,
+                                na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1 && K >= 0 && K <= 4, "bad sizes (K <= 4)");
+  NA_CHECK_ARG((size_t)N * kRayBlock * sizeof(float) <= 65536, "N too large (<= 256)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(alpha && weights && dalpha, "null pointer");
+  NA_CHECK_ARG(K == 0 || q, "q needed when K > 0");
+  hipLaunchKernelGGL(composite_bwd_kernel, grid1d(R, kRayBlock), dim3(kRayBlock),
+                     (size_t)N * kRayBlock * sizeof(float), (hipStream_t)stream, alpha, q,
+                     weights, R, N, K, dout, dweights, dalpha, dq);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_weighted_sum_fwd(const float* q, const float* w, int64_t R, int32_t N,
+                                   int32_t K, float* out, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1 && K >= 1, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(q && w && out, "null pointer");
+  hipLaunchKernelGGL(weighted_sum_fwd_kernel, grid1d(R * K), dim3(256), 0, (hipStream_t)stream, q,
+                     w, R, N, K, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_weighted_sum_bwd(const float* q, const float* w, int64_t R, int32_t N,
+                                   int32_t K, const float* dout, float* dq, float* dw,
+                                   na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1 && K >= 1, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(q && w && dout, "null pointer");
+  hipLaunchKernelGGL(weighted_sum_bwd_kernel, grid1d(R * N), dim3(256), 0, (hipStream_t)stream, q,
+                     w, R, N, K, dout, dq, dw);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_taps_points(const float* x, int64_t n, float eps, float* out,
+                              na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0, "bad size");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(x && out, "null pointer");
+  hipLaunchKernelGGL(taps_points_kernel, grid1d(n * 3), dim3(256), 0, (hipStream_t)stream, x, n,
+                     eps, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_taps_combine_fwd(const float* s_taps, const float* sdf, int64_t n, float eps,
+                                   float eps_sq, float* grad, float* hess, na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0, "bad size");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(s_taps && grad && (!hess || sdf), "null pointer");
+  hipLaunchKernelGGL(taps_combine_fwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream,
+                     s_taps, sdf, n, eps, eps_sq, grad, hess);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_taps_combine_bwd(int64_t n, float eps, float eps_sq, const float* dgrad,
+                                   const float* dhess, float* ds_taps, float* dsdf,
+                                   int32_t add_dsdf, na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0, "bad size");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(ds_taps, "null pointer");
+  hipLaunchKernelGGL(taps_combine_bwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, n,
+                     eps, eps_sq, dgrad, dhess, ds_taps, dsdf, add_dsdf);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_rgb_input_fwd(const float* x, const float* ray_unit, int64_t n_per_ray,
+                                const float* grad, int32_t normalize, int64_t n, int32_t col0,
+                                void* out, int32_t out_dtype, int64_t ld, na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0 && n_per_ray >= 1 && col0 >= 0 && ld >= col0 + 22, "bad sizes");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(x && ray_unit && grad && out, "null pointer");
+  if (out_dtype == NA_BF16)
+    hipLaunchKernelGGL(rgb_input_fwd_kernel<bf16_t>, grid1d(n), dim3(256), 0, (hipStream_t)stream,
+                       x, ray_unit, n_per_ray, grad, normalize, n, col0, (bf16_t*)out, ld);
+  else
+    hipLaunchKernelGGL(rgb_input_fwd_kernel<float>, grid1d(n), dim3(256), 0, (hipStream_t)stream,
+                       x, ray_unit, n_per_ray, grad, normalize, n, col0, (float*)out, ld);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_rgb_input_bwd(const float* grad, int32_t normalize, int64_t n, int32_t col0,
+                                const void* dy, int32_t dy_dtype, int64_t ld, float* dgrad,
+                                int32_t add, na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0 && ld >= col0 + 22, "bad sizes");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(grad && dy && dgrad, "null pointer");
+  if (dy_dtype == NA_BF16)
+    hipLaunchKernelGGL(rgb_input_bwd_kernel<bf16_t>, grid1d(n), dim3(256), 0, (hipStream_t)stream,
+                       grad, normalize, n, col0, (const bf16_t*)dy, ld, dgrad, add);
+  else
+    hipLaunchKernelGGL(rgb_input_bwd_kernel<float>, grid1d(n), dim3(256), 0, (hipStream_t)stream,
+                       grad, normalize, n, col0, (const float*)dy, ld, dgrad, add);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_sdf_losses_fwd(const float* grad, const float* hess, const uint8_t* outside,
+                                 int64_t n, int64_t n_per_ray, float* sums, na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0 && n_per_ray >= 1, "bad sizes");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(sums, "null pointer");
+  hipLaunchKernelGGL(sdf_losses_fwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, grad,
+                     hess, outside, n, n_per_ray, sums);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_sdf_losses_bwd(const float* grad, const float* hess, const uint8_t* outside,
+                                 int64_t n, int64_t n_per_ray, const float* w, float* dgrad,
+                                 int32_t add_dgrad, float* dhess, na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0 && n_per_ray >= 1, "bad sizes");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(w && (!dgrad || grad) && (!dhess || hess), "null pointer");
+  hipLaunchKernelGGL(sdf_losses_bwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, grad,
+                     hess, outside, n, n_per_ray, w, dgrad, add_dgrad, dhess);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}

@@ -1,0 +1,576 @@
+"""Autograd functions of the hot path, each backed by the gfx950 C-ABI kernels.
+
+Module boundaries follow the reference: one function per MLP
+(MLPforNeuralSDF, projects/neuralangelo/utils/mlp.py:55-69, together with the
+hash-grid encoding of projects/neuralangelo/utils/modules.py:76-95; and
+MLPwithSkipConnection + NeuralRGB input, modules.py:209-220), and one per
+renderer utility (NeuS alpha, model.py:275-298; compositing, render.py:87-112;
+4-tap combine, modules.py:157-175; eikonal/curvature, misc.py:74-89).
+
+Buffers in HBM (P points):
+  SDF input   X0 [P, K0p]  = [enc(L*F) | x(3) | 0 pad]  (compute dtype)
+  hidden      H0 [P, 256]  softplus output (compute dtype)
+  colour in   FB [P, 280]  = [feat(256) | x | SH16 | normal | 0 0]; the SDF
+              feature layer writes its output straight into columns 0..255.
+  colour hid  H1..H4 [P, 256] relu outputs
+Compute dtype is float32 or bfloat16 (MFMA inputs); accumulation, the SDF
+head, weight gradients and every per-ray quantity are float32.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import (NA_ACT_NONE, NA_ACT_RELU, NA_ACT_SIGMOID, NA_ACT_SOFTPLUS, NA_EPI_ATOMIC,
+                   NA_EPI_STORE)
+
+SOFTPLUS_BETA = 100.0
+RGB_LD = 280  # 256 features + 3 + 16 + 3, padded to a multiple of 8
+
+
+def _pad8(k):
+    return (k + 7) // 8 * 8
+
+
+def _code(t):
+    return _lib.dtype_code(t.dtype)
+
+
+def _s(t):
+    return _lib.stream_of(t)
+
+
+# ---------------------------------------------------------------------------
+# GEMM helper
+# ---------------------------------------------------------------------------
+def gemm(A, lda, trans_a, B, ldb, trans_b, M, N, K, C, ldc, epilogue=NA_EPI_STORE, bias=None,
+         act=NA_ACT_NONE, beta=SOFTPLUS_BETA, dact=NA_ACT_NONE, dact_y=None, ld_dact=0,
+         colsum=None, split_k=1, r1_a=None, r1_b=None):
+    """C[m,n] = epi(sum_k A(m,k) B(n,k)) via na_gemm (see include/neuralangelo_amd.h)."""
+    if A.dtype != B.dtype:
+        raise RuntimeError(f"gemm operands must share a dtype ({A.dtype} vs {B.dtype})")
+    a = _lib.GemmArgs()
+    a.M, a.N, a.K = int(M), int(N), int(K)
+    a.A, a.lda, a.trans_a = A.data_ptr(), int(lda), int(trans_a)
+    a.B, a.ldb, a.trans_b = B.data_ptr(), int(ldb), int(trans_b)
+    a.in_dtype = _code(A)
+    a.epilogue = epilogue
+    a.C, a.ldc, a.c_dtype = C.data_ptr(), int(ldc), _code(C)
+    a.bias = bias.data_ptr() if bias is not None else None
+    a.act, a.act_beta = act, beta
+    a.dact = dact
+    if dact_y is not None:
+        a.dact_y, a.ld_dact, a.dact_dtype = dact_y.data_ptr(), int(ld_dact), _code(dact_y)
+    a.r1_a = r1_a.data_ptr() if r1_a is not None else None
+    a.r1_b = r1_b.data_ptr() if r1_b is not None else None
+    a.colsum = colsum.data_ptr() if colsum is not None else None
+    a.split_k = int(split_k)
+    _lib.call("na_gemm", ctypes.byref(a), _s(A))
+
+
+def _split_for(rows):
+    """split-K factor for weight-gradient GEMMs reducing over `rows` points."""
+    return max(1, min(512, rows // 2048))
+
+
+def linear_weight_grad(dpre, X, ldx, K, dW):
+    """dW[n, k] += sum_m dpre[m, n] X[m, k]  (dpre [M, N] contiguous)."""
+    M, N = dpre.shape
+    if M == 0:
+        return
+    gemm(dpre, N, 1, X, ldx, 1, N, K, M, dW, dW.stride(0), epilogue=NA_EPI_ATOMIC,
+         split_k=_split_for(M))
+
+
+def act_grad(dy, y, act, out_dtype, colsum=None, n_cols=None):
+    """dy * act'(y) over the first n_cols columns (row strides taken from the tensors)."""
+    M = dy.shape[0]
+    N = dy.shape[1] if n_cols is None else n_cols
+    out = torch.empty(M, N, dtype=out_dtype, device=dy.device)
+    _lib.call("na_act_grad", _lib.ptr(dy), dy.stride(0), _code(dy), _lib.ptr(y), y.stride(0),
+              _code(y), M, N, act, SOFTPLUS_BETA, _lib.ptr(out), N, _code(out),
+              _lib.ptr(colsum), _s(dy))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# weight normalisation (torch.nn.utils.weight_norm, dim=0)
+# ---------------------------------------------------------------------------
+class WeightNorm(torch.autograd.Function):
+    """W = g * v / ||v||_row  (v [N,K], g [N,1] or [N])."""
+
+    @staticmethod
+    def forward(ctx, v, g):
+        _lib.require_cuda(v, g)
+        v = v.contiguous()
+        gf = g.reshape(-1).contiguous()
+        N, K = v.shape
+        W = torch.empty_like(v)
+        _lib.call("na_weight_norm_fwd", _lib.ptr(v), _lib.ptr(gf), N, K, _lib.ptr(W), None, 0, 0,
+                  _s(v))
+        ctx.save_for_backward(v, gf)
+        ctx.g_shape = g.shape
+        return W
+
+    @staticmethod
+    def backward(ctx, dW):
+        v, gf = ctx.saved_tensors
+        N, K = v.shape
+        dv = torch.zeros_like(v)
+        dg = torch.zeros_like(gf)
+        _lib.call("na_weight_norm_bwd", _lib.ptr(v), _lib.ptr(gf), _lib.ptr(dW.contiguous()), N, K,
+                  _lib.ptr(dv), _lib.ptr(dg), _s(v))
+        return dv, dg.view(ctx.g_shape)
+
+
+def weight_norm(v, g):
+    return WeightNorm.apply(v, g)
+
+
+# ---------------------------------------------------------------------------
+# SDF network: encoder + MLPforNeuralSDF
+# ---------------------------------------------------------------------------
+class SDFWeights:
+    """Layer weights of one step in kernel layout (built from the module's tensors).
+
+    W0 reference columns are [x(3), enc(LF)]; the kernel layout is
+    [enc(LF), x(3), 0 pad] so the encoder block starts at 0 (aligned).
+    """
+
+    def __init__(self, W0, b0, W1, b1, ws, bs, n_enc, cdt):
+        self.n_enc = n_enc
+        self.k0 = n_enc + 3
+        self.k0p = _pad8(self.k0)
+        self.cdt = cdt
+        dev = W0.device
+        W0p = torch.zeros(W0.shape[0], self.k0p, dtype=torch.float32, device=dev)
+        W0p[:, :n_enc] = W0[:, 3:]
+        W0p[:, n_enc:n_enc + 3] = W0[:, :3]
+        self.W0p = W0p.to(cdt).contiguous()
+        self.b0 = b0.contiguous().float()
+        self.W1 = W1.to(cdt).contiguous() if W1 is not None else None
+        self.b1 = b1.contiguous().float() if b1 is not None else None
+        self.ws = ws.reshape(1, -1).contiguous().float()
+        self.bs = bs.reshape(-1).contiguous().float()
+        self.ws_shape, self.bs_shape = ws.shape, bs.shape
+        self.hidden = W0.shape[0]
+
+    def unpermute_dW0(self, dW0p):
+        """[hidden, K0p] kernel-layout gradient -> reference layout [hidden, 3+LF]."""
+        n = self.n_enc
+        return torch.cat([dW0p[:, n:n + 3], dW0p[:, :n]], dim=1)
+
+
+def sdf_input(enc, points, cdt, k0p):
+    """X0 [P, K0p] = [enc | x | 1 | 0] for world points [P,3]."""
+    P = points.shape[0]
+    X0 = torch.empty(P, k0p, dtype=cdt, device=points.device)
+    table = enc.table_for_forward()
+    vmin, vmax = enc.vol_range
+    _lib.call("na_sdf_input_fwd", ctypes.byref(enc.desc), _lib.ptr(points), P, float(vmin),
+              float(vmax), _lib.ptr(table), _code(table), int(enc.active_levels), _lib.ptr(X0),
+              _code(X0), k0p, _s(points))
+    return X0
+
+
+def sdf_forward_nograd(enc, w, points):
+    """SDF value only, no autograd state (sampling passes)."""
+    P = points.shape[0]
+    X0 = sdf_input(enc, points, w.cdt, w.k0p)
+    H0 = torch.empty(P, w.hidden, dtype=w.cdt, device=points.device)
+    gemm(X0, w.k0p, 0, w.W0p, w.k0p, 0, P, w.hidden, w.k0p, H0, w.hidden, bias=w.b0,
+         act=NA_ACT_SOFTPLUS)
+    sdf = torch.empty(P, 1, dtype=torch.float32, device=points.device)
+    _lib.call("na_linear_narrow_fwd", _lib.ptr(H0), w.hidden, _code(H0), P, w.hidden,
+              _lib.ptr(w.ws), _lib.ptr(w.bs), 1, NA_ACT_NONE, _lib.ptr(sdf), _s(points))
+    return sdf
+
+
+class SDFNetFunction(torch.autograd.Function):
+    """SDF MLP (mlp.py:55-69), optionally fused with the encoder (modules.py:76-95).
+
+    enc given:  inp = world points [P,3]; the encoder builds X0 in-kernel and the
+                table gradient is returned for `table_params`.
+    enc None:   inp = X0 [P, K0p] already in kernel layout (compute dtype); the
+                input gradient dX0 [P, K0p] is returned.
+    Outputs: sdf [P,1] float32 and, with_feat, feat [P,256] as a view of a
+    [P,280] colour-input buffer (columns 256.. are filled by the colour net).
+    """
+
+    @staticmethod
+    def forward(ctx, inp, table_params, W0, b0, W1, b1, ws, bs, enc, cdt, with_feat, n_enc):
+        _lib.require_cuda(inp)
+        w = SDFWeights(W0, b0, W1 if with_feat else None, b1 if with_feat else None, ws, bs,
+                       n_enc, cdt)
+        dev = inp.device
+        if enc is not None:
+            points = inp.contiguous().float()
+            X0 = sdf_input(enc, points, cdt, w.k0p)
+        else:
+            points = torch.empty(0, device=dev)
+            X0 = inp.contiguous()
+            if X0.dtype != cdt or X0.shape[1] != w.k0p:
+                raise RuntimeError("prebuilt SDF input must be [P, K0p] in the compute dtype")
+        P = X0.shape[0]
+        H0 = torch.empty(P, w.hidden, dtype=cdt, device=dev)
+        gemm(X0, w.k0p, 0, w.W0p, w.k0p, 0, P, w.hidden, w.k0p, H0, w.hidden, bias=w.b0,
+             act=NA_ACT_SOFTPLUS)
+        sdf = torch.empty(P, 1, dtype=torch.float32, device=dev)
+        _lib.call("na_linear_narrow_fwd", _lib.ptr(H0), w.hidden, _code(H0), P, w.hidden,
+                  _lib.ptr(w.ws), _lib.ptr(w.bs), 1, NA_ACT_NONE, _lib.ptr(sdf), _s(X0))
+        FB = None
+        if with_feat:
+            FB = torch.empty(P, RGB_LD, dtype=cdt, device=dev)
+            gemm(H0, w.hidden, 0, w.W1, w.hidden, 0, P, w.hidden, w.hidden, FB, RGB_LD, bias=w.b1,
+                 act=NA_ACT_SOFTPLUS)
+        ctx.save_for_backward(points, X0, H0, FB if FB is not None else torch.empty(0, device=dev))
+        ctx.w, ctx.enc, ctx.with_feat = w, enc, with_feat
+        if with_feat:
+            return sdf, FB[:, :w.hidden]
+        return sdf, None
+
+    @staticmethod
+    def backward(ctx, dsdf, dfeat):
+        points, X0, H0, FB = ctx.saved_tensors
+        w, enc = ctx.w, ctx.enc
+        P, hid, cdt, dev = X0.shape[0], w.hidden, w.cdt, X0.device
+        dsdf = (torch.zeros(P, 1, device=dev) if dsdf is None else dsdf).contiguous().float()
+        dW1 = db1 = None
+        dH0 = torch.zeros(P, hid, dtype=torch.float32, device=dev)
+        if ctx.with_feat:
+            db1 = torch.zeros(hid, device=dev)
+            dW1 = torch.zeros(hid, hid, device=dev)
+            if dfeat is not None:
+                dpre1 = act_grad(dfeat, FB, NA_ACT_SOFTPLUS, cdt, colsum=db1, n_cols=hid)
+                linear_weight_grad(dpre1, H0, hid, hid, dW1)
+                gemm(dpre1, hid, 0, w.W1, hid, 1, P, hid, hid, dH0, hid)  # dH0 = dpre1 . W1
+        dws = torch.zeros(1, hid, device=dev)
+        dbs = torch.zeros(1, device=dev)
+        db0 = torch.zeros(hid, device=dev)
+        # dpre0 = (dH0 + dsdf * ws) * softplus'(H0); dws, dbs; db0 = column sums of dpre0
+        _lib.call("na_linear_narrow_bwd", _lib.ptr(H0), hid, _code(H0), P, hid, _lib.ptr(w.ws), 1,
+                  _lib.ptr(dsdf), _lib.ptr(dws), _lib.ptr(dbs), _lib.ptr(dH0), hid, _code(dH0), 1,
+                  _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS, SOFTPLUS_BETA, _lib.ptr(db0),
+                  _s(X0))
+        dpre0 = dH0 if cdt == torch.float32 else dH0.to(cdt)
+        dW0p = torch.zeros(hid, w.k0p, device=dev)
+        linear_weight_grad(dpre0, X0, w.k0p, w.k0p, dW0p)
+        dW0 = w.unpermute_dW0(dW0p)
+        dtable = dinp = None
+        if enc is not None and ctx.needs_input_grad[1]:
+            n_enc = w.n_enc
+            dX = torch.empty(P, n_enc, dtype=torch.float32, device=dev)
+            gemm(dpre0, hid, 0, w.W0p, w.k0p, 1, P, n_enc, hid, dX, n_enc)
+            dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
+            vmin, vmax = enc.vol_range
+            _lib.call("na_sdf_input_bwd", ctypes.byref(enc.desc), _lib.ptr(points), P, float(vmin),
+                      float(vmax), _lib.ptr(dX), _code(dX), n_enc, _lib.ptr(dtable),
+                      int(enc.active_levels), _s(X0))
+        if enc is None and ctx.needs_input_grad[0]:
+            dinp = torch.empty(P, w.k0p, dtype=torch.float32, device=dev)
+            gemm(dpre0, hid, 0, w.W0p, w.k0p, 1, P, w.k0p, hid, dinp, w.k0p)
+        return (dinp, dtable, dW0, db0, dW1, db1, dws.view(w.ws_shape), dbs.view(w.bs_shape),
+                None, None, None, None)
+
+
+# ---------------------------------------------------------------------------
+# colour network: input builder + MLPwithSkipConnection + sigmoid
+# ---------------------------------------------------------------------------
+class RGBWeights:
+    """Colour MLP weights in kernel layout.  Reference input columns are
+    [x(3), SH(16), normal(3), feat(256)]; kernel columns [feat, x, SH, normal, 0, 0]."""
+
+    def __init__(self, Ws, bs, cdt, n_feat=256):
+        self.cdt = cdt
+        self.n_feat = n_feat
+        W0 = Ws[0]
+        W0p = torch.zeros(W0.shape[0], RGB_LD, dtype=torch.float32, device=W0.device)
+        W0p[:, :n_feat] = W0[:, 22:]
+        W0p[:, n_feat:n_feat + 22] = W0[:, :22]
+        self.W = [W0p.to(cdt).contiguous()] + [W.to(cdt).contiguous() for W in Ws[1:-1]]
+        self.W_last = Ws[-1].contiguous().float()
+        self.b = [b.contiguous().float() for b in bs]
+        self.hidden = W0.shape[0]
+
+    def unpermute_dW0(self, dW0p):
+        n = self.n_feat
+        return torch.cat([dW0p[:, n:n + 22], dW0p[:, :n]], dim=1)
+
+
+class RGBNetFunction(torch.autograd.Function):
+    """Colour MLP (nerf_util.py:189-196) with the NeuralRGB input (modules.py:209-220).
+
+    build_input 1 or 2: feat [P,256] (ideally the SDF function's view of its
+      [P,280] buffer), points [P,3], ray_unit [R,3] (n_per_ray samples per ray)
+      and normals_src [P,3] -> the input row [feat | x | SH16 | normal | 0 0] is
+      assembled on the GPU; 1: normals_src are raw gradients, normalised in the
+      kernel (F.normalize), 2: normals_src are used as given.
+    build_input 0: feat is the whole prebuilt input [P,280] in kernel layout.
+    final_act: NA_ACT_SIGMOID (NeuralRGB) or NA_ACT_NONE (bare MLP).
+    """
+
+    @staticmethod
+    def forward(ctx, feat, points, ray_unit, normals_src, n_per_ray, build_input, final_act, cdt,
+                *wb):
+        n_lin = len(wb) // 2
+        Ws, bs = list(wb[:n_lin]), list(wb[n_lin:])
+        w = RGBWeights(Ws, bs, cdt)
+        P = feat.shape[0]
+        dev = feat.device
+        hid = w.hidden
+        if build_input:
+            if feat.stride(0) == RGB_LD and feat.dtype == cdt and feat.stride(1) == 1:
+                FB = torch.as_strided(feat, (P, RGB_LD), (RGB_LD, 1))
+            else:
+                FB = torch.empty(P, RGB_LD, dtype=cdt, device=dev)
+                FB[:, :w.n_feat] = feat
+            points = points.contiguous().float()
+            ray_unit = ray_unit.contiguous().float()
+            normals_src = normals_src.contiguous().float()
+            _lib.call("na_rgb_input_fwd", _lib.ptr(points), _lib.ptr(ray_unit), int(n_per_ray),
+                      _lib.ptr(normals_src), int(build_input == 1), P, w.n_feat, _lib.ptr(FB),
+                      _code(FB), RGB_LD, _s(feat))
+        else:
+            FB = feat.contiguous()
+            if FB.shape[1] != RGB_LD or FB.dtype != cdt:
+                raise RuntimeError("prebuilt colour input must be [P, 280] in the compute dtype")
+            normals_src = torch.empty(0, device=dev)
+        Hs = []
+        X, ldx, K = FB, RGB_LD, RGB_LD
+        for li in range(n_lin - 1):
+            H = torch.empty(P, hid, dtype=cdt, device=dev)
+            gemm(X, ldx, 0, w.W[li], K, 0, P, hid, K, H, hid, bias=w.b[li], act=NA_ACT_RELU)
+            Hs.append(H)
+            X, ldx, K = H, hid, hid
+        out = torch.empty(P, 3, dtype=torch.float32, device=dev)
+        _lib.call("na_linear_narrow_fwd", _lib.ptr(X), hid, _code(X), P, hid, _lib.ptr(w.W_last),
+                  _lib.ptr(w.b[-1]), 3, int(final_act), _lib.ptr(out), _s(feat))
+        ctx.save_for_backward(FB, normals_src, out, *Hs)
+        ctx.w, ctx.n_lin, ctx.build, ctx.final_act = w, n_lin, build_input, final_act
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        FB, normals_src, out, *Hs = ctx.saved_tensors
+        w, n_lin = ctx.w, ctx.n_lin
+        P, hid, cdt, dev = FB.shape[0], w.hidden, w.cdt, FB.device
+        dpre = dout.float()
+        if ctx.final_act == NA_ACT_SIGMOID:
+            dpre = dpre * out * (1 - out)
+        dpre = dpre.contiguous()
+        dWs = [None] * n_lin
+        dbs = [torch.zeros(b.shape[0], device=dev) for b in w.b]
+        dWs[-1] = torch.zeros(3, hid, device=dev)
+        dH = torch.zeros(P, hid, dtype=torch.float32, device=dev)
+        _lib.call("na_linear_narrow_bwd", _lib.ptr(Hs[-1]), hid, _code(Hs[-1]), P, hid,
+                  _lib.ptr(w.W_last), 3, _lib.ptr(dpre), _lib.ptr(dWs[-1]), _lib.ptr(dbs[-1]),
+                  _lib.ptr(dH), hid, _code(dH), 0, _lib.ptr(Hs[-1]), hid, _code(Hs[-1]),
+                  NA_ACT_RELU, SOFTPLUS_BETA, _lib.ptr(dbs[-2]), _s(FB))
+        dpre = dH if cdt == torch.float32 else dH.to(cdt)
+        for li in range(n_lin - 2, 0, -1):
+            X = Hs[li - 1]
+            dWs[li] = torch.zeros(hid, hid, device=dev)
+            linear_weight_grad(dpre, X, hid, hid, dWs[li])
+            dprev = torch.empty(P, hid, dtype=cdt, device=dev)
+            gemm(dpre, hid, 0, w.W[li], hid, 1, P, hid, hid, dprev, hid, dact=NA_ACT_RELU,
+                 dact_y=X, ld_dact=hid, colsum=dbs[li - 1])
+            dpre = dprev
+        dW0p = torch.zeros(hid, RGB_LD, device=dev)
+        linear_weight_grad(dpre, FB, RGB_LD, RGB_LD, dW0p)
+        dWs[0] = w.unpermute_dW0(dW0p)
+        dFB = torch.empty(P, RGB_LD, dtype=torch.float32, device=dev)
+        gemm(dpre, hid, 0, w.W[0], RGB_LD, 1, P, RGB_LD, hid, dFB, RGB_LD)
+        dnormals = None
+        if ctx.build:
+            dnormals = torch.empty(P, 3, dtype=torch.float32, device=dev)
+            _lib.call("na_rgb_input_bwd", _lib.ptr(normals_src), int(ctx.build == 1), P, w.n_feat,
+                      _lib.ptr(dFB), _code(dFB), RGB_LD, _lib.ptr(dnormals), 0, _s(FB))
+            dfeat = dFB[:, :w.n_feat]
+        else:
+            dfeat = dFB
+        return (dfeat, None, None, dnormals, None, None, None, None, *dWs, *dbs)
+
+
+# ---------------------------------------------------------------------------
+# 4-tap numerical gradient
+# ---------------------------------------------------------------------------
+def taps_points(x, eps):
+    x = x.contiguous().float()
+    n = x.shape[0]
+    out = torch.empty(4, n, 3, dtype=torch.float32, device=x.device)
+    _lib.call("na_taps_points", _lib.ptr(x), n, float(eps), _lib.ptr(out), _s(x))
+    return out
+
+
+class TapsCombine(torch.autograd.Function):
+    """s_taps [4, n], sdf [n] -> gradient [n,3], hessian [n,3] (or None)."""
+
+    @staticmethod
+    def forward(ctx, s_taps, sdf, eps, eps_sq, training):
+        s_taps = s_taps.contiguous().float()
+        n = s_taps.shape[1]
+        grad = torch.empty(n, 3, device=s_taps.device)
+        hess = torch.empty(n, 3, device=s_taps.device) if training else None
+        sdf_c = sdf.contiguous().float() if training else None
+        _lib.call("na_taps_combine_fwd", _lib.ptr(s_taps), _lib.ptr(sdf_c), n, float(eps),
+                  float(eps_sq), _lib.ptr(grad), _lib.ptr(hess), _s(s_taps))
+        ctx.n, ctx.eps, ctx.eps_sq, ctx.training = n, eps, eps_sq, training
+        if hess is None:
+            return grad, torch.empty(0, device=s_taps.device)
+        return grad, hess
+
+    @staticmethod
+    def backward(ctx, dgrad, dhess):
+        n = ctx.n
+        dev = dgrad.device if dgrad is not None else dhess.device
+        ds = torch.empty(4, n, device=dev)
+        dsdf = torch.empty(n, device=dev) if ctx.training else None
+        dg = dgrad.contiguous().float() if dgrad is not None else None
+        dh = dhess.contiguous().float() if (dhess is not None and ctx.training) else None
+        _lib.call("na_taps_combine_bwd", n, float(ctx.eps), float(ctx.eps_sq), _lib.ptr(dg),
+                  _lib.ptr(dh), _lib.ptr(ds), _lib.ptr(dsdf), 0, _lib.stream_of(ds))
+        return ds, dsdf, None, None, None
+
+
+# ---------------------------------------------------------------------------
+# NeuS alpha
+# ---------------------------------------------------------------------------
+class NeusAlpha(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, ray_unit, sdfs, gradients, dists, far, outside, s_var, progress, anneal_end,
+                outside_val):
+        R, N = sdfs.shape
+        ray_unit = ray_unit.contiguous().float()
+        sdfs = sdfs.contiguous().float()
+        gradients = gradients.contiguous().float()
+        dists = dists.contiguous().float()
+        far = far.contiguous().float() if far is not None else None
+        out_u8 = outside.contiguous().to(torch.uint8) if outside is not None else None
+        alpha = torch.empty(R, N, device=sdfs.device)
+        _lib.call("na_neus_alpha_fwd", _lib.ptr(ray_unit), _lib.ptr(sdfs), _lib.ptr(gradients),
+                  _lib.ptr(dists), _lib.ptr(far), _lib.ptr(out_u8), float(outside_val),
+                  _lib.ptr(s_var), float(progress), float(anneal_end), R, N, _lib.ptr(alpha),
+                  _s(sdfs))
+        ctx.save_for_backward(ray_unit, sdfs, gradients, dists,
+                              far if far is not None else torch.empty(0, device=sdfs.device),
+                              out_u8 if out_u8 is not None else torch.empty(0, device=sdfs.device),
+                              s_var)
+        ctx.has_far, ctx.has_out = far is not None, out_u8 is not None
+        ctx.progress, ctx.anneal_end, ctx.outside_val = progress, anneal_end, outside_val
+        return alpha
+
+    @staticmethod
+    def backward(ctx, dalpha):
+        ray_unit, sdfs, gradients, dists, far, out_u8, s_var = ctx.saved_tensors
+        R, N = sdfs.shape
+        dsdf = torch.empty_like(sdfs)
+        dgrad = torch.empty_like(gradients)
+        ds_var = torch.zeros(1, device=sdfs.device)
+        _lib.call("na_neus_alpha_bwd", _lib.ptr(ray_unit), _lib.ptr(sdfs), _lib.ptr(gradients),
+                  _lib.ptr(dists), _lib.ptr(far) if ctx.has_far else None,
+                  _lib.ptr(out_u8) if ctx.has_out else None, float(ctx.outside_val),
+                  _lib.ptr(s_var), float(ctx.progress), float(ctx.anneal_end), R, N,
+                  _lib.ptr(dalpha.contiguous().float()), _lib.ptr(dsdf), _lib.ptr(dgrad),
+                  _lib.ptr(ds_var), _s(sdfs))
+        return (None, dsdf, dgrad, None, None, None, ds_var.view_as(s_var), None, None, None)
+
+
+# ---------------------------------------------------------------------------
+# alpha compositing + composite
+# ---------------------------------------------------------------------------
+class Composite(torch.autograd.Function):
+    """alphas [R,N], q [R,N,K] (or None) -> weights [R,N], out [R,K]."""
+
+    @staticmethod
+    def forward(ctx, alphas, q):
+        R, N = alphas.shape
+        alphas = alphas.contiguous().float()
+        K = 0 if q is None else q.shape[-1]
+        qc = q.contiguous().float() if q is not None else None
+        weights = torch.empty(R, N, device=alphas.device)
+        out = torch.empty(R, K, device=alphas.device)
+        _lib.call("na_composite_fwd", _lib.ptr(alphas), _lib.ptr(qc), R, N, K, _lib.ptr(weights),
+                  _lib.ptr(out) if K else None, _s(alphas))
+        ctx.save_for_backward(alphas, qc if qc is not None else torch.empty(0, device=alphas.device),
+                              weights)
+        ctx.K = K
+        return weights, out
+
+    @staticmethod
+    def backward(ctx, dweights, dout):
+        alphas, qc, weights = ctx.saved_tensors
+        R, N = alphas.shape
+        K = ctx.K
+        dalpha = torch.empty_like(alphas)
+        dq = torch.empty(R, N, K, device=alphas.device) if (K and ctx.needs_input_grad[1]) else None
+        _lib.call("na_composite_bwd", _lib.ptr(alphas), _lib.ptr(qc) if K else None,
+                  _lib.ptr(weights), R, N, K,
+                  _lib.ptr(dout.contiguous().float()) if (dout is not None and K) else None,
+                  _lib.ptr(dweights.contiguous().float()) if dweights is not None else None,
+                  _lib.ptr(dalpha), _lib.ptr(dq), _s(alphas))
+        return dalpha, dq
+
+
+class WeightedSum(torch.autograd.Function):
+    """quantities [..., N, K] x weights [..., N, 1] -> [..., K]  (render.composite)."""
+
+    @staticmethod
+    def forward(ctx, q, w):
+        N, K = q.shape[-2], q.shape[-1]
+        lead = q.shape[:-2]
+        qc = q.reshape(-1, N, K).contiguous().float()
+        wc = w.reshape(-1, N).contiguous().float()
+        R = qc.shape[0]
+        out = torch.empty(R, K, device=qc.device)
+        _lib.call("na_weighted_sum_fwd", _lib.ptr(qc), _lib.ptr(wc), R, N, K, _lib.ptr(out), _s(qc))
+        ctx.save_for_backward(qc, wc)
+        ctx.shapes = (q.shape, w.shape)
+        return out.view(*lead, K)
+
+    @staticmethod
+    def backward(ctx, dout):
+        qc, wc = ctx.saved_tensors
+        R, N, K = qc.shape
+        dq = torch.empty_like(qc) if ctx.needs_input_grad[0] else None
+        dw = torch.empty_like(wc) if ctx.needs_input_grad[1] else None
+        _lib.call("na_weighted_sum_bwd", _lib.ptr(qc), _lib.ptr(wc), R, N, K,
+                  _lib.ptr(dout.reshape(R, K).contiguous().float()), _lib.ptr(dq), _lib.ptr(dw),
+                  _s(qc))
+        return (dq.view(ctx.shapes[0]) if dq is not None else None,
+                dw.view(ctx.shapes[1]) if dw is not None else None)
+
+
+# ---------------------------------------------------------------------------
+# eikonal + curvature losses
+# ---------------------------------------------------------------------------
+class SDFLosses(torch.autograd.Function):
+    """gradients [n,3], hessians [n,3] (or None), outside [R] -> (eikonal, curvature) means."""
+
+    @staticmethod
+    def forward(ctx, gradients, hessians, outside, n_per_ray):
+        n = gradients.shape[0]
+        g = gradients.contiguous().float()
+        h = hessians.contiguous().float() if hessians is not None else None
+        o = outside.contiguous().to(torch.uint8) if outside is not None else None
+        sums = torch.zeros(2, device=g.device)
+        _lib.call("na_sdf_losses_fwd", _lib.ptr(g), _lib.ptr(h), _lib.ptr(o), n, int(n_per_ray),
+                  _lib.ptr(sums), _s(g))
+        ctx.save_for_backward(g, h if h is not None else torch.empty(0, device=g.device),
+                              o if o is not None else torch.empty(0, device=g.device))
+        ctx.has_h, ctx.has_o, ctx.n_per_ray = h is not None, o is not None, n_per_ray
+        means = sums / max(n, 1)
+        return means[0], means[1]
+
+    @staticmethod
+    def backward(ctx, deik, dcurv):
+        g, h, o = ctx.saved_tensors
+        n = g.shape[0]
+        dev = g.device
+        w = torch.stack([deik if deik is not None else torch.zeros((), device=dev),
+                         dcurv if dcurv is not None else torch.zeros((), device=dev)]).float()
+        dg = torch.empty_like(g)
+        dh = torch.empty_like(h) if ctx.has_h else None
+        _lib.call("na_sdf_losses_bwd", _lib.ptr(g), _lib.ptr(h) if ctx.has_h else None,
+                  _lib.ptr(o) if ctx.has_o else None, n, int(ctx.n_per_ray), _lib.ptr(w),
+                  _lib.ptr(dg), 0, _lib.ptr(dh), _s(g))
+        return dg, dh, None, None

@@ -1,0 +1,167 @@
+"""Drop-in Model for the object render path (projects/neuralangelo/model.py:24-298).
+
+Covers the hot path with the background model disabled (the BASELINE
+configurations); every per-sample operator runs on the HIP kernels:
+  get_dist_bounds + sample_dists      -> one fused bounds/coarse kernel
+  sample_dists_all / _hierarchical    -> SDF passes + one resample/merge kernel per round
+  neural_sdf.forward / compute_grads  -> fused encoder + SDF MLP (GEMM kernels), 4-tap combine
+  neural_rgb.forward                  -> fused input builder (SH, normalise) + colour MLP
+  compute_neus_alphas                 -> NeuS alpha kernel (outside fill inside)
+  alpha_compositing_weights+composite -> one fused compositing kernel
+"""
+from collections import defaultdict
+
+import torch
+
+from . import functional as Fn
+from . import nerf_util
+from .modules import NeuralRGB, NeuralSDF, _get
+
+
+def default_config(log2_T=22, F=8, levels=16, n_coarse=64, n_fine=16, n_hier=2):
+    """The BASELINE hot-path configuration (base.yaml:38-115 with background off)."""
+    return dict(
+        object=dict(
+            sdf=dict(
+                mlp=dict(num_layers=1, hidden_dim=256, skip=[], activ="softplus",
+                         activ_params=dict(beta=100), geometric_init=True, weight_norm=True,
+                         out_bias=0.5, inside_out=False),
+                encoding=dict(type="hashgrid", levels=levels,
+                              hashgrid=dict(min_logres=5, max_logres=11, dict_size=log2_T, dim=F,
+                                            range=[-2, 2]),
+                              coarse2fine=dict(enabled=True, init_active_level=4, step=5000)),
+                gradient=dict(mode="numerical", taps=4)),
+            rgb=dict(mlp=dict(num_layers=4, hidden_dim=256, skip=[], activ="relu_", activ_params={},
+                              weight_norm=True),
+                     mode="idr", encoding_view=dict(type="spherical", levels=3)),
+            s_var=dict(init_val=3.0, anneal_end=0.1)),
+        background=dict(enabled=False, white=False),
+        render=dict(rand_rays=512, num_samples=dict(coarse=n_coarse, fine=n_fine, background=32),
+                    num_sample_hierarchy=n_hier, stratified=True),
+        appear_embed=dict(enabled=False, dim=8),
+    )
+
+
+class Model(torch.nn.Module):
+
+    def __init__(self, cfg_model, cfg_data=None, compute_dtype=torch.float32):
+        super().__init__()
+        self.cfg_render = _get(cfg_model, "render")
+        bg = _get(cfg_model, "background")
+        if _get(bg, "enabled", False):
+            raise NotImplementedError("the background NeRF is outside the accelerated path")
+        self.white_background = _get(bg, "white", False)
+        self.with_background = False
+        obj = _get(cfg_model, "object")
+        self.anneal_end = _get(_get(obj, "s_var"), "anneal_end")
+        self.outside_val = 1000. * (-1 if _get(_get(_get(obj, "sdf"), "mlp"), "inside_out") else 1)
+        self.compute_dtype = compute_dtype
+        self.neural_sdf = NeuralSDF(_get(obj, "sdf"), compute_dtype=compute_dtype)
+        self.neural_rgb = NeuralRGB(_get(obj, "rgb"), feat_dim=_get(_get(_get(obj, "sdf"), "mlp"),
+                                                                      "hidden_dim"),
+                                    appear_embed=_get(cfg_model, "appear_embed"),
+                                    compute_dtype=compute_dtype)
+        self.s_var = torch.nn.Parameter(torch.tensor(_get(_get(obj, "s_var"), "init_val"),
+                                                     dtype=torch.float32))
+        self.progress = 1.0
+
+    # -- sampling --------------------------------------------------------------
+    def _n_samples(self):
+        ns = _get(self.cfg_render, "num_samples")
+        return _get(ns, "coarse"), _get(ns, "fine"), _get(self.cfg_render, "num_sample_hierarchy")
+
+    @torch.no_grad()
+    def sample_dists_all(self, center, ray_unit, coarse):
+        """model.py:232-248 from coarse dists [R, Nc] -> sorted dists [R, N]."""
+        _, n_fine, n_hier = self._n_samples()
+        dists = coarse
+        if n_hier > 0:
+            pts = center[:, None, :] + ray_unit[:, None, :] * dists[..., None]
+            sdfs = self.neural_sdf.sdf_nograd(pts)[..., 0]
+        for h in range(n_hier):
+            merged, fine, pc, pf = nerf_util.hierarchical_round(dists, sdfs, 64 * 2 ** h, n_fine)
+            if h != n_hier - 1:
+                pts = center[:, None, :] + ray_unit[:, None, :] * fine[..., None]
+                sdf_fine = self.neural_sdf.sdf_nograd(pts)[..., 0]
+                sdfs = nerf_util.scatter_merge(sdfs, pc, sdf_fine, pf)
+            dists = merged
+        return dists
+
+    # -- rendering ---------------------------------------------------------------
+    def render_rays(self, center, ray_unit, sample_idx=None, stratified=False, rands=None,
+                    dists=None):
+        """center, ray_unit [R,3] (or [B,R,3]) -> output dict (model.py:124-156).
+
+        rands: optional [R, Nc] stratification offsets; dists: optional [R, N]
+        samples to render instead of sampling (parity testing)."""
+        lead = center.shape[:-1]
+        c = center.reshape(-1, 3).contiguous().float()
+        v = ray_unit.reshape(-1, 3).contiguous().float()
+        n_coarse, _, _ = self._n_samples()
+        if rands is None and stratified:
+            rands = torch.rand(c.shape[0], n_coarse, device=c.device)
+        with torch.no_grad():
+            near, far, outside, coarse = nerf_util.bounds_and_coarse(c, v, n_coarse, rands)
+        out = self.render_rays_object(c, v, near, far, outside, coarse, dists=dists)
+        rgb = out["rgb"]
+        if self.white_background:
+            rgb = rgb + (1 - out["weights"].sum(dim=1, keepdim=False)[..., None])
+        R = c.shape[0]
+        N = out["dists"].shape[1]
+
+        def shp(t, tail):
+            return t.view(*lead, *tail)
+        res = dict(
+            rgb=shp(rgb, (3,)), opacity=shp(out["opacity"], (1,)) if out["opacity"] is not None else None,
+            outside=shp(outside, (1,)), dists=shp(out["dists"], (N, 1)),
+            weights=shp(out["weights"], (N, 1)),
+            gradient=shp(out["gradient"], (3,)) if out["gradient"] is not None else None,
+            gradients=shp(out["gradients"], (N, 3)),
+            hessians=shp(out["hessians"], (N, 3)) if out["hessians"] is not None else None,
+            sdfs=shp(out["sdfs"], (N,)), alphas=shp(out["alphas"], (N,)),
+            near=shp(near, (1,)), far=shp(far, (1,)))
+        del R
+        return res
+
+    def render_rays_object(self, center, ray_unit, near, far, outside, coarse, dists=None):
+        if dists is None:
+            dists = self.sample_dists_all(center, ray_unit, coarse)  # [R, N]
+        dists = dists.contiguous().float()
+        R, N = dists.shape
+        points = center[:, None, :] + ray_unit[:, None, :] * dists[..., None]  # [R,N,3]
+        pts = points.reshape(-1, 3)
+        sdfs, feats = self.neural_sdf.forward(pts)  # [P,1], [P,256] buffer view
+        # sdfs[outside] = outside_val (model.py:163): the fill blocks the gradient
+        fill = outside[:, None].expand(R, N).reshape(-1, 1)
+        sdfs_filled = torch.where(fill, torch.full_like(sdfs, self.outside_val), sdfs)
+        gradients, hessians = self.neural_sdf.compute_gradients(pts, training=self.training,
+                                                                sdf=sdfs_filled)
+        rgbs = self.neural_rgb.forward(points, gradients.view(R, N, 3), ray_unit, feats, None,
+                                       normals_are_gradients=True)  # [R,N,3]
+        alphas = Fn.NeusAlpha.apply(ray_unit, sdfs.view(R, N), gradients.view(R, N, 3), dists, far,
+                                    outside, self.s_var, float(self.progress),
+                                    float(self.anneal_end), float(self.outside_val))
+        weights, rgb = Fn.Composite.apply(alphas, rgbs)
+        opacity = gradient = None
+        if not self.training:
+            opacity = weights.sum(dim=1, keepdim=True)
+            gradient = Fn.WeightedSum.apply(gradients.view(R, N, 3), weights[..., None])
+        return dict(rgb=rgb, rgbs=rgbs, sdfs=sdfs_filled.view(R, N), dists=dists, alphas=alphas,
+                    weights=weights, opacity=opacity, gradient=gradient,
+                    gradients=gradients.view(R, N, 3),
+                    hessians=hessians.view(R, N, 3) if hessians is not None else None)
+
+    def forward(self, data):
+        return self.render_rays(data["center"], data["ray_unit"], stratified=True,
+                                rands=data.get("rands"))
+
+    @torch.no_grad()
+    def inference(self, center, ray_unit, chunk=4096):
+        self.eval()
+        out = defaultdict(list)
+        for i in range(0, center.shape[0], chunk):
+            o = self.render_rays(center[i:i + chunk], ray_unit[i:i + chunk], stratified=False)
+            for k, v in o.items():
+                if v is not None:
+                    out[k].append(v)
+        return {k: torch.cat(v, dim=0) for k, v in out.items()}

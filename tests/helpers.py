@@ -1,0 +1,44 @@
+"""Shared test helpers: copy a product model's parameters into oracle form."""
+import torch
+
+from oracle import field as ofield
+from oracle import hashgrid as hg
+
+
+def oracle_params(model, dtype=torch.float64):
+    sdf, rgb = model.neural_sdf, model.neural_rgb
+    t = {}
+    l0, l1 = sdf.mlp.linears
+    t["W0v"], t["W0g"], t["b0"] = l0.weight_v, l0.weight_g, l0.bias
+    t["W1v"], t["W1g"], t["b1"] = l1.weight_v, l1.weight_g, l1.bias
+    t["ws"], t["bs"] = sdf.mlp.linear_sdf.weight, sdf.mlp.linear_sdf.bias
+    for i, lin in enumerate(rgb.mlp.linears):
+        t[f"R{i}v"], t[f"R{i}g"], t[f"R{i}b"] = lin.weight_v, lin.weight_g, lin.bias
+    return ofield.FieldParams({k: v.detach().cpu().to(dtype).requires_grad_(True)
+                               for k, v in t.items()})
+
+
+def oracle_field(model, P, dtype=torch.float64, table_dtype=None):
+    enc = model.neural_sdf.tcnn_encoding
+    lt = hg.level_table(enc.n_levels, enc.n_features_per_level, enc.log2_hashmap_size,
+                        enc.base_resolution, enc.per_level_scale)
+    table = enc.params.detach().cpu().to(dtype).requires_grad_(True)
+    return ofield.SDFField(lt, table, P, vol_range=enc.vol_range,
+                           active_levels=model.neural_sdf.active_levels, dtype=dtype,
+                           table_dtype=table_dtype), table
+
+
+def randomize_model(model, seed=0, table_scale=0.05):
+    """Non-degenerate parameters: random table and W0 encoder block (the geometric
+    init zeroes it, which would hide encoder gradients)."""
+    g = torch.Generator().manual_seed(seed)
+    with torch.no_grad():
+        enc = model.neural_sdf.tcnn_encoding
+        enc.params.copy_((torch.rand(enc.n_params, generator=g) * 2 - 1) * table_scale)
+        l0 = model.neural_sdf.mlp.linears[0]
+        v = l0.weight_v.detach().cpu().clone()
+        v[:, 3:] = torch.randn(v.shape[0], v.shape[1] - 3, generator=g) * 0.05
+        l0.weight_v.copy_(v)
+        l0.weight_g.copy_(v.norm(dim=1, keepdim=True))
+        for lin in model.neural_sdf.mlp.linears:
+            lin.bias.copy_(torch.randn(lin.bias.shape, generator=g) * 0.01)

@@ -1,0 +1,166 @@
+"""Dense-layer kernels (GEMM + epilogues, narrow layers, weight norm) vs float64 torch on the CPU."""
+import pytest
+import torch
+
+from neuralangelo_amd import _lib
+from neuralangelo_amd import functional as Fn
+from neuralangelo_amd._lib import NA_ACT_RELU, NA_ACT_SIGMOID, NA_ACT_SOFTPLUS
+
+pytestmark = pytest.mark.gpu
+
+BETA = 100.0
+
+
+def _act(v, act):
+    if act == NA_ACT_RELU:
+        return torch.relu(v)
+    if act == NA_ACT_SOFTPLUS:
+        return torch.where(v * BETA > 20, v, torch.log1p(torch.exp(v * BETA)) / BETA)
+    if act == NA_ACT_SIGMOID:
+        return torch.sigmoid(v)
+    return v
+
+
+def _dact(y, act):
+    if act == NA_ACT_RELU:
+        return (y > 0).double()
+    if act == NA_ACT_SOFTPLUS:
+        return torch.where(y * BETA > 20, torch.ones_like(y), -torch.expm1(-y * BETA))
+    if act == NA_ACT_SIGMOID:
+        return y * (1 - y)
+    return torch.ones_like(y)
+
+
+def _rand(shape, seed, scale=1.0):
+    g = torch.Generator().manual_seed(seed)
+    return torch.randn(*shape, generator=g) * scale
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(1000, 200, 136), (257, 256, 256), (64, 24, 280)])
+def test_gemm_layouts(cuda, dt, ta, tb, M, N, K):
+    if ta and M % 8:
+        M += 8 - M % 8  # a transposed operand's leading dimension must stay 16-byte aligned
+    A = _rand((M, K), 1).to(dt)
+    B = _rand((N, K), 2).to(dt)
+    ref = A.double() @ B.double().t()
+    Ad = (A.t().contiguous() if ta else A).to(cuda)
+    Bd = (B.t().contiguous() if tb else B).to(cuda)
+    C = torch.empty(M, N, device=cuda)
+    Fn.gemm(Ad, M if ta else K, ta, Bd, N if tb else K, tb, M, N, K, C, N)
+    err = (C.cpu().double() - ref).abs().max().item()
+    tol = 1e-4 * (K ** 0.5) if dt == torch.float32 else 1e-5 * (K ** 0.5)
+    assert err <= tol, err
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("act", [NA_ACT_SOFTPLUS, NA_ACT_RELU, NA_ACT_SIGMOID])
+def test_gemm_bias_activation_epilogue(cuda, dt, act):
+    M, N, K = 777, 256, 136
+    A = _rand((M, K), 3, 0.1).to(dt)
+    B = _rand((N, K), 4, 0.1).to(dt)
+    bias = _rand((N,), 5, 0.05)
+    ref = _act(A.double() @ B.double().t() + bias.double(), act)
+    for cdt in (torch.float32, dt):
+        C = torch.empty(M, N, dtype=cdt, device=cuda)
+        Fn.gemm(A.to(cuda), K, 0, B.to(cuda), K, 0, M, N, K, C, N, bias=bias.to(cuda), act=act)
+        got = C.cpu().double()
+        rel = 2 ** -7 if cdt == torch.bfloat16 else 1e-5
+        assert ((got - ref).abs() <= rel * ref.abs() + 1e-5).all()
+
+
+def test_gemm_dact_colsum_rank1(cuda):
+    M, N, K = 500, 256, 256
+    A = _rand((M, K), 6, 0.1)
+    W = _rand((K, N), 7, 0.1)  # B(n,k) = W[k, n] (trans_b)
+    Y = torch.nn.functional.softplus(_rand((M, N), 8, 0.01), beta=BETA)
+    r1a, r1b = _rand((M,), 9), _rand((N,), 10)
+    v = A.double() @ W.double() + r1a.double()[:, None] * r1b.double()[None]
+    ref = v * _dact(Y.double(), NA_ACT_SOFTPLUS)
+    C = torch.empty(M, N, device=cuda)
+    cs = torch.zeros(N, device=cuda)
+    Fn.gemm(A.to(cuda), K, 0, W.to(cuda), N, 1, M, N, K, C, N, dact=NA_ACT_SOFTPLUS,
+            dact_y=Y.to(cuda), ld_dact=N, colsum=cs, r1_a=r1a.to(cuda), r1_b=r1b.to(cuda))
+    assert (C.cpu().double() - ref).abs().max() < 1e-4
+    assert (cs.cpu().double() - ref.sum(0)).abs().max() < 1e-3
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.bfloat16])
+def test_weight_grad_split_k(cuda, dt):
+    M, N, K = 20000, 256, 136
+    dpre = _rand((M, N), 11, 0.1).to(dt)
+    X = _rand((M, K), 12, 0.1).to(dt)
+    ref = dpre.double().t() @ X.double()
+    dW = torch.zeros(N, K, device=cuda)
+    Fn.linear_weight_grad(dpre.to(cuda), X.to(cuda), K, K, dW)
+    err = (dW.cpu().double() - ref).abs().max().item()
+    assert err < 1e-3 * (M ** 0.5) * 0.01 + 1e-3, err
+
+
+def test_narrow_forward_backward(cuda):
+    M, K, N = 3000, 256, 3
+    X = torch.relu(_rand((M, K), 13, 0.1))
+    W = _rand((N, K), 14, 0.1)
+    b = _rand((N,), 15, 0.1)
+    out = torch.empty(M, N, device=cuda)
+    _lib.call("na_linear_narrow_fwd", _lib.ptr(X.to(cuda)), K, 0, M, K, _lib.ptr(W.to(cuda)),
+              _lib.ptr(b.to(cuda)), N, NA_ACT_SIGMOID, _lib.ptr(out), _lib.stream_of(out))
+    ref = torch.sigmoid(X.double() @ W.double().t() + b.double())
+    assert (out.cpu().double() - ref).abs().max() < 1e-6
+    dpre = _rand((M, N), 16)
+    Xd = X.to(cuda)
+    dW = torch.zeros(N, K, device=cuda)
+    db = torch.zeros(N, device=cuda)
+    dX = _rand((M, K), 17).to(cuda)
+    dX0 = dX.clone()
+    cs = torch.zeros(K, device=cuda)
+    _lib.call("na_linear_narrow_bwd", _lib.ptr(Xd), K, 0, M, K, _lib.ptr(W.to(cuda)), N,
+              _lib.ptr(dpre.to(cuda)), _lib.ptr(dW), _lib.ptr(db), _lib.ptr(dX), K, 0, 1,
+              _lib.ptr(Xd), K, 0, NA_ACT_RELU, BETA, _lib.ptr(cs), _lib.stream_of(dW))
+    refW = dpre.double().t() @ X.double()
+    refdX = (dX0.cpu().double() + dpre.double() @ W.double()) * (X.double() > 0)
+    assert (dW.cpu().double() - refW).abs().max() < 1e-3
+    assert (db.cpu().double() - dpre.double().sum(0)).abs().max() < 1e-3
+    assert (dX.cpu().double() - refdX).abs().max() < 1e-5
+    assert (cs.cpu().double() - refdX.sum(0)).abs().max() < 1e-3
+
+
+def test_weight_norm_matches_torch(cuda):
+    v = _rand((256, 131), 18)
+    g = _rand((256, 1), 19).abs() + 0.5
+    lin = torch.nn.utils.weight_norm(torch.nn.Linear(131, 256).double())
+    with torch.no_grad():
+        lin.weight_v.copy_(v.double())
+        lin.weight_g.copy_(g.double())
+    vd = v.to(cuda).requires_grad_(True)
+    gd = g.to(cuda).requires_grad_(True)
+    W = Fn.weight_norm(vd, gd)
+    Wref = torch._weight_norm(lin.weight_v, lin.weight_g, 0)
+    assert (W.detach().cpu().double() - Wref.detach()).abs().max() < 1e-6
+    dW = _rand((256, 131), 20)
+    W.backward(dW.to(cuda))
+    gv, gg = torch.autograd.grad((Wref * dW.double()).sum(), [lin.weight_v, lin.weight_g])
+    assert (vd.grad.cpu().double() - gv).abs().max() < 1e-5
+    assert (gd.grad.cpu().double() - gg).abs().max() < 1e-5
+
+
+def test_act_grad(cuda):
+    M, N = 999, 256
+    dy = _rand((M, 280), 21)
+    y = torch.nn.functional.softplus(_rand((M, 280), 22, 0.02), beta=BETA)
+    cs = torch.zeros(N, device=cuda)
+    out = Fn.act_grad(dy.to(cuda), y.to(cuda), NA_ACT_SOFTPLUS, torch.float32, colsum=cs, n_cols=N)
+    ref = dy[:, :N].double() * _dact(y[:, :N].double(), NA_ACT_SOFTPLUS)
+    assert (out.cpu().double() - ref).abs().max() < 1e-5
+    assert (cs.cpu().double() - ref.sum(0)).abs().max() < 1e-3
+
+
+def test_gemm_rejects_misaligned(cuda):
+    A = torch.zeros(10, 6, device=cuda)
+    C = torch.zeros(10, 10, device=cuda)
+    with pytest.raises(RuntimeError, match="multiple"):
+        Fn.gemm(A, 6, 0, A, 6, 0, 10, 10, 6, C, 10)
+    with pytest.raises(RuntimeError, match="split_k"):
+        Fn.gemm(torch.zeros(8, 8, device=cuda), 8, 0, torch.zeros(8, 8, device=cuda), 8, 0, 8, 8, 8,
+                C, 10, split_k=2)

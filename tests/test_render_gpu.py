@@ -1,0 +1,205 @@
+"""Per-ray kernels vs the CPU oracle (sample ordering bit-exact on identical inputs)."""
+import numpy as np
+import pytest
+import torch
+
+from neuralangelo_amd import functional as Fn
+from neuralangelo_amd import misc, nerf_util, render
+from oracle import render as orr
+
+pytestmark = pytest.mark.gpu
+
+
+def _rays(R, seed, radius=3.0, spread=0.35):
+    g = torch.Generator().manual_seed(seed)
+    c = torch.randn(R, 3, generator=g)
+    c = c / c.norm(dim=-1, keepdim=True) * radius
+    d = -c / radius + spread * torch.randn(R, 3, generator=g)
+    d = d / d.norm(dim=-1, keepdim=True)
+    return c, d
+
+
+def test_bounds_and_coarse_samples(cuda):
+    R, N = 3000, 64
+    c, v = _rays(R, 0, spread=0.6)  # some rays miss the unit sphere
+    g = torch.Generator().manual_seed(1)
+    u = torch.rand(R, N, generator=g)
+    near, far, outside, dists = nerf_util.bounds_and_coarse(c.to(cuda), v.to(cuda), N, u.to(cuda))
+    on, of, oo = orr.get_dist_bounds(c, v)
+    assert torch.equal(outside.cpu(), oo[:, 0])
+    assert outside.any() and (~outside).any()
+    assert torch.allclose(near.cpu(), on[:, 0], rtol=1e-6, atol=1e-6)
+    assert torch.allclose(far.cpu(), of[:, 0], rtol=1e-6, atol=1e-6)
+    # identical near/far in, identical op order: coarse distances bit-exact
+    od = orr.sample_dists(near.cpu()[:, None], far.cpu()[:, None], N, u)[..., 0]
+    assert torch.equal(dists.cpu(), od)
+    _, _, _, d05 = nerf_util.bounds_and_coarse(c.to(cuda), v.to(cuda), N, None)
+    assert torch.equal(d05.cpu(), orr.sample_dists(near.cpu()[:, None], far.cpu()[:, None], N)[..., 0])
+
+
+def _weights_cases(R, N, seed):
+    g = torch.Generator().manual_seed(seed)
+    w = torch.rand(R, N - 1, generator=g) ** 4
+    w[0] = 0.0                        # all-zero row -> idx = N, duplicated last bin
+    w[1] = 0.0
+    w[1, 7] = 1.0                     # delta pdf
+    w[2] = 1.0                        # uniform pdf
+    w[3, ::2] = 0.0                   # zero runs (ties in the cdf)
+    return w
+
+
+@pytest.mark.parametrize("N,Nf", [(64, 16), (80, 16), (33, 7)])
+def test_sample_pdf_bit_exact(cuda, N, Nf):
+    R = 2000
+    g = torch.Generator().manual_seed(2)
+    bins = torch.sort(torch.rand(R, N, generator=g) * 2 + 1, dim=-1).values
+    w = _weights_cases(R, N, 3)
+    got = nerf_util.sample_dists_from_pdf(bins.to(cuda)[..., None], w.to(cuda), Nf)[..., 0].cpu()
+    ref = orr.sample_dists_from_pdf(bins[..., None], w, Nf)[..., 0]
+    assert torch.equal(got, ref)
+    # closed forms: uniform pdf -> bins interpolated at the u_j; delta -> one interval
+    assert (got[1] >= bins[1, 7]).all() and (got[1] <= bins[1, 8]).all()
+    assert (got[0] == bins[0, -1]).all()
+
+
+def test_hierarchical_round(cuda):
+    R, N, Nf = 1500, 64, 16
+    c, v = _rays(R, 4)
+    near, far, outside = orr.get_dist_bounds(c, v)
+    dists = orr.sample_dists(near, far, N)[..., 0]
+    # sdf of a sphere of radius 0.5 along the rays
+    pts = c[:, None, :] + v[:, None, :] * dists[..., None]
+    sdfs = pts.norm(dim=-1) - 0.5
+    merged, fine, pc, pf = nerf_util.hierarchical_round(dists.to(cuda), sdfs.to(cuda), 64.0, Nf)
+    merged, fine, pc, pf = merged.cpu(), fine.cpu(), pc.cpu().long(), pf.cpu().long()
+    ref_fine = orr.sample_dists_hierarchical(dists[..., None], sdfs[..., None], 64.0, Nf)[..., 0]
+    # the interval alphas use exp(); allow last-bit differences in the resampled positions
+    assert torch.allclose(fine, ref_fine, rtol=1e-5, atol=1e-5)
+    # the merge is a stable sort of [coarse, fine] (ties keep coarse first)
+    cat = torch.cat([dists, fine], dim=-1)
+    srt, idx = torch.sort(cat, dim=-1, stable=True)
+    assert torch.equal(merged, srt)
+    pos = torch.empty_like(idx)
+    pos.scatter_(1, idx, torch.arange(N + Nf).expand(R, -1))
+    assert torch.equal(pc, pos[:, :N]) and torch.equal(pf, pos[:, N:])
+    # scatter_merge reproduces the sdf gather of model.py:246-247
+    s_fine = torch.randn(R, Nf)
+    sm = nerf_util.scatter_merge(sdfs.to(cuda), pc.int().to(cuda), s_fine.to(cuda), pf.int().to(cuda))
+    assert torch.equal(sm.cpu(), torch.cat([sdfs, s_fine], -1).gather(1, idx))
+
+
+def _neus_inputs(R, N, seed):
+    c, v = _rays(R, seed)
+    near, far, outside = orr.get_dist_bounds(c, v)
+    g = torch.Generator().manual_seed(seed + 1)
+    dists = orr.sample_dists(near, far, N, torch.rand(R, N, generator=g))[..., 0]
+    sdfs = torch.randn(R, N, generator=g) * 0.05
+    grads = torch.randn(R, N, 3, generator=g)
+    return c, v, near, far, outside, dists, sdfs, grads
+
+
+@pytest.mark.parametrize("progress", [0.02, 1.0])
+def test_neus_alpha_fwd_bwd(cuda, progress):
+    R, N = 512, 96
+    c, v, near, far, outside, dists, sdfs, grads = _neus_inputs(R, N, 5)
+    outside[:7] = True
+    s_var = torch.tensor(0.3 * 10)
+    # oracle in float64 with the reference's outside fill
+    sd64 = sdfs.double().clone().requires_grad_(True)
+    gr64 = grads.double().clone().requires_grad_(True)
+    sv64 = s_var.double().clone().requires_grad_(True)
+    filled = torch.where(outside.expand(R, N), torch.full_like(sd64, 1000.0), sd64)
+    a_ref = orr.compute_neus_alphas(v.double(), filled[..., None], gr64, dists.double()[..., None],
+                                    far.double()[..., None], sv64, progress, 0.1)
+    sdd = sdfs.to(cuda).requires_grad_(True)
+    grd = grads.to(cuda).requires_grad_(True)
+    svd = s_var.to(cuda).requires_grad_(True)
+    a = Fn.NeusAlpha.apply(v.to(cuda), sdd, grd, dists.to(cuda), far[:, 0].to(cuda),
+                           outside[:, 0].to(cuda), svd, progress, 0.1, 1000.0)
+    assert (a.detach().cpu().double() - a_ref.detach()).abs().max() < 2e-5
+    up = torch.randn(R, N, generator=torch.Generator().manual_seed(6))
+    a.backward(up.to(cuda))
+    gs, gg, gv = torch.autograd.grad((a_ref * up.double()).sum(), [sd64, gr64, sv64])
+    for got, ref in ((sdd.grad, gs), (grd.grad, gg), (svd.grad, gv)):
+        got = got.cpu().double()
+        assert (got - ref).abs().max() <= 1e-3 * ref.abs().max() + 1e-6
+
+
+def test_composite_fwd_bwd_and_closed_forms(cuda):
+    R, N = 700, 96
+    g = torch.Generator().manual_seed(7)
+    alpha = torch.rand(R, N, generator=g) ** 3
+    alpha[0] = 0.3                 # geometric series: sum w = 1 - 0.7^N
+    alpha[1, 10] = 1.0             # opaque sample: all later weights exactly 0
+    alpha[2] = 1.0
+    q = torch.rand(R, N, 3, generator=g)
+    a64 = alpha.double().requires_grad_(True)
+    q64 = q.double().requires_grad_(True)
+    w_ref = orr.alpha_compositing_weights(a64)
+    out_ref = orr.composite(q64, w_ref)
+    ad = alpha.to(cuda).requires_grad_(True)
+    qd = q.to(cuda).requires_grad_(True)
+    w, out = Fn.Composite.apply(ad, qd)
+    assert (w.detach().cpu().double() - w_ref[..., 0].detach()).abs().max() < 1e-5
+    assert (out.detach().cpu().double() - out_ref.detach()).abs().max() < 1e-5
+    wc = w.detach().cpu().double()
+    assert abs(wc[0].sum().item() - (1 - 0.7 ** N)) < 1e-5
+    assert (wc[1, 11:] == 0).all() and wc[2, 0] == 1.0 and (wc[2, 1:] == 0).all()
+    g1 = torch.randn(R, 3, generator=g)
+    g2 = torch.randn(R, N, generator=g)
+    (out * g1.to(cuda)).sum().add((w * g2.to(cuda)).sum()).backward()
+    ga, gq = torch.autograd.grad((out_ref * g1.double()).sum() + (w_ref[..., 0] * g2.double()).sum(),
+                                 [a64, q64])
+    assert (ad.grad.cpu().double() - ga).abs().max() < 1e-4
+    assert (qd.grad.cpu().double() - gq).abs().max() < 1e-5
+    # alpha_compositing_weights drop-in and weighted-sum composite
+    w2 = render.alpha_compositing_weights(alpha.to(cuda))
+    assert w2.shape == (R, N, 1) and torch.allclose(w2[..., 0], w.detach())
+    comp = render.composite(q.to(cuda), w2)
+    assert (comp.cpu().double() - orr.composite(q.double(), w2.cpu().double())).abs().max() < 1e-5
+
+
+def test_taps_and_losses(cuda):
+    n = 4000
+    g = torch.Generator().manual_seed(8)
+    x = torch.randn(n, 3, generator=g) * 0.4
+    eps64 = np.float64(1.0 / 2048) / np.sqrt(3)
+    eps, eps_sq = float(np.float32(eps64)), float(np.float32(eps64 ** 2))
+    tp = Fn.taps_points(x.to(cuda), eps).cpu()
+    ks = [torch.tensor(k, dtype=torch.float32) for k in ([1, -1, -1], [-1, -1, 1], [-1, 1, -1], [1, 1, 1])]
+    for i, k in enumerate(ks):
+        assert torch.equal(tp[i], x + k * eps64)
+    # sphere sdf at the taps: gradient -> x/|x|, laplacian -> 2/|x|
+    s = (tp.double().norm(dim=-1) - 0.5).float()
+    sdf = (x.double().norm(dim=-1) - 0.5).float()
+    s_d = s.to(cuda).requires_grad_(True)
+    c_d = sdf.to(cuda).requires_grad_(True)
+    grad, hess = Fn.TapsCombine.apply(s_d, c_d, eps, eps_sq, True)
+    gref = (ks[0] * s[0][:, None] + ks[1] * s[1][:, None] + ks[2] * s[2][:, None] +
+            ks[3] * s[3][:, None]) / (4.0 * eps64)
+    href = ((s[0] + s[1] + s[2] + s[3]) / 2.0 - 2 * sdf) / eps64 ** 2
+    assert torch.allclose(grad.detach().cpu(), gref, rtol=1e-5, atol=1e-5)
+    assert torch.allclose(hess.detach().cpu()[:, 0], href / 3.0, rtol=1e-4, atol=1e-2)
+    unit = x / x.norm(dim=-1, keepdim=True)
+    assert (grad.detach().cpu() - unit).abs().max() < 2e-2  # float32 sdf cancellation
+    # losses vs oracle (float64) incl. outside masking and NaN handling
+    R = 100
+    outside = torch.zeros(R, dtype=torch.bool)
+    outside[:5] = True
+    gg = torch.randn(n, 3, generator=g)
+    hh = torch.randn(n, 3, generator=g)
+    gg[17] = float("nan")
+    g64, h64 = gg.double().requires_grad_(True), hh.double().requires_grad_(True)
+    e_ref = orr.eikonal_loss(g64.view(R, -1, 3), outside[:, None])
+    c_ref = orr.curvature_loss(h64.view(R, -1, 3), outside[:, None])
+    gd, hd = gg.to(cuda).requires_grad_(True), hh.to(cuda).requires_grad_(True)
+    e, cc = misc.sdf_losses(gd.view(R, -1, 3), hd.view(R, -1, 3), outside.to(cuda))
+    assert abs(e.item() - e_ref.item()) < 1e-5 * abs(e_ref.item()) + 1e-6
+    assert abs(cc.item() - c_ref.item()) < 1e-5 * abs(c_ref.item()) + 1e-6
+    (0.1 * e + 5e-4 * cc).backward()
+    rg, rh = torch.autograd.grad(0.1 * e_ref + 5e-4 * c_ref, [g64, h64])
+    got = gd.grad.cpu().double()
+    assert torch.equal(got.isnan(), rg.isnan())  # NaN input propagates as in torch autograd
+    fin = ~rg.isnan()
+    assert (got[fin] - rg[fin]).abs().max() < 1e-8
+    assert (hd.grad.cpu().double() - rh).abs().max() < 1e-9
