@@ -298,6 +298,52 @@ int na_sdf_losses_bwd(const float* grad, const float* hess, const uint8_t* outsi
                       int64_t n, int64_t n_per_ray, const float* w, float* dgrad,
                       int32_t add_dgrad, float* dhess, na_stream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * AdamW step (replaces torch.optim.AdamW built at imaginaire/trainers/utils/
+ * get_trainer.py:106-150 and stepped at imaginaire/trainers/base.py:454-457,
+ * lr 1e-3 / wd 1e-2 from projects/neuralangelo/configs/base.yaml:117-121).
+ * Same update as torch's: p *= 1 - lr*wd; m = lerp(m, g, 1-beta1);
+ * v = beta2 v + (1-beta2) g^2; p -= lr/bc1 * m / (sqrt(v)/sqrt(bc2) + eps),
+ * bc_i = 1 - beta_i^step.  If lowp != NULL the bf16 copy of the updated
+ * parameter is written in the same pass (the hash table's gather copy).
+ * ------------------------------------------------------------------------- */
+int na_adamw_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                  float lr, float beta1, float beta2, float eps, float weight_decay,
+                  int64_t step, void* lowp, int32_t lowp_dtype, na_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Marching cubes over a block-partitioned lattice.  Replaces the per-block
+ * loop of projects/neuralangelo/utils/mesh.py:24-49 (LatticeGrid blocks of
+ * block_res+1 points, :68-100; mcubes.marching_cubes(sdf, 0.) per block,
+ * :119-121; V*intv + xyz[0,0,0], :123) and, with block_res <= 0 (one block
+ * spanning the volume), mcubes.marching_cubes(volume, iso) itself.
+ *
+ * volume: f32 [nx, ny, nz] C order, the whole lattice.  Blocks hold block_res
+ * cells per axis (the last block of an axis may hold fewer) and are visited
+ * x-major, as the reference's block index.  A cell's cube index has bit m set
+ * iff value[m] <= iso.  Output = the concatenation, in block order, of each
+ * block's mesh: float64 vertices [nv, 3] and int64 faces [nf, 3] indexing the
+ * concatenated vertex list.  Within a block the order is that of the
+ * sequential algorithm (cells i-major; per cell the owned edges 6, 5, 10, then
+ * 0,1,2,3,4,7,8,9,11 on the block boundary; triangles in table order).
+ * Vertices are block-local array-index coordinates, or, when the three f32
+ * axis arrays (the lattice coordinates, length nx/ny/nz) are given,
+ * local * intv + axis[block origin] in float64.
+ *
+ * Two calls: na_mc_count classifies, scans and writes totals[0] = nv,
+ * totals[1] = nf (device int64[2]); after the caller sizes the outputs,
+ * na_mc_emit writes them.  The workspace (na_mc_workspace_size bytes, caller
+ * owned, 256-byte aligned) carries state from count to emit.
+ * ------------------------------------------------------------------------- */
+int64_t na_mc_workspace_size(int64_t nx, int64_t ny, int64_t nz, int64_t block_res);
+int na_mc_count(const float* volume, int64_t nx, int64_t ny, int64_t nz, int64_t block_res,
+                double iso, void* workspace, int64_t ws_bytes, int64_t* totals,
+                na_stream_t stream);
+int na_mc_emit(const float* volume, int64_t nx, int64_t ny, int64_t nz, int64_t block_res,
+               double iso, void* workspace, int64_t ws_bytes, const float* axis_x,
+               const float* axis_y, const float* axis_z, double intv, double* vertices,
+               int64_t* faces, na_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

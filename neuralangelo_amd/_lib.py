@@ -102,6 +102,10 @@ _SIGNATURES = {
     "na_rgb_input_bwd": (ctypes.c_int, [_P, _I32, _I64, _I32, _P, _I32, _I64, _P, _I32, _P]),
     "na_sdf_losses_fwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P]),
     "na_sdf_losses_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P, _I32, _P, _P]),
+    "na_adamw_step": (ctypes.c_int, [_P, _P, _P, _P, _I64, _F32, _F32, _F32, _F32, _F32, _I64, _P, _I32, _P]),
+    "na_mc_workspace_size": (_I64, [_I64, _I64, _I64, _I64]),
+    "na_mc_count": (ctypes.c_int, [_P, _I64, _I64, _I64, _I64, _F64, _P, _I64, _P, _P]),
+    "na_mc_emit": (ctypes.c_int, [_P, _I64, _I64, _I64, _I64, _F64, _P, _I64, _P, _P, _P, _F64, _P, _P, _P]),
 }
 
 _lib = None
@@ -153,11 +157,22 @@ def dtype_code(dtype):
         raise RuntimeError(f"neuralangelo_amd: unsupported dtype {dtype}") from None
 
 
+class _TensorPtr(ctypes.c_void_p):
+    """A c_void_p that keeps its tensor alive while the argument list exists.
+
+    ``call(..., ptr(x.contiguous()), ...)`` would otherwise drop the temporary
+    before the kernel is enqueued, letting the caching allocator hand its block
+    to a later argument.  Holding the reference until the C call returns is
+    enough: allocations made after the launch are stream-ordered behind it."""
+
+
 def ptr(t):
-    """Raw device pointer of a tensor (None -> NULL)."""
+    """Raw device pointer of a tensor (None -> NULL); keeps the tensor alive."""
     if t is None:
         return None
-    return ctypes.c_void_p(t.data_ptr())
+    p = _TensorPtr(t.data_ptr())
+    p.tensor = t
+    return p
 
 
 def stream_of(t=None):

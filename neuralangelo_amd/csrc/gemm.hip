@@ -20,7 +20,7 @@ namespace na {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
 typedef __attribute__((ext_vector_type(16))) float f32x16_t;
 
-constexpr int GBM = 128, GBN = 128, GBK = 32, GNT = 256;
+constexpr int GBM = 128, GBN = 128, GNT = 256;
 
 struct EpiParams {
   int epilogue;
@@ -95,63 +95,96 @@ __device__ inline void epilogue_store(const EpiParams& e, int64_t m, int64_t n, 
   }
 }
 
-// Stage a [ROWS x GBK] operand tile into LDS laid out [row][k] (k contiguous).
-//   TRANS = false: element (r,k) at X[(r0+r)*ld + k0+k]   (k contiguous in memory)
-//   TRANS = true : element (r,k) at X[(k0+k)*ld + r0+r]   (r contiguous in memory)
-template <typename T, bool TRANS, int LDK>
-__device__ inline void stage_tile(T* __restrict__ s, const T* __restrict__ X, int64_t ld,
-                                  int64_t r0, int64_t rmax, int64_t k0, int64_t kend, int tid) {
-  constexpr int VEC = 16 / (int)sizeof(T);             // elements per 16-byte vector
-  constexpr int NVEC = GBM * GBK / VEC / GNT;          // vectors per thread
+// Per element type: K-tile depth, LDS row pitch (elements) and MFMA step.
+//   bf16: BK = 32, pitch 40 (80 B rows keep 16-byte MFMA operand reads aligned)
+//   fp32: BK = 16, pitch 17 (odd pitch: conflict-free single-float operand reads)
+template <typename T> struct GemmCfg;
+template <> struct GemmCfg<bf16_t> {
+  static constexpr int BK = 32, LDK = 40;
+};
+template <> struct GemmCfg<float> {
+  static constexpr int BK = 16, LDK = 17;
+};
+
+// Global -> register stage of one [128 x BK] operand tile, then register -> LDS
+// ([row][k] layout, k contiguous).  Consecutive threads read consecutive
+// 16-byte chunks of the contiguous dimension.
+//   TRANS = false: element (r,k) at X[(r0+r)*ld + k0+k]   (k contiguous)
+//   TRANS = true : element (r,k) at X[(k0+k)*ld + r0+r]   (r contiguous)
+template <typename T, bool TRANS>
+struct TileLoader {
+  static constexpr int BK = GemmCfg<T>::BK;
+  static constexpr int LDK = GemmCfg<T>::LDK;
+  static constexpr int VEC = 16 / (int)sizeof(T);
+  static constexpr int NVEC = GBM * BK / VEC / GNT;
+  union Vec {
+    uint4 q;
+    T e[VEC];
+  };
+  Vec reg[NVEC];
+
+  __device__ inline void load(const T* __restrict__ X, int64_t ld, int64_t r0, int64_t rmax,
+                              int64_t k0, int64_t kend, int tid) {
 #pragma unroll
-  for (int j = 0; j < NVEC; ++j) {
-    int v = tid * NVEC + j;
-    T vals[VEC];
-    if (!TRANS) {
-      int r = v / (GBK / VEC);
-      int k = (v % (GBK / VEC)) * VEC;
-      int64_t gr = r0 + r, gk = k0 + k;
-      if (gr < rmax && gk < kend) {
-        uint4 q = *reinterpret_cast<const uint4*>(X + gr * ld + gk);
-        *reinterpret_cast<uint4*>(vals) = q;
+    for (int j = 0; j < NVEC; ++j) {
+      const int v = j * GNT + tid;
+      if (!TRANS) {
+        const int r = v / (BK / VEC), k = (v % (BK / VEC)) * VEC;
+        const int64_t gr = r0 + r, gk = k0 + k;
+        if (gr < rmax && gk < kend) {
+          reg[j].q = *reinterpret_cast<const uint4*>(X + gr * ld + gk);
+        } else {
+          reg[j].q = make_uint4(0, 0, 0, 0);
+        }
       } else {
+        const int k = v / (GBM / VEC), r = (v % (GBM / VEC)) * VEC;
+        const int64_t gr = r0 + r, gk = k0 + k;
+        if (gk < kend && gr + VEC <= rmax) {
+          reg[j].q = *reinterpret_cast<const uint4*>(X + gk * ld + gr);
+        } else {
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) vals[e] = T(0);
+          for (int e = 0; e < VEC; ++e)
+            reg[j].e[e] = (gk < kend && gr + e < rmax) ? X[gk * ld + gr + e] : T(0);
+        }
       }
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) s[r * LDK + k + e] = vals[e];
-    } else {
-      int k = v / (GBM / VEC);
-      int r = (v % (GBM / VEC)) * VEC;
-      int64_t gr = r0 + r, gk = k0 + k;
-      if (gk < kend && gr + VEC <= rmax) {
-        uint4 q = *reinterpret_cast<const uint4*>(X + gk * ld + gr);
-        *reinterpret_cast<uint4*>(vals) = q;
-      } else {
-#pragma unroll
-        for (int e = 0; e < VEC; ++e)
-          vals[e] = (gk < kend && gr + e < rmax) ? X[gk * ld + gr + e] : T(0);
-      }
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) s[(r + e) * LDK + k] = vals[e];
     }
   }
-}
 
-template <typename T> struct MfmaTraits;
-template <> struct MfmaTraits<bf16_t> {
-  static constexpr int LDK = GBK + 8;
-  __device__ static inline void step(const bf16_t* As, const bf16_t* Bs, int wm, int wn, int lane,
-                                     f32x16_t acc[2][2]) {
+  __device__ inline void store(T* __restrict__ s, int tid) const {
 #pragma unroll
-    for (int kk = 0; kk < GBK; kk += 16) {
+    for (int j = 0; j < NVEC; ++j) {
+      const int v = j * GNT + tid;
+      if (!TRANS) {
+        const int r = v / (BK / VEC), k = (v % (BK / VEC)) * VEC;
+        if constexpr (sizeof(T) == 2) {
+          *reinterpret_cast<uint4*>(s + r * LDK + k) = reg[j].q;
+        } else {
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) s[r * LDK + k + e] = reg[j].e[e];
+        }
+      } else {
+        const int k = v / (GBM / VEC), r = (v % (GBM / VEC)) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) s[(r + e) * LDK + k] = reg[j].e[e];
+      }
+    }
+  }
+};
+
+template <typename T> struct MfmaStep;
+template <> struct MfmaStep<bf16_t> {
+  static constexpr int LDK = GemmCfg<bf16_t>::LDK;
+  __device__ static inline void run(const bf16_t* As, const bf16_t* Bs, int wm, int wn, int lane,
+                                    f32x16_t acc[2][2]) {
+#pragma unroll
+    for (int kk = 0; kk < GemmCfg<bf16_t>::BK; kk += 16) {
       bf16x8_t a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const bf16_t* pa = As + (wm * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5) * 8;
-        const bf16_t* pb = Bs + (wn * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5) * 8;
-        a[i] = *reinterpret_cast<const bf16x8_t*>(pa);
-        b[i] = *reinterpret_cast<const bf16x8_t*>(pb);
+        a[i] = *reinterpret_cast<const bf16x8_t*>(
+            As + (wm * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5) * 8);
+        b[i] = *reinterpret_cast<const bf16x8_t*>(
+            Bs + (wn * 64 + i * 32 + (lane & 31)) * LDK + kk + (lane >> 5) * 8);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -161,12 +194,12 @@ template <> struct MfmaTraits<bf16_t> {
     }
   }
 };
-template <> struct MfmaTraits<float> {
-  static constexpr int LDK = GBK + 1;
-  __device__ static inline void step(const float* As, const float* Bs, int wm, int wn, int lane,
-                                     f32x16_t acc[2][2]) {
+template <> struct MfmaStep<float> {
+  static constexpr int LDK = GemmCfg<float>::LDK;
+  __device__ static inline void run(const float* As, const float* Bs, int wm, int wn, int lane,
+                                    f32x16_t acc[2][2]) {
 #pragma unroll
-    for (int kk = 0; kk < GBK; kk += 2) {
+    for (int kk = 0; kk < GemmCfg<float>::BK; kk += 2) {
       float a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
@@ -182,11 +215,27 @@ template <> struct MfmaTraits<float> {
   }
 };
 
+// Two LDS stages: the next K-tile is fetched into registers while the MFMAs
+// consume the current stage, then written to the other stage; one barrier per
+// K-tile.  The epilogue stages each wave's accumulators through LDS (reusing
+// the operand buffers) so that every accumulator register is addressed
+// statically and the global stores run along rows (64 consecutive columns per
+// wave instruction).
+template <typename T> struct GemmSmem {
+  static constexpr int LDK = GemmCfg<T>::LDK;
+  static constexpr int OPERAND_BYTES = 2 * (GBM + GBN) * LDK * (int)sizeof(T);
+  static constexpr int CPITCH = 65;                       // floats per staged row
+  static constexpr int C_BYTES = 4 * 32 * CPITCH * 4;    // 4 waves x 32 rows
+  static constexpr int BYTES = OPERAND_BYTES > C_BYTES ? OPERAND_BYTES : C_BYTES;
+};
+
 template <typename T, bool TA, bool TB>
 __global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
-  constexpr int LDK = MfmaTraits<T>::LDK;
-  __shared__ __attribute__((aligned(16))) T As[GBM * LDK];
-  __shared__ __attribute__((aligned(16))) T Bs[GBN * LDK];
+  constexpr int BK = GemmCfg<T>::BK;
+  constexpr int LDK = GemmCfg<T>::LDK;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[GemmSmem<T>::BYTES];
+  T* As0 = reinterpret_cast<T*>(smem);
+  T* Bs0 = As0 + 2 * GBM * LDK;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t m0 = (int64_t)blockIdx.x * GBM;
@@ -204,38 +253,61 @@ __global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-  for (int64_t k0 = kbeg; k0 < kend; k0 += GBK) {
-    stage_tile<T, TA, LDK>(As, A, p.lda, m0, p.M, k0, kend, tid);
-    stage_tile<T, TB, LDK>(Bs, B, p.ldb, n0, p.N, k0, kend, tid);
+  TileLoader<T, TA> la;
+  TileLoader<T, TB> lb;
+  if (kbeg < kend) {
+    la.load(A, p.lda, m0, p.M, kbeg, kend, tid);
+    lb.load(B, p.ldb, n0, p.N, kbeg, kend, tid);
+    la.store(As0, tid);
+    lb.store(Bs0, tid);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if (more) {
+      la.load(A, p.lda, m0, p.M, k0 + BK, kend, tid);
+      lb.load(B, p.ldb, n0, p.N, k0 + BK, kend, tid);
+    }
+    MfmaStep<T>::run(As0 + buf * GBM * LDK, Bs0 + buf * GBN * LDK, wm, wn, lane, acc);
+    if (more) {
+      la.store(As0 + (buf ^ 1) * GBM * LDK, tid);
+      lb.store(Bs0 + (buf ^ 1) * GBN * LDK, tid);
+    }
     __syncthreads();
-    MfmaTraits<T>::step(As, Bs, wm, wn, lane, acc);
-    __syncthreads();
+    buf ^= 1;
   }
 
+  // ---- epilogue: per half (i), stage a 32 x 64 accumulator block per wave ----
+  constexpr int CP = GemmSmem<T>::CPITCH;
+  float* Cw = reinterpret_cast<float*>(smem) + wave * 32 * CP;
   const EpiParams& e = p.e;
+  const int64_t n = n0 + wn * 64 + lane;
+  float csum = 0.0f;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int64_t n = n0 + wn * 64 + j * 32 + (lane & 31);
-    float csum = 0.0f;
+  for (int i = 0; i < 2; ++i) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int col = j * 32 + (lane & 31);
+        Cw[row * CP + col] = acc[i][j][r];
+      }
+    __syncthreads();
     if (n < p.N) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int64_t m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          if (m < p.M) {
-            float v = epilogue_value(e, m, n, acc[i][j][r]);
-            csum += v;
-            epilogue_store(e, m, n, v);
-          }
-        }
+      const int64_t mb = m0 + wm * 64 + i * 32;
+      for (int rr = 0; rr < 32; ++rr) {
+        const int64_t m = mb + rr;
+        if (m >= p.M) break;
+        float v = epilogue_value(e, m, n, Cw[rr * CP + lane]);
+        csum += v;
+        epilogue_store(e, m, n, v);
       }
     }
-    if (e.colsum) {
-      csum += __shfl_xor(csum, 32);
-      if (lane < 32 && n < p.N) atomic_add_f32(e.colsum + n, csum);
-    }
+    __syncthreads();
   }
+  if (e.colsum && n < p.N) atomic_add_f32(e.colsum + n, csum);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,97 +315,190 @@ __global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
 // ---------------------------------------------------------------------------
 constexpr int kNarrowMax = 4;
 
-template <typename TX>
+// Vector load of 8 consecutive elements as floats (16 B of bf16 / 32 B of fp32).
+__device__ inline void load8(const bf16_t* __restrict__ p, float v[8]) {
+  uint4 q = *reinterpret_cast<const uint4*>(p);
+  uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[2 * i] = bf16_to_f32((bf16_t)(w[i] & 0xffffu));
+    v[2 * i + 1] = bf16_to_f32((bf16_t)(w[i] >> 16));
+  }
+}
+__device__ inline void load8(const float* __restrict__ p, float v[8]) {
+  float4 a = *reinterpret_cast<const float4*>(p);
+  float4 b = *reinterpret_cast<const float4*>(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+// Row-parallel narrow layer: a half-wave (32 lanes) owns one row; lane l covers
+// k = 8*l + 256*c (c < KC) with 16/32-byte vector loads; each wave keeps two
+// rows per iteration and unrolls two iterations so four rows are in flight.
+template <typename TX, int KC>
 __global__ void __launch_bounds__(256)
-narrow_fwd_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
-                  const float* __restrict__ W, const float* __restrict__ b, int N, int act,
-                  float* __restrict__ out) {
+narrow_fwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
+                      const float* __restrict__ W, const float* __restrict__ b, int N, int act,
+                      float* __restrict__ out) {
   const int lane = threadIdx.x & 63;
+  const int hl = lane & 31, half = lane >> 5;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t m = wave; m < M; m += nwaves) {
-    float s[kNarrowMax] = {0.f, 0.f, 0.f, 0.f};
-    for (int64_t k = lane; k < K; k += 64) {
-      float x = scalar_io<TX>::load(X + m * ldx + k);
+  float w[kNarrowMax][KC][8];
+#pragma unroll
+  for (int n = 0; n < kNarrowMax; ++n)
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t k = 8 * hl + 256 * c + e;
+        w[n][c][e] = (n < N && k < K) ? W[n * K + k] : 0.0f;
+      }
+  const int64_t pairs = (M + 1) / 2;
+  for (int64_t pr = wave; pr < pairs; pr += 2 * nwaves) {
+    float s[2][kNarrowMax];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int64_t m = 2 * (pr + u * nwaves) + half;
+#pragma unroll
+      for (int n = 0; n < kNarrowMax; ++n) s[u][n] = 0.0f;
+      if (m < M) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int64_t k = 8 * hl + 256 * c;
+          if (k < K) {
+            float x[8];
+            load8(X + m * ldx + k, x);
+#pragma unroll
+            for (int n = 0; n < kNarrowMax; ++n)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) s[u][n] = fmaf(x[e], w[n][c][e], s[u][n]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int n = 0; n < kNarrowMax; ++n)
-        if (n < N) s[n] = fmaf(x, W[n * K + k], s[n]);
-    }
 #pragma unroll
-    for (int n = 0; n < kNarrowMax; ++n) {
+        for (int o = 16; o > 0; o >>= 1) s[u][n] += __shfl_xor(s[u][n], o);
+    if (hl == 0) {
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) s[n] += __shfl_xor(s[n], o);
-    }
-    if (lane == 0) {
-      for (int n = 0; n < N; ++n) out[m * N + n] = act_forward(act, s[n] + b[n], 1.0f);
+      for (int u = 0; u < 2; ++u) {
+        const int64_t m = 2 * (pr + u * nwaves) + half;
+        if (m < M)
+          for (int n = 0; n < N; ++n) out[m * N + n] = act_forward(act, s[u][n] + b[n], 1.0f);
+      }
     }
   }
 }
 
-template <typename TX, typename TDX>
+template <typename TX, typename TDX, int KC>
 __global__ void __launch_bounds__(256)
-narrow_bwd_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
-                  const float* __restrict__ W, int N, const float* __restrict__ dpre,
-                  float* __restrict__ dW, float* __restrict__ db, TDX* __restrict__ dX,
-                  int64_t lddx, int add_dx, const void* Y, int64_t ldy, int y_dtype, int dact,
-                  float beta, float* __restrict__ dx_colsum) {
-  // each lane owns k = lane + 64*j, j < KJ; dW partials stay in registers
-  constexpr int KJ = 8;  // K <= 512
+narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
+                      const float* __restrict__ W, int N, const float* __restrict__ dpre,
+                      float* __restrict__ dW, float* __restrict__ db, TDX* __restrict__ dX,
+                      int64_t lddx, int add_dx, const void* Y, int64_t ldy, int y_dtype,
+                      int dact, float beta, float* __restrict__ dx_colsum) {
   const int lane = threadIdx.x & 63;
+  const int hl = lane & 31, half = lane >> 5;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  float gw[kNarrowMax][KJ];
+  float w[kNarrowMax][KC][8], gw[kNarrowMax][KC][8], cs[KC][8];
   float gb[kNarrowMax] = {0.f, 0.f, 0.f, 0.f};
-  float cs[KJ];
-#pragma unroll
-  for (int j = 0; j < KJ; ++j) cs[j] = 0.0f;
 #pragma unroll
   for (int n = 0; n < kNarrowMax; ++n)
 #pragma unroll
-    for (int j = 0; j < KJ; ++j) gw[n][j] = 0.0f;
-  for (int64_t m = wave; m < M; m += nwaves) {
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t k = 8 * hl + 256 * c + e;
+        w[n][c][e] = (n < N && k < K) ? W[n * K + k] : 0.0f;
+        gw[n][c][e] = 0.0f;
+        if (n == 0) cs[c][e] = 0.0f;
+      }
+  for (int64_t m = 2 * wave + half; m < M; m += 2 * nwaves) {
     float d[kNarrowMax];
 #pragma unroll
-    for (int n = 0; n < kNarrowMax; ++n) d[n] = n < N ? dpre[m * N + n] : 0.0f;
+    for (int n = 0; n < kNarrowMax; ++n) {
+      d[n] = n < N ? dpre[m * N + n] : 0.0f;
+      gb[n] += d[n];
+    }
 #pragma unroll
-    for (int n = 0; n < kNarrowMax; ++n) gb[n] += d[n];
-#pragma unroll
-    for (int j = 0; j < KJ; ++j) {
-      int64_t k = lane + 64 * j;
+    for (int c = 0; c < KC; ++c) {
+      const int64_t k = 8 * hl + 256 * c;
       if (k < K) {
-        float x = scalar_io<TX>::load(X + m * ldx + k);
-        float dx = 0.0f;
+        float x[8];
+        load8(X + m * ldx + k, x);
+        float dx[8];
 #pragma unroll
-        for (int n = 0; n < kNarrowMax; ++n) {
-          if (n < N) {
-            gw[n][j] = fmaf(d[n], x, gw[n][j]);
-            dx = fmaf(d[n], W[n * K + k], dx);
+        for (int e = 0; e < 8; ++e) {
+          dx[e] = 0.0f;
+#pragma unroll
+          for (int n = 0; n < kNarrowMax; ++n) {
+            gw[n][c][e] = fmaf(d[n], x[e], gw[n][c][e]);
+            dx[e] = fmaf(d[n], w[n][c][e], dx[e]);
           }
         }
         if (dX) {
-          if (add_dx) dx += scalar_io<TDX>::load(dX + m * lddx + k);
-          if (dact) dx *= act_deriv_from_y(dact, load_as_f32(Y, y_dtype, m * ldy + k), beta);
-          scalar_io<TDX>::store(dX + m * lddx + k, dx);
-          cs[j] += dx;
+          TDX* pdx = dX + m * lddx + k;
+          if (add_dx) {
+            float old[8];
+            load8(pdx, old);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) dx[e] += old[e];
+          }
+          if (dact) {
+            float yv[8];
+            if (y_dtype == NA_BF16) load8((const bf16_t*)Y + m * ldy + k, yv);
+            else load8((const float*)Y + m * ldy + k, yv);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) dx[e] *= act_deriv_from_y(dact, yv[e], beta);
+          }
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            scalar_io<TDX>::store(pdx + e, dx[e]);
+            cs[c][e] += dx[e];
+          }
         }
       }
     }
   }
-  for (int n = 0; n < N; ++n) {
+  // combine the two half-waves (same k ranges), then one atomic per element
 #pragma unroll
-    for (int j = 0; j < KJ; ++j) {
-      int64_t k = lane + 64 * j;
-      if (k < K) atomic_add_f32(dW + n * K + k, gw[n][j]);
-    }
-    float s = gb[n];
-    if (lane == 0) atomic_add_f32(db + n, s);
+  for (int n = 0; n < kNarrowMax; ++n) {
+    gb[n] += __shfl_xor(gb[n], 32);
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) gw[n][c][e] += __shfl_xor(gw[n][c][e], 32);
   }
-  if (dX && dx_colsum) {
 #pragma unroll
-    for (int j = 0; j < KJ; ++j) {
-      int64_t k = lane + 64 * j;
-      if (k < K) atomic_add_f32(dx_colsum + k, cs[j]);
+  for (int c = 0; c < KC; ++c)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) cs[c][e] += __shfl_xor(cs[c][e], 32);
+  if (half == 0) {
+    for (int n = 0; n < N; ++n) {
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int64_t k = 8 * hl + 256 * c + e;
+          if (k < K) atomic_add_f32(dW + n * K + k, gw[n][c][e]);
+        }
     }
+    if (dX && dx_colsum) {
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int64_t k = 8 * hl + 256 * c + e;
+          if (k < K) atomic_add_f32(dx_colsum + k, cs[c][e]);
+        }
+    }
+    if (hl == 0)
+      for (int n = 0; n < N; ++n) atomic_add_f32(db + n, gb[n]);
   }
 }
 
@@ -450,10 +615,6 @@ static int launch_gemm(const na_gemm_args* a, const GemmParams& p, dim3 grid, hi
   return NA_OK;
 }
 
-static inline int num_waves_for(int64_t rows) {
-  int64_t w = rows < 4096 ? rows : 4096;
-  return (int)(w < 1 ? 1 : w);
-}
 
 }  // namespace na
 
@@ -482,8 +643,9 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   GemmParams p;
   p.M = a->M; p.N = a->N; p.K = a->K;
   p.A = a->A; p.lda = a->lda; p.B = a->B; p.ldb = a->ldb;
-  int64_t kc = ceil_div(ceil_div(a->K, split), GBK) * GBK;
-  if (kc < GBK) kc = GBK;
+  const int64_t bk = a->in_dtype == NA_BF16 ? GemmCfg<bf16_t>::BK : GemmCfg<float>::BK;
+  int64_t kc = ceil_div(ceil_div(a->K, split), bk) * bk;
+  if (kc < bk) kc = bk;
   p.k_chunk = kc;
   int64_t nsplit = a->K == 0 ? 1 : ceil_div(a->K, kc);
   p.e.epilogue = a->epilogue; p.e.C = a->C; p.e.ldc = a->ldc; p.e.c_dtype = a->c_dtype;
@@ -496,6 +658,11 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   return launch_gemm<float>(a, p, grid, s);
 }
 
+static bool narrow_vec_ok(int64_t K, int64_t ld, const void* X, int dtype) {
+  const int64_t align = dtype == NA_BF16 ? 16 : 32;
+  return K % 8 == 0 && K <= 512 && ld % 8 == 0 && ((uintptr_t)X % align) == 0;
+}
+
 extern "C" int na_linear_narrow_fwd(const void* X, int64_t ldx, int32_t x_dtype, int64_t M,
                                     int64_t K, const float* W, const float* b, int32_t N,
                                     int32_t act, float* out, na_stream_t stream) {
@@ -503,15 +670,21 @@ extern "C" int na_linear_narrow_fwd(const void* X, int64_t ldx, int32_t x_dtype,
   NA_CHECK_ARG(M >= 0 && K >= 1, "bad sizes");
   if (M == 0) return NA_OK;
   NA_CHECK_ARG(X && W && b && out, "null pointer");
-  int waves = num_waves_for(M);
+  NA_CHECK_ARG(narrow_vec_ok(K, ldx, X, x_dtype),
+               "narrow layer needs K % 8 == 0, K <= 512, ldx % 8 == 0 and aligned rows");
+  int64_t waves = ceil_div(ceil_div(M, 2), 2);
+  if (waves > 16384) waves = 16384;
   dim3 grid((unsigned)ceil_div(waves, 4));
   hipStream_t s = (hipStream_t)stream;
-  if (x_dtype == NA_BF16)
-    hipLaunchKernelGGL(narrow_fwd_kernel<bf16_t>, grid, dim3(256), 0, s, (const bf16_t*)X, ldx, M,
-                       K, W, b, N, act, out);
-  else
-    hipLaunchKernelGGL(narrow_fwd_kernel<float>, grid, dim3(256), 0, s, (const float*)X, ldx, M,
-                       K, W, b, N, act, out);
+#define NA_NFWD(TX, KC)                                                                    \
+  hipLaunchKernelGGL((narrow_fwd_vec_kernel<TX, KC>), grid, dim3(256), 0, s, (const TX*)X, ldx, \
+                     M, K, W, b, N, act, out)
+  if (x_dtype == NA_BF16) {
+    if (K <= 256) NA_NFWD(bf16_t, 1); else NA_NFWD(bf16_t, 2);
+  } else {
+    if (K <= 256) NA_NFWD(float, 1); else NA_NFWD(float, 2);
+  }
+#undef NA_NFWD
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
@@ -523,23 +696,34 @@ extern "C" int na_linear_narrow_bwd(const void* X, int64_t ldx, int32_t x_dtype,
                                     int32_t dact, float act_beta, float* dx_colsum,
                                     na_stream_t stream) {
   NA_CHECK_ARG(N >= 1 && N <= kNarrowMax, "N must be in [1, 4]");
-  NA_CHECK_ARG(K >= 1 && K <= 512, "K must be in [1, 512]");
-  NA_CHECK_ARG(M >= 0, "bad sizes");
+  NA_CHECK_ARG(M >= 0 && K >= 1, "bad sizes");
   if (M == 0) return NA_OK;
   NA_CHECK_ARG(X && W && dpre && dW && db, "null pointer");
   NA_CHECK_ARG(!dact || Y, "dact needs Y");
-  int waves = num_waves_for(M);
+  NA_CHECK_ARG(narrow_vec_ok(K, ldx, X, x_dtype) && (!dX || narrow_vec_ok(K, lddx, dX, dx_dtype)) &&
+                   (!dact || narrow_vec_ok(K, ldy, Y, y_dtype)),
+               "narrow layer needs K % 8 == 0, K <= 512, leading dims % 8 == 0 and aligned rows");
+  int64_t waves = ceil_div(M, 16);
+  if (waves > 4096) waves = 4096;
   dim3 grid((unsigned)ceil_div(waves, 4));
   hipStream_t s = (hipStream_t)stream;
-#define NA_NARROW_BWD(TX, TDX)                                                             \
-  hipLaunchKernelGGL((narrow_bwd_kernel<TX, TDX>), grid, dim3(256), 0, s, (const TX*)X, ldx, M, \
-                     K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, Y, ldy, y_dtype, dact,   \
-                     act_beta, dx_colsum)
-  if (x_dtype == NA_BF16 && dx_dtype == NA_BF16) NA_NARROW_BWD(bf16_t, bf16_t);
-  else if (x_dtype == NA_BF16) NA_NARROW_BWD(bf16_t, float);
-  else if (dx_dtype == NA_BF16) NA_NARROW_BWD(float, bf16_t);
-  else NA_NARROW_BWD(float, float);
-#undef NA_NARROW_BWD
+#define NA_NBWD(TX, TDX, KC)                                                                  \
+  hipLaunchKernelGGL((narrow_bwd_vec_kernel<TX, TDX, KC>), grid, dim3(256), 0, s, (const TX*)X, \
+                     ldx, M, K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, Y, ldy, y_dtype,     \
+                     dact, act_beta, dx_colsum)
+  const bool x16 = x_dtype == NA_BF16, d16 = dx_dtype == NA_BF16;
+  if (K <= 256) {
+    if (x16 && d16) NA_NBWD(bf16_t, bf16_t, 1);
+    else if (x16) NA_NBWD(bf16_t, float, 1);
+    else if (d16) NA_NBWD(float, bf16_t, 1);
+    else NA_NBWD(float, float, 1);
+  } else {
+    if (x16 && d16) NA_NBWD(bf16_t, bf16_t, 2);
+    else if (x16) NA_NBWD(bf16_t, float, 2);
+    else if (d16) NA_NBWD(float, bf16_t, 2);
+    else NA_NBWD(float, float, 2);
+  }
+#undef NA_NBWD
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -1,0 +1,97 @@
+// Fused AdamW over a flat parameter tensor (HBM-bound: 7 streams of 4 bytes
+// per parameter, plus 2 bytes for the optional bf16 copy).
+#include <math.h>
+
+#include "common.h"
+
+namespace na {
+
+struct AdamArgs {
+  float lr, beta1, beta2, eps, decay;   // decay = 1 - lr * weight_decay
+  float step_size, inv_bc2_sqrt;        // lr / bc1, 1 / sqrt(bc2)
+};
+
+__device__ inline float lerp_torch(float a, float b, float w) {
+  // torch.lerp: a + w (b - a) for |w| < 0.5, else b - (b - a)(1 - w)
+  return fabsf(w) < 0.5f ? a + w * (b - a) : b - (b - a) * (1.0f - w);
+}
+
+template <bool LOWP>
+__global__ void __launch_bounds__(256)
+adamw_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+             float* __restrict__ v, int64_t n, AdamArgs a, bf16_t* __restrict__ lowp) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i < n; i += stride) {
+    if (i + 4 <= n) {
+      float4 pp = *reinterpret_cast<float4*>(p + i);
+      float4 gg = *reinterpret_cast<const float4*>(g + i);
+      float4 mm = *reinterpret_cast<float4*>(m + i);
+      float4 vv = *reinterpret_cast<float4*>(v + i);
+      float* P = &pp.x; const float* G = &gg.x; float* Mv = &mm.x; float* V = &vv.x;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float pe = P[e] * a.decay;
+        float me = lerp_torch(Mv[e], G[e], 1.0f - a.beta1);
+        float ve = V[e] * a.beta2 + (1.0f - a.beta2) * G[e] * G[e];
+        float den = sqrtf(ve) * a.inv_bc2_sqrt + a.eps;
+        pe = pe - a.step_size * me / den;
+        P[e] = pe; Mv[e] = me; V[e] = ve;
+      }
+      *reinterpret_cast<float4*>(p + i) = pp;
+      *reinterpret_cast<float4*>(m + i) = mm;
+      *reinterpret_cast<float4*>(v + i) = vv;
+      if (LOWP) {
+        uint2 w;
+        w.x = (uint32_t)f32_to_bf16(pp.x) | ((uint32_t)f32_to_bf16(pp.y) << 16);
+        w.y = (uint32_t)f32_to_bf16(pp.z) | ((uint32_t)f32_to_bf16(pp.w) << 16);
+        *reinterpret_cast<uint2*>(lowp + i) = w;
+      }
+    } else {
+      for (int64_t j = i; j < n; ++j) {
+        float pe = p[j] * a.decay;
+        float me = lerp_torch(m[j], g[j], 1.0f - a.beta1);
+        float ve = v[j] * a.beta2 + (1.0f - a.beta2) * g[j] * g[j];
+        float den = sqrtf(ve) * a.inv_bc2_sqrt + a.eps;
+        pe = pe - a.step_size * me / den;
+        p[j] = pe; m[j] = me; v[j] = ve;
+        if (LOWP) lowp[j] = f32_to_bf16(pe);
+      }
+    }
+  }
+}
+
+}  // namespace na
+
+using namespace na;
+
+extern "C" int na_adamw_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                             int64_t n, float lr, float beta1, float beta2, float eps,
+                             float weight_decay, int64_t step, void* lowp, int32_t lowp_dtype,
+                             na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0 && step >= 1, "bad sizes / step");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(param && grad && exp_avg && exp_avg_sq, "null pointer");
+  NA_CHECK_ARG(((uintptr_t)param | (uintptr_t)grad | (uintptr_t)exp_avg | (uintptr_t)exp_avg_sq) % 16 == 0,
+               "buffers must be 16-byte aligned");
+  NA_CHECK_ARG(!lowp || lowp_dtype == NA_BF16, "lowp copy must be bf16");
+  NA_CHECK_ARG(!lowp || (uintptr_t)lowp % 8 == 0, "lowp copy must be 8-byte aligned");
+  AdamArgs a;
+  a.lr = lr; a.beta1 = beta1; a.beta2 = beta2; a.eps = eps;
+  a.decay = (float)(1.0 - (double)lr * (double)weight_decay);
+  const double bc1 = 1.0 - pow((double)beta1, (double)step);
+  const double bc2 = 1.0 - pow((double)beta2, (double)step);
+  a.step_size = (float)((double)lr / bc1);
+  a.inv_bc2_sqrt = (float)(1.0 / sqrt(bc2));
+  int64_t threads = ceil_div(n, 4);
+  int64_t blocks = ceil_div(threads, 256);
+  if (blocks > 8192) blocks = 8192;
+  hipStream_t s = (hipStream_t)stream;
+  if (lowp)
+    hipLaunchKernelGGL(adamw_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, param, grad,
+                       exp_avg, exp_avg_sq, n, a, (bf16_t*)lowp);
+  else
+    hipLaunchKernelGGL(adamw_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, param, grad,
+                       exp_avg, exp_avg_sq, n, a, (bf16_t*)nullptr);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}

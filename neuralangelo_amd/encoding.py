@@ -130,6 +130,21 @@ class Encoding(torch.nn.Module):
             self._lowp_cache = (key, p.detach().to(self.dtype))
         return self._lowp_cache[1]
 
+    def lowp_shadow(self):
+        """The bf16 gather copy (allocated on first use), or None for an fp32 table."""
+        if self.dtype == torch.float32:
+            return None
+        p = self.params
+        if self._lowp_cache is None or self._lowp_cache[1].device != p.device:
+            self._lowp_cache = ((p.data_ptr(), p._version), p.detach().to(self.dtype))
+        return self._lowp_cache[1]
+
+    def mark_shadow_fresh(self):
+        """Declare the bf16 copy up to date with the master (written by the optimizer)."""
+        if self._lowp_cache is not None:
+            p = self.params
+            self._lowp_cache = ((p.data_ptr(), p._version), self._lowp_cache[1])
+
     def _forward_raw(self, x, params):
         _lib.require_cuda(x, params)
         if x.dim() != 2 or x.shape[1] != 3:

@@ -1,0 +1,55 @@
+"""Fused AdamW on the GPU (torch.optim.AdamW semantics, one kernel pass per tensor).
+
+Replaces the optimizer the reference builds at
+imaginaire/trainers/utils/get_trainer.py:106-150 (AdamW, lr 1e-3, wd 1e-2,
+projects/neuralangelo/configs/base.yaml:117-121).  Parameters registered with
+``set_lowp_shadow`` (the hash table when it is gathered in bf16) get their
+bf16 copy refreshed in the same pass.
+"""
+import torch
+
+from . import _lib
+
+
+class AdamW(torch.optim.Optimizer):
+
+    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=1e-2):
+        super().__init__(params, dict(lr=lr, betas=betas, eps=eps, weight_decay=weight_decay))
+        self._shadows = {}
+
+    def set_lowp_shadow(self, param, holder):
+        """holder.lowp_shadow() -> (bf16 tensor) / holder.mark_shadow_fresh(): kept in sync."""
+        self._shadows[id(param)] = holder
+
+    @torch.no_grad()
+    def step(self, closure=None):
+        loss = closure() if closure is not None else None
+        for group in self.param_groups:
+            b1, b2 = group["betas"]
+            for p in group["params"]:
+                if p.grad is None:
+                    continue
+                _lib.require_cuda(p)
+                if p.dtype != torch.float32 or not p.is_contiguous():
+                    raise RuntimeError("fused AdamW expects contiguous float32 parameters")
+                g = p.grad
+                if g.dtype != torch.float32 or not g.is_contiguous():
+                    g = g.float().contiguous()
+                st = self.state[p]
+                if not st:
+                    st["step"] = 0
+                    st["exp_avg"] = torch.zeros_like(p)
+                    st["exp_avg_sq"] = torch.zeros_like(p)
+                st["step"] += 1
+                holder = self._shadows.get(id(p))
+                lowp = holder.lowp_shadow() if holder is not None else None
+                _lib.call("na_adamw_step", _lib.ptr(p), _lib.ptr(g), _lib.ptr(st["exp_avg"]),
+                          _lib.ptr(st["exp_avg_sq"]), p.numel(), float(group["lr"]), float(b1),
+                          float(b2), float(group["eps"]), float(group["weight_decay"]),
+                          int(st["step"]), _lib.ptr(lowp), _lib.NA_BF16, _lib.stream_of(p))
+                # the kernel wrote through a raw pointer: bump the version counter so
+                # caches keyed on it (the encoder's bf16 copy) see the update
+                torch.autograd.graph.increment_version(p)
+                if holder is not None:
+                    holder.mark_shadow_fresh()
+        return loss

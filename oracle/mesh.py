@@ -1,0 +1,136 @@
+"""Oracle: marching cubes + block-wise mesh extraction (CPU).  TEST INFRASTRUCTURE.
+
+marching_cubes  -> the C restatement in oracle/mc_oracle.c (PyMCubes algorithm,
+                   projects/neuralangelo/utils/mesh.py:121)
+extract_mesh    -> projects/neuralangelo/utils/mesh.py:24-49 (LatticeGrid blocks
+                   of block_res+1 points, :68-100), marching_cubes per block
+                   (:119-133), sphere filter (:136-148), concatenation without
+                   vertex welding (:44-46)
+
+Pinned by (tests/test_oracle_mesh.py): every one of the 256 single-cube cases
+against the case table's crossed-edge sets and triangle counts; watertightness
+and orientability on random volumes (each interior edge shared by exactly two
+faces with opposite orientation); a sphere volume gives Euler characteristic
+2 and vertices within the interpolation error of the sphere.  Exact PyMCubes
+vertex order is unpinned (PyMCubes is absent and unpinned).
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from . import build as _build
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_build.LIB):
+            _build.build()
+        lib = ctypes.CDLL(_build.LIB)
+        lib.mc_oracle.restype = ctypes.c_int
+        lib.mc_oracle.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                  ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        _lib = lib
+    return _lib
+
+
+def marching_cubes(volume, isovalue=0.0):
+    """volume [X,Y,Z] -> (vertices float64 [nv,3] index coords, faces int64 [nf,3])."""
+    vol = np.ascontiguousarray(np.asarray(volume, dtype=np.float32))
+    nx, ny, nz = vol.shape
+    lib = _load()
+    nv, nf = ctypes.c_int64(), ctypes.c_int64()
+    st = lib.mc_oracle(vol.ctypes.data, nx, ny, nz, float(isovalue), None, None,
+                       ctypes.byref(nv), ctypes.byref(nf))
+    if st:
+        raise RuntimeError("mc_oracle failed")
+    V = np.zeros((nv.value, 3), np.float64)
+    F = np.zeros((nf.value, 3), np.int64)
+    st = lib.mc_oracle(vol.ctypes.data, nx, ny, nz, float(isovalue), V.ctypes.data, F.ctypes.data,
+                       ctypes.byref(nv), ctypes.byref(nf))
+    if st:
+        raise RuntimeError("mc_oracle failed")
+    return V, F
+
+
+def lattice_marching_cubes(vol, block_res, isovalue=0.0, axes=None, intv=1.0):
+    """The reference's per-block loop (mesh.py:24-49, :82-96 slicing, :121-123) on a
+    stored lattice volume, without the sphere filter: blocks of block_res+1
+    points, x-major; each block's V (index coords, or V*intv + axes[block
+    origin] when axes are given) and F + running vertex offset, concatenated."""
+    vol = np.asarray(vol, np.float32)
+    n = vol.shape
+    nb = [int(np.ceil(d / block_res)) for d in n]
+    Vs, Fs, off = [], [], 0
+    for bx in range(nb[0]):
+        for by in range(nb[1]):
+            for bz in range(nb[2]):
+                xi, yi, zi = bx * block_res, by * block_res, bz * block_res
+                blk = vol[xi:xi + block_res + 1, yi:yi + block_res + 1, zi:zi + block_res + 1]
+                V, F = marching_cubes(blk, isovalue)
+                if V.shape[0] == 0:
+                    continue
+                if axes is not None:
+                    V = V * intv + np.array([axes[0][xi], axes[1][yi], axes[2][zi]], np.float32)
+                Vs.append(V)
+                Fs.append(F + off)
+                off += V.shape[0]
+    if not Vs:
+        return np.zeros((0, 3)), np.zeros((0, 3), np.int64)
+    return np.concatenate(Vs), np.concatenate(Fs)
+
+
+def lattice_blocks(bounds, intv, block_res):
+    """LatticeGrid (mesh.py:68-100): yields (block index, xyz [bx,by,bz,3] float32)."""
+    (x0, x1), (y0, y1), (z0, z1) = bounds
+    xg = torch.arange(x0, x1, intv)
+    yg = torch.arange(y0, y1, intv)
+    zg = torch.arange(z0, z1, intv)
+    nbx = int(np.ceil(len(xg) / block_res))
+    nby = int(np.ceil(len(yg) / block_res))
+    nbz = int(np.ceil(len(zg) / block_res))
+    for idx in range(nbx * nby * nbz):
+        bx = idx // (nby * nbz)
+        by = (idx // nbz) % nby
+        bz = idx % nbz
+        xi, yi, zi = bx * block_res, by * block_res, bz * block_res
+        x, y, z = torch.meshgrid(xg[xi:xi + block_res + 1], yg[yi:yi + block_res + 1],
+                                 zg[zi:zi + block_res + 1], indexing="ij")
+        yield idx, torch.stack([x, y, z], dim=-1)
+
+
+def filter_points_outside_bounding_sphere(V, F):
+    """mesh.py:136-148 on plain arrays."""
+    mask = np.linalg.norm(V, axis=-1) < 1.0
+    if not np.any(mask):
+        return np.zeros((0, 3)), np.zeros((0, 3), np.int64)
+    indices = np.ones(len(V), dtype=np.int64) * -1
+    indices[mask] = np.arange(mask.sum())
+    fmask = mask[F[:, 0]] & mask[F[:, 1]] & mask[F[:, 2]]
+    return V[mask], indices[F[fmask]]
+
+
+def extract_mesh(sdf_func, bounds, intv, block_res=64):
+    """mesh.py:24-49: per block, MC on sdf_func(xyz) (already negated by the caller,
+    extract_mesh.py:85), V*intv + xyz[0,0,0], sphere filter, concatenation."""
+    Vs, Fs, off = [], [], 0
+    for _, xyz in lattice_blocks(bounds, intv, block_res):
+        sdf = sdf_func(xyz)[..., 0]
+        V, F = marching_cubes(sdf.numpy(), 0.0)
+        if V.shape[0] == 0:
+            continue
+        V = V * intv + xyz[0, 0, 0].numpy()
+        V, F = filter_points_outside_bounding_sphere(V, F)
+        if V.shape[0] == 0:
+            continue
+        Vs.append(V)
+        Fs.append(F + off)
+        off += V.shape[0]
+    if not Vs:
+        return np.zeros((0, 3)), np.zeros((0, 3), np.int64)
+    return np.concatenate(Vs), np.concatenate(Fs)

@@ -104,19 +104,20 @@ def test_narrow_forward_backward(cuda):
     W = _rand((N, K), 14, 0.1)
     b = _rand((N,), 15, 0.1)
     out = torch.empty(M, N, device=cuda)
-    _lib.call("na_linear_narrow_fwd", _lib.ptr(X.to(cuda)), K, 0, M, K, _lib.ptr(W.to(cuda)),
-              _lib.ptr(b.to(cuda)), N, NA_ACT_SIGMOID, _lib.ptr(out), _lib.stream_of(out))
+    Xd, Wd, bd = X.to(cuda), W.to(cuda), b.to(cuda)
+    _lib.call("na_linear_narrow_fwd", _lib.ptr(Xd), K, 0, M, K, _lib.ptr(Wd), _lib.ptr(bd), N,
+              NA_ACT_SIGMOID, _lib.ptr(out), _lib.stream_of(out))
     ref = torch.sigmoid(X.double() @ W.double().t() + b.double())
     assert (out.cpu().double() - ref).abs().max() < 1e-6
     dpre = _rand((M, N), 16)
-    Xd = X.to(cuda)
+    dpre_d = dpre.to(cuda)
     dW = torch.zeros(N, K, device=cuda)
     db = torch.zeros(N, device=cuda)
     dX = _rand((M, K), 17).to(cuda)
     dX0 = dX.clone()
     cs = torch.zeros(K, device=cuda)
-    _lib.call("na_linear_narrow_bwd", _lib.ptr(Xd), K, 0, M, K, _lib.ptr(W.to(cuda)), N,
-              _lib.ptr(dpre.to(cuda)), _lib.ptr(dW), _lib.ptr(db), _lib.ptr(dX), K, 0, 1,
+    _lib.call("na_linear_narrow_bwd", _lib.ptr(Xd), K, 0, M, K, _lib.ptr(Wd), N,
+              _lib.ptr(dpre_d), _lib.ptr(dW), _lib.ptr(db), _lib.ptr(dX), K, 0, 1,
               _lib.ptr(Xd), K, 0, NA_ACT_RELU, BETA, _lib.ptr(cs), _lib.stream_of(dW))
     refW = dpre.double().t() @ X.double()
     refdX = (dX0.cpu().double() + dpre.double() @ W.double()) * (X.double() > 0)

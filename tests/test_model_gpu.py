@@ -79,8 +79,10 @@ def test_train_step_matches_oracle(setup, cuda):
     l_ref, parts = orr.train_loss(o, target.double())
     assert abs(loss_ren.item() - parts["render"].item()) < 1e-5
     assert abs(eik.item() - parts["eikonal"].item()) < 1e-3 * abs(parts["eikonal"].item())
-    assert (out["rgb"].detach().cpu().double() - o["rgb"].detach()).abs().max() < 1e-5
-    assert (out["alphas"].detach().cpu().double() - o["alphas"].detach()).abs().max() < 1e-4
+    # FP32 vs FP64: the 4-tap gradient divides float32 SDF rounding (~6e-8) by
+    # eps (~2.8e-4); it enters alpha through the cosine term, then the colour
+    assert (out["rgb"].detach().cpu().double() - o["rgb"].detach()).abs().max() < 2e-4
+    assert (out["alphas"].detach().cpu().double() - o["alphas"].detach()).abs().max() < 1e-3
     names = ["W0v", "b0", "W1v", "ws", "R0v", "R2v", "R4v"]
     refs = torch.autograd.grad(l_ref, [table, sv] + [P[k] for k in names], allow_unused=True)
     sdf_mod, rgb_mod = m.neural_sdf, m.neural_rgb
