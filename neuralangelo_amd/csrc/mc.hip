@@ -29,20 +29,42 @@ namespace na {
 
 constexpr int kMcBlock = 256;
 constexpr int kMcItems = 4;
+constexpr int kMcWaves = kMcBlock / 64;
 constexpr int kMcTile = kMcBlock * kMcItems;  // cells per tile
 constexpr int kScanGroup = kMcBlock * kMcItems;  // tiles per scan group
 
-__constant__ int8_t dMcTri[256][16] = NA_MC_TRI_TABLE_INIT;
-__constant__ uint16_t dMcEdge[256] = NA_MC_EDGE_TABLE_INIT;
+// Bourke corner offsets and edge endpoints; kOrder = per-cell vertex order.
+constexpr int kCx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+constexpr int kCy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+constexpr int kCz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+constexpr int kEa[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+constexpr int kEb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
+constexpr int kOrder[12] = {6, 5, 10, 0, 1, 2, 3, 4, 7, 8, 9, 11};
 
-__device__ const int kCx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
-__device__ const int kCy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
-__device__ const int kCz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-__device__ const int kEa[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
-__device__ const int kEb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
-__device__ const int kOrder[12] = {6, 5, 10, 0, 1, 2, 3, 4, 7, 8, 9, 11};
-// position of edge e in kOrder
-__device__ const int kRankOf[12] = {3, 4, 5, 6, 7, 1, 0, 8, 9, 10, 2, 11};
+// Edges that precede edge e in kOrder, as a bit mask.
+constexpr int before_mask(int e) {
+  int m = 0;
+  for (int q = 0; q < 12 && kOrder[q] != e; ++q) m |= 1 << kOrder[q];
+  return m;
+}
+
+// Case word: crossed-edge mask (bits 0..11) | triangle count << 12.
+struct McCaseTable {
+  uint16_t w[256];
+};
+constexpr McCaseTable make_case_table() {
+  McCaseTable t{};
+  constexpr uint16_t edge[256] = NA_MC_EDGE_TABLE_INIT;
+  constexpr int8_t tri[256][16] = NA_MC_TRI_TABLE_INIT;
+  for (int c = 0; c < 256; ++c) {
+    int n = 0;
+    while (n < 5 && tri[c][3 * n] >= 0) ++n;
+    t.w[c] = (uint16_t)(edge[c] | (n << 12));
+  }
+  return t;
+}
+__constant__ McCaseTable dMcCase = make_case_table();
+__constant__ int8_t dMcTri[256][16] = NA_MC_TRI_TABLE_INIT;
 
 struct McLattice {
   const float* v;
@@ -60,15 +82,17 @@ struct McBlock {
   int64_t ncell;
 };
 
-__device__ inline McBlock block_of(const McLattice& g, int64_t b) {
+__device__ inline McBlock block_of(const McLattice& g, uint32_t b) {
+  // block counts per axis are < 2^31 (the host checks the tile count)
   McBlock k;
-  const int64_t bz = b % g.nb[2];
-  const int64_t t = b / g.nb[2];
-  const int64_t by = t % g.nb[1];
-  const int64_t bx = t / g.nb[1];
-  k.ox = bx * g.br[0];
-  k.oy = by * g.br[1];
-  k.oz = bz * g.br[2];
+  const uint32_t nbz = (uint32_t)g.nb[2], nby = (uint32_t)g.nb[1];
+  const uint32_t bz = b % nbz;
+  const uint32_t t = b / nbz;
+  const uint32_t by = t % nby;
+  const uint32_t bx = t / nby;
+  k.ox = (int64_t)bx * g.br[0];
+  k.oy = (int64_t)by * g.br[1];
+  k.oz = (int64_t)bz * g.br[2];
   k.cx = min(g.br[0], g.nx - 1 - k.ox);
   k.cy = min(g.br[1], g.ny - 1 - k.oy);
   k.cz = min(g.br[2], g.nz - 1 - k.oz);
@@ -76,42 +100,54 @@ __device__ inline McBlock block_of(const McLattice& g, int64_t b) {
   return k;
 }
 
-__device__ inline float vox(const McLattice& g, int64_t x, int64_t y, int64_t z) {
-  return g.v[(x * g.ny + y) * g.nz + z];
-}
-
-__device__ inline int cube_of(const McLattice& g, int64_t x, int64_t y, int64_t z) {
-  int c = 0;
-#pragma unroll
-  for (int m = 0; m < 8; ++m)
-    if ((double)vox(g, x + kCx[m], y + kCy[m], z + kCz[m]) <= g.iso) c |= 1 << m;
-  return c;
-}
+// Block-local cell coordinates of this thread's cells, base + it*256 + thread,
+// advanced in mixed radix (cz, cy) without per-cell division.  I is the
+// index type of block-local cell numbers (uint32_t when they fit).
+template <typename I>
+struct CellWalk {
+  I i, j, k, lc;
+  I cy, cz, di, dj, dk;
+  __device__ CellWalk(const McBlock& b, int64_t base) {
+    cy = (I)b.cy;
+    cz = (I)b.cz;
+    lc = (I)base + (I)threadIdx.x;
+    k = lc % cz;
+    const I t = lc / cz;
+    j = t % cy;
+    i = t / cy;
+    dk = (I)kMcBlock % cz;
+    const I s = (I)kMcBlock / cz;
+    dj = s % cy;
+    di = s / cy;
+  }
+  __device__ void next() {
+    lc += (I)kMcBlock;
+    k += dk;
+    I cj = 0;
+    if (k >= cz) {
+      k -= cz;
+      cj = 1;
+    }
+    j += dj + cj;
+    I ci = 0;
+    if (j >= cy) {
+      j -= cy;
+      ci = 1;
+    }
+    i += di + ci;
+  }
+};
 
 // bit e set if edge e is owned by block-local cell (i,j,k)
-__device__ inline int owned_mask(int64_t i, int64_t j, int64_t k) {
+__device__ inline int owned_mask(bool i0, bool j0, bool k0) {
   int m = (1 << 6) | (1 << 5) | (1 << 10);
-  if (j == 0 && k == 0) m |= 1 << 0;
-  if (k == 0) m |= (1 << 1) | (1 << 2);
-  if (i == 0 && k == 0) m |= 1 << 3;
-  if (j == 0) m |= (1 << 4) | (1 << 9);
-  if (i == 0) m |= (1 << 7) | (1 << 11);
-  if (i == 0 && j == 0) m |= 1 << 8;
+  if (j0 && k0) m |= 1 << 0;
+  if (k0) m |= (1 << 1) | (1 << 2);
+  if (i0 && k0) m |= 1 << 3;
+  if (j0) m |= (1 << 4) | (1 << 9);
+  if (i0) m |= (1 << 7) | (1 << 11);
+  if (i0 && j0) m |= 1 << 8;
   return m;
-}
-
-__device__ inline int tri_count(int cube) {
-  int n = 0;
-  while (n < 5 && dMcTri[cube][3 * n] >= 0) ++n;
-  return n;
-}
-
-__device__ inline void unravel(int64_t c, int64_t cy, int64_t cz, int64_t& i, int64_t& j,
-                               int64_t& k) {
-  k = c % cz;
-  const int64_t t = c / cz;
-  j = t % cy;
-  i = t / cy;
 }
 
 // Inclusive scan across the 64 lanes of a wavefront.
@@ -125,23 +161,33 @@ __device__ inline int wave_incl_scan(int x) {
   return x;
 }
 
-// Exclusive scan of one value per thread in thread order across the
-// workgroup; *total gets the workgroup sum.  sh: kMcBlock/64 ints of LDS.
-__device__ inline int block_excl_scan(int x, int* sh, int* total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int inc = wave_incl_scan(x);
-  if (lane == 63) sh[w] = inc;
-  __syncthreads();
-  int before = 0, sum = 0;
+__device__ inline int wave_sum(int x) {
 #pragma unroll
-  for (int q = 0; q < kMcBlock / 64; ++q) {
-    const int s = sh[q];
-    before += q < w ? s : 0;
-    sum += s;
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+// Tile-order exclusive prefix of per-item counts.  Tile order is item-major,
+// then wave, then lane.  inc[it] holds the wave-inclusive scan of item it;
+// wsum is LDS [kMcItems][kMcWaves] of wave totals (written by lane 63).
+// Returns the tile total; pre[it] gets the exclusive prefix.
+__device__ inline int tile_prefix(const int (&inc)[kMcItems], const int (&cnt)[kMcItems],
+                                  int (&wsum)[kMcItems][kMcWaves], int (&pre)[kMcItems]) {
+  const int w = threadIdx.x >> 6;
+  int carry = 0;
+#pragma unroll
+  for (int it = 0; it < kMcItems; ++it) {
+    int before = 0, sum = 0;
+#pragma unroll
+    for (int q = 0; q < kMcWaves; ++q) {
+      const int s = wsum[it][q];
+      before += q < w ? s : 0;
+      sum += s;
+    }
+    pre[it] = carry + before + inc[it] - cnt[it];
+    carry += sum;
   }
-  __syncthreads();
-  *total = sum;
-  return before + inc - x;
+  return carry;
 }
 
 __device__ inline int64_t block_excl_scan64(int64_t x, int64_t* sh, int64_t* total) {
@@ -156,7 +202,7 @@ __device__ inline int64_t block_excl_scan64(int64_t x, int64_t* sh, int64_t* tot
   __syncthreads();
   int64_t before = 0, sum = 0;
 #pragma unroll
-  for (int q = 0; q < kMcBlock / 64; ++q) {
+  for (int q = 0; q < kMcWaves; ++q) {
     const int64_t s = sh[q];
     before += q < w ? s : 0;
     sum += s;
@@ -167,61 +213,86 @@ __device__ inline int64_t block_excl_scan64(int64_t x, int64_t* sh, int64_t* tot
 }
 
 // ---------------------------------------------------------------------------
-// 1. classify: one workgroup per tile.  Cells of a tile are visited as
-//    local index base + it*256 + thread (coalesced volume reads); the vertex
-//    prefix is accumulated in that order.
+// 1. classify: one workgroup per tile.  Cells of a tile are visited as local
+//    index base + it*256 + thread (coalesced volume reads).  Tiles without a
+//    crossed edge write only their zero counts.
 // ---------------------------------------------------------------------------
+template <typename I>
 __global__ void __launch_bounds__(kMcBlock)
 mc_classify_kernel(McLattice g, uint8_t* __restrict__ cidx, uint16_t* __restrict__ vpre,
                    int* __restrict__ tile_v, int* __restrict__ tile_f) {
-  __shared__ int sh[kMcBlock / 64];
-  __shared__ int shf[kMcBlock / 64];
-  const int64_t gt = blockIdx.x;
-  const int64_t b = gt / g.tpb;
-  const int64_t base = (gt % g.tpb) * kMcTile;
+  __shared__ int wv[kMcItems][kMcWaves];
+  __shared__ int wf[kMcWaves];
+  const uint32_t gt = blockIdx.x;
+  const uint32_t tpb = (uint32_t)g.tpb;
+  const uint32_t b = gt / tpb;
+  const int64_t base = (int64_t)(gt - b * tpb) * kMcTile;
   const McBlock k = block_of(g, b);
-  if (base >= k.ncell) {  // trailing tile of a ragged block (uniform branch)
+  if (base >= k.ncell) {
     if (threadIdx.x == 0) {
       tile_v[gt] = 0;
       tile_f[gt] = 0;
     }
     return;
   }
-  int cube[kMcItems], pre[kMcItems];
-  int carry = 0, nf_sum = 0;
+  const int64_t sy = g.nz, sx = g.ny * g.nz;
+  CellWalk<I> cw(k, base);
+  int cube[kMcItems], nv[kMcItems], inc[kMcItems];
+  int nf = 0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int it = 0; it < kMcItems; ++it) {
-    const int64_t lc = base + it * kMcBlock + threadIdx.x;
-    int c = 0, nv = 0;
-    if (lc < k.ncell) {
-      int64_t i, j, l;
-      unravel(lc, k.cy, k.cz, i, j, l);
-      c = cube_of(g, k.ox + i, k.oy + j, k.oz + l);
-      nv = __popc(dMcEdge[c] & owned_mask(i, j, l));
-      nf_sum += tri_count(c);
+    const bool valid = (int64_t)cw.lc < k.ncell;
+    const float* p = valid ? g.v + ((k.ox + (int64_t)cw.i) * g.ny + (k.oy + (int64_t)cw.j)) * g.nz +
+                                 (k.oz + (int64_t)cw.k)
+                           : g.v;
+    float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f;
+    if (valid) {
+      a00 = p[0];
+      a01 = p[sy];
+      a10 = p[sx];
+      a11 = p[sx + sy];
+    }
+    // values at k+1 come from the next lane when it holds cell k+1 of the same row
+    float b00 = __shfl_down(a00, 1), b01 = __shfl_down(a01, 1);
+    float b10 = __shfl_down(a10, 1), b11 = __shfl_down(a11, 1);
+    const bool next_ok = lane < 63 && (cw.k + 1 < cw.cz) && ((int64_t)cw.lc + 1 < k.ncell);
+    if (valid && !next_ok) {
+      b00 = p[1];
+      b01 = p[sy + 1];
+      b10 = p[sx + 1];
+      b11 = p[sx + sy + 1];
+    }
+    int c = 0, n = 0;
+    if (valid) {
+      const double iso = g.iso;
+      c = ((double)a00 <= iso) | (((double)a10 <= iso) << 1) | (((double)a11 <= iso) << 2) |
+          (((double)a01 <= iso) << 3) | (((double)b00 <= iso) << 4) | (((double)b10 <= iso) << 5) |
+          (((double)b11 <= iso) << 6) | (((double)b01 <= iso) << 7);
+      const int word = dMcCase.w[c];
+      n = __popc(word & 0xfff & owned_mask(cw.i == 0, cw.j == 0, cw.k == 0));
+      nf += word >> 12;
     }
     cube[it] = c;
-    int tot;
-    pre[it] = carry + block_excl_scan(nv, sh, &tot);
-    carry += tot;
+    nv[it] = n;
+    inc[it] = wave_incl_scan(n);
+    if (lane == 63) wv[it][w] = inc[it];
+    cw.next();
   }
-  // triangle total of the tile
-  int f = nf_sum;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o);
-  if ((threadIdx.x & 63) == 0) shf[threadIdx.x >> 6] = f;
+  nf = wave_sum(nf);
+  if (lane == 0) wf[w] = nf;
   __syncthreads();
+  int pre[kMcItems];
+  const int tv = tile_prefix(inc, nv, wv, pre);
   int tf = 0;
 #pragma unroll
-  for (int q = 0; q < kMcBlock / 64; ++q) tf += shf[q];
-  const int tv = carry;
+  for (int q = 0; q < kMcWaves; ++q) tf += wf[q];
   if (tv != 0 || tf != 0) {
-    const int64_t slot0 = gt * kMcTile;
+    const int64_t slot0 = (int64_t)gt * kMcTile + threadIdx.x;
 #pragma unroll
     for (int it = 0; it < kMcItems; ++it) {
-      const int64_t s = slot0 + it * kMcBlock + threadIdx.x;
-      cidx[s] = (uint8_t)cube[it];
-      vpre[s] = (uint16_t)pre[it];
+      cidx[slot0 + it * kMcBlock] = (uint8_t)cube[it];
+      vpre[slot0 + it * kMcBlock] = (uint16_t)pre[it];
     }
   }
   if (threadIdx.x == 0) {
@@ -240,7 +311,7 @@ __global__ void __launch_bounds__(kMcBlock)
 mc_scan_tiles_kernel(int64_t ntiles, const int* __restrict__ tile_v, const int* __restrict__ tile_f,
                      int64_t* __restrict__ toff_v, int64_t* __restrict__ toff_f,
                      int64_t* __restrict__ gsum_v, int64_t* __restrict__ gsum_f) {
-  __shared__ int64_t sh[kMcBlock / 64];
+  __shared__ int64_t sh[kMcWaves];
   const int64_t base = (int64_t)blockIdx.x * kScanGroup + threadIdx.x * kMcItems;
   int64_t lv[kMcItems], lf[kMcItems], sv = 0, sf = 0;
 #pragma unroll
@@ -273,7 +344,7 @@ mc_scan_tiles_kernel(int64_t ntiles, const int* __restrict__ tile_v, const int* 
 __global__ void __launch_bounds__(kMcBlock)
 mc_scan_groups_kernel(int64_t ngroups, int64_t* __restrict__ gsum_v, int64_t* __restrict__ gsum_f,
                       int64_t* __restrict__ totals) {
-  __shared__ int64_t sh[kMcBlock / 64];
+  __shared__ int64_t sh[kMcWaves];
   int64_t carry_v = 0, carry_f = 0;
   for (int64_t c0 = 0; c0 < ngroups; c0 += kMcBlock) {
     const int64_t c = c0 + threadIdx.x;
@@ -308,131 +379,170 @@ __device__ inline double interp(double x1, double x2, double f1, double f2, doub
   return r + x1;
 }
 
-// Owner cell and owner-side edge of edge e of cell (i,j,k) (which the cell
-// does not own); block-local coordinates.
-__device__ inline void edge_owner(int e, int64_t i, int64_t j, int64_t k, int64_t& oi,
-                                  int64_t& oj, int64_t& ok, int& oe) {
-  const int a = kEa[e], b = kEb[e];
-  const int64_t px = i + min(kCx[a], kCx[b]), py = j + min(kCy[a], kCy[b]),
-                pz = k + min(kCz[a], kCz[b]);
-  const int axis = e < 8 ? (e & 1) : 2;
-  if (axis == 0) {  // x edge at (px, py, pz)
+struct McWorld {  // optional index -> world transform (V * intv + axis[block origin])
+  const float* ax[3];
+  double intv;
+};
+
+struct McOffsets {
+  const uint8_t* cidx;
+  const uint16_t* vpre;
+  const int64_t* toff_v;
+  const int64_t* gsum_v;
+};
+
+// Global index of the vertex on edge e of cell (i,j,k), which that cell does
+// not own: found in the owner cell (same block) from its cube index, the
+// owner tile's offset and the owner's in-tile prefix.
+template <int E, typename I>
+__device__ inline int64_t shared_vertex(const McOffsets& o, int64_t tile0, I i, I j, I k, I cy,
+                                        I cz) {
+  constexpr int a = kEa[E], b = kEb[E];
+  constexpr int lx = kCx[a] < kCx[b] ? kCx[a] : kCx[b];
+  constexpr int ly = kCy[a] < kCy[b] ? kCy[a] : kCy[b];
+  constexpr int lz = kCz[a] < kCz[b] ? kCz[a] : kCz[b];
+  constexpr int axis = E < 8 ? (E & 1) : 2;
+  const I px = i + lx, py = j + ly, pz = k + lz;  // low end of the lattice edge
+  I oi, oj, ok;
+  int oe;
+  if (axis == 0) {
     oi = px;
     if (py >= 1 && pz >= 1) { oj = py - 1; ok = pz - 1; oe = 6; }
     else if (py == 0 && pz == 0) { oj = 0; ok = 0; oe = 0; }
     else if (pz == 0) { oj = py - 1; ok = 0; oe = 2; }
     else { oj = 0; ok = pz - 1; oe = 4; }
-  } else if (axis == 1) {  // y edge
+  } else if (axis == 1) {
     oj = py;
     if (px >= 1 && pz >= 1) { oi = px - 1; ok = pz - 1; oe = 5; }
     else if (px == 0 && pz == 0) { oi = 0; ok = 0; oe = 3; }
     else if (pz == 0) { oi = px - 1; ok = 0; oe = 1; }
     else { oi = 0; ok = pz - 1; oe = 7; }
-  } else {  // z edge
+  } else {
     ok = pz;
     if (px >= 1 && py >= 1) { oi = px - 1; oj = py - 1; oe = 10; }
     else if (px == 0 && py == 0) { oi = 0; oj = 0; oe = 8; }
     else if (py == 0) { oi = px - 1; oj = 0; oe = 9; }
     else { oi = 0; oj = py - 1; oe = 11; }
   }
+  const int64_t olc = ((int64_t)oi * cy + oj) * cz + ok;
+  const int64_t ot = tile0 + olc / kMcTile;
+  const int64_t oslot = tile0 * kMcTile + olc;
+  const int oown = dMcCase.w[o.cidx[oslot]] & owned_mask(oi == 0, oj == 0, ok == 0);
+  int bm = 0;
+  switch (oe) {  // edges before oe in kOrder
+    case 6: bm = before_mask(6); break;
+    case 5: bm = before_mask(5); break;
+    case 10: bm = before_mask(10); break;
+    case 0: bm = before_mask(0); break;
+    case 1: bm = before_mask(1); break;
+    case 2: bm = before_mask(2); break;
+    case 3: bm = before_mask(3); break;
+    case 4: bm = before_mask(4); break;
+    case 7: bm = before_mask(7); break;
+    case 8: bm = before_mask(8); break;
+    case 9: bm = before_mask(9); break;
+    default: bm = before_mask(11); break;
+  }
+  return o.toff_v[ot] + o.gsum_v[ot / kScanGroup] + o.vpre[oslot] + __popc(oown & bm);
 }
 
-// rank of edge e among the owned crossed edges of a cell, in kOrder order
-__device__ inline int owned_rank(int e, int crossed_owned) {
-  int r = 0;
-  const int pos = kRankOf[e];
-  for (int q = 0; q < pos; ++q)
-    if (crossed_owned & (1 << kOrder[q])) ++r;
-  return r;
+template <int Q, typename I>
+__device__ inline void emit_edge(const McLattice& g, const McWorld& wt, const McOffsets& o,
+                                 const double (&org)[3], const float* p, int64_t sx, int64_t sy,
+                                 int64_t tile0, I i, I j, I k, I cy, I cz, int edges, int own,
+                                 int64_t vb, double* __restrict__ V, int64_t* sidx) {
+  constexpr int e = kOrder[Q];
+  if (!(edges & (1 << e))) return;
+  if (own & (1 << e)) {
+    constexpr int a = kEa[e], b = kEb[e];
+    constexpr int axis = e < 8 ? (e & 1) : 2;
+    const int64_t oa = kCx[a] * sx + kCy[a] * sy + kCz[a];
+    const int64_t ob = kCx[b] * sx + kCy[b] * sy + kCz[b];
+    const double fa = (double)p[oa], fb = (double)p[ob];
+    double q[3] = {(double)(i + kCx[a]), (double)(j + kCy[a]), (double)(k + kCz[a])};
+    const double xb = axis == 0 ? (double)(i + kCx[b]) : axis == 1 ? (double)(j + kCy[b])
+                                                                   : (double)(k + kCz[b]);
+    q[axis] = interp(q[axis], xb, fa, fb, g.iso);
+    if (wt.ax[0]) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double s = q[d] * wt.intv;
+        q[d] = s + org[d];
+      }
+    }
+    const int64_t vi = vb + __popc(own & edges & before_mask(e));
+    double* out = V + 3 * vi;
+    out[0] = q[0];
+    out[1] = q[1];
+    out[2] = q[2];
+    sidx[e * kMcBlock] = vi;
+  } else {
+    sidx[e * kMcBlock] = shared_vertex<e>(o, tile0, i, j, k, cy, cz);
+  }
 }
 
-struct McWorld {  // optional index -> world transform (V * intv + axis[block origin])
-  const float* ax[3];
-  double intv;
-};
-
+template <typename I>
 __global__ void __launch_bounds__(kMcBlock)
-mc_emit_kernel(McLattice g, McWorld wt, const uint8_t* __restrict__ cidx,
-               const uint16_t* __restrict__ vpre, const int* __restrict__ tile_v,
-               const int* __restrict__ tile_f, const int64_t* __restrict__ toff_v,
-               const int64_t* __restrict__ toff_f, const int64_t* __restrict__ gsum_v,
-               const int64_t* __restrict__ gsum_f, double* __restrict__ V,
-               int64_t* __restrict__ F) {
-  __shared__ int sh[kMcBlock / 64];
-  const int64_t gt = blockIdx.x;
-  if (tile_v[gt] == 0 && tile_f[gt] == 0) return;  // uniform
-  const int64_t b = gt / g.tpb;
-  const int64_t base = (gt % g.tpb) * kMcTile;
+mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, int64_t ntiles,
+               const int* __restrict__ tile_v, const int* __restrict__ tile_f,
+               const int64_t* __restrict__ toff_f, const int64_t* __restrict__ gsum_f,
+               double* __restrict__ V, int64_t* __restrict__ F) {
+  __shared__ int wf[kMcItems][kMcWaves];
+  __shared__ int64_t sidx_all[12 * kMcBlock];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t gt = blockIdx.x;
+  if (tile_v[gt] == 0 && tile_f[gt] == 0) return;
+  const int64_t sy = g.nz, sx = g.ny * g.nz;
+  int64_t* sidx = sidx_all + threadIdx.x;
+  const uint32_t tpb = (uint32_t)g.tpb;
+  const uint32_t b = gt / tpb;
+  const int64_t base = (int64_t)(gt - b * tpb) * kMcTile;
   const McBlock k = block_of(g, b);
-  const int64_t vbase = toff_v[gt] + gsum_v[gt / kScanGroup];
-  int64_t fcarry = toff_f[gt] + gsum_f[gt / kScanGroup];
-  const int64_t tile0 = b * g.tpb;  // first tile of this block
+  const int64_t vbase = o.toff_v[gt] + o.gsum_v[gt / kScanGroup];
+  const int64_t fbase = toff_f[gt] + gsum_f[gt / kScanGroup];
+  const int64_t tile0 = (int64_t)b * tpb;
   double org[3] = {0.0, 0.0, 0.0};
   if (wt.ax[0]) {
     org[0] = (double)wt.ax[0][k.ox];
     org[1] = (double)wt.ax[1][k.oy];
     org[2] = (double)wt.ax[2][k.oz];
   }
-  for (int it = 0; it < kMcItems; ++it) {
-    const int64_t lc = base + it * kMcBlock + threadIdx.x;
-    const int64_t slot = gt * kMcTile + it * kMcBlock + threadIdx.x;
-    int cube = 0;
-    if (lc < k.ncell) cube = cidx[slot];
-    const int nf = tri_count(cube);
-    int ftot;
-    const int64_t fb = fcarry + block_excl_scan(nf, sh, &ftot);
-    fcarry += ftot;
-    const int edges = dMcEdge[cube];
-    if (!edges) continue;
-    int64_t i, j, l;
-    unravel(lc, k.cy, k.cz, i, j, l);
-    const int own = edges & owned_mask(i, j, l);
-    int64_t idx[12];
-    const int64_t vb = vbase + vpre[slot];
-    int rank = 0;
-    for (int q = 0; q < 12; ++q) {
-      const int e = kOrder[q];
-      if (!(edges & (1 << e))) continue;
-      if (own & (1 << e)) {
-        const int a = kEa[e], c = kEb[e];
-        const int64_t p[3] = {i + kCx[a], j + kCy[a], l + kCz[a]};
-        const int64_t pb[3] = {i + kCx[c], j + kCy[c], l + kCz[c]};
-        const int ax = e < 8 ? (e & 1) : 2;
-        const double fa = (double)vox(g, k.ox + p[0], k.oy + p[1], k.oz + p[2]);
-        const double fc = (double)vox(g, k.ox + pb[0], k.oy + pb[1], k.oz + pb[2]);
-        double o[3] = {(double)p[0], (double)p[1], (double)p[2]};
-        o[ax] = interp((double)p[ax], (double)pb[ax], fa, fc, g.iso);
-        if (wt.ax[0]) {
+  const int64_t slot0 = (int64_t)gt * kMcTile + threadIdx.x;
+  int cube[kMcItems], nf[kMcItems], inc[kMcItems];
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            const double s = o[d] * wt.intv;
-            o[d] = s + org[d];
-          }
-        }
-        double* out = V + 3 * (vb + rank);
-        out[0] = o[0];
-        out[1] = o[1];
-        out[2] = o[2];
-        idx[e] = vb + rank;
-        ++rank;
-      } else {
-        int64_t oi, oj, ok;
-        int oe;
-        edge_owner(e, i, j, l, oi, oj, ok, oe);
-        const int64_t olc = (oi * k.cy + oj) * k.cz + ok;
-        const int64_t ot = tile0 + olc / kMcTile;
-        const int64_t oslot = tile0 * kMcTile + olc;
-        const int ocube = cidx[oslot];
-        const int oown = dMcEdge[ocube] & owned_mask(oi, oj, ok);
-        idx[e] = toff_v[ot] + gsum_v[ot / kScanGroup] + vpre[oslot] + owned_rank(oe, oown);
-      }
-    }
-    int64_t fo = fb;
-    for (int q = 0; q < 15 && dMcTri[cube][q] >= 0; q += 3) {
+  for (int it = 0; it < kMcItems; ++it) {
+    const bool in = base + it * kMcBlock + (int64_t)threadIdx.x < k.ncell;
+    cube[it] = in ? o.cidx[slot0 + it * kMcBlock] : 0;
+    nf[it] = dMcCase.w[cube[it]] >> 12;
+    inc[it] = wave_incl_scan(nf[it]);
+    if (lane == 63) wf[it][w] = inc[it];
+  }
+  __syncthreads();
+  int fpre[kMcItems];
+  tile_prefix(inc, nf, wf, fpre);
+  CellWalk<I> cw(k, base);
+#pragma unroll 1
+  for (int it = 0; it < kMcItems; ++it, cw.next()) {
+    const int edges = dMcCase.w[cube[it]] & 0xfff;
+    if (!edges) continue;
+    const I i = cw.i, j = cw.j, kk = cw.k;
+    const int own = edges & owned_mask(i == 0, j == 0, kk == 0);
+    const float* p = g.v + ((k.ox + (int64_t)i) * g.ny + (k.oy + (int64_t)j)) * g.nz +
+                     (k.oz + (int64_t)kk);
+    const int64_t vb = vbase + o.vpre[slot0 + it * kMcBlock];
+#define NA_MC_EDGE(Q) \
+  emit_edge<Q>(g, wt, o, org, p, sx, sy, tile0, i, j, kk, cw.cy, cw.cz, edges, own, vb, V, sidx)
+    NA_MC_EDGE(0); NA_MC_EDGE(1); NA_MC_EDGE(2); NA_MC_EDGE(3);
+    NA_MC_EDGE(4); NA_MC_EDGE(5); NA_MC_EDGE(6); NA_MC_EDGE(7);
+    NA_MC_EDGE(8); NA_MC_EDGE(9); NA_MC_EDGE(10); NA_MC_EDGE(11);
+#undef NA_MC_EDGE
+    int64_t fo = fbase + fpre[it];
+    const int8_t* tri = dMcTri[cube[it]];
+    for (int q3 = 0; q3 < 15 && tri[q3] >= 0; q3 += 3) {
       int64_t* out = F + 3 * fo;
-      out[0] = idx[dMcTri[cube][q]];
-      out[1] = idx[dMcTri[cube][q + 1]];
-      out[2] = idx[dMcTri[cube][q + 2]];
+      out[0] = sidx[tri[q3] * kMcBlock];
+      out[1] = sidx[tri[q3 + 1] * kMcBlock];
+      out[2] = sidx[tri[q3 + 2] * kMcBlock];
       ++fo;
     }
   }
@@ -449,6 +559,7 @@ struct McPlan {
   McLattice g;
   int64_t ntiles, ngroups;
   bool empty;
+  bool small;  // block-local cell numbers (plus one tile of slack) fit in uint32
 };
 
 static McPlan mc_plan(const float* v, int64_t nx, int64_t ny, int64_t nz, int64_t block_res,
@@ -469,6 +580,7 @@ static McPlan mc_plan(const float* v, int64_t nx, int64_t ny, int64_t nz, int64_
   p.g.tpb = ceil_div(p.g.br[0] * p.g.br[1] * p.g.br[2], (int64_t)kMcTile);
   p.ntiles = p.g.nb[0] * p.g.nb[1] * p.g.nb[2] * p.g.tpb;
   p.ngroups = ceil_div(p.ntiles, (int64_t)kScanGroup);
+  p.small = p.g.tpb * kMcTile + kMcBlock < (int64_t)UINT32_MAX;
   return p;
 }
 
@@ -536,8 +648,12 @@ extern "C" int na_mc_count(const float* volume, int64_t nx, int64_t ny, int64_t 
   NA_CHECK_ARG((int64_t)mc_ws_layout(p, nullptr, nullptr) <= ws_bytes, "workspace too small");
   McWorkspace w;
   mc_ws_layout(p, workspace, &w);
-  hipLaunchKernelGGL(mc_classify_kernel, dim3((unsigned)p.ntiles), dim3(kMcBlock), 0, s, p.g,
-                     w.cidx, w.vpre, w.tile_v, w.tile_f);
+  if (p.small)
+    hipLaunchKernelGGL(mc_classify_kernel<uint32_t>, dim3((unsigned)p.ntiles), dim3(kMcBlock), 0,
+                       s, p.g, w.cidx, w.vpre, w.tile_v, w.tile_f);
+  else
+    hipLaunchKernelGGL(mc_classify_kernel<uint64_t>, dim3((unsigned)p.ntiles), dim3(kMcBlock), 0,
+                       s, p.g, w.cidx, w.vpre, w.tile_v, w.tile_f);
   NA_CHECK_LAUNCH();
   hipLaunchKernelGGL(mc_scan_tiles_kernel, dim3((unsigned)p.ngroups), dim3(kMcBlock), 0, s,
                      p.ntiles, w.tile_v, w.tile_f, w.toff_v, w.toff_f, w.gsum_v, w.gsum_f);
@@ -563,9 +679,14 @@ extern "C" int na_mc_emit(const float* volume, int64_t nx, int64_t ny, int64_t n
   mc_ws_layout(p, workspace, &w);
   McWorld wt{{axis_x, axis_y, axis_z}, intv};
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(mc_emit_kernel, dim3((unsigned)p.ntiles), dim3(kMcBlock), 0, s, p.g, wt,
-                     w.cidx, w.vpre, w.tile_v, w.tile_f, w.toff_v, w.toff_f, w.gsum_v, w.gsum_f,
-                     vertices, faces);
+  McOffsets o{w.cidx, w.vpre, w.toff_v, w.gsum_v};
+  const dim3 grid((unsigned)p.ntiles);
+  if (p.small)
+    hipLaunchKernelGGL(mc_emit_kernel<uint32_t>, grid, dim3(kMcBlock), 0, s, p.g, wt, o, p.ntiles,
+                       w.tile_v, w.tile_f, w.toff_f, w.gsum_f, vertices, faces);
+  else
+    hipLaunchKernelGGL(mc_emit_kernel<uint64_t>, grid, dim3(kMcBlock), 0, s, p.g, wt, o, p.ntiles,
+                       w.tile_v, w.tile_f, w.toff_f, w.gsum_f, vertices, faces);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
