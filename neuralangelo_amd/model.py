@@ -87,6 +87,46 @@ class Model(torch.nn.Module):
             dists = merged
         return dists
 
+    # -- reference-signature operators (model.py:207-298) ------------------------
+    @torch.no_grad()
+    def get_dist_bounds(self, center, ray_unit):
+        """model.py:207-213: [..., 3] -> near, far, outside [..., 1] (unit sphere)."""
+        lead = center.shape[:-1]
+        near, far, outside, _ = nerf_util.bounds_and_coarse(center.reshape(-1, 3),
+                                                            ray_unit.reshape(-1, 3), 1)
+        return near.view(*lead, 1), far.view(*lead, 1), outside.view(*lead, 1)
+
+    def sample_dists_from_pdf(self, bin, weights):
+        """model.py:229-230 delegate: fine samples per the configured count."""
+        return nerf_util.sample_dists_from_pdf(bin, weights, _get(_get(self.cfg_render,
+                                                                       "num_samples"), "fine"))
+
+    @torch.no_grad()
+    def sample_dists_hierarchical(self, dists, sdfs, inv_s, robust=True, eps=1e-5):
+        """model.py:250-267: dists [...,N,1], sdfs [...,N,1] -> dists_fine [...,Nf,1]."""
+        if not robust or eps != 1e-5:
+            raise NotImplementedError("only the robust estimator with eps=1e-5 is implemented")
+        lead, N = dists.shape[:-2], dists.shape[-2]
+        n_fine = _get(_get(self.cfg_render, "num_samples"), "fine")
+        _, fine, _, _ = nerf_util.hierarchical_round(dists.reshape(-1, N).float(),
+                                                     sdfs.reshape(-1, N).float(), inv_s, n_fine)
+        return fine.view(*lead, n_fine, 1)
+
+    def compute_neus_alphas(self, ray_unit, sdfs, gradients, dists, dist_far=None, progress=1.,
+                            eps=1e-5):
+        """model.py:275-292: ray_unit [...,3], sdfs [...,N,1], gradients [...,N,3],
+        dists [...,N,1], dist_far [...,1,1] or None (1e10) -> alphas [...,N]."""
+        if eps != 1e-5:
+            raise NotImplementedError("the NeuS alpha kernel uses eps = 1e-5")
+        lead, N = sdfs.shape[:-2], sdfs.shape[-2]
+        R = int(torch.tensor(lead).prod().item()) if len(lead) else 1
+        far = dist_far.reshape(R) if dist_far is not None else None
+        alphas = Fn.NeusAlpha.apply(ray_unit.reshape(R, 3), sdfs.reshape(R, N),
+                                    gradients.reshape(R, N, 3), dists.reshape(R, N), far, None,
+                                    self.s_var, float(progress), float(self.anneal_end),
+                                    float(self.outside_val))
+        return alphas.view(*lead, N)
+
     # -- rendering ---------------------------------------------------------------
     def render_rays(self, center, ray_unit, sample_idx=None, stratified=False, rands=None,
                     dists=None):

@@ -93,3 +93,29 @@ def test_train_step_matches_oracle(setup, cuda):
     for name, a, b in zip(["table", "s_var"] + names, got, refs):
         cs = _cos(a.cpu(), b)
         assert cs > 0.999, (name, cs)
+
+
+def test_reference_signature_operators(setup, cuda):
+    """Model.get_dist_bounds / compute_neus_alphas / sample_dists_hierarchical with the
+    reference's [B,R,N,...] shapes (model.py:207-298) against the oracle."""
+    m = setup
+    B, R, N = 2, 24, 40
+    g = torch.Generator().manual_seed(9)
+    c, v = _rays(B * R, 9)
+    c, v = c.view(B, R, 3), v.view(B, R, 3)
+    near, far, outside = m.get_dist_bounds(c.to(cuda), v.to(cuda))
+    on, of, oo = orr.get_dist_bounds(c, v)
+    assert torch.allclose(near.cpu(), on, atol=1e-6) and torch.equal(outside.cpu(), oo)
+    dists = torch.sort(torch.rand(B, R, N, 1, generator=g) * 2 + 1, dim=2).values
+    sdfs = torch.randn(B, R, N, 1, generator=g) * 0.3
+    grads = torch.randn(B, R, N, 3, generator=g)
+    dfar = dists[:, :, -1:, :] + 0.1
+    a = m.compute_neus_alphas(v.to(cuda), sdfs.to(cuda), grads.to(cuda), dists.to(cuda),
+                              dfar.to(cuda), progress=0.05)
+    ref = orr.compute_neus_alphas(v, sdfs, grads, dists, dfar, m.s_var.detach().cpu(), 0.05,
+                                  m.anneal_end)
+    assert a.shape == (B, R, N) and (a.cpu() - ref).abs().max() < 2e-6
+    fine = m.sample_dists_hierarchical(dists.to(cuda), sdfs.to(cuda), 64.0)
+    ref = orr.sample_dists_hierarchical(dists, sdfs, 64.0, 16)
+    assert fine.shape == (B, R, 16, 1)
+    assert (fine.cpu() - ref).abs().max() < 1e-5
