@@ -191,6 +191,19 @@ int na_sdf_input_bwd(const na_grid_desc* desc, const float* points, int64_t n, f
                      float vmax, const void* dy, int32_t dy_dtype, int64_t ld,
                      float* grad_table, int32_t active_levels, na_stream_t stream);
 
+/* Encoder backward of one numerical-gradient point set in a single pass: the
+ * centre evaluation (modules.py:84-86, reached from model.py:160) and the four
+ * tetrahedral taps x + eps*k_i, k = (1,-1,-1), (-1,-1,1), (-1,1,-1), (1,1,1)
+ * (modules.py:157-175) -- five tcnn backward calls in the reference.
+ * points: centre points [n,3] (world); dy_centre [n, ld] and dy_taps [4n, ld]
+ * (tap-major) hold the encoder gradient in columns 0..L*F-1.  Contributions of
+ * the five points to a shared corner are summed before one atomic add into
+ * grad_table (accumulated; the caller zeroes it). */
+int na_sdf_taps_bwd(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
+                    float vmax, float eps, const void* dy_centre, const void* dy_taps,
+                    int32_t dy_dtype, int64_t ld, float* grad_table, int32_t active_levels,
+                    na_stream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Per-ray operators.  R rays, N samples per ray, row-major [R, N(, K)].
  * ------------------------------------------------------------------------- */

@@ -121,6 +121,7 @@ struct TileLoader {
     uint4 q;
     T e[VEC];
   };
+  static_assert(sizeof(T) != 2 || NVEC == 2, "bf16 transposed loads pair two k rows");
   Vec reg[NVEC];
 
   __device__ inline void load(const T* __restrict__ X, int64_t ld, int64_t r0, int64_t rmax,
@@ -135,6 +136,17 @@ struct TileLoader {
           reg[j].q = *reinterpret_cast<const uint4*>(X + gr * ld + gk);
         } else {
           reg[j].q = make_uint4(0, 0, 0, 0);
+        }
+      } else if constexpr (sizeof(T) == 2) {
+        // bf16: thread t owns rows 8*(t%16)..+7 at k = 2*(t/16) + j
+        const int r = (tid % (GBM / VEC)) * VEC, k = 2 * (tid / (GBM / VEC)) + j;
+        const int64_t gr = r0 + r, gk = k0 + k;
+        if (gk < kend && gr + VEC <= rmax) {
+          reg[j].q = *reinterpret_cast<const uint4*>(X + gk * ld + gr);
+        } else {
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            reg[j].e[e] = (gk < kend && gr + e < rmax) ? X[gk * ld + gr + e] : T(0);
         }
       } else {
         const int k = v / (GBM / VEC), r = (v % (GBM / VEC)) * VEC;
@@ -161,6 +173,21 @@ struct TileLoader {
         } else {
 #pragma unroll
           for (int e = 0; e < VEC; ++e) s[r * LDK + k + e] = reg[j].e[e];
+        }
+      } else if constexpr (sizeof(T) == 2) {
+        // transpose the thread's 2 (k) x 8 (r) block in registers: one 32-bit
+        // LDS store per row holds the (k, k+1) pair
+        if (j == 0) {
+          const int r = (tid % (GBM / VEC)) * VEC, k = 2 * (tid / (GBM / VEC));
+          const uint32_t a[4] = {reg[0].q.x, reg[0].q.y, reg[0].q.z, reg[0].q.w};
+          const uint32_t b[4] = {reg[1].q.x, reg[1].q.y, reg[1].q.z, reg[1].q.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t lo = (a[e] & 0xffffu) | (b[e] << 16);
+            const uint32_t hi = (a[e] >> 16) | (b[e] & 0xffff0000u);
+            *reinterpret_cast<uint32_t*>(s + (r + 2 * e) * LDK + k) = lo;
+            *reinterpret_cast<uint32_t*>(s + (r + 2 * e + 1) * LDK + k) = hi;
+          }
         }
       } else {
         const int k = v / (GBM / VEC), r = (v % (GBM / VEC)) * VEC;
@@ -214,6 +241,44 @@ template <> struct MfmaStep<float> {
     }
   }
 };
+
+// Epilogue shared by the GEMM kernels: per half (i), each wave stages a 32 x 64
+// accumulator block through LDS (`smem`, at least 4*32*65 floats, free to
+// reuse) so that every accumulator register is addressed statically and the
+// global stores run along rows (64 consecutive columns per wave instruction).
+__device__ inline void gemm_epilogue(const GemmParams& p, unsigned char* smem,
+                                     f32x16_t (&acc)[2][2], int64_t m0, int64_t n0, int wm,
+                                     int wn, int wave, int lane) {
+  constexpr int CP = 65;
+  float* Cw = reinterpret_cast<float*>(smem) + wave * 32 * CP;
+  const EpiParams& e = p.e;
+  const int64_t n = n0 + wn * 64 + lane;
+  float csum = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int col = j * 32 + (lane & 31);
+        Cw[row * CP + col] = acc[i][j][r];
+      }
+    __syncthreads();
+    if (n < p.N) {
+      const int64_t mb = m0 + wm * 64 + i * 32;
+      for (int rr = 0; rr < 32; ++rr) {
+        const int64_t m = mb + rr;
+        if (m >= p.M) break;
+        float v = epilogue_value(e, m, n, Cw[rr * CP + lane]);
+        csum += v;
+        epilogue_store(e, m, n, v);
+      }
+    }
+    __syncthreads();
+  }
+  if (e.colsum && n < p.N) atomic_add_f32(e.colsum + n, csum);
+}
 
 // Two LDS stages: the next K-tile is fetched into registers while the MFMAs
 // consume the current stage, then written to the other stage; one barrier per
@@ -278,36 +343,7 @@ __global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
     buf ^= 1;
   }
 
-  // ---- epilogue: per half (i), stage a 32 x 64 accumulator block per wave ----
-  constexpr int CP = GemmSmem<T>::CPITCH;
-  float* Cw = reinterpret_cast<float*>(smem) + wave * 32 * CP;
-  const EpiParams& e = p.e;
-  const int64_t n = n0 + wn * 64 + lane;
-  float csum = 0.0f;
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int col = j * 32 + (lane & 31);
-        Cw[row * CP + col] = acc[i][j][r];
-      }
-    __syncthreads();
-    if (n < p.N) {
-      const int64_t mb = m0 + wm * 64 + i * 32;
-      for (int rr = 0; rr < 32; ++rr) {
-        const int64_t m = mb + rr;
-        if (m >= p.M) break;
-        float v = epilogue_value(e, m, n, Cw[rr * CP + lane]);
-        csum += v;
-        epilogue_store(e, m, n, v);
-      }
-    }
-    __syncthreads();
-  }
-  if (e.colsum && n < p.N) atomic_add_f32(e.colsum + n, csum);
+  gemm_epilogue(p, smem, acc, m0, n0, wm, wn, wave, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -601,10 +637,121 @@ weight_norm_bwd_kernel(const float* __restrict__ v, const float* __restrict__ g,
 // ---------------------------------------------------------------------------
 // host dispatch
 // ---------------------------------------------------------------------------
+// bf16 GEMM, both operands k-contiguous (A[m*lda + k], B[n*ldb + k]): a
+// four-stage pipeline of direct global -> LDS loads (global_load_lds_dwordx4,
+// no register staging).  Stage t+3 is in flight while the MFMAs consume
+// stage t, which hides HBM latency behind three K-tiles of MFMA work.
+//
+// Stage layout: [128 rows][32 k] bf16 per operand, 64-byte rows, with the four
+// 16-byte chunks of row r stored at chunk slot c ^ ((r >> 2) & 3).  One wave
+// load instruction fills 1 KB (16 rows); lane l lands at slot l & 3 of row
+// l >> 2 and fetches the global chunk that belongs there.  The MFMA operand
+// reads (16 B per lane, 32 rows per half-wave) then touch 16 distinct bank
+// groups per 16 lanes.  Chunks past M/N or K read a zero block instead.
+// ---------------------------------------------------------------------------
+constexpr int kNNStages = 4;
+constexpr int kNNBK = 32;
+constexpr int kNNStageElems = GBM * kNNBK;  // per operand
+
+__device__ uint4 gemm_zero_chunks[64];  // zero source for out-of-range chunks
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ inline void nn_issue_stage(const bf16_t* __restrict__ X, int64_t ld, int64_t r0,
+                                      int64_t rmax, int64_t k0, int64_t kend, bf16_t* stage,
+                                      int wave, int lane) {
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int q = wave * 2 + u;            // 16-row block of the stage
+    const int row = q * 16 + (lane >> 2);
+    const int chunk = (lane & 3) ^ ((row >> 2) & 3);
+    const int64_t gr = r0 + row, gk = k0 + chunk * 8;
+    const void* src = (gr < rmax && gk < kend) ? (const void*)(X + gr * ld + gk)
+                                               : (const void*)(gemm_zero_chunks + lane);
+    __builtin_amdgcn_global_load_lds(src, (lds_void_t*)(stage + q * 16 * kNNBK), 16, 0, 0);
+  }
+}
+
+// Wait until at most `pending` of this wave's stage loads are outstanding
+// (4 load instructions per stage).  vmcnt is an immediate operand.
+__device__ inline void nn_wait_stages(int pending) {
+  // gfx9 s_waitcnt immediate: vmcnt[3:0], expcnt[6:4] = 7, lgkmcnt[11:8] = 15
+  if (pending >= 2) __builtin_amdgcn_s_waitcnt(0xF70 | 8);
+  else if (pending == 1) __builtin_amdgcn_s_waitcnt(0xF70 | 4);
+  else __builtin_amdgcn_s_waitcnt(0xF70);
+}
+
+__global__ void __launch_bounds__(GNT)
+gemm_bf16_nn_kernel(GemmParams p) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[kNNStages * 2 * kNNStageElems * 2];
+  bf16_t* st0 = reinterpret_cast<bf16_t*>(smem);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t m0 = (int64_t)blockIdx.x * GBM;
+  const int64_t n0 = (int64_t)blockIdx.y * GBN;
+  const int64_t kbeg = (int64_t)blockIdx.z * p.k_chunk;
+  const int64_t kend = kbeg + p.k_chunk < p.K ? kbeg + p.k_chunk : p.K;
+  const bf16_t* A = (const bf16_t*)p.A;
+  const bf16_t* B = (const bf16_t*)p.B;
+  const int nk = kbeg < kend ? (int)((kend - kbeg + kNNBK - 1) / kNNBK) : 0;
+
+  f32x16_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  auto issue = [&](int t) {
+    bf16_t* sa = st0 + (t % kNNStages) * 2 * kNNStageElems;
+    const int64_t k0 = kbeg + (int64_t)t * kNNBK;
+    nn_issue_stage(A, p.lda, m0, p.M, k0, kend, sa, wave, lane);
+    nn_issue_stage(B, p.ldb, n0, p.N, k0, kend, sa + kNNStageElems, wave, lane);
+  };
+#pragma unroll
+  for (int t = 0; t < kNNStages - 1; ++t)
+    if (t < nk) issue(t);
+
+  for (int t = 0; t < nk; ++t) {
+    const int ahead = nk - 1 - t;  // stages issued after t, capped by the pipeline depth
+    nn_wait_stages(ahead < kNNStages - 2 ? ahead : kNNStages - 2);
+    __builtin_amdgcn_s_barrier();   // stage t complete in every wave; stage t-1 consumed
+    if (t + kNNStages - 1 < nk) issue(t + kNNStages - 1);
+    const bf16_t* sa = st0 + (t % kNNStages) * 2 * kNNStageElems;
+    const bf16_t* sb = sa + kNNStageElems;
+#pragma unroll
+    for (int kk = 0; kk < kNNBK; kk += 16) {
+      bf16x8_t a[2], b[2];
+      const int ck = (kk >> 3) + (lane >> 5);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int ra = wm * 64 + i * 32 + (lane & 31);
+        const int rb = wn * 64 + i * 32 + (lane & 31);
+        a[i] = *reinterpret_cast<const bf16x8_t*>(sa + ra * kNNBK + ((ck ^ ((ra >> 2) & 3)) << 3));
+        b[i] = *reinterpret_cast<const bf16x8_t*>(sb + rb * kNNBK + ((ck ^ ((rb >> 2) & 3)) << 3));
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0xF70);
+  __syncthreads();  // all operand reads done before the epilogue reuses the stages
+  gemm_epilogue(p, smem, acc, m0, n0, wm, wn, wave, lane);
+}
+
+// ---------------------------------------------------------------------------
 template <typename T>
 static int launch_gemm(const na_gemm_args* a, const GemmParams& p, dim3 grid, hipStream_t s) {
-  if (!a->trans_a && !a->trans_b)
-    hipLaunchKernelGGL((gemm_kernel<T, false, false>), grid, dim3(GNT), 0, s, p);
+  if (!a->trans_a && !a->trans_b) {
+    if constexpr (sizeof(T) == 2)
+      hipLaunchKernelGGL(gemm_bf16_nn_kernel, grid, dim3(GNT), 0, s, p);
+    else
+      hipLaunchKernelGGL((gemm_kernel<T, false, false>), grid, dim3(GNT), 0, s, p);
+  }
   else if (!a->trans_a && a->trans_b)
     hipLaunchKernelGGL((gemm_kernel<T, false, true>), grid, dim3(GNT), 0, s, p);
   else if (a->trans_a && a->trans_b)

@@ -245,6 +245,109 @@ hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
   }
 }
 
+// Backward of the five encoder evaluations of one numerical-gradient point
+// (the centre and its four tetrahedral taps, modules.py:84-86 and :157-175)
+// in one pass.  The taps lie less than one cell from the centre at every
+// active level, so the union of the five points' corners fits a 3x3x3 box of
+// lattice points; the five contributions to each corner are summed in
+// registers and each touched corner gets one atomic.  When a level's cells
+// spread wider (an epsilon larger than the finest active cell), the thread
+// falls back to one atomic per point and corner.
+constexpr float kTapDir[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+
+template <int F, typename TD>
+__global__ void __launch_bounds__(kBlock)
+hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
+                         float vrange, float eps, const TD* __restrict__ dyc,
+                         const TD* __restrict__ dyt, int64_t ld, float* __restrict__ grad) {
+  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int l = blockIdx.y;
+  const int64_t i = tid / F;
+  const int f = (int)(tid % F);
+  if (i >= n) return;
+  const int64_t col = (int64_t)l * F + f;
+  float g[5];
+  g[0] = scalar_io<TD>::load(dyc + i * ld + col);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) g[1 + t] = scalar_io<TD>::load(dyt + ((int64_t)t * n + i) * ld + col);
+  if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && g[3] == 0.0f && g[4] == 0.0f) return;
+  const LevelParams p = level_params(d, l);
+  int cell[5][3];
+  float fr[5][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float xc = x[i * 3 + k];
+    uint32_t c;
+    pos_fract(normalize_coord(xc, vmin, vrange), p.scale, c, fr[0][k]);
+    cell[0][k] = (int)c;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float xt = xc + kTapDir[t][k] * eps;  // the tap point, as the forward built it
+      pos_fract(normalize_coord(xt, vmin, vrange), p.scale, c, fr[1 + t][k]);
+      cell[1 + t][k] = (int)c;
+    }
+  }
+  float* gtab = grad + (size_t)p.off * F + f;
+  int lo[3];
+  bool box = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int a = cell[0][k], b = cell[0][k];
+#pragma unroll
+    for (int q = 1; q < 5; ++q) {
+      a = min(a, cell[q][k]);
+      b = max(b, cell[q][k]);
+    }
+    lo[k] = a;
+    box = box && (b - a <= 1);
+  }
+  if (!box) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (g[q] == 0.0f) continue;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t id = corner_index((uint32_t)cell[q][0] + (c & 1),
+                                         (uint32_t)cell[q][1] + ((c >> 1) & 1),
+                                         (uint32_t)cell[q][2] + ((c >> 2) & 1), p);
+        atomic_add_f32(gtab + (size_t)id * F, corner_weight(c, fr[q]) * g[q]);
+      }
+    }
+    return;
+  }
+  // per point and axis: offset of its cell in the box (0 or 1)
+  int off[5][3];
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) off[q][k] = cell[q][k] - lo[k];
+#pragma unroll
+  for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+      for (int ox = 0; ox < 3; ++ox) {
+        float acc = 0.0f;
+        bool touched = false;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const int dx = ox - off[q][0], dy = oy - off[q][1], dz = oz - off[q][2];
+          if (g[q] != 0.0f && (unsigned)dx <= 1u && (unsigned)dy <= 1u && (unsigned)dz <= 1u) {
+            float w = dx ? fr[q][0] : 1.0f - fr[q][0];
+            w *= dy ? fr[q][1] : 1.0f - fr[q][1];
+            w *= dz ? fr[q][2] : 1.0f - fr[q][2];
+            acc += w * g[q];
+            touched = true;
+          }
+        }
+        if (touched) {
+          const uint32_t id = corner_index((uint32_t)(lo[0] + ox), (uint32_t)(lo[1] + oy),
+                                           (uint32_t)(lo[2] + oz), p);
+          atomic_add_f32(gtab + (size_t)id * F, acc);
+        }
+      }
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
@@ -337,6 +440,29 @@ static int run_bwd(const BwdLaunch& a, int ddt) {
   }
 }
 
+template <int F, typename TD>
+static int launch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, float vmin,
+                           float vrange, float eps, const void* dyc, const void* dyt, int64_t ld,
+                           float* grad, int active, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(n * F, kBlock), active);
+  hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD>), grid, dim3(kBlock), 0, s, *d, x, n, vmin,
+                     vrange, eps, (const TD*)dyc, (const TD*)dyt, ld, grad);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+template <int F>
+static int dispatch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, float vmin,
+                             float vrange, float eps, const void* dyc, const void* dyt, int ddt,
+                             int64_t ld, float* grad, int active, hipStream_t s) {
+  if (ddt == NA_F32)
+    return launch_taps_bwd<F, float>(d, x, n, vmin, vrange, eps, dyc, dyt, ld, grad, active, s);
+  if (ddt == NA_BF16)
+    return launch_taps_bwd<F, bf16_t>(d, x, n, vmin, vrange, eps, dyc, dyt, ld, grad, active, s);
+  set_error("hash-grid taps backward: unsupported dy dtype");
+  return NA_ERR_UNSUPPORTED;
+}
+
 }  // namespace na
 
 using namespace na;
@@ -419,4 +545,32 @@ extern "C" int na_sdf_input_bwd(const na_grid_desc* desc, const float* points, i
   BwdLaunch a{desc, points, n, vmin, vmax - vmin, dy, ld, grad_table, active,
               (hipStream_t)stream};
   return run_bwd(a, dy_dtype);
+}
+
+extern "C" int na_sdf_taps_bwd(const na_grid_desc* desc, const float* points, int64_t n,
+                               float vmin, float vmax, float eps, const void* dy_centre,
+                               const void* dy_taps, int32_t dy_dtype, int64_t ld,
+                               float* grad_table, int32_t active_levels, na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(points && dy_centre && dy_taps && grad_table, "null pointer");
+  NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
+  NA_CHECK_ARG(ld >= (int64_t)desc->n_levels * desc->n_features, "ld must be >= L*F");
+  NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
+  const int active = active_levels < desc->n_levels ? active_levels : desc->n_levels;
+  if (active <= 0) return NA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const float vr = vmax - vmin;
+  switch (desc->n_features) {
+    case 1: return dispatch_taps_bwd<1>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
+                                        dy_dtype, ld, grad_table, active, s);
+    case 2: return dispatch_taps_bwd<2>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
+                                        dy_dtype, ld, grad_table, active, s);
+    case 4: return dispatch_taps_bwd<4>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
+                                        dy_dtype, ld, grad_table, active, s);
+    default: return dispatch_taps_bwd<8>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
+                                         dy_dtype, ld, grad_table, active, s);
+  }
 }

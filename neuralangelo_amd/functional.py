@@ -68,6 +68,11 @@ def gemm(A, lda, trans_a, B, ldb, trans_b, M, N, K, C, ldc, epilogue=NA_EPI_STOR
     _lib.call("na_gemm", ctypes.byref(a), _s(A))
 
 
+def _t(W):
+    """W^T, contiguous: input-gradient GEMMs then read both operands along k."""
+    return W.t().contiguous()
+
+
 def _split_for(rows):
     """split-K factor for weight-gradient GEMMs reducing over `rows` points."""
     return max(1, min(512, rows // 2048))
@@ -186,6 +191,65 @@ def sdf_forward_nograd(enc, w, points):
     return sdf
 
 
+def _sdf_mlp_forward(X0, w, with_feat):
+    """H0 = softplus(X0 W0p^T + b0), sdf = H0 ws^T + bs, FB[:, :hid] = softplus(H0 W1^T + b1)."""
+    P, dev, cdt = X0.shape[0], X0.device, w.cdt
+    H0 = torch.empty(P, w.hidden, dtype=cdt, device=dev)
+    gemm(X0, w.k0p, 0, w.W0p, w.k0p, 0, P, w.hidden, w.k0p, H0, w.hidden, bias=w.b0,
+         act=NA_ACT_SOFTPLUS)
+    sdf = torch.empty(P, 1, dtype=torch.float32, device=dev)
+    _lib.call("na_linear_narrow_fwd", _lib.ptr(H0), w.hidden, _code(H0), P, w.hidden,
+              _lib.ptr(w.ws), _lib.ptr(w.bs), 1, NA_ACT_NONE, _lib.ptr(sdf), _s(X0))
+    FB = None
+    if with_feat:
+        FB = torch.empty(P, RGB_LD, dtype=cdt, device=dev)
+        gemm(H0, w.hidden, 0, w.W1, w.hidden, 0, P, w.hidden, w.hidden, FB, RGB_LD, bias=w.b1,
+             act=NA_ACT_SOFTPLUS)
+    return H0, sdf, FB
+
+
+class _SDFGrads:
+    """Weight-gradient accumulators of the SDF MLP (kernel layout, fp32)."""
+
+    def __init__(self, w, dev, with_feat):
+        hid = w.hidden
+        self.dW0p = torch.zeros(hid, w.k0p, device=dev)
+        self.db0 = torch.zeros(hid, device=dev)
+        self.dws = torch.zeros(1, hid, device=dev)
+        self.dbs = torch.zeros(1, device=dev)
+        self.dW1 = torch.zeros(hid, hid, device=dev) if with_feat else None
+        self.db1 = torch.zeros(hid, device=dev) if with_feat else None
+
+
+def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc):
+    """Backward through the SDF MLP; weight gradients accumulate into `acc`.
+    Returns the first layer's pre-activation gradient dpre0 [P, hid] (compute dtype)."""
+    P, hid, cdt, dev = X0.shape[0], w.hidden, w.cdt, X0.device
+    dsdf = (torch.zeros(P, 1, device=dev) if dsdf is None else dsdf).reshape(P, 1)
+    dsdf = dsdf.contiguous().float()
+    dH0 = torch.zeros(P, hid, dtype=torch.float32, device=dev)
+    if FB is not None and dfeat is not None:
+        dpre1 = act_grad(dfeat, FB, NA_ACT_SOFTPLUS, cdt, colsum=acc.db1, n_cols=hid)
+        linear_weight_grad(dpre1, H0, hid, hid, acc.dW1)
+        gemm(dpre1, hid, 0, _t(w.W1), hid, 0, P, hid, hid, dH0, hid)  # dH0 = dpre1 . W1
+    # dpre0 = (dH0 + dsdf * ws) * softplus'(H0); dws, dbs; db0 = column sums of dpre0
+    _lib.call("na_linear_narrow_bwd", _lib.ptr(H0), hid, _code(H0), P, hid, _lib.ptr(w.ws), 1,
+              _lib.ptr(dsdf), _lib.ptr(acc.dws), _lib.ptr(acc.dbs), _lib.ptr(dH0), hid,
+              _code(dH0), 1, _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS, SOFTPLUS_BETA,
+              _lib.ptr(acc.db0), _s(X0))
+    dpre0 = dH0 if cdt == torch.float32 else dH0.to(cdt)
+    linear_weight_grad(dpre0, X0, w.k0p, w.k0p, acc.dW0p)
+    return dpre0
+
+
+def _encoder_grad(dpre0, w):
+    """dX (encoder columns) = dpre0 . W0p[:, :n_enc]  ->  [P, n_enc] fp32."""
+    P = dpre0.shape[0]
+    dX = torch.empty(P, w.n_enc, dtype=torch.float32, device=dpre0.device)
+    gemm(dpre0, w.hidden, 0, _t(w.W0p), w.hidden, 0, P, w.n_enc, w.hidden, dX, w.n_enc)
+    return dX
+
+
 class SDFNetFunction(torch.autograd.Function):
     """SDF MLP (mlp.py:55-69), optionally fused with the encoder (modules.py:76-95).
 
@@ -211,18 +275,7 @@ class SDFNetFunction(torch.autograd.Function):
             X0 = inp.contiguous()
             if X0.dtype != cdt or X0.shape[1] != w.k0p:
                 raise RuntimeError("prebuilt SDF input must be [P, K0p] in the compute dtype")
-        P = X0.shape[0]
-        H0 = torch.empty(P, w.hidden, dtype=cdt, device=dev)
-        gemm(X0, w.k0p, 0, w.W0p, w.k0p, 0, P, w.hidden, w.k0p, H0, w.hidden, bias=w.b0,
-             act=NA_ACT_SOFTPLUS)
-        sdf = torch.empty(P, 1, dtype=torch.float32, device=dev)
-        _lib.call("na_linear_narrow_fwd", _lib.ptr(H0), w.hidden, _code(H0), P, w.hidden,
-                  _lib.ptr(w.ws), _lib.ptr(w.bs), 1, NA_ACT_NONE, _lib.ptr(sdf), _s(X0))
-        FB = None
-        if with_feat:
-            FB = torch.empty(P, RGB_LD, dtype=cdt, device=dev)
-            gemm(H0, w.hidden, 0, w.W1, w.hidden, 0, P, w.hidden, w.hidden, FB, RGB_LD, bias=w.b1,
-                 act=NA_ACT_SOFTPLUS)
+        H0, sdf, FB = _sdf_mlp_forward(X0, w, with_feat)
         ctx.save_for_backward(points, X0, H0, FB if FB is not None else torch.empty(0, device=dev))
         ctx.w, ctx.enc, ctx.with_feat = w, enc, with_feat
         if with_feat:
@@ -233,44 +286,72 @@ class SDFNetFunction(torch.autograd.Function):
     def backward(ctx, dsdf, dfeat):
         points, X0, H0, FB = ctx.saved_tensors
         w, enc = ctx.w, ctx.enc
-        P, hid, cdt, dev = X0.shape[0], w.hidden, w.cdt, X0.device
-        dsdf = (torch.zeros(P, 1, device=dev) if dsdf is None else dsdf).contiguous().float()
-        dW1 = db1 = None
-        dH0 = torch.zeros(P, hid, dtype=torch.float32, device=dev)
-        if ctx.with_feat:
-            db1 = torch.zeros(hid, device=dev)
-            dW1 = torch.zeros(hid, hid, device=dev)
-            if dfeat is not None:
-                dpre1 = act_grad(dfeat, FB, NA_ACT_SOFTPLUS, cdt, colsum=db1, n_cols=hid)
-                linear_weight_grad(dpre1, H0, hid, hid, dW1)
-                gemm(dpre1, hid, 0, w.W1, hid, 1, P, hid, hid, dH0, hid)  # dH0 = dpre1 . W1
-        dws = torch.zeros(1, hid, device=dev)
-        dbs = torch.zeros(1, device=dev)
-        db0 = torch.zeros(hid, device=dev)
-        # dpre0 = (dH0 + dsdf * ws) * softplus'(H0); dws, dbs; db0 = column sums of dpre0
-        _lib.call("na_linear_narrow_bwd", _lib.ptr(H0), hid, _code(H0), P, hid, _lib.ptr(w.ws), 1,
-                  _lib.ptr(dsdf), _lib.ptr(dws), _lib.ptr(dbs), _lib.ptr(dH0), hid, _code(dH0), 1,
-                  _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS, SOFTPLUS_BETA, _lib.ptr(db0),
-                  _s(X0))
-        dpre0 = dH0 if cdt == torch.float32 else dH0.to(cdt)
-        dW0p = torch.zeros(hid, w.k0p, device=dev)
-        linear_weight_grad(dpre0, X0, w.k0p, w.k0p, dW0p)
-        dW0 = w.unpermute_dW0(dW0p)
+        P, dev = X0.shape[0], X0.device
+        acc = _SDFGrads(w, dev, ctx.with_feat)
+        dpre0 = _sdf_mlp_backward(X0, H0, FB if ctx.with_feat else None, w, dsdf, dfeat, acc)
+        dW0 = w.unpermute_dW0(acc.dW0p)
         dtable = dinp = None
         if enc is not None and ctx.needs_input_grad[1]:
-            n_enc = w.n_enc
-            dX = torch.empty(P, n_enc, dtype=torch.float32, device=dev)
-            gemm(dpre0, hid, 0, w.W0p, w.k0p, 1, P, n_enc, hid, dX, n_enc)
+            dX = _encoder_grad(dpre0, w)
             dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
             vmin, vmax = enc.vol_range
             _lib.call("na_sdf_input_bwd", ctypes.byref(enc.desc), _lib.ptr(points), P, float(vmin),
-                      float(vmax), _lib.ptr(dX), _code(dX), n_enc, _lib.ptr(dtable),
+                      float(vmax), _lib.ptr(dX), _code(dX), w.n_enc, _lib.ptr(dtable),
                       int(enc.active_levels), _s(X0))
         if enc is None and ctx.needs_input_grad[0]:
             dinp = torch.empty(P, w.k0p, dtype=torch.float32, device=dev)
-            gemm(dpre0, hid, 0, w.W0p, w.k0p, 1, P, w.k0p, hid, dinp, w.k0p)
-        return (dinp, dtable, dW0, db0, dW1, db1, dws.view(w.ws_shape), dbs.view(w.bs_shape),
-                None, None, None, None)
+            gemm(dpre0, w.hidden, 0, _t(w.W0p), w.hidden, 0, P, w.k0p, w.hidden, dinp, w.k0p)
+        return (dinp, dtable, dW0, acc.db0, acc.dW1, acc.db1, acc.dws.view(w.ws_shape),
+                acc.dbs.view(w.bs_shape), None, None, None, None)
+
+
+class SDFCentreTapsFunction(torch.autograd.Function):
+    """The SDF field at the sample points and at their four tetrahedral taps
+    (modules.py:84-95 for the centre, modules.py:157-175 for the taps) as one node.
+
+    Forward: sdf [P,1], feat [P,256] (view of the colour-input buffer) and
+    s_taps [4,P].  Backward runs both MLP backward passes into shared weight
+    accumulators and scatters the five encoder gradients of each point with one
+    merged kernel (na_sdf_taps_bwd)."""
+
+    @staticmethod
+    def forward(ctx, points, table_params, W0, b0, W1, b1, ws, bs, enc, cdt, n_enc, eps):
+        _lib.require_cuda(points)
+        w = SDFWeights(W0, b0, W1, b1, ws, bs, n_enc, cdt)
+        pts = points.contiguous().float()
+        P = pts.shape[0]
+        X0c = sdf_input(enc, pts, cdt, w.k0p)
+        H0c, sdf, FB = _sdf_mlp_forward(X0c, w, True)
+        tp = taps_points(pts, eps).view(-1, 3)
+        X0t = sdf_input(enc, tp, cdt, w.k0p)
+        H0t, s_t, _ = _sdf_mlp_forward(X0t, w, False)
+        del tp
+        ctx.save_for_backward(pts, X0c, H0c, FB, X0t, H0t)
+        ctx.w, ctx.enc, ctx.eps = w, enc, float(eps)
+        return sdf, FB[:, :w.hidden], s_t.view(4, P)
+
+    @staticmethod
+    def backward(ctx, dsdf, dfeat, ds_taps):
+        pts, X0c, H0c, FB, X0t, H0t = ctx.saved_tensors
+        w, enc = ctx.w, ctx.enc
+        P, dev = pts.shape[0], pts.device
+        acc = _SDFGrads(w, dev, True)
+        dpre_c = _sdf_mlp_backward(X0c, H0c, FB, w, dsdf, dfeat, acc)
+        dpre_t = _sdf_mlp_backward(X0t, H0t, None, w,
+                                   ds_taps.reshape(4 * P, 1) if ds_taps is not None else None,
+                                   None, acc)
+        dtable = None
+        if ctx.needs_input_grad[1]:
+            dXc = _encoder_grad(dpre_c, w)
+            dXt = _encoder_grad(dpre_t, w)
+            dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
+            vmin, vmax = enc.vol_range
+            _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
+                      float(vmax), ctx.eps, _lib.ptr(dXc), _lib.ptr(dXt), _code(dXc), w.n_enc,
+                      _lib.ptr(dtable), int(enc.active_levels), _s(pts))
+        dW0 = w.unpermute_dW0(acc.dW0p)
+        return (None, dtable, dW0, acc.db0, acc.dW1, acc.db1, acc.dws.view(w.ws_shape),
+                acc.dbs.view(w.bs_shape), None, None, None, None)
 
 
 # ---------------------------------------------------------------------------
@@ -372,14 +453,14 @@ class RGBNetFunction(torch.autograd.Function):
             dWs[li] = torch.zeros(hid, hid, device=dev)
             linear_weight_grad(dpre, X, hid, hid, dWs[li])
             dprev = torch.empty(P, hid, dtype=cdt, device=dev)
-            gemm(dpre, hid, 0, w.W[li], hid, 1, P, hid, hid, dprev, hid, dact=NA_ACT_RELU,
+            gemm(dpre, hid, 0, _t(w.W[li]), hid, 0, P, hid, hid, dprev, hid, dact=NA_ACT_RELU,
                  dact_y=X, ld_dact=hid, colsum=dbs[li - 1])
             dpre = dprev
         dW0p = torch.zeros(hid, RGB_LD, device=dev)
         linear_weight_grad(dpre, FB, RGB_LD, RGB_LD, dW0p)
         dWs[0] = w.unpermute_dW0(dW0p)
         dFB = torch.empty(P, RGB_LD, dtype=torch.float32, device=dev)
-        gemm(dpre, hid, 0, w.W[0], RGB_LD, 1, P, RGB_LD, hid, dFB, RGB_LD)
+        gemm(dpre, hid, 0, _t(w.W[0]), hid, 0, P, RGB_LD, hid, dFB, RGB_LD)
         dnormals = None
         if ctx.build:
             dnormals = torch.empty(P, 3, dtype=torch.float32, device=dev)

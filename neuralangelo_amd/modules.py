@@ -91,6 +91,30 @@ class NeuralSDF(torch.nn.Module):
     def sdf(self, points_3D):
         return self.forward(points_3D, with_sdf=True, with_feat=False)[0]
 
+    def forward_with_taps(self, points):
+        """forward(points) and the four tap SDFs of compute_gradients in one autograd node.
+
+        points [P,3] -> sdf [P,1], feat [P,256] (buffer view), s_taps [4,P].  Equals
+        forward() followed by sdf(x + eps*k_i) for the tetrahedral k_i
+        (modules.py:157-175); the backward merges the five encoder scatters."""
+        grad_cfg = _get(self.cfg_sdf, "gradient")
+        if _get(grad_cfg, "mode", "numerical") != "numerical" or _get(grad_cfg, "taps", 4) != 4:
+            raise NotImplementedError("only numerical 4-tap gradients are supported")
+        W0, b0, W1, b1, ws, bs = self.mlp.weights()
+        self.tcnn_encoding.active_levels = self.active_levels
+        eps, _ = self.tap_eps()
+        return Fn.SDFCentreTapsFunction.apply(points.reshape(-1, 3), self.tcnn_encoding.params,
+                                              W0, b0, W1, b1, ws, bs, self.tcnn_encoding,
+                                              self.compute_dtype, self.tcnn_encoding.n_output_dims,
+                                              eps)
+
+    def gradients_from_taps(self, s_taps, sdf, training):
+        """The 4-tap combination of compute_gradients (modules.py:163-175) on given taps."""
+        eps, eps_sq = self.tap_eps()
+        grad, hess = Fn.TapsCombine.apply(s_taps, sdf.reshape(-1) if training else None, eps,
+                                          eps_sq, training)
+        return grad, (hess if training else None)
+
     @torch.no_grad()
     def sdf_nograd(self, points_3D):
         """SDF values without autograd bookkeeping (sampling passes)."""

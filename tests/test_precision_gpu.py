@@ -1,0 +1,60 @@
+"""Mixed-precision error against the FP64 oracle at the 2^22 x 16 x 8 configuration:
+BF16 table copy and BF16 MFMA with FP32 masters, on 65,536 random points in
+[-1, 1]^3 (SDF, 4-tap gradient, colour) -- the BF16 accuracy figures of the
+mixed-precision train step."""
+import pytest
+import torch
+import torch.nn.functional as torch_F
+
+from neuralangelo_amd.model import Model, default_config
+from oracle import field as ofield
+from tests.helpers import oracle_field, oracle_params, randomize_model
+
+pytestmark = pytest.mark.gpu
+
+# tolerances: FP32 path (rounding only) and BF16 path (BF16 table, weights and activations)
+TOL = {torch.float32: dict(sdf=2e-5, grad=2e-3, rgb=2e-5),
+       torch.bfloat16: dict(sdf=5e-3, grad=5e-2, rgb=2e-2)}
+
+
+@pytest.fixture(scope="module")
+def setup(cuda):
+    torch.manual_seed(0)
+    m32 = Model(default_config(log2_T=22, F=8), compute_dtype=torch.float32)
+    randomize_model(m32, 3, table_scale=0.05)
+    m32.neural_sdf.active_levels = m32.neural_sdf.anneal_levels = 16
+    m32.neural_sdf.set_normal_epsilon()
+    P = oracle_params(m32)
+    fld, _ = oracle_field(m32, P)
+    g = torch.Generator().manual_seed(11)
+    x = torch.rand(65536, 3, generator=g) * 2 - 1
+    v = torch_F.normalize(torch.randn(65536, 3, generator=g), dim=-1)
+    with torch.no_grad():
+        s_ref, f_ref = fld.forward(x.double())
+        g_ref, _ = fld.compute_gradients(x.double(), m32.neural_sdf.normal_eps, training=False)
+        rgb_ref = ofield.neural_rgb(x.double(), torch_F.normalize(g_ref, dim=-1), v.double(),
+                                    f_ref, P)
+    return m32, x, v, s_ref, g_ref, rgb_ref
+
+
+@pytest.mark.parametrize("cdt", [torch.float32, torch.bfloat16])
+def test_error_against_fp64_oracle(setup, cuda, cdt):
+    m32, x, v, s_ref, g_ref, rgb_ref = setup
+    m = Model(default_config(log2_T=22, F=8), compute_dtype=cdt)
+    m.load_state_dict(m32.state_dict())
+    m = m.to(cuda).eval()
+    m.neural_sdf.active_levels = m.neural_sdf.anneal_levels = 16
+    m.neural_sdf.set_normal_epsilon()
+    with torch.no_grad():
+        sdf, feat, s_taps = m.neural_sdf.forward_with_taps(x.to(cuda))
+        grad, _ = m.neural_sdf.gradients_from_taps(s_taps, sdf, False)
+        rgb = m.neural_rgb.forward(x.to(cuda)[:, None], grad[:, None], v.to(cuda), feat, None,
+                                   normals_are_gradients=True)[:, 0]
+    e_sdf = (sdf.cpu().double() - s_ref).abs().max().item()
+    gr = g_ref.norm(dim=-1, keepdim=True)
+    e_grad = ((grad.cpu().double() - g_ref).norm(dim=-1, keepdim=True) / gr).max().item()
+    e_rgb = (rgb.float().cpu().double() - rgb_ref).abs().max().item()
+    print(f"\n{cdt}: max |sdf err| {e_sdf:.3e}  max rel grad err {e_grad:.3e}  "
+          f"max |rgb err| {e_rgb:.3e}")
+    t = TOL[cdt]
+    assert e_sdf <= t["sdf"] and e_grad <= t["grad"] and e_rgb <= t["rgb"], (e_sdf, e_grad, e_rgb)

@@ -1,0 +1,83 @@
+"""Merged centre + 4-tap encoder backward (na_sdf_taps_bwd) against five separate
+scatters, and the fused SDF node against the unfused forward + compute_gradients."""
+import ctypes
+
+import pytest
+import torch
+
+from neuralangelo_amd import _lib
+from neuralangelo_amd import functional as Fn
+from neuralangelo_amd.encoding import Encoding
+from neuralangelo_amd.model import Model, default_config
+from tests.helpers import randomize_model
+
+pytestmark = pytest.mark.gpu
+
+
+def _enc(log2_T, F, cuda):
+    return Encoding(3, dict(otype="HashGrid", n_levels=16, n_features_per_level=F,
+                            log2_hashmap_size=log2_T, base_resolution=32,
+                            per_level_scale=2 ** 0.4)).to(cuda)
+
+
+@pytest.mark.parametrize("eps,active,dt,F", [(1 / 2048 / 3 ** 0.5, 16, torch.float32, 8),
+                                             (1 / 2048 / 3 ** 0.5, 16, torch.bfloat16, 8),
+                                             (1 / 256 / 3 ** 0.5, 11, torch.float32, 4),
+                                             (0.05, 16, torch.float32, 2)])
+def test_merged_scatter_equals_five_scatters(cuda, eps, active, dt, F):
+    print("taps bwd", eps, active, dt, F, flush=True)
+    enc = _enc(16, F, cuda)
+    n, LF = 4000, 16 * F
+    g = torch.Generator().manual_seed(0)
+    pts = (torch.rand(n, 3, generator=g) * 3.8 - 1.9).to(cuda)
+    dyc = torch.randn(n, LF, generator=g).to(cuda).to(dt)
+    dyt = torch.randn(4 * n, LF, generator=g).to(cuda).to(dt)
+    dyc[::7] = 0  # zero rows are skipped
+    s = _lib.stream_of(pts)
+    ref = torch.zeros(enc.n_params, device=cuda)
+    tp = Fn.taps_points(pts, eps).view(-1, 3)
+    for x, dy in ((pts, dyc), (tp, dyt)):
+        _lib.call("na_sdf_input_bwd", ctypes.byref(enc.desc), _lib.ptr(x), x.shape[0], -2.0, 2.0,
+                  _lib.ptr(dy), _lib.dtype_code(dy.dtype), LF, _lib.ptr(ref), active, s)
+    got = torch.zeros(enc.n_params, device=cuda)
+    _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), n, -2.0, 2.0, float(eps),
+              _lib.ptr(dyc), _lib.ptr(dyt), _lib.dtype_code(dt), LF, _lib.ptr(got), active, s)
+    torch.cuda.synchronize()
+    scale = ref.abs().max().item()
+    assert (got - ref).abs().max().item() <= 2e-5 * scale
+    assert torch.equal(got != 0, ref != 0) or ((got != 0) ^ (ref != 0)).sum() < 1e-4 * ref.numel()
+    off = enc.offsets[active] * F
+    assert got[off:].abs().max().item() == 0.0  # inactive levels untouched
+
+
+@pytest.mark.parametrize("cdt", [torch.float32, torch.bfloat16])
+def test_fused_node_matches_unfused_path(cuda, cdt):
+    torch.manual_seed(0)
+    m = Model(default_config(log2_T=15), compute_dtype=cdt)
+    randomize_model(m, 2)
+    m = m.to(cuda).train()
+    m.neural_sdf.active_levels = m.neural_sdf.anneal_levels = 16
+    m.neural_sdf.set_normal_epsilon()
+    sdf_mod = m.neural_sdf
+    g = torch.Generator().manual_seed(1)
+    pts = (torch.rand(3000, 3, generator=g) * 1.6 - 0.8).to(cuda)
+    w_feat = torch.randn(3000, 256, generator=g).to(cuda)
+    w_grad = torch.randn(3000, 3, generator=g).to(cuda)
+
+    def loss_of(sdf, feat, grad, hess):
+        return (sdf.sum() + (feat.float() * w_feat).sum() * 1e-2 + (grad * w_grad).sum()
+                + 1e-3 * hess.abs().sum())
+
+    params = [sdf_mod.tcnn_encoding.params] + list(sdf_mod.mlp.parameters())
+    sdf, feat = sdf_mod.forward(pts)
+    grad, hess = sdf_mod.compute_gradients(pts, training=True, sdf=sdf)
+    ref = torch.autograd.grad(loss_of(sdf, feat, grad, hess), params)
+    sdf2, feat2, s_taps = sdf_mod.forward_with_taps(pts)
+    grad2, hess2 = sdf_mod.gradients_from_taps(s_taps, sdf2, True)
+    assert torch.equal(sdf, sdf2) and torch.equal(grad, grad2) and torch.equal(hess, hess2)
+    got = torch.autograd.grad(loss_of(sdf2, feat2, grad2, hess2), params)
+    for a, b in zip(got, ref):
+        a, b = a.double().flatten(), b.double().flatten()
+        cos = (a @ b / (a.norm() * b.norm() + 1e-300)).item()
+        assert cos > 0.999999, cos
+        assert (a - b).abs().max() <= 1e-4 * b.abs().max() + 1e-12
