@@ -105,7 +105,7 @@ def encoder_roofline(model, dev, n_points=786432, iters=20):
     pts = (c[:R, None, :] + v[:R, None, :] * t[None, :, None]).reshape(-1, 3)
     pts = pts.clamp(-1.5, 1.5).contiguous().to(dev)
     cdt = model.compute_dtype
-    k0p = Fn._pad8(enc.n_output_dims + 3)
+    k0p = Fn.sdf_input_width(enc.n_output_dims, cdt)
     for _ in range(3):
         Fn.sdf_input(enc, pts, cdt, k0p)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -261,6 +261,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison run")
     ap.add_argument("--no-mc", action="store_true", help="skip the mesh-extraction leg")
     ap.add_argument("--mc-res", type=int, default=1024)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one device copy of MIB MiB (a known-size dispatch that "
+                         "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of "
@@ -270,6 +273,13 @@ def main():
     dev = torch.device("cuda", 0)
     from neuralangelo_amd import _lib
     _lib.load()
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty(n, device=dev)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     model, dt_bf16, loss_bf16 = time_train(torch.bfloat16, dev, args.rays, args.steps, args.warmup)
     roof = encoder_roofline(model, dev)
     del model

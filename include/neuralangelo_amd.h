@@ -139,9 +139,31 @@ typedef struct na_gemm_args {
   const float* r1_b;
   float* colsum;
   int32_t split_k;
+  /* optional extras (NULL = off; store epilogue, split_k = 1):
+   *   z_out:    fp32 copy of the pre-activation value (after bias) [M, ld_z];
+   *   zc:       delta mode -- the accumulator is an increment dz of a stored
+   *             fp32 pre-activation z0 = zc[(m % zc_rows) * ld_zc + n]; the
+   *             output is act(z0 + dz) (act must be NA_ACT_SOFTPLUS, no bias);
+   *   head_w:   a narrow head fused into the epilogue: head_out[part * M + m] =
+   *             sum over the tile's columns n of head_w[n] * h, h = the
+   *             activated output, or act(z0 + dz) - act(z0) in delta mode
+   *             (evaluated without cancellation); part = 2 * (n / 128) +
+   *             (n % 128) / 64, i.e. 2 * ceil(N / 128) partial sums per row. */
+  float* z_out;
+  int64_t ld_z;
+  const float* zc;
+  int64_t ld_zc;
+  int64_t zc_rows;
+  const float* head_w;
+  float* head_out;
 } na_gemm_args;
 
 int na_gemm(const na_gemm_args* args, na_stream_t stream);
+
+/* out[m] = (bias ? bias[0] : 0) + sum_{p < nparts} parts[p*M + m] (fixed order):
+ * completes the fused narrow head of na_gemm (head_out partial sums). */
+int na_head_sum(const float* parts, int32_t nparts, int64_t M, const float* bias, float* out,
+                na_stream_t stream);
 
 /* Narrow output layer (N <= 4): out[m,n] = act(sum_k X[m*ldx+k] W[n*K+k] + b[n]),
  * X of x_dtype, W/b fp32, out fp32 [M, N].  Used for the SDF head
@@ -177,7 +199,10 @@ int na_weight_norm_bwd(const float* v, const float* g, const float* dW, int64_t 
                        float* dv, float* dg, na_stream_t stream);
 
 /* ---------------------------------------------------------------------------
- * SDF network input: [enc(L*F) | x(3) | zero pad] per point, row stride ld.
+ * SDF network input per point, row stride ld: [enc(L*F) | x(3) | zero pad] for an
+ * fp32 output; [enc(L*F) | hi(3) | lo(3) | hi(3) | zero pad] for a bf16 output,
+ * hi = bf16(x), lo = bf16(x - hi), so the point keeps 16 mantissa bits when the
+ * first-layer weight columns are laid out [W_hi | W_hi | W_lo].
  * Fuses the range normalisation of projects/neuralangelo/utils/modules.py:82-83,
  * the encoder (modules.py:84-86), the coarse-to-fine mask (:91-93) and the
  * concatenation with the raw points (:94); the encoder block comes first so
@@ -203,6 +228,17 @@ int na_sdf_taps_bwd(const na_grid_desc* desc, const float* points, int64_t n, fl
                     float vmax, float eps, const void* dy_centre, const void* dy_taps,
                     int32_t dy_dtype, int64_t ld, float* grad_table, int32_t active_levels,
                     na_stream_t stream);
+
+/* SDF network inputs of a numerical-gradient point set for the increment
+ * formulation (bf16): x0_centre [n, ld] and x0_taps [4n, ld] (tap-major) as
+ * na_sdf_input_fwd writes them for x and x + eps*k_i (k_i as na_sdf_taps_bwd),
+ * and d_taps [4n, ld] = x0_taps - x0_centre evaluated in fp32 before rounding
+ * (the point columns hold the exact offset).  Replaces the encoder calls of
+ * modules.py:84-86 and :157-175 for one point set; all outputs bf16. */
+int na_sdf_taps_input_fwd(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
+                          float vmax, float eps, const void* table, int32_t table_dtype,
+                          int32_t active_levels, void* x0_centre, void* x0_taps, void* d_taps,
+                          int64_t ld, na_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Per-ray operators.  R rays, N samples per ray, row-major [R, N(, K)].
@@ -282,6 +318,17 @@ int na_taps_combine_fwd(const float* s_taps, const float* sdf, int64_t n, float 
 int na_taps_combine_bwd(int64_t n, float eps, float eps_sq, const float* dgrad,
                         const float* dhess, float* ds_taps, float* dsdf, int32_t add_dsdf,
                         na_stream_t stream);
+
+/* Increment form of the 4-tap combination (modules.py:163-175): ds [4, n] holds
+ * sdf(x + eps k_i) - sdf(x), sc [n] the centre SDF as evaluated and sdf [n] the
+ * value passed to compute_gradients (outside fill included).  grad [n,3] =
+ * sum_i k_i ds_i / (4 eps); hess [n,3] = h/3 with h = (sum_i ds_i / 2 + 2 (sc -
+ * sdf)) / eps_sq (algebraically the reference's ((sum s_i)/2 - 2 sdf)/eps^2).
+ * hess may be NULL (evaluation).  Backward writes dds [4,n], dsc [n], dsdf [n]. */
+int na_taps_delta_fwd(const float* ds, const float* sc, const float* sdf, int64_t n, float eps,
+                      float eps_sq, float* grad, float* hess, na_stream_t stream);
+int na_taps_delta_bwd(int64_t n, float eps, float eps_sq, const float* dgrad, const float* dhess,
+                      float* dds, float* dsc, float* dsdf, na_stream_t stream);
 
 /* Colour-MLP input (projects/neuralangelo/utils/modules.py:209-220, SH degree 3
  * from projects/neuralangelo/utils/spherical_harmonics.py:47-71).  Row layout

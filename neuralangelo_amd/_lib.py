@@ -57,6 +57,9 @@ class GemmArgs(ctypes.Structure):
         ("dact", ctypes.c_int32),
         ("r1_a", ctypes.c_void_p), ("r1_b", ctypes.c_void_p), ("colsum", ctypes.c_void_p),
         ("split_k", ctypes.c_int32),
+        ("z_out", ctypes.c_void_p), ("ld_z", ctypes.c_int64),
+        ("zc", ctypes.c_void_p), ("ld_zc", ctypes.c_int64), ("zc_rows", ctypes.c_int64),
+        ("head_w", ctypes.c_void_p), ("head_out", ctypes.c_void_p),
     ]
 
 
@@ -78,6 +81,7 @@ _SIGNATURES = {
     "na_hashgrid_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _I32, _P]),
     "na_hashgrid_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _P]),
     "na_gemm": (ctypes.c_int, [_PGEMM, _P]),
+    "na_head_sum": (ctypes.c_int, [_P, _I32, _I64, _P, _P, _P]),
     "na_linear_narrow_fwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P, _I32, _I32, _P, _P]),
     "na_linear_narrow_bwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _I32, _P, _P, _P, _P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _F32, _P, _P]),
     "na_act_grad": (ctypes.c_int, [_P, _I64, _I32, _P, _I64, _I32, _I64, _I64, _I32, _F32, _P, _I64, _I32, _P, _P]),
@@ -86,6 +90,7 @@ _SIGNATURES = {
     "na_sdf_input_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _P, _I32, _I32, _P, _I32, _I64, _P]),
     "na_sdf_input_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _P, _I32, _I64, _P, _I32, _P]),
     "na_sdf_taps_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _P, _I32, _I64, _P, _I32, _P]),
+    "na_sdf_taps_input_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _I32, _I32, _P, _P, _P, _I64, _P]),
     "na_ray_bounds_sample": (ctypes.c_int, [_P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P]),
     "na_sample_pdf": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
     "na_hier_resample": (ctypes.c_int, [_P, _P, _I64, _I32, _F32, _I32, _P, _P, _P, _P, _P]),
@@ -99,6 +104,8 @@ _SIGNATURES = {
     "na_taps_points": (ctypes.c_int, [_P, _I64, _F32, _P, _P]),
     "na_taps_combine_fwd": (ctypes.c_int, [_P, _P, _I64, _F32, _F32, _P, _P, _P]),
     "na_taps_combine_bwd": (ctypes.c_int, [_I64, _F32, _F32, _P, _P, _P, _P, _I32, _P]),
+    "na_taps_delta_fwd": (ctypes.c_int, [_P, _P, _P, _I64, _F32, _F32, _P, _P, _P]),
+    "na_taps_delta_bwd": (ctypes.c_int, [_I64, _F32, _F32, _P, _P, _P, _P, _P, _P]),
     "na_rgb_input_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _I64, _I32, _P, _I32, _I64, _P]),
     "na_rgb_input_bwd": (ctypes.c_int, [_P, _I32, _I64, _I32, _P, _I32, _I64, _P, _I32, _P]),
     "na_sdf_losses_fwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P]),

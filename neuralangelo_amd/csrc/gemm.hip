@@ -37,6 +37,13 @@ struct EpiParams {
   const float* r1_a;
   const float* r1_b;
   float* colsum;
+  float* z_out;
+  int64_t ld_z;
+  const float* zc;
+  int64_t ld_zc;
+  int64_t zc_rows;
+  const float* head_w;
+  float* head_out;
 };
 
 struct GemmParams {
@@ -72,8 +79,28 @@ __device__ inline float act_deriv_from_y(int act, float y, float beta) {
   return 1.0f;
 }
 
+// softplus_beta(z0 + dz) - softplus_beta(z0) without cancellation: below the
+// linear threshold it is log1p(sigmoid(beta z0) * expm1(beta dz)) / beta.
+__device__ inline float softplus_delta(float z0, float dz, float beta) {
+  const float z1 = z0 + dz;
+  const float a0 = z0 * beta, a1 = z1 * beta;
+  if (a0 > 20.0f && a1 > 20.0f) return z1 - z0;
+  if (a0 > 20.0f || a1 > 20.0f)
+    return act_forward(NA_ACT_SOFTPLUS, z1, beta) - act_forward(NA_ACT_SOFTPLUS, z0, beta);
+  const float sig = 1.0f / (1.0f + expf(-a0));
+  return log1pf(sig * expm1f(beta * dz)) / beta;
+}
+
 __device__ inline float load_as_f32(const void* p, int dtype, int64_t i) {
   return dtype == NA_BF16 ? bf16_to_f32(((const bf16_t*)p)[i]) : ((const float*)p)[i];
+}
+
+// Everything of the epilogue after the bias: act, rank-1 term, derivative factor.
+__device__ inline float epilogue_value_nobias(const EpiParams& e, int64_t m, int64_t n, float v) {
+  if (e.act) v = act_forward(e.act, v, e.beta);
+  if (e.r1_a) v += e.r1_a[m] * e.r1_b[n];
+  if (e.dact) v *= act_deriv_from_y(e.dact, load_as_f32(e.dact_y, e.dact_dtype, m * e.ld_dact + n), e.beta);
+  return v;
 }
 
 __device__ inline float epilogue_value(const EpiParams& e, int64_t m, int64_t n, float v) {
@@ -253,6 +280,9 @@ __device__ inline void gemm_epilogue(const GemmParams& p, unsigned char* smem,
   float* Cw = reinterpret_cast<float*>(smem) + wave * 32 * CP;
   const EpiParams& e = p.e;
   const int64_t n = n0 + wn * 64 + lane;
+  const bool head = e.head_w != nullptr;
+  const int64_t part = 2 * (int64_t)blockIdx.y + wn;
+  const float hw = (head && n < p.N) ? e.head_w[n] : 0.0f;
   float csum = 0.0f;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -265,14 +295,37 @@ __device__ inline void gemm_epilogue(const GemmParams& p, unsigned char* smem,
         Cw[row * CP + col] = acc[i][j][r];
       }
     __syncthreads();
+    const int64_t mb = m0 + wm * 64 + i * 32;
     if (n < p.N) {
-      const int64_t mb = m0 + wm * 64 + i * 32;
       for (int rr = 0; rr < 32; ++rr) {
         const int64_t m = mb + rr;
         if (m >= p.M) break;
-        float v = epilogue_value(e, m, n, Cw[rr * CP + lane]);
+        float v = Cw[rr * CP + lane];
+        float hv;
+        if (e.zc) {  // delta mode: v is an increment of the stored pre-activation
+          const float z0 = e.zc[(m % e.zc_rows) * e.ld_zc + n];
+          hv = softplus_delta(z0, v, e.beta);
+          v = act_forward(NA_ACT_SOFTPLUS, z0 + v, e.beta);
+        } else {
+          if (e.bias) v += e.bias[n];
+          if (e.z_out) e.z_out[m * e.ld_z + n] = v;
+          v = epilogue_value_nobias(e, m, n, v);
+          hv = v;
+        }
         csum += v;
         epilogue_store(e, m, n, v);
+        if (head) Cw[rr * CP + lane] = hw * hv;
+      }
+    } else if (head) {
+      for (int rr = 0; rr < 32; ++rr) Cw[rr * CP + lane] = 0.0f;
+    }
+    if (head) {
+      __syncthreads();
+      // lanes 0..31 each reduce one row of the wave's 64 columns (fixed order)
+      if (lane < 32 && mb + lane < p.M) {
+        float sum = 0.0f;
+        for (int c = 0; c < 64; ++c) sum += Cw[lane * CP + c];
+        e.head_out[part * p.M + mb + lane] = sum;
       }
     }
     __syncthreads();
@@ -344,6 +397,17 @@ __global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
   }
 
   gemm_epilogue(p, smem, acc, m0, n0, wm, wn, wave, lane);
+}
+
+// out[m] = (bias ? bias[0] : 0) + sum_p parts[p*M + m], parts summed in order.
+__global__ void __launch_bounds__(256)
+head_sum_kernel(const float* __restrict__ parts, int nparts, int64_t M,
+                const float* __restrict__ bias, float* __restrict__ out) {
+  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (m >= M) return;
+  float v = bias ? bias[0] : 0.0f;
+  for (int q = 0; q < nparts; ++q) v += parts[q * M + m];
+  out[m] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -786,6 +850,13 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
                "split_k > 1 allows no nonlinear epilogue terms");
   NA_CHECK_ARG(!a->dact || a->dact_y, "dact needs dact_y");
   NA_CHECK_ARG(!a->r1_a || a->r1_b, "r1_a needs r1_b");
+  const bool extras = a->z_out || a->zc || a->head_w;
+  NA_CHECK_ARG(!extras || (split == 1 && a->epilogue == NA_EPI_STORE),
+               "z_out / zc / head need split_k = 1 and the store epilogue");
+  NA_CHECK_ARG(!a->head_w || a->head_out, "head_w needs head_out");
+  NA_CHECK_ARG(!a->zc || (a->act == NA_ACT_SOFTPLUS && !a->bias && !a->dact && !a->r1_a &&
+                          !a->z_out && a->zc_rows >= 1),
+               "delta mode needs softplus, no bias/dact/r1/z_out, zc_rows >= 1");
   NA_CHECK_ARG((a->M + GBM - 1) / GBM < 0x7fffffffLL, "M too large");
   GemmParams p;
   p.M = a->M; p.N = a->N; p.K = a->K;
@@ -799,6 +870,8 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   p.e.bias = a->bias; p.e.act = a->act; p.e.beta = a->act_beta;
   p.e.dact_y = a->dact_y; p.e.ld_dact = a->ld_dact; p.e.dact_dtype = a->dact_dtype; p.e.dact = a->dact;
   p.e.r1_a = a->r1_a; p.e.r1_b = a->r1_b; p.e.colsum = a->colsum;
+  p.e.z_out = a->z_out; p.e.ld_z = a->ld_z; p.e.zc = a->zc; p.e.ld_zc = a->ld_zc;
+  p.e.zc_rows = a->zc_rows; p.e.head_w = a->head_w; p.e.head_out = a->head_out;
   dim3 grid((unsigned)ceil_div(a->M, GBM), (unsigned)ceil_div(a->N, GBN), (unsigned)nsplit);
   hipStream_t s = (hipStream_t)stream;
   if (a->in_dtype == NA_BF16) return launch_gemm<bf16_t>(a, p, grid, s);
@@ -919,6 +992,17 @@ extern "C" int na_weight_norm_bwd(const float* v, const float* g, const float* d
   NA_CHECK_ARG(v && g && dW && dv && dg, "null pointer");
   hipLaunchKernelGGL(weight_norm_bwd_kernel, dim3((unsigned)N), dim3(256), 0, (hipStream_t)stream,
                      v, g, dW, K, dv, dg);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_head_sum(const float* parts, int32_t nparts, int64_t M, const float* bias,
+                           float* out, na_stream_t stream) {
+  NA_CHECK_ARG(M >= 0 && nparts >= 1, "bad sizes");
+  if (M == 0) return NA_OK;
+  NA_CHECK_ARG(parts && out, "null pointer");
+  hipLaunchKernelGGL(head_sum_kernel, dim3((unsigned)ceil_div(M, (int64_t)256)), dim3(256), 0,
+                     (hipStream_t)stream, parts, nparts, M, bias, out);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -185,10 +185,27 @@ hashgrid_fwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
   const int L = d.n_levels;
   TO* o = out + i * ld + l * F;
   if (write_extras && l == 0) {
-    // [enc(L*F) | x(3) | zero pad up to ld]
+    // fp32: [enc(L*F) | x(3) | zero pad up to ld]
+    // bf16: [enc(L*F) | hi(3) | lo(3) | hi(3) | zero pad], hi = bf16(x),
+    //       lo = bf16(x - hi): x is carried to 16 mantissa bits, and with the
+    //       weight columns [W_hi | W_hi | W_lo] the product keeps
+    //       x_hi W_hi + x_lo W_hi + x_hi W_lo
     TO* e = out + i * ld + L * F;
-    for (int k = 0; k < 3; ++k) scalar_io<TO>::store(e + k, x[i * 3 + k]);
-    for (int64_t c = L * F + 3; c < ld; ++c) scalar_io<TO>::store(out + i * ld + c, 0.0f);
+    int64_t c0 = L * F + 3;
+    if constexpr (sizeof(TO) == 2) {
+      for (int k = 0; k < 3; ++k) {
+        const float xv = x[i * 3 + k];
+        const bf16_t hi = f32_to_bf16(xv);
+        const float lo = xv - bf16_to_f32(hi);
+        e[k] = hi;
+        e[3 + k] = f32_to_bf16(lo);
+        e[6 + k] = hi;
+      }
+      c0 = L * F + 9;
+    } else {
+      for (int k = 0; k < 3; ++k) scalar_io<TO>::store(e + k, x[i * 3 + k]);
+    }
+    for (int64_t c = c0; c < ld; ++c) scalar_io<TO>::store(out + i * ld + c, 0.0f);
   }
   float acc[F];
 #pragma unroll
@@ -245,6 +262,104 @@ hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
   }
 }
 
+constexpr float kTapDir[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+
+// Centre + tap inputs of the SDF network for the increment formulation of the
+// 4-tap gradient (bf16 compute).  Per point: X0c (centre row), X0t (tap rows,
+// tap-major [4n]) and D = X0t - X0c evaluated in fp32 before the bf16 cast,
+// so the increments keep bf16 precision relative to their own (small) size.
+// Point columns use the split layout [hi | lo | hi]; for D they carry the
+// exact fp32 offset x_t - x_c.  One thread per (point, level).
+__device__ inline void split_store(bf16_t* e, const float v[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const bf16_t hi = f32_to_bf16(v[k]);
+    e[k] = hi;
+    e[3 + k] = f32_to_bf16(v[k] - bf16_to_f32(hi));
+    e[6 + k] = hi;
+  }
+}
+
+template <int F, typename TT>
+__global__ void __launch_bounds__(kBlock)
+sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
+                      float vrange, float eps, const TT* __restrict__ table,
+                      bf16_t* __restrict__ X0c, bf16_t* __restrict__ X0t,
+                      bf16_t* __restrict__ D, int64_t ld, int active_levels) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int l = blockIdx.y;
+  if (i >= n) return;
+  const int L = d.n_levels;
+  float xc[3], xt[4][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    xc[k] = x[i * 3 + k];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) xt[t][k] = xc[k] + kTapDir[t][k] * eps;
+  }
+  if (l == 0) {
+    const int64_t c0 = (int64_t)L * F;
+    split_store(X0c + i * ld + c0, xc);
+    for (int64_t c = c0 + 9; c < ld; ++c) X0c[i * ld + c] = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t r = (int64_t)t * n + i;
+      float dx[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dx[k] = xt[t][k] - xc[k];  // exact (Sterbenz)
+      split_store(X0t + r * ld + c0, xt[t]);
+      split_store(D + r * ld + c0, dx);
+      for (int64_t c = c0 + 9; c < ld; ++c) {
+        X0t[r * ld + c] = 0;
+        D[r * ld + c] = 0;
+      }
+    }
+  }
+  float ac[F], at[4][F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    ac[f] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) at[t][f] = 0.0f;
+  }
+  if (l < active_levels) {
+    const LevelParams p = level_params(d, l);
+    const TT* tab = table + (size_t)p.off * F;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const float* xq = q == 0 ? xc : xt[q - 1];
+      uint32_t cell[3];
+      float fr[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        pos_fract(normalize_coord(xq[k], vmin, vrange), p.scale, cell[k], fr[k]);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                                         cell[2] + ((c >> 2) & 1), p);
+        const float w = corner_weight(c, fr);
+        float v[F];
+        EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          if (q == 0) ac[f] = fmaf(w, v[f], ac[f]);
+          else at[q - 1][f] = fmaf(w, v[f], at[q - 1][f]);
+        }
+      }
+    }
+  }
+  RowStore<F, bf16_t>::store(X0c + i * ld + l * F, ac);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int64_t r = (int64_t)t * n + i;
+    float dv[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) dv[f] = at[t][f] - ac[f];
+    RowStore<F, bf16_t>::store(X0t + r * ld + l * F, at[t]);
+    RowStore<F, bf16_t>::store(D + r * ld + l * F, dv);
+  }
+}
+
 // Backward of the five encoder evaluations of one numerical-gradient point
 // (the centre and its four tetrahedral taps, modules.py:84-86 and :157-175)
 // in one pass.  The taps lie less than one cell from the centre at every
@@ -253,7 +368,6 @@ hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
 // registers and each touched corner gets one atomic.  When a level's cells
 // spread wider (an epsilon larger than the finest active cell), the thread
 // falls back to one atomic per point and corner.
-constexpr float kTapDir[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
 
 template <int F, typename TD>
 __global__ void __launch_bounds__(kBlock)
@@ -521,7 +635,8 @@ extern "C" int na_sdf_input_fwd(const na_grid_desc* desc, const float* points, i
   NA_CHECK_ARG(points && table && out, "null pointer");
   NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
   const int64_t LF = (int64_t)desc->n_levels * desc->n_features;
-  NA_CHECK_ARG(ld >= LF + 3, "ld must be >= L*F + 3");
+  NA_CHECK_ARG(ld >= LF + (out_dtype == NA_BF16 ? 9 : 3),
+               "ld must be >= L*F + 3 (fp32) or L*F + 9 (bf16)");
   NA_CHECK_ARG(ld % (out_dtype == NA_BF16 ? 8 : 4) == 0, "ld must keep rows 16-byte aligned");
   NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
   FwdLaunch a{desc, points, n, vmin, vmax - vmin, table, out, ld, 1, active_levels,
@@ -573,4 +688,41 @@ extern "C" int na_sdf_taps_bwd(const na_grid_desc* desc, const float* points, in
     default: return dispatch_taps_bwd<8>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
                                          dy_dtype, ld, grad_table, active, s);
   }
+}
+
+extern "C" int na_sdf_taps_input_fwd(const na_grid_desc* desc, const float* points, int64_t n,
+                                     float vmin, float vmax, float eps, const void* table,
+                                     int32_t table_dtype, int32_t active_levels, void* x0_centre,
+                                     void* x0_taps, void* d_taps, int64_t ld,
+                                     na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(points && table && x0_centre && x0_taps && d_taps, "null pointer");
+  NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
+  const int64_t LF = (int64_t)desc->n_levels * desc->n_features;
+  NA_CHECK_ARG(ld >= LF + 9 && ld % 8 == 0, "ld must be >= L*F + 9 and a multiple of 8");
+  NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
+  dim3 grid((unsigned)ceil_div(n, kBlock), desc->n_levels);
+  hipStream_t s = (hipStream_t)stream;
+  const float vr = vmax - vmin;
+#define NA_TAPS_IN(FF, TT)                                                                   \
+  hipLaunchKernelGGL((sdf_taps_input_kernel<FF, TT>), grid, dim3(kBlock), 0, s, *desc, points, \
+                     n, vmin, vr, eps, (const TT*)table, (bf16_t*)x0_centre, (bf16_t*)x0_taps,  \
+                     (bf16_t*)d_taps, ld, active_levels)
+  if (table_dtype != NA_F32 && table_dtype != NA_BF16) {
+    set_error("taps input: table must be f32 or bf16");
+    return NA_ERR_UNSUPPORTED;
+  }
+  const bool tb = table_dtype == NA_BF16;
+  switch (desc->n_features) {
+    case 1: if (tb) NA_TAPS_IN(1, bf16_t); else NA_TAPS_IN(1, float); break;
+    case 2: if (tb) NA_TAPS_IN(2, bf16_t); else NA_TAPS_IN(2, float); break;
+    case 4: if (tb) NA_TAPS_IN(4, bf16_t); else NA_TAPS_IN(4, float); break;
+    default: if (tb) NA_TAPS_IN(8, bf16_t); else NA_TAPS_IN(8, float); break;
+  }
+#undef NA_TAPS_IN
+  NA_CHECK_LAUNCH();
+  return NA_OK;
 }

@@ -384,6 +384,53 @@ taps_points_kernel(const float* __restrict__ x, int64_t n, float eps, float* __r
   for (int k = 0; k < 4; ++k) out[(k * n + p) * 3 + d] = xv + kTap[k][d] * eps;
 }
 
+// Increment form of the 4-tap combination: ds[t] = sdf(x + eps k_t) - sdf(x),
+// sc = sdf(x) as evaluated, sdf = the value the reference passes in (the outside
+// fill included).  gradient = sum_t k_t ds_t / (4 eps) (sum k_t = 0);
+// hessian trace h = (sum_t ds_t / 2 + 2 (sc - sdf)) / eps^2, returned as h/3 per
+// component -- the reference's ((sum s_t)/2 - 2 sdf)/eps^2 without subtracting
+// nearly equal numbers.
+__global__ void __launch_bounds__(256)
+taps_delta_fwd_kernel(const float* __restrict__ ds, const float* __restrict__ sc,
+                      const float* __restrict__ sdf, int64_t n, float eps, float eps_sq,
+                      float* __restrict__ grad, float* __restrict__ hess) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float d1 = ds[p], d2 = ds[n + p], d3 = ds[2 * n + p], d4 = ds[3 * n + p];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float g = kTap[0][k] * d1 + kTap[1][k] * d2 + kTap[2][k] * d3 + kTap[3][k] * d4;
+    grad[p * 3 + k] = g / (4.0f * eps);
+  }
+  if (hess) {
+    const float h = ((d1 + d2 + d3 + d4) / 2.0f + 2.0f * (sc[p] - sdf[p])) / eps_sq;
+    const float h3 = h / 3.0f;
+    hess[p * 3] = h3;
+    hess[p * 3 + 1] = h3;
+    hess[p * 3 + 2] = h3;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+taps_delta_bwd_kernel(int64_t n, float eps, float eps_sq, const float* __restrict__ dgrad,
+                      const float* __restrict__ dhess, float* __restrict__ dds,
+                      float* __restrict__ dsc, float* __restrict__ dsdf) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float dh = 0.0f;
+  if (dhess) dh = (dhess[p * 3] + dhess[p * 3 + 1] + dhess[p * 3 + 2]) / 3.0f / eps_sq;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float v = 0.5f * dh;
+    if (dgrad)
+      v += (kTap[t][0] * dgrad[p * 3] + kTap[t][1] * dgrad[p * 3 + 1] +
+            kTap[t][2] * dgrad[p * 3 + 2]) / (4.0f * eps);
+    dds[t * n + p] = v;
+  }
+  if (dsc) dsc[p] = 2.0f * dh;
+  if (dsdf) dsdf[p] = -2.0f * dh;
+}
+
 __global__ void __launch_bounds__(256)
 taps_combine_fwd_kernel(const float* __restrict__ s, const float* __restrict__ sdf, int64_t n,
                         float eps, float eps_sq, float* __restrict__ grad,
@@ -852,6 +899,30 @@ extern "C" int na_sdf_losses_bwd(const float* grad, const float* hess, const uin
   NA_CHECK_ARG(w && (!dgrad || grad) && (!dhess || hess), "null pointer");
   hipLaunchKernelGGL(sdf_losses_bwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, grad,
                      hess, outside, n, n_per_ray, w, dgrad, add_dgrad, dhess);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_taps_delta_fwd(const float* ds, const float* sc, const float* sdf, int64_t n,
+                                 float eps, float eps_sq, float* grad, float* hess,
+                                 na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0, "bad size");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(ds && grad && (!hess || (sc && sdf)), "null pointer");
+  hipLaunchKernelGGL(taps_delta_fwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, ds, sc,
+                     sdf, n, eps, eps_sq, grad, hess);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_taps_delta_bwd(int64_t n, float eps, float eps_sq, const float* dgrad,
+                                 const float* dhess, float* dds, float* dsc, float* dsdf,
+                                 na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0, "bad size");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(dds, "null pointer");
+  hipLaunchKernelGGL(taps_delta_bwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, n, eps,
+                     eps_sq, dgrad, dhess, dds, dsc, dsdf);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -19,6 +19,7 @@ head, weight gradients and every per-ray quantity are float32.
 import ctypes
 
 import torch
+import torch.nn.functional as torch_F
 
 from . import _lib
 from ._lib import (NA_ACT_NONE, NA_ACT_RELU, NA_ACT_SIGMOID, NA_ACT_SOFTPLUS, NA_EPI_ATOMIC,
@@ -45,7 +46,8 @@ def _s(t):
 # ---------------------------------------------------------------------------
 def gemm(A, lda, trans_a, B, ldb, trans_b, M, N, K, C, ldc, epilogue=NA_EPI_STORE, bias=None,
          act=NA_ACT_NONE, beta=SOFTPLUS_BETA, dact=NA_ACT_NONE, dact_y=None, ld_dact=0,
-         colsum=None, split_k=1, r1_a=None, r1_b=None):
+         colsum=None, split_k=1, r1_a=None, r1_b=None, z_out=None, zc=None, zc_rows=0,
+         head_w=None, head_out=None):
     """C[m,n] = epi(sum_k A(m,k) B(n,k)) via na_gemm (see include/neuralangelo_amd.h)."""
     if A.dtype != B.dtype:
         raise RuntimeError(f"gemm operands must share a dtype ({A.dtype} vs {B.dtype})")
@@ -65,7 +67,26 @@ def gemm(A, lda, trans_a, B, ldb, trans_b, M, N, K, C, ldc, epilogue=NA_EPI_STOR
     a.r1_b = r1_b.data_ptr() if r1_b is not None else None
     a.colsum = colsum.data_ptr() if colsum is not None else None
     a.split_k = int(split_k)
+    if z_out is not None:
+        a.z_out, a.ld_z = z_out.data_ptr(), int(z_out.stride(0))
+    if zc is not None:
+        a.zc, a.ld_zc, a.zc_rows = zc.data_ptr(), int(zc.stride(0)), int(zc_rows)
+    if head_w is not None:
+        a.head_w, a.head_out = head_w.data_ptr(), head_out.data_ptr()
     _lib.call("na_gemm", ctypes.byref(a), _s(A))
+
+
+def head_parts(N):
+    """Number of partial sums per row written by a fused GEMM head over N columns."""
+    return 2 * ((N + 127) // 128)
+
+
+def head_sum(parts, bias=None):
+    """[nparts, M] partial head sums -> [M] (plus bias[0])."""
+    nparts, M = parts.shape
+    out = torch.empty(M, device=parts.device)
+    _lib.call("na_head_sum", _lib.ptr(parts), nparts, M, _lib.ptr(bias), _lib.ptr(out), _s(parts))
+    return out
 
 
 def _t(W):
@@ -138,20 +159,34 @@ def weight_norm(v, g):
 class SDFWeights:
     """Layer weights of one step in kernel layout (built from the module's tensors).
 
-    W0 reference columns are [x(3), enc(LF)]; the kernel layout is
-    [enc(LF), x(3), 0 pad] so the encoder block starts at 0 (aligned).
+    W0 reference columns are [x(3), enc(LF)]; the kernel layout puts the encoder
+    block first (aligned):  fp32: [enc | x | 0 pad];  bf16: [enc | W_hi | W_hi |
+    W_lo | 0 pad] against the input [enc | x_hi | x_lo | x_hi | 0], which carries
+    the point and its weights to ~16 mantissa bits through the bf16 MFMA.
     """
 
     def __init__(self, W0, b0, W1, b1, ws, bs, n_enc, cdt):
         self.n_enc = n_enc
-        self.k0 = n_enc + 3
-        self.k0p = _pad8(self.k0)
+        self.split = cdt == torch.bfloat16
+        self.k0p = sdf_input_width(n_enc, cdt)
         self.cdt = cdt
         dev = W0.device
-        W0p = torch.zeros(W0.shape[0], self.k0p, dtype=torch.float32, device=dev)
-        W0p[:, :n_enc] = W0[:, 3:]
-        W0p[:, n_enc:n_enc + 3] = W0[:, :3]
-        self.W0p = W0p.to(cdt).contiguous()
+        n = n_enc
+        Wx = W0[:, :3].float()
+        if self.split:
+            Whi = Wx.to(cdt)
+            Wlo = (Wx - Whi.float()).to(cdt)
+            W0p = torch.zeros(W0.shape[0], self.k0p, dtype=cdt, device=dev)
+            W0p[:, :n] = W0[:, 3:].to(cdt)
+            W0p[:, n:n + 3] = Whi
+            W0p[:, n + 3:n + 6] = Whi
+            W0p[:, n + 6:n + 9] = Wlo
+            self.W0p = W0p.contiguous()
+        else:
+            W0p = torch.zeros(W0.shape[0], self.k0p, dtype=torch.float32, device=dev)
+            W0p[:, :n] = W0[:, 3:]
+            W0p[:, n:n + 3] = Wx
+            self.W0p = W0p.to(cdt).contiguous()
         self.b0 = b0.contiguous().float()
         self.W1 = W1.to(cdt).contiguous() if W1 is not None else None
         self.b1 = b1.contiguous().float() if b1 is not None else None
@@ -161,9 +196,34 @@ class SDFWeights:
         self.hidden = W0.shape[0]
 
     def unpermute_dW0(self, dW0p):
-        """[hidden, K0p] kernel-layout gradient -> reference layout [hidden, 3+LF]."""
+        """[hidden, K0p] kernel-layout gradient -> reference layout [hidden, 3+LF].
+        With the split layout d/dW_x = sum dpre (x_hi + x_lo): columns hi + lo."""
         n = self.n_enc
-        return torch.cat([dW0p[:, n:n + 3], dW0p[:, :n]], dim=1)
+        dWx = dW0p[:, n:n + 3] + dW0p[:, n + 3:n + 6] if self.split else dW0p[:, n:n + 3]
+        return torch.cat([dWx, dW0p[:, :n]], dim=1)
+
+    def build_input(self, x):
+        return reference_input_to_kernel(x, self.n_enc, self.cdt)
+
+
+def sdf_input_width(n_enc, cdt):
+    """Row width of the SDF input buffer: enc + point (+ split point for bf16), padded to 8."""
+    return _pad8(n_enc + (9 if cdt == torch.bfloat16 else 3))
+
+
+def reference_input_to_kernel(x, n_enc, cdt):
+    """Reference-order SDF input [P, 3+LF] -> kernel-layout X0 [P, K0p] with torch ops
+    (differentiable: the bf16 casts pass gradients through unchanged, so the point
+    gradient is the sum of its hi and lo columns' gradients)."""
+    enc, pt = x[:, 3:], x[:, :3]
+    if cdt == torch.bfloat16:
+        hi = pt.to(cdt)
+        lo = (pt.float() - hi.float()).to(cdt)
+        X0 = torch.cat([enc.to(cdt), hi, lo, hi], dim=1)
+    else:
+        X0 = torch.cat([enc, pt], dim=1).to(cdt)
+    k0p = sdf_input_width(n_enc, cdt)
+    return torch_F.pad(X0, (0, k0p - X0.shape[1])) if X0.shape[1] < k0p else X0
 
 
 def sdf_input(enc, points, cdt, k0p):
@@ -333,25 +393,124 @@ class SDFCentreTapsFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dsdf, dfeat, ds_taps):
         pts, X0c, H0c, FB, X0t, H0t = ctx.saved_tensors
-        w, enc = ctx.w, ctx.enc
-        P, dev = pts.shape[0], pts.device
-        acc = _SDFGrads(w, dev, True)
-        dpre_c = _sdf_mlp_backward(X0c, H0c, FB, w, dsdf, dfeat, acc)
-        dpre_t = _sdf_mlp_backward(X0t, H0t, None, w,
-                                   ds_taps.reshape(4 * P, 1) if ds_taps is not None else None,
-                                   None, acc)
-        dtable = None
-        if ctx.needs_input_grad[1]:
-            dXc = _encoder_grad(dpre_c, w)
-            dXt = _encoder_grad(dpre_t, w)
-            dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
-            vmin, vmax = enc.vol_range
-            _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
-                      float(vmax), ctx.eps, _lib.ptr(dXc), _lib.ptr(dXt), _code(dXc), w.n_enc,
-                      _lib.ptr(dtable), int(enc.active_levels), _s(pts))
-        dW0 = w.unpermute_dW0(acc.dW0p)
-        return (None, dtable, dW0, acc.db0, acc.dW1, acc.db1, acc.dws.view(w.ws_shape),
-                acc.dbs.view(w.bs_shape), None, None, None, None)
+        return _centre_taps_backward(ctx, pts, X0c, H0c, FB, X0t, H0t, dsdf, dfeat,
+                                     ds_taps) + (None,)
+
+
+class SDFCentreTapsDeltaFunction(torch.autograd.Function):
+    """BF16 form of SDFCentreTapsFunction with the tap SDFs as increments.
+
+    Forward: sdf [P,1], feat [P,256] and ds [4,P] = sdf(x + eps k_i) - sdf(x).
+    The encoder writes the tap inputs' differences from the centre input in
+    fp32 before rounding; the tap GEMM multiplies those increments and its
+    epilogue forms softplus(z_c + dz) - softplus(z_c) from the centre's fp32
+    pre-activation z_c, so each increment keeps bf16 precision relative to its
+    own size.  The SDF head runs in fp32 inside the GEMM epilogues.
+    Backward: the derivative of ds_i is that of sdf(x + eps k_i) minus that of
+    sdf(x); the tap passes use the ordinary backward with the stored tap inputs."""
+
+    @staticmethod
+    def forward(ctx, points, table_params, W0, b0, W1, b1, ws, bs, enc, n_enc, eps):
+        _lib.require_cuda(points)
+        cdt = torch.bfloat16
+        w = SDFWeights(W0, b0, W1, b1, ws, bs, n_enc, cdt)
+        pts = points.contiguous().float()
+        P, dev, hid = pts.shape[0], pts.device, w.hidden
+        X0c = torch.empty(P, w.k0p, dtype=cdt, device=dev)
+        X0t = torch.empty(4 * P, w.k0p, dtype=cdt, device=dev)
+        D = torch.empty(4 * P, w.k0p, dtype=cdt, device=dev)
+        table = enc.table_for_forward()
+        vmin, vmax = enc.vol_range
+        _lib.call("na_sdf_taps_input_fwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
+                  float(vmax), float(eps), _lib.ptr(table), _code(table), int(enc.active_levels),
+                  _lib.ptr(X0c), _lib.ptr(X0t), _lib.ptr(D), w.k0p, _s(pts))
+        nparts = head_parts(hid)
+        H0c = torch.empty(P, hid, dtype=cdt, device=dev)
+        Zc = torch.empty(P, hid, dtype=torch.float32, device=dev)
+        parts_c = torch.empty(nparts, P, device=dev)
+        gemm(X0c, w.k0p, 0, w.W0p, w.k0p, 0, P, hid, w.k0p, H0c, hid, bias=w.b0,
+             act=NA_ACT_SOFTPLUS, z_out=Zc, head_w=w.ws.view(-1), head_out=parts_c)
+        sdf = head_sum(parts_c, w.bs).view(P, 1)
+        FB = torch.empty(P, RGB_LD, dtype=cdt, device=dev)
+        gemm(H0c, hid, 0, w.W1, hid, 0, P, hid, hid, FB, RGB_LD, bias=w.b1, act=NA_ACT_SOFTPLUS)
+        H0t = torch.empty(4 * P, hid, dtype=cdt, device=dev)
+        parts_t = torch.empty(nparts, 4 * P, device=dev)
+        gemm(D, w.k0p, 0, w.W0p, w.k0p, 0, 4 * P, hid, w.k0p, H0t, hid, act=NA_ACT_SOFTPLUS,
+             zc=Zc, zc_rows=P, head_w=w.ws.view(-1), head_out=parts_t)
+        ds = head_sum(parts_t).view(4, P)
+        del D, Zc
+        ctx.save_for_backward(pts, X0c, H0c, FB, X0t, H0t)
+        ctx.w, ctx.enc, ctx.eps = w, enc, float(eps)
+        return sdf, FB[:, :hid], ds
+
+    @staticmethod
+    def backward(ctx, dsdf, dfeat, dds):
+        pts, X0c, H0c, FB, X0t, H0t = ctx.saved_tensors
+        P = pts.shape[0]
+        if dds is not None:
+            dds = dds.contiguous().float()
+            dsdf = (torch.zeros(P, 1, device=pts.device) if dsdf is None else dsdf.float())
+            dsdf = dsdf.reshape(P, 1) - dds.sum(dim=0).view(P, 1)
+        return _centre_taps_backward(ctx, pts, X0c, H0c, FB, X0t, H0t, dsdf, dfeat, dds)
+
+
+def _centre_taps_backward(ctx, pts, X0c, H0c, FB, X0t, H0t, dsdf, dfeat, ds_taps):
+    """Shared backward of the centre + tap SDF nodes (gradient of sdf(x) and of the
+    four sdf(x + eps k_i)); returns the gradients of (points, table, W0, b0, W1, b1,
+    ws, bs, enc, cdt/n_enc, n_enc/eps)."""
+    w, enc = ctx.w, ctx.enc
+    P, dev = pts.shape[0], pts.device
+    acc = _SDFGrads(w, dev, True)
+    dpre_c = _sdf_mlp_backward(X0c, H0c, FB, w, dsdf, dfeat, acc)
+    dpre_t = _sdf_mlp_backward(X0t, H0t, None, w,
+                               ds_taps.reshape(4 * P, 1) if ds_taps is not None else None,
+                               None, acc)
+    dtable = None
+    if ctx.needs_input_grad[1]:
+        dXc = _encoder_grad(dpre_c, w)
+        dXt = _encoder_grad(dpre_t, w)
+        dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
+        vmin, vmax = enc.vol_range
+        _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
+                  float(vmax), ctx.eps, _lib.ptr(dXc), _lib.ptr(dXt), _code(dXc), w.n_enc,
+                  _lib.ptr(dtable), int(enc.active_levels), _s(pts))
+    dW0 = w.unpermute_dW0(acc.dW0p)
+    return (None, dtable, dW0, acc.db0, acc.dW1, acc.db1, acc.dws.view(w.ws_shape),
+            acc.dbs.view(w.bs_shape), None, None, None)
+
+
+class TapsDelta(torch.autograd.Function):
+    """ds [4,n] increments, sc [n] centre SDF, sdf [n] reference input -> gradient
+    [n,3] and hessian [n,3] (or None), as TapsCombine (modules.py:163-175)."""
+
+    @staticmethod
+    def forward(ctx, ds, sc, sdf, eps, eps_sq, training):
+        ds = ds.contiguous().float()
+        n = ds.shape[1]
+        dev = ds.device
+        grad = torch.empty(n, 3, device=dev)
+        hess = torch.empty(n, 3, device=dev) if training else None
+        sc_ = sc.reshape(-1).contiguous().float() if training else None
+        sdf_ = sdf.reshape(-1).contiguous().float() if training else None
+        _lib.call("na_taps_delta_fwd", _lib.ptr(ds), _lib.ptr(sc_), _lib.ptr(sdf_), n, float(eps),
+                  float(eps_sq), _lib.ptr(grad), _lib.ptr(hess), _s(ds))
+        ctx.n, ctx.eps, ctx.eps_sq, ctx.training = n, eps, eps_sq, training
+        if hess is None:
+            return grad, torch.empty(0, device=dev)
+        return grad, hess
+
+    @staticmethod
+    def backward(ctx, dgrad, dhess):
+        n = ctx.n
+        dev = dgrad.device if dgrad is not None else dhess.device
+        dds = torch.empty(4, n, device=dev)
+        dsc = torch.empty(n, device=dev) if ctx.training else None
+        dsdf = torch.empty(n, device=dev) if ctx.training else None
+        dg = dgrad.contiguous().float() if dgrad is not None else None
+        dh = dhess.contiguous().float() if (dhess is not None and ctx.training) else None
+        _lib.call("na_taps_delta_bwd", n, float(ctx.eps), float(ctx.eps_sq), _lib.ptr(dg),
+                  _lib.ptr(dh), _lib.ptr(dds), _lib.ptr(dsc), _lib.ptr(dsdf), _lib.stream_of(dds))
+        return dds, dsc, dsdf, None, None, None
 
 
 # ---------------------------------------------------------------------------

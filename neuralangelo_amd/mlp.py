@@ -99,11 +99,9 @@ class MLPforNeuralSDF(torch.nn.Module):
         shape = input.shape[:-1]
         x = input.reshape(-1, input.shape[-1])
         n_enc = x.shape[1] - 3
-        k0p = Fn._pad8(n_enc + 3)
-        # kernel layout [enc | x | 0 pad]; built with differentiable torch ops so the
-        # input gradient (if any) flows back through the column permutation
-        X0 = torch_F.pad(torch.cat([x[:, 3:], x[:, :3]], dim=1), (0, k0p - n_enc - 3))
-        X0 = X0.to(self.compute_dtype)
+        # kernel layout (encoder block first; split point columns in bf16), built with
+        # differentiable torch ops so the input gradient flows back through it
+        X0 = Fn.reference_input_to_kernel(x, n_enc, self.compute_dtype)
         W0, b0, W1, b1, ws, bs = self.weights()
         sdf, feat = Fn.SDFNetFunction.apply(X0, None, W0, b0, W1, b1, ws, bs, None,
                                             self.compute_dtype, with_feat, n_enc)

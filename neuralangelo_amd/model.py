@@ -172,12 +172,12 @@ class Model(torch.nn.Module):
         pts = points.reshape(-1, 3)
         # centre SDF + features and the four tap SDFs of the numerical gradient in
         # one node (model.py:160-166 and modules.py:157-175)
-        sdfs, feats, s_taps = self.neural_sdf.forward_with_taps(pts)  # [P,1], [P,256], [4,P]
+        sdfs, feats, taps, delta = self.neural_sdf.forward_with_taps(pts)  # [P,1], [P,256], [4,P]
         # sdfs[outside] = outside_val (model.py:163): the fill blocks the gradient
         fill = outside[:, None].expand(R, N).reshape(-1, 1)
         sdfs_filled = torch.where(fill, torch.full_like(sdfs, self.outside_val), sdfs)
-        gradients, hessians = self.neural_sdf.gradients_from_taps(s_taps, sdfs_filled,
-                                                                  self.training)
+        gradients, hessians = self.neural_sdf.gradients_from_taps(taps, sdfs, sdfs_filled,
+                                                                  self.training, delta)
         rgbs = self.neural_rgb.forward(points, gradients.view(R, N, 3), ray_unit, feats, None,
                                        normals_are_gradients=True)  # [R,N,3]
         alphas = Fn.NeusAlpha.apply(ray_unit, sdfs.view(R, N), gradients.view(R, N, 3), dists, far,

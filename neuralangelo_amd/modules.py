@@ -94,25 +94,40 @@ class NeuralSDF(torch.nn.Module):
     def forward_with_taps(self, points):
         """forward(points) and the four tap SDFs of compute_gradients in one autograd node.
 
-        points [P,3] -> sdf [P,1], feat [P,256] (buffer view), s_taps [4,P].  Equals
-        forward() followed by sdf(x + eps*k_i) for the tetrahedral k_i
-        (modules.py:157-175); the backward merges the five encoder scatters."""
+        points [P,3] -> (sdf [P,1], feat [P,256] buffer view, taps [4,P], delta).  FP32:
+        taps are sdf(x + eps*k_i) for the tetrahedral k_i (modules.py:157-175), delta
+        False.  BF16: taps are the increments sdf(x + eps*k_i) - sdf(x), delta True
+        (evaluated without cancellation).  The backward merges the five encoder
+        scatters of each point."""
         grad_cfg = _get(self.cfg_sdf, "gradient")
         if _get(grad_cfg, "mode", "numerical") != "numerical" or _get(grad_cfg, "taps", 4) != 4:
             raise NotImplementedError("only numerical 4-tap gradients are supported")
         W0, b0, W1, b1, ws, bs = self.mlp.weights()
         self.tcnn_encoding.active_levels = self.active_levels
         eps, _ = self.tap_eps()
-        return Fn.SDFCentreTapsFunction.apply(points.reshape(-1, 3), self.tcnn_encoding.params,
-                                              W0, b0, W1, b1, ws, bs, self.tcnn_encoding,
-                                              self.compute_dtype, self.tcnn_encoding.n_output_dims,
-                                              eps)
+        enc = self.tcnn_encoding
+        pts = points.reshape(-1, 3)
+        if self.compute_dtype == torch.bfloat16:
+            sdf, feat, ds = Fn.SDFCentreTapsDeltaFunction.apply(
+                pts, enc.params, W0, b0, W1, b1, ws, bs, enc, enc.n_output_dims, eps)
+            return sdf, feat, ds, True
+        sdf, feat, s_taps = Fn.SDFCentreTapsFunction.apply(
+            pts, enc.params, W0, b0, W1, b1, ws, bs, enc, self.compute_dtype, enc.n_output_dims,
+            eps)
+        return sdf, feat, s_taps, False
 
-    def gradients_from_taps(self, s_taps, sdf, training):
-        """The 4-tap combination of compute_gradients (modules.py:163-175) on given taps."""
+    def gradients_from_taps(self, taps, sdf_centre, sdf, training, delta):
+        """The 4-tap combination of compute_gradients (modules.py:163-175) on the taps
+        of forward_with_taps; sdf is the value the reference passes in (outside fill
+        included), sdf_centre the evaluated centre SDF."""
         eps, eps_sq = self.tap_eps()
-        grad, hess = Fn.TapsCombine.apply(s_taps, sdf.reshape(-1) if training else None, eps,
-                                          eps_sq, training)
+        if delta:
+            grad, hess = Fn.TapsDelta.apply(taps, sdf_centre.reshape(-1),
+                                            sdf.reshape(-1) if training else None, eps, eps_sq,
+                                            training)
+        else:
+            grad, hess = Fn.TapsCombine.apply(taps, sdf.reshape(-1) if training else None, eps,
+                                              eps_sq, training)
         return grad, (hess if training else None)
 
     @torch.no_grad()

@@ -12,9 +12,12 @@ from tests.helpers import oracle_field, oracle_params, randomize_model
 
 pytestmark = pytest.mark.gpu
 
-# tolerances: FP32 path (rounding only) and BF16 path (BF16 table, weights and activations)
-TOL = {torch.float32: dict(sdf=2e-5, grad=2e-3, rgb=2e-5),
-       torch.bfloat16: dict(sdf=5e-3, grad=5e-2, rgb=2e-2)}
+# Tolerances.  sdf and rgb: max absolute error.  grad: max |g - g_ref| over the mean
+# |g_ref| (the 4-tap difference quotient amplifies value errors by 1/(4 eps)).
+# The FP32 path is the reference's arithmetic; BF16 must stay within the 1e-3
+# SDF-error target of the mixed-precision step.
+TOL = {torch.float32: dict(sdf=2e-5, grad=3e-2, rgb=2e-5),
+       torch.bfloat16: dict(sdf=1e-3, grad=1e-1, rgb=2e-2)}
 
 
 @pytest.fixture(scope="module")
@@ -46,15 +49,18 @@ def test_error_against_fp64_oracle(setup, cuda, cdt):
     m.neural_sdf.active_levels = m.neural_sdf.anneal_levels = 16
     m.neural_sdf.set_normal_epsilon()
     with torch.no_grad():
-        sdf, feat, s_taps = m.neural_sdf.forward_with_taps(x.to(cuda))
-        grad, _ = m.neural_sdf.gradients_from_taps(s_taps, sdf, False)
+        sdf, feat, taps, delta = m.neural_sdf.forward_with_taps(x.to(cuda))
+        grad, _ = m.neural_sdf.gradients_from_taps(taps, sdf, sdf, False, delta)
         rgb = m.neural_rgb.forward(x.to(cuda)[:, None], grad[:, None], v.to(cuda), feat, None,
                                    normals_are_gradients=True)[:, 0]
     e_sdf = (sdf.cpu().double() - s_ref).abs().max().item()
-    gr = g_ref.norm(dim=-1, keepdim=True)
-    e_grad = ((grad.cpu().double() - g_ref).norm(dim=-1, keepdim=True) / gr).max().item()
+    gerr = (grad.cpu().double() - g_ref).norm(dim=-1)
+    gmean = g_ref.norm(dim=-1).mean().item()
+    e_grad = gerr.max().item() / gmean
     e_rgb = (rgb.float().cpu().double() - rgb_ref).abs().max().item()
-    print(f"\n{cdt}: max |sdf err| {e_sdf:.3e}  max rel grad err {e_grad:.3e}  "
+    cos = torch_F.cosine_similarity(grad.cpu().double(), g_ref, dim=-1)
+    print(f"\n{cdt}: max |sdf err| {e_sdf:.3e}  grad err max/mean|g| {e_grad:.3e} "
+          f"(median {gerr.median().item() / gmean:.3e}, min cos {cos.min().item():.5f})  "
           f"max |rgb err| {e_rgb:.3e}")
     t = TOL[cdt]
     assert e_sdf <= t["sdf"] and e_grad <= t["grad"] and e_rgb <= t["rgb"], (e_sdf, e_grad, e_rgb)
