@@ -293,7 +293,10 @@ def main():
     else:
         m32 = build_model(torch.float32, dev)
     if not args.no_mc:
-        extra["mc"] = mc_leg(m32, dev, res=args.mc_res, cpu=not args.no_cpu)
+        # a fixed volume: the FP32 network at its seeded initialisation (not the
+        # trained state above, whose surface changes from run to run)
+        extra["mc"] = mc_leg(build_model(torch.float32, dev), dev, res=args.mc_res,
+                             cpu=not args.no_cpu)
     del m32
     torch.cuda.empty_cache()
     cpu = cpu_baseline() if not args.no_cpu else None

@@ -59,8 +59,13 @@ struct GemmParams {
 __device__ inline float act_forward(int act, float v, float beta) {
   if (act == NA_ACT_RELU) return v > 0.0f ? v : 0.0f;
   if (act == NA_ACT_SOFTPLUS) {
-    float z = v * beta;
-    return z > 20.0f ? v : log1pf(expf(z)) / beta;
+    // torch semantics: linear above the threshold (beta v > 20), else
+    // log1p(exp(beta v)) / beta, evaluated as max(v, 0) + log(1 + e^{-beta|v|}) / beta
+    // with the hardware exp/log (1 + t lies in (1, 2]: no cancellation)
+    const float z = v * beta;
+    if (z > 20.0f) return v;
+    const float t = __expf(-fabsf(z));
+    return fmaxf(v, 0.0f) + __logf(1.0f + t) * (1.0f / beta);
   }
   if (act == NA_ACT_SIGMOID) return 1.0f / (1.0f + expf(-v));
   return v;
@@ -72,23 +77,34 @@ __device__ inline float act_forward(int act, float v, float beta) {
 __device__ inline float act_deriv_from_y(int act, float y, float beta) {
   if (act == NA_ACT_RELU) return y > 0.0f ? 1.0f : 0.0f;
   if (act == NA_ACT_SOFTPLUS) {
-    float z = y * beta;
-    return z > 20.0f ? 1.0f : -expm1f(-z);
+    const float z = y * beta;
+    return z > 20.0f ? 1.0f : 1.0f - __expf(-z);
   }
   if (act == NA_ACT_SIGMOID) return y * (1.0f - y);
   return 1.0f;
 }
 
 // softplus_beta(z0 + dz) - softplus_beta(z0) without cancellation: below the
-// linear threshold it is log1p(sigmoid(beta z0) * expm1(beta dz)) / beta.
+// linear threshold it is log1p(sigmoid(beta z0) * expm1(beta dz)) / beta.  The
+// small-argument branches of expm1 and log1p use their Taylor series (relative
+// error < 1e-9 for |x| < 1/32), the rest the hardware exp/log.
+__device__ inline float expm1_fast(float x) {
+  if (fabsf(x) < 0.03125f) return x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f))));
+  return __expf(x) - 1.0f;
+}
+__device__ inline float log1p_fast(float y) {
+  if (fabsf(y) < 0.03125f)
+    return y * (1.0f - y * (0.5f - y * (1.0f / 3.0f - y * (0.25f - y * 0.2f))));
+  return __logf(1.0f + y);
+}
 __device__ inline float softplus_delta(float z0, float dz, float beta) {
   const float z1 = z0 + dz;
   const float a0 = z0 * beta, a1 = z1 * beta;
   if (a0 > 20.0f && a1 > 20.0f) return z1 - z0;
   if (a0 > 20.0f || a1 > 20.0f)
     return act_forward(NA_ACT_SOFTPLUS, z1, beta) - act_forward(NA_ACT_SOFTPLUS, z0, beta);
-  const float sig = 1.0f / (1.0f + expf(-a0));
-  return log1pf(sig * expm1f(beta * dz)) / beta;
+  const float sig = __frcp_rn(1.0f + __expf(-a0));
+  return log1p_fast(sig * expm1_fast(beta * dz)) * (1.0f / beta);
 }
 
 __device__ inline float load_as_f32(const void* p, int dtype, int64_t i) {
@@ -273,19 +289,45 @@ template <> struct MfmaStep<float> {
 // accumulator block through LDS (`smem`, at least 4*32*65 floats, free to
 // reuse) so that every accumulator register is addressed statically and the
 // global stores run along rows (64 consecutive columns per wave instruction).
-__device__ inline void gemm_epilogue(const GemmParams& p, unsigned char* smem,
-                                     f32x16_t (&acc)[2][2], int64_t m0, int64_t n0, int wm,
-                                     int wn, int wave, int lane) {
+// Epilogue shared by the GEMM kernels: per half (i), each wave stages a 32 x 64
+// accumulator block through LDS (`smem`, at least 4*32*65 floats, free to
+// reuse) so that every accumulator register is addressed statically and the
+// global stores run along rows (64 consecutive columns per wave instruction).
+// Rows go in batches of RB; the per-row operand of the next batch (the stored
+// activation for dact, or the stored pre-activation in delta mode) is loaded
+// while the current batch is processed.
+__device__ __attribute__((always_inline)) inline float epi_row_operand(const EpiParams& e,
+                                                                       int64_t m, int64_t n,
+                                                                       bool ok) {
+  if (!ok) return 0.0f;
+  if (e.zc) return e.zc[(m % e.zc_rows) * e.ld_zc + n];
+  if (e.dact) return load_as_f32(e.dact_y, e.dact_dtype, m * e.ld_dact + n);
+  return 0.0f;
+}
+
+template <int RB, bool PIPE>
+__device__ __attribute__((always_inline)) inline void gemm_epilogue(
+    const GemmParams& p, unsigned char* smem, f32x16_t (&acc)[2][2], int64_t m0, int64_t n0,
+    int wm, int wn, int wave, int lane) {
   constexpr int CP = 65;
   float* Cw = reinterpret_cast<float*>(smem) + wave * 32 * CP;
   const EpiParams& e = p.e;
   const int64_t n = n0 + wn * 64 + lane;
+  const bool ncol = n < p.N;
   const bool head = e.head_w != nullptr;
+  const bool rowop = PIPE && (e.zc || e.dact);
   const int64_t part = 2 * (int64_t)blockIdx.y + wn;
-  const float hw = (head && n < p.N) ? e.head_w[n] : 0.0f;
+  // per-column operands, loaded once
+  const float hw = (head && ncol) ? e.head_w[n] : 0.0f;
+  const float bias_n = (e.bias && ncol) ? e.bias[n] : 0.0f;
+  const float r1b_n = (e.r1_a && ncol) ? e.r1_b[n] : 0.0f;
   float csum = 0.0f;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
+    const int64_t mb = m0 + wm * 64 + i * 32;
+    float nxt[RB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) nxt[u] = rowop ? epi_row_operand(e, mb + u, n, ncol && mb + u < p.M) : 0.0f;
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -295,29 +337,48 @@ __device__ inline void gemm_epilogue(const GemmParams& p, unsigned char* smem,
         Cw[row * CP + col] = acc[i][j][r];
       }
     __syncthreads();
-    const int64_t mb = m0 + wm * 64 + i * 32;
-    if (n < p.N) {
-      for (int rr = 0; rr < 32; ++rr) {
-        const int64_t m = mb + rr;
-        if (m >= p.M) break;
-        float v = Cw[rr * CP + lane];
-        float hv;
-        if (e.zc) {  // delta mode: v is an increment of the stored pre-activation
-          const float z0 = e.zc[(m % e.zc_rows) * e.ld_zc + n];
-          hv = softplus_delta(z0, v, e.beta);
-          v = act_forward(NA_ACT_SOFTPLUS, z0 + v, e.beta);
-        } else {
-          if (e.bias) v += e.bias[n];
-          if (e.z_out) e.z_out[m * e.ld_z + n] = v;
-          v = epilogue_value_nobias(e, m, n, v);
-          hv = v;
-        }
-        csum += v;
-        epilogue_store(e, m, n, v);
-        if (head) Cw[rr * CP + lane] = hw * hv;
+    for (int rb = 0; rb < 32; rb += RB) {
+      float cur[RB], v[RB], hv[RB];
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        cur[u] = nxt[u];
+        v[u] = Cw[(rb + u) * CP + lane];
       }
-    } else if (head) {
-      for (int rr = 0; rr < 32; ++rr) Cw[rr * CP + lane] = 0.0f;
+      if (rowop && rb + RB < 32) {
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+          const int64_t m = mb + rb + RB + u;
+          nxt[u] = epi_row_operand(e, m, n, ncol && m < p.M);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        const int64_t m = mb + rb + u;
+        const bool ok = ncol && m < p.M;
+        float x = v[u], h = 0.0f;
+        if (ok) {
+          if (!PIPE) cur[u] = epi_row_operand(e, m, n, true);
+          if (e.zc) {  // delta mode: x is an increment of the stored pre-activation
+            const float z0 = cur[u];
+            h = softplus_delta(z0, x, e.beta);
+            x = act_forward(NA_ACT_SOFTPLUS, z0 + x, e.beta);
+          } else {
+            x += bias_n;
+            if (e.z_out) e.z_out[m * e.ld_z + n] = x;
+            if (e.act) x = act_forward(e.act, x, e.beta);
+            if (e.r1_a) x += e.r1_a[m] * r1b_n;
+            if (e.dact) x *= act_deriv_from_y(e.dact, cur[u], e.beta);
+            h = x;
+          }
+          csum += x;
+          epilogue_store(e, m, n, x);
+        }
+        hv[u] = hw * h;
+      }
+      if (head) {
+#pragma unroll
+        for (int u = 0; u < RB; ++u) Cw[(rb + u) * CP + lane] = hv[u];
+      }
     }
     if (head) {
       __syncthreads();
@@ -330,7 +391,7 @@ __device__ inline void gemm_epilogue(const GemmParams& p, unsigned char* smem,
     }
     __syncthreads();
   }
-  if (e.colsum && n < p.N) atomic_add_f32(e.colsum + n, csum);
+  if (e.colsum && ncol) atomic_add_f32(e.colsum + n, csum);
 }
 
 // Two LDS stages: the next K-tile is fetched into registers while the MFMAs
@@ -348,7 +409,7 @@ template <typename T> struct GemmSmem {
 };
 
 template <typename T, bool TA, bool TB>
-__global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
+__global__ void __launch_bounds__(GNT, 3) gemm_kernel(GemmParams p) {
   constexpr int BK = GemmCfg<T>::BK;
   constexpr int LDK = GemmCfg<T>::LDK;
   __shared__ __attribute__((aligned(16))) unsigned char smem[GemmSmem<T>::BYTES];
@@ -396,7 +457,7 @@ __global__ void __launch_bounds__(GNT) gemm_kernel(GemmParams p) {
     buf ^= 1;
   }
 
-  gemm_epilogue(p, smem, acc, m0, n0, wm, wn, wave, lane);
+  gemm_epilogue<4, false>(p, smem, acc, m0, n0, wm, wn, wave, lane);
 }
 
 // out[m] = (bias ? bias[0] : 0) + sum_p parts[p*M + m], parts summed in order.
@@ -804,7 +865,7 @@ gemm_bf16_nn_kernel(GemmParams p) {
   }
   __builtin_amdgcn_s_waitcnt(0xF70);
   __syncthreads();  // all operand reads done before the epilogue reuses the stages
-  gemm_epilogue(p, smem, acc, m0, n0, wm, wn, wave, lane);
+  gemm_epilogue<8, true>(p, smem, acc, m0, n0, wm, wn, wave, lane);
 }
 
 // ---------------------------------------------------------------------------

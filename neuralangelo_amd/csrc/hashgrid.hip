@@ -315,47 +315,68 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
       }
     }
   }
-  float ac[F], at[4][F];
+  if (l >= active_levels) {  // coarse-to-fine: inactive levels are zero in all outputs
+    float z[F];
 #pragma unroll
-  for (int f = 0; f < F; ++f) {
-    ac[f] = 0.0f;
+    for (int f = 0; f < F; ++f) z[f] = 0.0f;
+    RowStore<F, bf16_t>::store(X0c + i * ld + l * F, z);
 #pragma unroll
-    for (int t = 0; t < 4; ++t) at[t][f] = 0.0f;
-  }
-  if (l < active_levels) {
-    const LevelParams p = level_params(d, l);
-    const TT* tab = table + (size_t)p.off * F;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      const float* xq = q == 0 ? xc : xt[q - 1];
-      uint32_t cell[3];
-      float fr[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        pos_fract(normalize_coord(xq[k], vmin, vrange), p.scale, cell[k], fr[k]);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
-                                         cell[2] + ((c >> 2) & 1), p);
-        const float w = corner_weight(c, fr);
-        float v[F];
-        EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          if (q == 0) ac[f] = fmaf(w, v[f], ac[f]);
-          else at[q - 1][f] = fmaf(w, v[f], at[q - 1][f]);
-        }
-      }
+    for (int t = 0; t < 4; ++t) {
+      const int64_t r = (int64_t)t * n + i;
+      RowStore<F, bf16_t>::store(X0t + r * ld + l * F, z);
+      RowStore<F, bf16_t>::store(D + r * ld + l * F, z);
     }
+    return;
   }
-  RowStore<F, bf16_t>::store(X0c + i * ld + l * F, ac);
+  const LevelParams p = level_params(d, l);
+  const TT* tab = table + (size_t)p.off * F;
+  // centre first, then one tap at a time (two F-vectors live at once)
+  float ac[F];
+  {
+    uint32_t cell[3];
+    float fr[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      pos_fract(normalize_coord(xc[k], vmin, vrange), p.scale, cell[k], fr[k]);
+#pragma unroll
+    for (int f = 0; f < F; ++f) ac[f] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                                       cell[2] + ((c >> 2) & 1), p);
+      const float w = corner_weight(c, fr);
+      float v[F];
+      EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
+#pragma unroll
+      for (int f = 0; f < F; ++f) ac[f] = fmaf(w, v[f], ac[f]);
+    }
+    RowStore<F, bf16_t>::store(X0c + i * ld + l * F, ac);
+  }
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
+    uint32_t cell[3];
+    float fr[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      pos_fract(normalize_coord(xt[t][k], vmin, vrange), p.scale, cell[k], fr[k]);
+    float at[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) at[f] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                                       cell[2] + ((c >> 2) & 1), p);
+      const float w = corner_weight(c, fr);
+      float v[F];
+      EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
+#pragma unroll
+      for (int f = 0; f < F; ++f) at[f] = fmaf(w, v[f], at[f]);
+    }
     const int64_t r = (int64_t)t * n + i;
     float dv[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) dv[f] = at[t][f] - ac[f];
-    RowStore<F, bf16_t>::store(X0t + r * ld + l * F, at[t]);
+    for (int f = 0; f < F; ++f) dv[f] = at[f] - ac[f];
+    RowStore<F, bf16_t>::store(X0t + r * ld + l * F, at);
     RowStore<F, bf16_t>::store(D + r * ld + l * F, dv);
   }
 }

@@ -223,6 +223,7 @@ mc_classify_kernel(McLattice g, uint8_t* __restrict__ cidx, uint16_t* __restrict
                    int* __restrict__ tile_v, int* __restrict__ tile_f) {
   __shared__ int wv[kMcItems][kMcWaves];
   __shared__ int wf[kMcWaves];
+  __shared__ float stage[2 * (kMcTile + 1) * 2];  // 2 planes x (R+1) rows x (cz+1), R*cz = 1024
   const uint32_t gt = blockIdx.x;
   const uint32_t tpb = (uint32_t)g.tpb;
   const uint32_t b = gt / tpb;
@@ -240,28 +241,58 @@ mc_classify_kernel(McLattice g, uint8_t* __restrict__ cidx, uint16_t* __restrict
   int cube[kMcItems], nv[kMcItems], inc[kMcItems];
   int nf = 0;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // Staged path: when z-rows divide the tile (cz | 1024 and 1024 | cy*cz) the tile
+  // is R = 1024/cz whole rows of one plane; its 2 x (R+1) x (cz+1) lattice points
+  // are loaded once, row by row with coalesced loads, and every corner is read
+  // from LDS.  Other blocks (ragged edges) load corners from global memory.
+  const bool staged = (kMcTile % k.cz == 0) && ((k.cy * k.cz) % kMcTile == 0);
+  const int64_t zp = k.cz + 1;  // LDS row pitch (points per staged row)
+  int64_t jrow0 = 0;
+  if (staged) {
+    const int64_t R = kMcTile / k.cz;
+    const int64_t i0 = base / (k.cy * k.cz);
+    jrow0 = (base % (k.cy * k.cz)) / k.cz;
+    for (int64_t row = w; row < 2 * (R + 1); row += kMcWaves) {
+      const int64_t a = row / (R + 1), rr = row % (R + 1);
+      const float* src = g.v + ((k.ox + i0 + a) * g.ny + (k.oy + jrow0 + rr)) * g.nz + k.oz;
+      for (int64_t kk = lane; kk < zp; kk += 64) stage[row * zp + kk] = src[kk];
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int it = 0; it < kMcItems; ++it) {
     const bool valid = (int64_t)cw.lc < k.ncell;
-    const float* p = valid ? g.v + ((k.ox + (int64_t)cw.i) * g.ny + (k.oy + (int64_t)cw.j)) * g.nz +
-                                 (k.oz + (int64_t)cw.k)
-                           : g.v;
     float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f;
-    if (valid) {
-      a00 = p[0];
-      a01 = p[sy];
-      a10 = p[sx];
-      a11 = p[sx + sy];
-    }
-    // values at k+1 come from the next lane when it holds cell k+1 of the same row
-    float b00 = __shfl_down(a00, 1), b01 = __shfl_down(a01, 1);
-    float b10 = __shfl_down(a10, 1), b11 = __shfl_down(a11, 1);
-    const bool next_ok = lane < 63 && (cw.k + 1 < cw.cz) && ((int64_t)cw.lc + 1 < k.ncell);
-    if (valid && !next_ok) {
-      b00 = p[1];
-      b01 = p[sy + 1];
-      b10 = p[sx + 1];
-      b11 = p[sx + sy + 1];
+    float b00, b01, b10, b11;
+    if (staged) {
+      const int64_t R1 = kMcTile / k.cz + 1;
+      const int64_t jj = (int64_t)cw.j - jrow0;
+      const float* s0 = stage + jj * zp + (int64_t)cw.k;  // plane i0, row jj
+      const float* s1 = s0 + R1 * zp;                      // plane i0 + 1
+      a00 = s0[0];  a01 = s0[zp];  a10 = s1[0];  a11 = s1[zp];
+      b00 = s0[1];  b01 = s0[zp + 1];  b10 = s1[1];  b11 = s1[zp + 1];
+    } else {
+      const float* p = valid ? g.v + ((k.ox + (int64_t)cw.i) * g.ny + (k.oy + (int64_t)cw.j)) *
+                                         g.nz + (k.oz + (int64_t)cw.k)
+                             : g.v;
+      if (valid) {
+        a00 = p[0];
+        a01 = p[sy];
+        a10 = p[sx];
+        a11 = p[sx + sy];
+      }
+      // values at k+1 come from the next lane when it holds cell k+1 of the same row
+      b00 = __shfl_down(a00, 1);
+      b01 = __shfl_down(a01, 1);
+      b10 = __shfl_down(a10, 1);
+      b11 = __shfl_down(a11, 1);
+      const bool next_ok = lane < 63 && (cw.k + 1 < cw.cz) && ((int64_t)cw.lc + 1 < k.ncell);
+      if (valid && !next_ok) {
+        b00 = p[1];
+        b01 = p[sy + 1];
+        b10 = p[sx + 1];
+        b11 = p[sx + sy + 1];
+      }
     }
     int c = 0, n = 0;
     if (valid) {

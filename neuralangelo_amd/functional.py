@@ -287,17 +287,21 @@ def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc):
     P, hid, cdt, dev = X0.shape[0], w.hidden, w.cdt, X0.device
     dsdf = (torch.zeros(P, 1, device=dev) if dsdf is None else dsdf).reshape(P, 1)
     dsdf = dsdf.contiguous().float()
-    dH0 = torch.zeros(P, hid, dtype=torch.float32, device=dev)
-    if FB is not None and dfeat is not None:
+    has_feat = FB is not None and dfeat is not None
+    if has_feat:
+        dH0 = torch.empty(P, hid, dtype=torch.float32, device=dev)
         dpre1 = act_grad(dfeat, FB, NA_ACT_SOFTPLUS, cdt, colsum=acc.db1, n_cols=hid)
         linear_weight_grad(dpre1, H0, hid, hid, acc.dW1)
         gemm(dpre1, hid, 0, _t(w.W1), hid, 0, P, hid, hid, dH0, hid)  # dH0 = dpre1 . W1
+    else:
+        # no feature gradient: dpre0 is written directly in the compute dtype
+        dH0 = torch.empty(P, hid, dtype=cdt, device=dev)
     # dpre0 = (dH0 + dsdf * ws) * softplus'(H0); dws, dbs; db0 = column sums of dpre0
     _lib.call("na_linear_narrow_bwd", _lib.ptr(H0), hid, _code(H0), P, hid, _lib.ptr(w.ws), 1,
               _lib.ptr(dsdf), _lib.ptr(acc.dws), _lib.ptr(acc.dbs), _lib.ptr(dH0), hid,
-              _code(dH0), 1, _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS, SOFTPLUS_BETA,
-              _lib.ptr(acc.db0), _s(X0))
-    dpre0 = dH0 if cdt == torch.float32 else dH0.to(cdt)
+              _code(dH0), int(has_feat), _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS,
+              SOFTPLUS_BETA, _lib.ptr(acc.db0), _s(X0))
+    dpre0 = dH0 if dH0.dtype == cdt else dH0.to(cdt)
     linear_weight_grad(dpre0, X0, w.k0p, w.k0p, acc.dW0p)
     return dpre0
 
