@@ -160,6 +160,12 @@ typedef struct na_gemm_args {
 
 int na_gemm(const na_gemm_args* args, na_stream_t stream);
 
+/* out[n] += sum_m X[m*ld + n] (fp32 accumulation, X of x_dtype): the bias gradient
+ * of a linear layer from its pre-activation gradient (torch.nn.Linear backward,
+ * mlp.py:55-69 / nerf_util.py:189-196).  Accumulates: the caller zeroes out. */
+int na_colsum(const void* X, int64_t ld, int32_t x_dtype, int64_t M, int64_t N, float* out,
+              na_stream_t stream);
+
 /* out[m] = (bias ? bias[0] : 0) + sum_{p < nparts} parts[p*M + m] (fixed order):
  * completes the fused narrow head of na_gemm (head_out partial sums). */
 int na_head_sum(const float* parts, int32_t nparts, int64_t M, const float* bias, float* out,

@@ -81,6 +81,7 @@ _SIGNATURES = {
     "na_hashgrid_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _I32, _P]),
     "na_hashgrid_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _P]),
     "na_gemm": (ctypes.c_int, [_PGEMM, _P]),
+    "na_colsum": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P]),
     "na_head_sum": (ctypes.c_int, [_P, _I32, _I64, _P, _P, _P]),
     "na_linear_narrow_fwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P, _I32, _I32, _P, _P]),
     "na_linear_narrow_bwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _I32, _P, _P, _P, _P, _I64, _I32, _I32, _P, _I64, _I32, _I32, _F32, _P, _P]),

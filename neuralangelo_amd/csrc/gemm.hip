@@ -460,6 +460,30 @@ __global__ void __launch_bounds__(GNT, 3) gemm_kernel(GemmParams p) {
   gemm_epilogue<4, false>(p, smem, acc, m0, n0, wm, wn, wave, lane);
 }
 
+// out[n] += sum_m X[m*ld + n] (fp32 accumulation): column sums for bias gradients.
+// One thread per column, workgroups over (column block, row slab); each thread
+// reduces its slab with 8 independent rows in flight, then adds one atomic, so
+// every output sees M / kColSlab atomics instead of one per GEMM tile.
+constexpr int kColSlab = 1024;
+
+template <typename TX>
+__global__ void __launch_bounds__(256)
+colsum_kernel(const TX* __restrict__ X, int64_t ld, int64_t M, int64_t N, float* __restrict__ out) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.y * kColSlab;
+  if (n >= N) return;
+  const int64_t r1 = r0 + kColSlab < M ? r0 + kColSlab : M;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int64_t m = r0;
+  for (; m + 8 <= r1; m += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += scalar_io<TX>::load(X + (m + u) * ld + n);
+  }
+  for (; m < r1; ++m) acc[0] += scalar_io<TX>::load(X + m * ld + n);
+  const float s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  atomic_add_f32(out + n, s);
+}
+
 // out[m] = (bias ? bias[0] : 0) + sum_p parts[p*M + m], parts summed in order.
 __global__ void __launch_bounds__(256)
 head_sum_kernel(const float* __restrict__ parts, int nparts, int64_t M,
@@ -869,6 +893,504 @@ gemm_bf16_nn_kernel(GemmParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent BF16 NN GEMM for tall-skinny products (M >> N, K of a few
+// hundred): C = epi(A[M,K] . B[N,K]^T) with the store epilogue.  Each
+// workgroup walks the tiles blockIdx.x, blockIdx.x + gridDim.x, ... and its
+// direct-to-LDS load pipeline (NS stages of 32-wide K-slices, same swizzled
+// layout as above) runs across tile boundaries: the first K-slices of the next
+// tile are in flight while the current tile's epilogue runs.  A tile is 128
+// rows x BN = 64*WN columns (BN = 256 covers a 256-wide layer, so A is read
+// from HBM once); 2 x WN waves each own 64 x 64 outputs in 32x32x16 MFMA
+// accumulators.
+//
+// Epilogue: each wave stages one 32 x 32 accumulator block at a time in a
+// private LDS region (pitch 36 floats = 4 x odd, so the 16 rows read by a
+// 16-lane group fall in distinct banks) and reads it back as 8 consecutive
+// columns of one row per lane; every global load / store of the epilogue then
+// moves 16 bytes, and bf16 results use the hardware pack-convert (RNE).
+// ---------------------------------------------------------------------------
+constexpr int kPEP = 36;
+constexpr int kPEpiFloats = 32 * kPEP;
+
+template <int WN> struct PersistTile {
+  static constexpr int NW = 2 * WN, BN = 64 * WN, ROWS = GBM + BN;
+  static constexpr int NS = WN == 4 ? 5 : 3;                 // 1 resp. 2 workgroups per CU
+  static constexpr int STAGE_EL = ROWS * kNNBK;
+  static constexpr int LOADS = ROWS / 16 / NW;               // 1 KB loads per wave per stage
+  static constexpr int LDS_BYTES = NS * STAGE_EL * 2 + NW * kPEpiFloats * 4;
+};
+
+typedef __attribute__((ext_vector_type(8))) float f32x8_t;
+
+// Stage loads: one 1 KB buffer-to-LDS load per 16 rows.  Lane l lands at slot
+// l & 3 of row l >> 2 and fetches the chunk that belongs there; its byte offset
+// within the tile (row * ld * 2 + chunk * 16) is fixed for the whole kernel.
+// The buffer's record count ends at the last row of the matrix, so rows past
+// M (or N) read zeros; K-tail chunks get an offset past the record count.
+constexpr uint32_t kOobOffset = 0x80000000u;
+
+__device__ inline __amdgpu_buffer_rsrc_t tile_rsrc(const bf16_t* X, int64_t ld, int64_t rows,
+                                                   int64_t r0, int64_t k0) {
+  const int64_t bytes = (rows - r0) * ld * 2 - k0 * 2;
+  const int n = bytes > 0x7fffffffLL ? 0x7fffffff : (bytes > 0 ? (int)bytes : 0);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(X + r0 * ld + k0), 0, n, 0x00020000);
+}
+
+template <int WN>
+__device__ inline void persist_issue(const __amdgpu_buffer_rsrc_t ra, const __amdgpu_buffer_rsrc_t rb,
+                                     const uint32_t (&voff)[PersistTile<WN>::LOADS],
+                                     const int (&kofs)[PersistTile<WN>::LOADS], int krem,
+                                     bf16_t* stage, int wave) {
+  using PT = PersistTile<WN>;
+#pragma unroll
+  for (int u = 0; u < PT::LOADS; ++u) {
+    const int q = wave * PT::LOADS + u;  // 16-row block of the stage: A rows, then B rows
+    const uint32_t off = kofs[u] < krem ? voff[u] : kOobOffset;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(q < GBM / 16 ? ra : rb,
+                                             (lds_void_t*)(stage + q * 16 * kNNBK), 16, off, 0, 0, 0);
+  }
+}
+
+// Wait until at most `ahead` stages (LOADS instructions each) issued after the
+// one needed now are outstanding.  vmcnt is an immediate operand.
+template <int LOADS>
+__device__ inline void persist_wait(int ahead) {
+  switch (ahead) {
+    case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    case 1: __builtin_amdgcn_s_waitcnt(0xF70 | (1 * LOADS)); break;
+    case 2: __builtin_amdgcn_s_waitcnt(0xF70 | (2 * LOADS)); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF70 | (3 * LOADS)); break;
+  }
+}
+
+__device__ inline void vload8(const float* p, float (&v)[8]) {
+  const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ inline void vload8(const bf16_t* p, float (&v)[8]) {
+  const uint4 a = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    v[2 * c] = __uint_as_float(w[c] << 16);
+    v[2 * c + 1] = __uint_as_float(w[c] & 0xffff0000u);
+  }
+}
+__device__ inline void vstore8(float* p, const float (&v)[8]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ inline void vstore8(bf16_t* p, const float (&v)[8]) {
+  f32x8_t x;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) x[c] = v[c];
+  *reinterpret_cast<bf16x8_t*>(p) = __builtin_convertvector(x, bf16x8_t);
+}
+// 8 columns of a row: vector access when FULL, else only the first `nc`
+template <bool FULL, typename T>
+__device__ inline void load_cols(const T* p, int nc, float (&v)[8]) {
+  if (FULL) { vload8(p, v); return; }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = c < nc ? scalar_io<T>::load(p + c) : 0.0f;
+}
+template <bool FULL, typename T>
+__device__ inline void store_cols(T* p, int nc, const float (&v)[8]) {
+  if (FULL) { vstore8(p, v); return; }
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (c < nc) scalar_io<T>::store(p + c, v[c]);
+}
+
+// Epilogue kinds of the persistent kernel (the products of the SDF and colour
+// MLPs); other combinations run on the general kernel.
+enum PersistKind : int {
+  PK_PLAIN = 0,         // C = acc
+  PK_BIAS_RELU = 1,     // C = relu(acc + b)
+  PK_BIAS_SOFTPLUS = 2, // C = softplus(acc + b) [z_out = acc + b] [head]
+  PK_DACT_RELU = 3,     // C = acc * [y > 0]     [colsum]
+  PK_DELTA = 4,         // C = softplus(zc + acc), head over softplus increments
+};
+
+// softplus_beta with a select for the linear region (no branch); ib = 1 / beta
+__device__ inline float softplus_sel(float v, float beta, float ib) {
+  const float z = v * beta;
+  const float r = fmaxf(v, 0.0f) + __logf(1.0f + __expf(-fabsf(z))) * ib;
+  return z > 20.0f ? v : r;
+}
+
+// softplus_delta with the small-argument series chosen by select; only the
+// linear-region case (either argument above the threshold) branches.
+__device__ inline float softplus_delta_sel(float z0, float dz, float beta, float ib) {
+  const float z1 = z0 + dz;
+  const float a0 = z0 * beta, a1 = z1 * beta;
+  if (a0 > 20.0f || a1 > 20.0f)
+    return (a0 > 20.0f && a1 > 20.0f) ? z1 - z0 : softplus_sel(z1, beta, ib) - softplus_sel(z0, beta, ib);
+  const float sig = __frcp_rn(1.0f + __expf(-a0));
+  const float u = beta * dz;
+  const float em = fabsf(u) < 0.03125f ? u * (1.0f + u * (0.5f + u * (1.0f / 6.0f + u * (1.0f / 24.0f))))
+                                       : __expf(u) - 1.0f;
+  const float y = sig * em;
+  const float lp = fabsf(y) < 0.03125f
+                       ? y * (1.0f - y * (0.5f - y * (1.0f / 3.0f - y * (0.25f - y * 0.2f))))
+                       : __logf(1.0f + y);
+  return lp * ib;
+}
+
+// One 32 x 32 block of the wave, staged in eb: lane (rl, cg) handles columns
+// nb .. nb+7 of rows mrow0 + rl and mrow0 + 16 + rl.  cs collects column sums,
+// hs the head partial of each of the two rows.
+template <int KIND, bool FULL>
+__device__ __attribute__((always_inline)) inline void persist_rows(
+    const GemmParams& p, const float* eb, int64_t mrow0, int64_t nb, int nc, int rl, int cg,
+    const float (&cb)[8], const float (&hw8)[8], float (&hs)[2], float (&cs)[8], int64_t zrow0,
+    float ib) {
+  const EpiParams& e = p.e;
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    const int rr = hh * 16 + rl;
+    const int64_t m = mrow0 + rr;
+    float x[8];
+    {
+      const float4 a = *reinterpret_cast<const float4*>(eb + rr * kPEP + cg * 8);
+      const float4 b = *reinterpret_cast<const float4*>(eb + rr * kPEP + cg * 8 + 4);
+      x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    }
+    if (!FULL && (m >= p.M || nc <= 0)) continue;
+    float hv[8];
+    if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) x[c] += cb[c];
+      if (KIND == PK_BIAS_SOFTPLUS && e.z_out) store_cols<FULL>(e.z_out + m * e.ld_z + nb, nc, x);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        x[c] = KIND == PK_BIAS_RELU ? fmaxf(x[c], 0.0f) : softplus_sel(x[c], e.beta, ib);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) hv[c] = x[c];
+    } else if constexpr (KIND == PK_DACT_RELU) {
+      float y[8];
+      if (e.dact_dtype == NA_BF16)
+        load_cols<FULL>((const bf16_t*)e.dact_y + m * e.ld_dact + nb, nc, y);
+      else
+        load_cols<FULL>((const float*)e.dact_y + m * e.ld_dact + nb, nc, y);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) x[c] = y[c] > 0.0f ? x[c] : 0.0f;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) cs[c] += x[c];  // columns past N hold 0
+    } else if constexpr (KIND == PK_DELTA) {
+      // row m reads stored row m mod zc_rows; zrow0 + rr is that index before the reduction
+      const int64_t off = zrow0 + rr;
+      const int64_t mz = e.zc_rows >= GBM ? (off >= e.zc_rows ? off - e.zc_rows : off)
+                                          : (int64_t)((uint32_t)m % (uint32_t)e.zc_rows);
+      float z0[8];
+      load_cols<FULL>(e.zc + mz * e.ld_zc + nb, nc, z0);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        hv[c] = softplus_delta_sel(z0[c], x[c], e.beta, ib);
+        x[c] = softplus_sel(z0[c] + x[c], e.beta, ib);
+      }
+    }
+    if (e.c_dtype == NA_BF16) store_cols<FULL>((bf16_t*)e.C + m * e.ldc + nb, nc, x);
+    else store_cols<FULL>((float*)e.C + m * e.ldc + nb, nc, x);
+    if constexpr (KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA) {
+      if (e.head_w) {
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s += hw8[c] * hv[c];
+        hs[hh] += s;
+      }
+    }
+  }
+}
+
+__device__ __attribute__((always_inline)) inline void persist_stage(float* eb, const f32x16_t& a, int lane) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) eb[(8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * kPEP + (lane & 31)] = a[r];
+}
+
+// The four 32 x 32 blocks go through one (rolled) loop so that each kind
+// carries a single copy of its element math per bounds mode.
+template <int WN, int KIND>
+__device__ __attribute__((always_inline)) inline void persist_epilogue(
+    const GemmParams& p, float* eb, f32x16_t (&acc)[2][2], int64_t m0, int64_t n0, int wm, int wn,
+    int lane, float (&csum)[2][8], int64_t zbase, float ib) {
+  // zbase = m0 mod zc_rows (increment mode)
+  const EpiParams& e = p.e;
+  const int rl = lane & 15, cg = lane >> 4;
+  const int64_t mw = m0 + wm * 64, nw = n0 + wn * 64;
+  const bool full = mw + 64 <= p.M && nw + 64 <= p.N;  // wave-uniform
+  constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA;
+  float hs[2] = {0.0f, 0.0f};
+#pragma unroll 1
+  for (int blk = 0; blk < 4; ++blk) {
+    const int i = blk >> 1, j = blk & 1;
+    if (blk == 0) persist_stage(eb, acc[0][0], lane);
+    else if (blk == 1) persist_stage(eb, acc[0][1], lane);
+    else if (blk == 2) persist_stage(eb, acc[1][0], lane);
+    else persist_stage(eb, acc[1][1], lane);
+    const int64_t nb = nw + j * 32 + cg * 8;
+    const int64_t ncl = p.N - nb;
+    const int nc = ncl >= 8 ? 8 : (ncl > 0 ? (int)ncl : 0);
+    float cb[8], hw8[8], cs[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) cb[c] = hw8[c] = cs[c] = 0.0f;
+    const int64_t zr = zbase + wm * 64 + i * 32;
+    if (full) {
+      if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) load_cols<true>(e.bias + nb, 8, cb);
+      if constexpr (kHead) { if (e.head_w) load_cols<true>(e.head_w + nb, 8, hw8); }
+      persist_rows<KIND, true>(p, eb, mw + i * 32, nb, 8, rl, cg, cb, hw8, hs, cs, zr, ib);
+    } else {
+      if (nc > 0) {
+        if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) load_cols<false>(e.bias + nb, nc, cb);
+        if constexpr (kHead) { if (e.head_w) load_cols<false>(e.head_w + nb, nc, hw8); }
+      }
+      persist_rows<KIND, false>(p, eb, mw + i * 32, nb, nc, rl, cg, cb, hw8, hs, cs, zr, ib);
+    }
+    if constexpr (KIND == PK_DACT_RELU) {
+      if (j == 0) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) csum[0][c] += cs[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) csum[1][c] += cs[c];
+      }
+    }
+    if constexpr (kHead) {
+      if (e.head_w && j == 1) {
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+          float s = hs[hh];
+          s += __shfl_xor(s, 16);
+          s += __shfl_xor(s, 32);
+          // parts = one per 64 columns, 2 * ceil(N / 128) of them (zero where past N)
+          const int64_t part = n0 / 64 + wn, m = mw + i * 32 + hh * 16 + rl;
+          if (cg == 0 && m < p.M && part < 2 * ((p.N + 127) / 128)) e.head_out[part * p.M + m] = s;
+          hs[hh] = 0.0f;
+        }
+      }
+    }
+  }
+}
+
+// Column sums of the workgroup: each wave reduces its 64 columns over the 16
+// lanes that share a column group, the two waves of a column block combine
+// through LDS, and one atomic per column leaves the workgroup.  Called by every
+// wave of the workgroup (it holds a barrier).
+template <int WN>
+__device__ inline void persist_flush_colsum(float* colsum, int64_t N, int64_t ncol0, int lane,
+                                            float (&csum)[2][8], float* epi_base, int wm, int wn) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      float s = csum[j][c];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      s += __shfl_xor(s, 4);
+      s += __shfl_xor(s, 8);
+      csum[j][c] = s;
+    }
+  float* partner = epi_base + (WN + wn) * kPEpiFloats;  // region of the wm = 1 wave
+  if (wm == 1 && (lane & 15) == 0) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) partner[(lane >> 4) * 16 + j * 8 + c] = csum[j][c];
+  }
+  __syncthreads();
+  if (wm == 0 && (lane & 15) == 0) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int64_t n = ncol0 + j * 32 + (lane >> 4) * 8 + c;
+        const float s = csum[j][c] + partner[(lane >> 4) * 16 + j * 8 + c];
+        if (n < N) atomic_add_f32(colsum + n, s);
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) csum[j][c] = 0.0f;
+}
+
+template <int WN, int KIND>
+__global__ void __launch_bounds__(128 * WN)
+gemm_bf16_nn_persist_kernel(GemmParams p) {
+  using PT = PersistTile<WN>;
+  constexpr int NS = PT::NS;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[PT::LDS_BYTES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  bf16_t* st0 = reinterpret_cast<bf16_t*>(smem);
+  float* eb = reinterpret_cast<float*>(smem + NS * PT::STAGE_EL * 2) + wave * kPEpiFloats;
+  const bf16_t* A = (const bf16_t*)p.A;
+  const bf16_t* B = (const bf16_t*)p.B;
+  // tile indices fit 32 bits (M < 2^31 is checked on the host)
+  const uint32_t n_mt = (uint32_t)((p.M + GBM - 1) / GBM);
+  const uint32_t n_tiles = n_mt * (uint32_t)((p.N + PT::BN - 1) / PT::BN);
+  const int nk = (int)((p.K + kNNBK - 1) / kNNBK);
+  const int64_t my_tiles = blockIdx.x < n_tiles ? (n_tiles - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  const int64_t total = my_tiles * nk;
+
+  // per-lane byte offsets of this wave's stage loads within a tile, and the K
+  // offset of the lane's chunk (a chunk is loaded when kofs < K - k0)
+  uint32_t voff[PT::LOADS];
+  int kofs[PT::LOADS];
+#pragma unroll
+  for (int u = 0; u < PT::LOADS; ++u) {
+    const int q = wave * PT::LOADS + u;
+    const int row = q * 16 + (lane >> 2);
+    const int chunk = (lane & 3) ^ ((row >> 2) & 3);
+    const int64_t ld = q < GBM / 16 ? p.lda : p.ldb;
+    const int lrow = q < GBM / 16 ? row : row - GBM;
+    voff[u] = (uint32_t)(lrow * ld * 2 + chunk * 16);
+    kofs[u] = chunk * 8;
+  }
+
+  // issue cursor: next stage to load
+  int64_t issued = 0;
+  uint32_t is_tile = blockIdx.x;
+  int is_t = 0, is_slot = 0;
+  int64_t is_m0 = (int64_t)(is_tile % n_mt) * GBM, is_n0 = (int64_t)(is_tile / n_mt) * PT::BN;
+  auto issue_next = [&]() {
+    const int64_t k0 = (int64_t)is_t * kNNBK;
+    const int64_t kr = p.K - k0;
+    persist_issue<WN>(tile_rsrc(A, p.lda, p.M, is_m0, k0), tile_rsrc(B, p.ldb, p.N, is_n0, k0), voff,
+                      kofs, kr < kNNBK ? (int)kr : kNNBK, st0 + is_slot * PT::STAGE_EL, wave);
+    ++issued;
+    is_slot = is_slot + 1 == NS ? 0 : is_slot + 1;
+    if (++is_t == nk) {
+      is_t = 0;
+      is_tile += gridDim.x;
+      is_m0 = (int64_t)(is_tile % n_mt) * GBM;
+      is_n0 = (int64_t)(is_tile / n_mt) * PT::BN;
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < NS - 1; ++s)
+    if (issued < total) issue_next();
+
+  float csum[2][8];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) csum[j][c] = 0.0f;
+  const float ib = 1.0f / p.e.beta;
+  int slot = 0;
+  int64_t g = 0;
+  for (int64_t jt = 0; jt < my_tiles; ++jt) {
+    const uint32_t tile = blockIdx.x + (uint32_t)jt * gridDim.x;
+    const int64_t m0 = (int64_t)(tile % n_mt) * GBM, n0 = (int64_t)(tile / n_mt) * PT::BN;
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    for (int t = 0; t < nk; ++t, ++g) {
+      persist_wait<PT::LOADS>((int)(issued - g - 1));
+      __builtin_amdgcn_s_barrier();  // stage g landed in every wave; stage g-1 consumed
+      if (issued < total) issue_next();
+      const bf16_t* sa = st0 + slot * PT::STAGE_EL;
+      const bf16_t* sb = sa + GBM * kNNBK;
+      slot = slot + 1 == NS ? 0 : slot + 1;
+#pragma unroll
+      for (int kk = 0; kk < kNNBK; kk += 16) {
+        bf16x8_t a[2], b[2];
+        const int ck = (kk >> 3) + (lane >> 5);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const int ra = wm * 64 + i * 32 + (lane & 31);
+          const int rb = wn * 64 + i * 32 + (lane & 31);
+          a[i] = *reinterpret_cast<const bf16x8_t*>(sa + ra * kNNBK + ((ck ^ ((ra >> 2) & 3)) << 3));
+          b[i] = *reinterpret_cast<const bf16x8_t*>(sb + rb * kNNBK + ((ck ^ ((rb >> 2) & 3)) << 3));
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    const int64_t zbase = KIND == PK_DELTA ? (int64_t)((uint32_t)m0 % (uint32_t)p.e.zc_rows) : 0;
+    persist_epilogue<WN, KIND>(p, eb, acc, m0, n0, wm, wn, lane, csum, zbase, ib);
+    if constexpr (KIND == PK_DACT_RELU) {
+      if (p.e.colsum) {
+        const uint32_t nxt = tile + gridDim.x;
+        if (jt + 1 == my_tiles || nxt / n_mt != tile / n_mt)
+          persist_flush_colsum<WN>(p.e.colsum, p.N, n0 + wn * 64, lane, csum,
+                                   reinterpret_cast<float*>(smem + NS * PT::STAGE_EL * 2), wm, wn);
+      }
+    }
+  }
+}
+
+// Workgroups per CU and CU count for the persistent launch, per device.
+struct PersistOcc { int cus = 0, wg2 = 0, wg4 = 0; };
+static PersistOcc g_persist_occ[64];
+
+static int persist_grid(int WN, int64_t n_tiles) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  PersistOcc& o = g_persist_occ[dev];
+  if (o.cus == 0) {
+    int cus = 0, b2 = 0, b4 = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, gemm_bf16_nn_persist_kernel<2, PK_PLAIN>, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b4, gemm_bf16_nn_persist_kernel<4, PK_PLAIN>, 512, 0) != hipSuccess) return 0;
+    o.wg2 = b2; o.wg4 = b4; o.cus = cus;
+  }
+  const int64_t per = (int64_t)o.cus * (WN == 4 ? o.wg4 : o.wg2);
+  if (per <= 0) return 0;
+  return (int)(n_tiles < per ? n_tiles : per);
+}
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+// Epilogue kind of the persistent kernel for these arguments, or -1 when the
+// general kernel runs them (other operand types, split-K, atomic epilogue,
+// unaligned rows, or epilogue terms outside the kinds).
+static int persist_kind(const na_gemm_args* a, int split) {
+  if (a->in_dtype != NA_BF16 || a->trans_a || a->trans_b) return -1;
+  if (a->epilogue != NA_EPI_STORE || split != 1 || a->K < 1) return -1;
+  if (a->M >= 0x7fffffffLL || a->lda * 2 * GBM >= 0x7fffffffLL || a->ldb * 2 * 256 >= 0x7fffffffLL) return -1;
+  const int cb = a->c_dtype == NA_BF16 ? 2 : 4;
+  if (!aligned16(a->C) || (a->ldc * cb) % 16) return -1;
+  if (a->r1_a) return -1;
+  if (a->bias && !aligned16(a->bias)) return -1;
+  if (a->head_w && !aligned16(a->head_w)) return -1;
+  if (a->z_out && (!aligned16(a->z_out) || (a->ld_z * 4) % 16)) return -1;
+  if (a->zc) {
+    if (!aligned16(a->zc) || (a->ld_zc * 4) % 16 || a->colsum || a->zc_rows >= 0x7fffffffLL) return -1;
+    return PK_DELTA;  // softplus, no bias / dact / z_out: checked by the caller
+  }
+  if (a->dact) {
+    const int db = a->dact_dtype == NA_BF16 ? 2 : 4;
+    if (a->dact != NA_ACT_RELU || a->act || a->bias || a->z_out || a->head_w) return -1;
+    if (!aligned16(a->dact_y) || (a->ld_dact * db) % 16) return -1;
+    return PK_DACT_RELU;
+  }
+  if (a->colsum) return -1;
+  if (!a->bias && !a->act && !a->z_out && !a->head_w) return PK_PLAIN;
+  if (a->bias && a->act == NA_ACT_RELU && !a->z_out && !a->head_w) return PK_BIAS_RELU;
+  if (a->bias && a->act == NA_ACT_SOFTPLUS) return PK_BIAS_SOFTPLUS;
+  return -1;
+}
+
+template <int WN>
+static void launch_persist(int kind, int grid, const GemmParams& p, hipStream_t s) {
+  const dim3 g(grid), b(128 * WN);
+  switch (kind) {
+    case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_PLAIN>), g, b, 0, s, p); break;
+    case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_BIAS_RELU>), g, b, 0, s, p); break;
+    case PK_BIAS_SOFTPLUS: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;
+    case PK_DACT_RELU: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_DACT_RELU>), g, b, 0, s, p); break;
+    default: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_DELTA>), g, b, 0, s, p); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
 template <typename T>
 static int launch_gemm(const na_gemm_args* a, const GemmParams& p, dim3 grid, hipStream_t s) {
   if (!a->trans_a && !a->trans_b) {
@@ -935,6 +1457,18 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   p.e.zc_rows = a->zc_rows; p.e.head_w = a->head_w; p.e.head_out = a->head_out;
   dim3 grid((unsigned)ceil_div(a->M, GBM), (unsigned)ceil_div(a->N, GBN), (unsigned)nsplit);
   hipStream_t s = (hipStream_t)stream;
+  const int kind = persist_kind(a, (int)nsplit);
+  if (kind >= 0) {
+    const int wn = a->N > 128 ? 4 : 2;
+    const int64_t n_tiles = ceil_div(a->M, GBM) * ceil_div(a->N, (int64_t)(64 * wn));
+    const int pg = persist_grid(wn, n_tiles);
+    if (pg > 0) {
+      if (wn == 4) launch_persist<4>(kind, pg, p, s);
+      else launch_persist<2>(kind, pg, p, s);
+      NA_CHECK_LAUNCH();
+      return NA_OK;
+    }
+  }
   if (a->in_dtype == NA_BF16) return launch_gemm<bf16_t>(a, p, grid, s);
   return launch_gemm<float>(a, p, grid, s);
 }
@@ -1064,6 +1598,26 @@ extern "C" int na_head_sum(const float* parts, int32_t nparts, int64_t M, const 
   NA_CHECK_ARG(parts && out, "null pointer");
   hipLaunchKernelGGL(head_sum_kernel, dim3((unsigned)ceil_div(M, (int64_t)256)), dim3(256), 0,
                      (hipStream_t)stream, parts, nparts, M, bias, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_colsum(const void* X, int64_t ld, int32_t x_dtype, int64_t M, int64_t N,
+                         float* out, na_stream_t stream) {
+  NA_CHECK_ARG(M >= 0 && N >= 0 && ld >= N, "bad sizes");
+  if (M == 0 || N == 0) return NA_OK;
+  NA_CHECK_ARG(X && out, "null pointer");
+  NA_CHECK_ARG(ceil_div(M, (int64_t)kColSlab) < 65536, "M too large");
+  dim3 grid((unsigned)ceil_div(N, (int64_t)256), (unsigned)ceil_div(M, (int64_t)kColSlab));
+  hipStream_t s = (hipStream_t)stream;
+  if (x_dtype == NA_BF16)
+    hipLaunchKernelGGL(colsum_kernel<bf16_t>, grid, dim3(256), 0, s, (const bf16_t*)X, ld, M, N, out);
+  else if (x_dtype == NA_F32)
+    hipLaunchKernelGGL(colsum_kernel<float>, grid, dim3(256), 0, s, (const float*)X, ld, M, N, out);
+  else {
+    set_error("colsum: x dtype must be f32 or bf16");
+    return NA_ERR_UNSUPPORTED;
+  }
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -12,9 +12,10 @@
 //
 // Layout: table is AoS per entry ([entries][F]), levels concatenated; the
 // output is [n, L*F] row-major (level-major within a row), which is what the
-// SDF MLP consumes.  Forward: one thread per (point, level), blockIdx.y =
-// level so a workgroup gathers from one level's table (keeps that level's
-// working set in L2/MALL).  Backward: one thread per (point, feature) so the F
+// SDF MLP consumes.  Forward: one thread per (point, level), level-minor, so
+// the L lanes of a point write its output row in one wave instruction (whole
+// cache lines; a level-major grid writes each row 16 bytes at a time from
+// workgroups launched far apart).  Backward: one thread per (point, feature) so the F
 // atomics of one corner are adjacent lanes of one instruction and coalesce
 // into a single L2 atomic request.
 #include "common.h"
@@ -179,10 +180,12 @@ __global__ void __launch_bounds__(kBlock)
 hashgrid_fwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
                     float vrange, const TT* __restrict__ table, TO* __restrict__ out, int64_t ld,
                     int write_extras, int active_levels) {
-  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  int l = blockIdx.y;
-  if (i >= n) return;
+  // level-minor mapping: the L lanes of one point write one output row together
   const int L = d.n_levels;
+  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = tid / L;
+  const int l = (int)(tid % L);
+  if (i >= n) return;
   TO* o = out + i * ld + l * F;
   if (write_extras && l == 0) {
     // fp32: [enc(L*F) | x(3) | zero pad up to ld]
@@ -286,10 +289,13 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
                       float vrange, float eps, const TT* __restrict__ table,
                       bf16_t* __restrict__ X0c, bf16_t* __restrict__ X0t,
                       bf16_t* __restrict__ D, int64_t ld, int active_levels) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int l = blockIdx.y;
-  if (i >= n) return;
+  // level-minor mapping: the L lanes of one point write its rows' level segments
+  // together, so each output row is written whole by one wave instruction
   const int L = d.n_levels;
+  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = tid / L;
+  const int l = (int)(tid % L);
+  if (i >= n) return;
   float xc[3], xt[4][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -510,7 +516,8 @@ struct FwdLaunch {
 
 template <int F, typename TT, typename TO>
 static int launch_fwd(const FwdLaunch& a) {
-  dim3 grid((unsigned)ceil_div(a.n, kBlock), a.d->n_levels);
+  NA_CHECK_ARG(a.n <= (int64_t)0x7fffffff * kBlock / a.d->n_levels, "n too large");
+  dim3 grid((unsigned)ceil_div(a.n * a.d->n_levels, (int64_t)kBlock));
   hipLaunchKernelGGL((hashgrid_fwd_kernel<F, TT, TO>), grid, dim3(kBlock), 0, a.s, *a.d, a.x, a.n,
                      a.vmin, a.vrange, (const TT*)a.table, (TO*)a.out, a.ld, a.extras, a.active);
   NA_CHECK_LAUNCH();
@@ -725,7 +732,8 @@ extern "C" int na_sdf_taps_input_fwd(const na_grid_desc* desc, const float* poin
   const int64_t LF = (int64_t)desc->n_levels * desc->n_features;
   NA_CHECK_ARG(ld >= LF + 9 && ld % 8 == 0, "ld must be >= L*F + 9 and a multiple of 8");
   NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
-  dim3 grid((unsigned)ceil_div(n, kBlock), desc->n_levels);
+  NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / desc->n_levels, "n too large");
+  dim3 grid((unsigned)ceil_div(n * desc->n_levels, (int64_t)kBlock));
   hipStream_t s = (hipStream_t)stream;
   const float vr = vmax - vmin;
 #define NA_TAPS_IN(FF, TT)                                                                   \

@@ -165,3 +165,53 @@ def test_gemm_rejects_misaligned(cuda):
     with pytest.raises(RuntimeError, match="split_k"):
         Fn.gemm(torch.zeros(8, 8, device=cuda), 8, 0, torch.zeros(8, 8, device=cuda), 8, 0, 8, 8, 8,
                 C, 10, split_k=2)
+
+
+@pytest.mark.parametrize("M,N,K", [(70001, 256, 144), (140001, 128, 256), (3001, 200, 280),
+                                   (999, 24, 136), (1000, 300, 256)])
+def test_gemm_nn_bf16_fused_epilogues(cuda, M, N, K):
+    """bf16 NN products with every store-epilogue term the MLPs use, at row counts that
+    give each workgroup several tiles, ragged M and N, and N below / above one tile."""
+    bf = torch.bfloat16
+    A = _rand((M, K), 20, 0.1).to(bf)
+    B = _rand((N, K), 21, 0.1).to(bf)
+    bias = _rand((N,), 22, 0.05)
+    hw = _rand((N,), 23)
+    acc = A.double() @ B.double().t()
+    nparts = Fn.head_parts(N)
+    Ad, Bd = A.to(cuda), B.to(cuda)
+
+    # forward: bias + softplus, pre-activation out, fused head, bf16 C
+    z = acc + bias.double()
+    y = _act(z, NA_ACT_SOFTPLUS)
+    C = torch.empty(M, N, dtype=bf, device=cuda)
+    zo = torch.empty(M, N, device=cuda)
+    ho = torch.full((nparts, M), float("nan"), device=cuda)
+    Fn.gemm(Ad, K, 0, Bd, K, 0, M, N, K, C, N, bias=bias.to(cuda), act=NA_ACT_SOFTPLUS,
+            z_out=zo, head_w=hw.to(cuda), head_out=ho)
+    assert ((C.cpu().double() - y).abs() <= 2 ** -7 * y.abs() + 1e-5).all()
+    assert (zo.cpu().double() - z).abs().max() < 1e-5
+    head = ho.cpu().double().sum(0)
+    assert (head - y @ hw.double()).abs().max() < 1e-3
+
+    # backward: relu derivative from a bf16 activation, column sums, fp32 C
+    Y = torch.relu(_rand((M, N), 24)).to(bf)
+    ref = acc * (Y.double() > 0)
+    C32 = torch.empty(M, N, device=cuda)
+    cs = torch.zeros(N, device=cuda)
+    Fn.gemm(Ad, K, 0, Bd, K, 0, M, N, K, C32, N, dact=NA_ACT_RELU, dact_y=Y.to(cuda), ld_dact=N,
+            colsum=cs)
+    assert (C32.cpu().double() - ref).abs().max() < 1e-5
+    assert (cs.cpu().double() - ref.sum(0)).abs().max() < 1e-5 * M ** 0.5 + 1e-4
+
+    # increment mode: x = softplus(zc[m % rows] + acc), head over softplus increments
+    rows = (M + 3) // 4
+    zc = _rand((rows, N), 25, 0.02)
+    z0 = zc.double()[torch.arange(M) % rows]
+    y1 = _act(z0 + acc, NA_ACT_SOFTPLUS)
+    dsp = y1 - _act(z0, NA_ACT_SOFTPLUS)
+    ho.fill_(float("nan"))
+    Fn.gemm(Ad, K, 0, Bd, K, 0, M, N, K, C, N, act=NA_ACT_SOFTPLUS, zc=zc.to(cuda), zc_rows=rows,
+            head_w=hw.to(cuda), head_out=ho)
+    assert ((C.cpu().double() - y1).abs() <= 2 ** -7 * y1.abs() + 1e-5).all()
+    assert (ho.cpu().double().sum(0) - dsp @ hw.double()).abs().max() < 1e-3
