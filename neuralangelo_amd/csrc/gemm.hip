@@ -1326,6 +1326,166 @@ gemm_bf16_nn_persist_kernel(GemmParams p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Weight-gradient GEMM: C[n, k] += sum_m A[m, n] B[m, k] (bf16 in, fp32 out),
+// A = [M, lda] and B = [M, ldb] row-major, M (the points) much larger than N
+// and K (the layer widths).  Each workgroup owns a C tile of 256 x 64*KB in
+// MFMA accumulators (8 waves: 4 along n x 2 along k, 2 x KB 32x32 blocks per
+// wave) and reduces one contiguous range of rows; one atomic add per output
+// element leaves the workgroup at the end.
+//
+// Rows stream through NS LDS stages of 32 rows (buffer-to-LDS loads, 16 B per
+// lane, rows past the range or columns past N / K read zeros).  A stage keeps
+// the row-major layout of memory with a row pitch of PA = 512 B (A) and PB
+// bytes (B), and 16-byte chunk c of row r stored at slot c ^ 4*(r % 4).  The
+// MFMA operands need 8 consecutive rows of one column per lane, which
+// ds_read_tr16_b64 delivers from this layout: in each 16-lane group, source
+// lane j addresses row r0 + j/4 and the 4 columns c0 + 4*(j%4) .. +3, and lane i
+// receives column c0 + i of rows r0 .. r0 + 3.  The swizzle puts the four rows
+// of a read on distinct banks.
+// ---------------------------------------------------------------------------
+typedef short v4s_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s_t lds_v4s_t;
+
+constexpr int kWgPA = 512;
+
+template <int KB> struct WgTile {
+  static constexpr int BK = 64 * KB;
+  static constexpr int PB = ((2 * BK + 255) / 256) * 256;
+  static constexpr int STAGE_BYTES = 32 * (kWgPA + PB);
+  static constexpr int NS = 150000 / STAGE_BYTES < 4 ? 150000 / STAGE_BYTES : 4;
+  static constexpr int LOADS = STAGE_BYTES / 1024 / 8;  // 1 KB loads per wave per stage
+  static_assert(STAGE_BYTES % 8192 == 0, "stage must split into 8 waves of 1 KB loads");
+};
+
+__device__ inline bf16x8_t tr_pair(unsigned char* p0, unsigned char* p1) {
+  const v4s_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s_t*)p0);
+  const v4s_t hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s_t*)p1);
+  typedef short v8s_t __attribute__((ext_vector_type(8)));
+  const v8s_t v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  return __builtin_bit_cast(bf16x8_t, v);
+}
+
+// One 32-row slice of A and B into a stage; rows from `me` on read zeros.
+template <int KB>
+__device__ inline void wgrad_issue(const bf16_t* A, int64_t lda, const bf16_t* B, int64_t ldb,
+                                   int64_t m, int64_t me, const uint32_t (&voff)[WgTile<KB>::LOADS],
+                                   const bool (&isa)[WgTile<KB>::LOADS], unsigned char* st, int wave) {
+  using WT = WgTile<KB>;
+  const int64_t rem = me - m;
+  const int64_t ab = rem * lda * 2, bb = rem * ldb * 2;
+  const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(A + m * lda), 0, ab > 0x7fffffffLL ? 0x7fffffff : (int)ab, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(B + m * ldb), 0, bb > 0x7fffffffLL ? 0x7fffffff : (int)bb, 0x00020000);
+#pragma unroll
+  for (int u = 0; u < WT::LOADS; ++u)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(isa[u] ? ra : rb,
+                                             (lds_void_t*)(st + (wave * WT::LOADS + u) * 1024), 16,
+                                             voff[u], 0, 0, 0);
+}
+
+template <int KB>
+__global__ void __launch_bounds__(512)
+gemm_bf16_wgrad_kernel(const bf16_t* __restrict__ A, int64_t lda, const bf16_t* __restrict__ B,
+                       int64_t ldb, int64_t M, int64_t N, int64_t K, int64_t rows_per,
+                       float* __restrict__ C, int64_t ldc) {
+  using WT = WgTile<KB>;
+  constexpr int NS = WT::NS, PB = WT::PB;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[NS * WT::STAGE_BYTES];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wn = wave >> 1, wk = wave & 1;
+  const int64_t n0 = (int64_t)blockIdx.y * 256, k0 = (int64_t)blockIdx.z * WT::BK;
+  const int64_t mb = (int64_t)blockIdx.x * rows_per;
+  const int64_t me = mb + rows_per < M ? mb + rows_per : M;
+  const int nsl = mb < me ? (int)((me - mb + 31) / 32) : 0;
+
+  // per-lane load offsets (bytes within a 32-row slice) and validity
+  uint32_t voff[WT::LOADS];
+  bool isa[WT::LOADS];
+#pragma unroll
+  for (int u = 0; u < WT::LOADS; ++u) {
+    const int off = (wave * WT::LOADS + u) * 1024 + lane * 16;  // byte offset in the stage
+    const bool a = off < 32 * kWgPA;
+    const int loc = a ? off : off - 32 * kWgPA;
+    const int pitch = a ? kWgPA : PB;
+    const int row = loc / pitch, slot = (loc % pitch) / 16;
+    const int chunk = slot ^ (4 * (row & 3));
+    const int64_t col = (a ? n0 : k0) + chunk * 8;
+    const bool ok = col < (a ? N : K) && chunk * 8 < (a ? 256 : WT::BK);
+    voff[u] = ok ? (uint32_t)(row * (a ? lda : ldb) * 2 + col * 2) : kOobOffset;
+    isa[u] = (wave * WT::LOADS + u) * 1024 < 32 * kWgPA;  // wave-uniform per u
+  }
+  auto issue = [&](int sl) {
+    wgrad_issue<KB>(A, lda, B, ldb, mb + (int64_t)sl * 32, me, voff, isa,
+                    smem + (sl % NS) * WT::STAGE_BYTES, wave);
+  };
+
+  // per-lane LDS read offsets: row (L>>5)*8 + ((L>>2)&3), column group 16*((L>>4)&1) + 4*(L&3)
+  const int rrow = (lane >> 5) * 8 + ((lane >> 2) & 3);
+  const int swz = 4 * ((lane >> 2) & 3);
+  const int csub = 16 * ((lane >> 4) & 1) + 4 * (lane & 3);  // column within a 32-block
+  int aoff[2], boff[KB];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = wn * 64 + i * 32 + csub;
+    aoff[i] = rrow * kWgPA + (((c >> 3) ^ swz) << 4) + (c & 7) * 2;
+  }
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    const int c = wk * 32 * KB + j * 32 + csub;
+    boff[j] = 32 * kWgPA + rrow * PB + (((c >> 3) ^ swz) << 4) + (c & 7) * 2;
+  }
+
+  f32x16_t acc[2][KB];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < KB; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+#pragma unroll
+  for (int s = 0; s < NS - 1; ++s)
+    if (s < nsl) issue(s);
+  for (int sl = 0; sl < nsl; ++sl) {
+    const int ahead = nsl - 1 - sl < NS - 2 ? nsl - 1 - sl : NS - 2;
+    persist_wait<WT::LOADS>(ahead);
+    __builtin_amdgcn_s_barrier();  // slice sl landed in every wave; slice sl-1 consumed
+    if (sl + NS - 1 < nsl) issue(sl + NS - 1);
+    unsigned char* st = smem + (sl % NS) * WT::STAGE_BYTES;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {  // 16 rows per MFMA step
+      bf16x8_t a[2], b[KB];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = tr_pair(st + aoff[i] + kk * 16 * kWgPA, st + aoff[i] + (kk * 16 + 4) * kWgPA);
+#pragma unroll
+      for (int j = 0; j < KB; ++j)
+        b[j] = tr_pair(st + boff[j] + kk * 16 * PB, st + boff[j] + (kk * 16 + 4) * PB);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  if (nsl == 0) return;
+  // acc[i][j][r]: n = n0 + wn*64 + i*32 + 8*(r>>2) + 4*(lane>>5) + (r&3), k = k0 + wk*32*KB + j*32 + (lane&31)
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    const int64_t k = k0 + wk * 32 * KB + j * 32 + (lane & 31);
+    if (k >= K) continue;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t n = n0 + wn * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (n < N) atomic_add_f32(C + n * ldc + k, acc[i][j][r]);
+      }
+  }
+}
+
 // Workgroups per CU and CU count for the persistent launch, per device.
 struct PersistOcc { int cus = 0, wg2 = 0, wg4 = 0; };
 static PersistOcc g_persist_occ[64];
@@ -1390,6 +1550,47 @@ static void launch_persist(int kind, int grid, const GemmParams& p, hipStream_t 
   }
 }
 
+// Weight-gradient products (both operands transposed, bf16, atomic fp32
+// output, no epilogue terms) run on the row-range reduction kernel.
+static bool wgrad_ok(const na_gemm_args* a) {
+  return a->in_dtype == NA_BF16 && a->trans_a && a->trans_b && a->epilogue == NA_EPI_ATOMIC &&
+         a->c_dtype == NA_F32 && !a->bias && !a->act && !a->dact && !a->r1_a && !a->colsum &&
+         !a->z_out && !a->zc && !a->head_w && a->K >= 1 && a->lda * 2 * 32 < 0x7fffffffLL &&
+         a->ldb * 2 * 32 < 0x7fffffffLL;
+}
+
+static int launch_wgrad(const na_gemm_args* a, hipStream_t s) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  // output [a->M, a->N] = sum over a->K rows; kernel names: n = output row, k = output column
+  const int64_t No = a->M, Ko = a->N, Mr = a->K;
+  const int kb = Ko <= 128 ? 2 : (Ko <= 192 ? 3 : (Ko <= 256 ? 4 : 5));
+  const int64_t bk = 64 * kb;
+  const int64_t tiles = ceil_div(No, (int64_t)256) * ceil_div(Ko, bk);
+  int64_t splits = cus / tiles;
+  if (splits < 1) splits = 1;
+  int64_t rows_per = ceil_div(ceil_div(Mr, splits), (int64_t)32) * 32;
+  if (rows_per < 128) rows_per = 128;
+  splits = ceil_div(Mr, rows_per);
+  NA_CHECK_ARG(splits < 0x7fffffffLL && ceil_div(No, (int64_t)256) < 65536 && ceil_div(Ko, bk) < 65536,
+               "weight-gradient grid too large");
+  const dim3 grid((unsigned)splits, (unsigned)ceil_div(No, (int64_t)256), (unsigned)ceil_div(Ko, bk));
+  const bf16_t* A = (const bf16_t*)a->A;
+  const bf16_t* B = (const bf16_t*)a->B;
+  float* C = (float*)a->C;
+#define NA_WG(KB) hipLaunchKernelGGL((gemm_bf16_wgrad_kernel<KB>), grid, dim3(512), 0, s, A, a->lda, B, \
+                                     a->ldb, Mr, No, Ko, rows_per, C, a->ldc)
+  if (kb == 2) NA_WG(2);
+  else if (kb == 3) NA_WG(3);
+  else if (kb == 4) NA_WG(4);
+  else NA_WG(5);
+#undef NA_WG
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
 // ---------------------------------------------------------------------------
 template <typename T>
 static int launch_gemm(const na_gemm_args* a, const GemmParams& p, dim3 grid, hipStream_t s) {
@@ -1423,7 +1624,7 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   const int vec = a->in_dtype == NA_BF16 ? 8 : 4;
   NA_CHECK_ARG(a->lda % vec == 0 && a->ldb % vec == 0, "leading dimensions must be multiples of 8 (bf16) / 4 (fp32)");
   NA_CHECK_ARG(((uintptr_t)a->A & 15) == 0 && ((uintptr_t)a->B & 15) == 0, "operands must be 16-byte aligned");
-  NA_CHECK_ARG(a->K % vec == 0, "K must be a multiple of 8 (bf16) / 4 (fp32)");
+  NA_CHECK_ARG(a->K % vec == 0 || wgrad_ok(a), "K must be a multiple of 8 (bf16) / 4 (fp32)");
   NA_CHECK_ARG(a->epilogue == NA_EPI_STORE || a->epilogue == NA_EPI_ATOMIC, "bad epilogue");
   NA_CHECK_ARG(a->epilogue == NA_EPI_ATOMIC || a->c_dtype == NA_F32 || a->c_dtype == NA_BF16, "bad c_dtype");
   NA_CHECK_ARG(a->epilogue == NA_EPI_STORE || a->c_dtype == NA_F32, "atomic epilogue needs fp32 C");
@@ -1457,6 +1658,7 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   p.e.zc_rows = a->zc_rows; p.e.head_w = a->head_w; p.e.head_out = a->head_out;
   dim3 grid((unsigned)ceil_div(a->M, GBM), (unsigned)ceil_div(a->N, GBN), (unsigned)nsplit);
   hipStream_t s = (hipStream_t)stream;
+  if (wgrad_ok(a)) return launch_wgrad(a, s);
   const int kind = persist_kind(a, (int)nsplit);
   if (kind >= 0) {
     const int wn = a->N > 128 ? 4 : 2;

@@ -215,3 +215,20 @@ def test_gemm_nn_bf16_fused_epilogues(cuda, M, N, K):
             head_w=hw.to(cuda), head_out=ho)
     assert ((C.cpu().double() - y1).abs() <= 2 ** -7 * y1.abs() + 1e-5).all()
     assert (ho.cpu().double().sum(0) - dsp @ hw.double()).abs().max() < 1e-3
+
+
+@pytest.mark.parametrize("M,N,K", [(60001, 256, 280), (50000, 256, 256), (20000, 256, 144),
+                                   (777, 64, 136), (40000, 296, 24), (33, 256, 256)])
+def test_weight_grad_row_reduction(cuda, M, N, K):
+    """dW[n, k] += sum_m dpre[m, n] X[m, k] in bf16 with fp32 accumulation: every
+    output-tile width, ragged row ranges, N above one tile, and accumulation into a
+    non-zero output."""
+    bf = torch.bfloat16
+    dpre = _rand((M, N), 30, 0.1).to(bf)
+    X = _rand((M, K), 31, 0.1).to(bf)
+    dW0 = _rand((N, K), 32)
+    ref = dW0.double() + dpre.double().t() @ X.double()
+    dW = dW0.to(cuda)
+    Fn.linear_weight_grad(dpre.to(cuda), X.to(cuda), K, K, dW)
+    err = (dW.cpu().double() - ref).abs().max().item()
+    assert err < 1e-3 * (M ** 0.5) * 0.01 + 1e-4, err
