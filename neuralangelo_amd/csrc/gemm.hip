@@ -498,6 +498,19 @@ head_sum_kernel(const float* __restrict__ parts, int nparts, int64_t M,
 // ---------------------------------------------------------------------------
 // narrow layers (N <= 4)
 // ---------------------------------------------------------------------------
+// 8 consecutive values to memory in one 16-byte (bf16) or two 16-byte (fp32) stores
+__device__ inline void store8v(float* p, const float (&v)[8]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ inline void store8v(bf16_t* p, const float (&v)[8]) {
+  typedef __attribute__((ext_vector_type(8))) float f8_t;
+  f8_t x;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) x[c] = v[c];
+  *reinterpret_cast<bf16x8_t*>(p) = __builtin_convertvector(x, bf16x8_t);
+}
+
 constexpr int kNarrowMax = 4;
 
 // Vector load of 8 consecutive elements as floats (16 B of bf16 / 32 B of fp32).
@@ -579,8 +592,11 @@ narrow_fwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
   }
 }
 
-template <typename TX, typename TDX, int KC>
-__global__ void __launch_bounds__(256)
+// Workgroup size of the narrow backward (waves).
+template <int NM> struct NarrowBwdCfg { static constexpr int WPG = 8; };
+
+template <typename TX, typename TDX, int KC, int NM>
+__global__ void __launch_bounds__(64 * NarrowBwdCfg<NM>::WPG)
 narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
                       const float* __restrict__ W, int N, const float* __restrict__ dpre,
                       float* __restrict__ dW, float* __restrict__ db, TDX* __restrict__ dX,
@@ -590,10 +606,12 @@ narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
   const int hl = lane & 31, half = lane >> 5;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  float w[kNarrowMax][KC][8], gw[kNarrowMax][KC][8], cs[KC][8];
-  float gb[kNarrowMax] = {0.f, 0.f, 0.f, 0.f};
+  float w[NM][KC][8], gw[NM][KC][8], cs[KC][8];
+  float gb[NM];
 #pragma unroll
-  for (int n = 0; n < kNarrowMax; ++n)
+  for (int n = 0; n < NM; ++n) gb[n] = 0.0f;
+#pragma unroll
+  for (int n = 0; n < NM; ++n)
 #pragma unroll
     for (int c = 0; c < KC; ++c)
 #pragma unroll
@@ -603,56 +621,74 @@ narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
         gw[n][c][e] = 0.0f;
         if (n == 0) cs[c][e] = 0.0f;
       }
-  for (int64_t m = 2 * wave + half; m < M; m += 2 * nwaves) {
-    float d[kNarrowMax];
+  // Rows go in batches of NB per half-wave: all loads of a batch are issued
+  // before its arithmetic.  When Y is X (the layer's own stored activation) the
+  // derivative reads the loaded X values.
+  constexpr int NB = (KC > 1 && NM > 1) ? 1 : 2;  // keeps the widest variant within registers
+  const int64_t stride = 2 * nwaves;
+  const bool y_is_x = dact && (const void*)X == Y && ldy == ldx &&
+                      y_dtype == (sizeof(TX) == 2 ? NA_BF16 : NA_F32);
+  for (int64_t mb = 2 * wave + half; mb < M; mb += stride * NB) {
+    float d[NB][NM], xv[NB][KC][8], yv[NB][KC][8], ov[NB][KC][8];
 #pragma unroll
-    for (int n = 0; n < kNarrowMax; ++n) {
-      d[n] = n < N ? dpre[m * N + n] : 0.0f;
-      gb[n] += d[n];
-    }
+    for (int r = 0; r < NB; ++r) {
+      const int64_t m = mb + r * stride;
+      const bool ok = m < M;
 #pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      const int64_t k = 8 * hl + 256 * c;
-      if (k < K) {
-        float x[8];
-        load8(X + m * ldx + k, x);
-        float dx[8];
+      for (int n = 0; n < NM; ++n) d[r][n] = (ok && n < N) ? dpre[m * N + n] : 0.0f;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          dx[e] = 0.0f;
+      for (int c = 0; c < KC; ++c) {
+        const int64_t k = 8 * hl + 256 * c;
+        const bool kok = ok && k < K;
 #pragma unroll
-          for (int n = 0; n < kNarrowMax; ++n) {
-            gw[n][c][e] = fmaf(d[n], x[e], gw[n][c][e]);
-            dx[e] = fmaf(d[n], w[n][c][e], dx[e]);
-          }
-        }
-        if (dX) {
-          TDX* pdx = dX + m * lddx + k;
-          if (add_dx) {
-            float old[8];
-            load8(pdx, old);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) dx[e] += old[e];
-          }
-          if (dact) {
-            float yv[8];
-            if (y_dtype == NA_BF16) load8((const bf16_t*)Y + m * ldy + k, yv);
-            else load8((const float*)Y + m * ldy + k, yv);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) dx[e] *= act_deriv_from_y(dact, yv[e], beta);
-          }
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            scalar_io<TDX>::store(pdx + e, dx[e]);
-            cs[c][e] += dx[e];
+        for (int e = 0; e < 8; ++e) xv[r][c][e] = yv[r][c][e] = ov[r][c][e] = 0.0f;
+        if (kok) {
+          load8(X + m * ldx + k, xv[r][c]);
+          if (dX && add_dx) load8(dX + m * lddx + k, ov[r][c]);
+          if (dX && dact && !y_is_x) {
+            if (y_dtype == NA_BF16) load8((const bf16_t*)Y + m * ldy + k, yv[r][c]);
+            else load8((const float*)Y + m * ldy + k, yv[r][c]);
           }
         }
       }
     }
-  }
-  // combine the two half-waves (same k ranges), then one atomic per element
 #pragma unroll
-  for (int n = 0; n < kNarrowMax; ++n) {
+    for (int r = 0; r < NB; ++r) {
+      const int64_t m = mb + r * stride;
+      if (m >= M) continue;
+#pragma unroll
+      for (int n = 0; n < NM; ++n) gb[n] += d[r][n];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        const int64_t k = 8 * hl + 256 * c;
+        if (k >= K) continue;
+        float dx[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          dx[e] = ov[r][c][e];
+#pragma unroll
+          for (int n = 0; n < NM; ++n) {
+            gw[n][c][e] = fmaf(d[r][n], xv[r][c][e], gw[n][c][e]);
+            dx[e] = fmaf(d[r][n], w[n][c][e], dx[e]);
+          }
+        }
+        if (dX) {
+          if (dact) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              dx[e] *= act_deriv_from_y(dact, y_is_x ? xv[r][c][e] : yv[r][c][e], beta);
+          }
+          store8v(dX + m * lddx + k, dx);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) cs[c][e] += dx[e];
+        }
+      }
+    }
+  }
+  // combine the two half-waves (same k ranges), then the waves of the
+  // workgroup through LDS; one atomic per element leaves the workgroup
+#pragma unroll
+  for (int n = 0; n < NM; ++n) {
     gb[n] += __shfl_xor(gb[n], 32);
 #pragma unroll
     for (int c = 0; c < KC; ++c)
@@ -663,7 +699,44 @@ narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
   for (int c = 0; c < KC; ++c)
 #pragma unroll
     for (int e = 0; e < 8; ++e) cs[c][e] += __shfl_xor(cs[c][e], 32);
-  if (half == 0) {
+  constexpr int PER = (NM + 1) * KC * 256 + NM;  // partials of one wave
+  constexpr int WPG = NarrowBwdCfg<NM>::WPG;
+  __shared__ float red[(WPG - 1) * PER];
+  const int wv = threadIdx.x >> 6;
+  if (wv > 0 && half == 0) {
+    float* r = red + (wv - 1) * PER;
+#pragma unroll
+    for (int n = 0; n < NM; ++n)
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) r[(n * KC + c) * 256 + 8 * hl + e] = gw[n][c][e];
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) r[(NM * KC + c) * 256 + 8 * hl + e] = cs[c][e];
+    if (hl == 0)
+#pragma unroll
+      for (int n = 0; n < NM; ++n) r[(NM + 1) * KC * 256 + n] = gb[n];
+  }
+  __syncthreads();
+  if (wv == 0 && half == 0) {
+#pragma unroll
+    for (int q = 0; q < WPG - 1; ++q) {
+      const float* r = red + q * PER;
+#pragma unroll
+      for (int n = 0; n < NM; ++n)
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) gw[n][c][e] += r[(n * KC + c) * 256 + 8 * hl + e];
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) cs[c][e] += r[(NM * KC + c) * 256 + 8 * hl + e];
+#pragma unroll
+      for (int n = 0; n < NM; ++n) gb[n] += r[(NM + 1) * KC * 256 + n];
+    }
     for (int n = 0; n < N; ++n) {
 #pragma unroll
       for (int c = 0; c < KC; ++c)
@@ -1007,7 +1080,7 @@ enum PersistKind : int {
   PK_PLAIN = 0,         // C = acc
   PK_BIAS_RELU = 1,     // C = relu(acc + b)
   PK_BIAS_SOFTPLUS = 2, // C = softplus(acc + b) [z_out = acc + b] [head]
-  PK_DACT_RELU = 3,     // C = acc * [y > 0]     [colsum]
+  PK_DACT = 3,          // C = (acc [+ r1_a r1_b]) * act'(y), relu or softplus  [colsum]
   PK_DELTA = 4,         // C = softplus(zc + acc), head over softplus increments
 };
 
@@ -1066,14 +1139,24 @@ __device__ __attribute__((always_inline)) inline void persist_rows(
         x[c] = KIND == PK_BIAS_RELU ? fmaxf(x[c], 0.0f) : softplus_sel(x[c], e.beta, ib);
 #pragma unroll
       for (int c = 0; c < 8; ++c) hv[c] = x[c];
-    } else if constexpr (KIND == PK_DACT_RELU) {
+    } else if constexpr (KIND == PK_DACT) {
       float y[8];
       if (e.dact_dtype == NA_BF16)
         load_cols<FULL>((const bf16_t*)e.dact_y + m * e.ld_dact + nb, nc, y);
       else
         load_cols<FULL>((const float*)e.dact_y + m * e.ld_dact + nb, nc, y);
+      if (e.r1_a) {
+        const float ra = e.r1_a[m];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) x[c] = y[c] > 0.0f ? x[c] : 0.0f;
+        for (int c = 0; c < 8; ++c) x[c] = fmaf(ra, cb[c], x[c]);  // cb holds r1_b here
+      }
+      if (e.dact == NA_ACT_RELU) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) x[c] = y[c] > 0.0f ? x[c] : 0.0f;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) x[c] *= act_deriv_from_y(NA_ACT_SOFTPLUS, y[c], e.beta);
+      }
 #pragma unroll
       for (int c = 0; c < 8; ++c) cs[c] += x[c];  // columns past N hold 0
     } else if constexpr (KIND == PK_DELTA) {
@@ -1136,16 +1219,18 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue(
     const int64_t zr = zbase + wm * 64 + i * 32;
     if (full) {
       if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) load_cols<true>(e.bias + nb, 8, cb);
+      if constexpr (KIND == PK_DACT) { if (e.r1_a) load_cols<true>(e.r1_b + nb, 8, cb); }
       if constexpr (kHead) { if (e.head_w) load_cols<true>(e.head_w + nb, 8, hw8); }
       persist_rows<KIND, true>(p, eb, mw + i * 32, nb, 8, rl, cg, cb, hw8, hs, cs, zr, ib);
     } else {
       if (nc > 0) {
         if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) load_cols<false>(e.bias + nb, nc, cb);
+        if constexpr (KIND == PK_DACT) { if (e.r1_a) load_cols<false>(e.r1_b + nb, nc, cb); }
         if constexpr (kHead) { if (e.head_w) load_cols<false>(e.head_w + nb, nc, hw8); }
       }
       persist_rows<KIND, false>(p, eb, mw + i * 32, nb, nc, rl, cg, cb, hw8, hs, cs, zr, ib);
     }
-    if constexpr (KIND == PK_DACT_RELU) {
+    if constexpr (KIND == PK_DACT) {
       if (j == 0) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) csum[0][c] += cs[c];
@@ -1315,7 +1400,7 @@ gemm_bf16_nn_persist_kernel(GemmParams p) {
     }
     const int64_t zbase = KIND == PK_DELTA ? (int64_t)((uint32_t)m0 % (uint32_t)p.e.zc_rows) : 0;
     persist_epilogue<WN, KIND>(p, eb, acc, m0, n0, wm, wn, lane, csum, zbase, ib);
-    if constexpr (KIND == PK_DACT_RELU) {
+    if constexpr (KIND == PK_DACT) {
       if (p.e.colsum) {
         const uint32_t nxt = tile + gridDim.x;
         if (jt + 1 == my_tiles || nxt / n_mt != tile / n_mt)
@@ -1517,7 +1602,7 @@ static int persist_kind(const na_gemm_args* a, int split) {
   if (a->M >= 0x7fffffffLL || a->lda * 2 * GBM >= 0x7fffffffLL || a->ldb * 2 * 256 >= 0x7fffffffLL) return -1;
   const int cb = a->c_dtype == NA_BF16 ? 2 : 4;
   if (!aligned16(a->C) || (a->ldc * cb) % 16) return -1;
-  if (a->r1_a) return -1;
+  if (a->r1_a && (!a->dact || !aligned16(a->r1_b))) return -1;
   if (a->bias && !aligned16(a->bias)) return -1;
   if (a->head_w && !aligned16(a->head_w)) return -1;
   if (a->z_out && (!aligned16(a->z_out) || (a->ld_z * 4) % 16)) return -1;
@@ -1527,9 +1612,11 @@ static int persist_kind(const na_gemm_args* a, int split) {
   }
   if (a->dact) {
     const int db = a->dact_dtype == NA_BF16 ? 2 : 4;
-    if (a->dact != NA_ACT_RELU || a->act || a->bias || a->z_out || a->head_w) return -1;
+    if ((a->dact != NA_ACT_RELU && a->dact != NA_ACT_SOFTPLUS) || a->act || a->bias || a->z_out ||
+        a->head_w)
+      return -1;
     if (!aligned16(a->dact_y) || (a->ld_dact * db) % 16) return -1;
-    return PK_DACT_RELU;
+    return PK_DACT;
   }
   if (a->colsum) return -1;
   if (!a->bias && !a->act && !a->z_out && !a->head_w) return PK_PLAIN;
@@ -1545,7 +1632,7 @@ static void launch_persist(int kind, int grid, const GemmParams& p, hipStream_t 
     case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_PLAIN>), g, b, 0, s, p); break;
     case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_BIAS_RELU>), g, b, 0, s, p); break;
     case PK_BIAS_SOFTPLUS: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;
-    case PK_DACT_RELU: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_DACT_RELU>), g, b, 0, s, p); break;
+    case PK_DACT: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_DACT>), g, b, 0, s, p); break;
     default: hipLaunchKernelGGL((gemm_bf16_nn_persist_kernel<WN, PK_DELTA>), g, b, 0, s, p); break;
   }
 }
@@ -1720,14 +1807,23 @@ extern "C" int na_linear_narrow_bwd(const void* X, int64_t ldx, int32_t x_dtype,
   NA_CHECK_ARG(narrow_vec_ok(K, ldx, X, x_dtype) && (!dX || narrow_vec_ok(K, lddx, dX, dx_dtype)) &&
                    (!dact || narrow_vec_ok(K, ldy, Y, y_dtype)),
                "narrow layer needs K % 8 == 0, K <= 512, leading dims % 8 == 0 and aligned rows");
+  // 2 workgroups per CU at most: every workgroup ends with one atomic per
+  // weight / column-sum element, so fewer, longer workgroups contend less
   int64_t waves = ceil_div(M, 16);
   if (waves > 4096) waves = 4096;
-  dim3 grid((unsigned)ceil_div(waves, 4));
+  dim3 grid8((unsigned)ceil_div(waves, 8));
   hipStream_t s = (hipStream_t)stream;
-#define NA_NBWD(TX, TDX, KC)                                                                  \
-  hipLaunchKernelGGL((narrow_bwd_vec_kernel<TX, TDX, KC>), grid, dim3(256), 0, s, (const TX*)X, \
-                     ldx, M, K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, Y, ldy, y_dtype,     \
-                     dact, act_beta, dx_colsum)
+#define NA_NBWD(TX, TDX, KC)                                                                   \
+  do {                                                                                         \
+    if (N == 1)                                                                                \
+      hipLaunchKernelGGL((narrow_bwd_vec_kernel<TX, TDX, KC, 1>), grid8, dim3(512), 0, s,      \
+                         (const TX*)X, ldx, M, K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, Y, \
+                         ldy, y_dtype, dact, act_beta, dx_colsum);                              \
+    else                                                                                       \
+      hipLaunchKernelGGL((narrow_bwd_vec_kernel<TX, TDX, KC, kNarrowMax>), grid8, dim3(512), 0,\
+                         s, (const TX*)X, ldx, M, K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, \
+                         Y, ldy, y_dtype, dact, act_beta, dx_colsum);                           \
+  } while (0)
   const bool x16 = x_dtype == NA_BF16, d16 = dx_dtype == NA_BF16;
   if (K <= 256) {
     if (x16 && d16) NA_NBWD(bf16_t, bf16_t, 1);

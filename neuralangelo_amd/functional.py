@@ -288,20 +288,23 @@ def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc):
     dsdf = (torch.zeros(P, 1, device=dev) if dsdf is None else dsdf).reshape(P, 1)
     dsdf = dsdf.contiguous().float()
     has_feat = FB is not None and dfeat is not None
+    dpre0 = torch.empty(P, hid, dtype=cdt, device=dev)
     if has_feat:
-        dH0 = torch.empty(P, hid, dtype=torch.float32, device=dev)
         dpre1 = act_grad(dfeat, FB, NA_ACT_SOFTPLUS, cdt, colsum=acc.db1, n_cols=hid)
         linear_weight_grad(dpre1, H0, hid, hid, acc.dW1)
-        gemm(dpre1, hid, 0, _t(w.W1), hid, 0, P, hid, hid, dH0, hid)  # dH0 = dpre1 . W1
+        # dpre0 = (dpre1 . W1 + dsdf ws) * softplus'(H0), db0 = column sums: one GEMM epilogue
+        gemm(dpre1, hid, 0, _t(w.W1), hid, 0, P, hid, hid, dpre0, hid, dact=NA_ACT_SOFTPLUS,
+             dact_y=H0, ld_dact=hid, r1_a=dsdf.view(-1), r1_b=w.ws.view(-1), colsum=acc.db0)
+        # dws = dsdf^T H0, dbs = sum dsdf (no input gradient from this call)
+        _lib.call("na_linear_narrow_bwd", _lib.ptr(H0), hid, _code(H0), P, hid, _lib.ptr(w.ws), 1,
+                  _lib.ptr(dsdf), _lib.ptr(acc.dws), _lib.ptr(acc.dbs), None, hid, _code(dpre0), 0,
+                  None, hid, _code(H0), NA_ACT_NONE, SOFTPLUS_BETA, None, _s(X0))
     else:
-        # no feature gradient: dpre0 is written directly in the compute dtype
-        dH0 = torch.empty(P, hid, dtype=cdt, device=dev)
-    # dpre0 = (dH0 + dsdf * ws) * softplus'(H0); dws, dbs; db0 = column sums of dpre0
-    _lib.call("na_linear_narrow_bwd", _lib.ptr(H0), hid, _code(H0), P, hid, _lib.ptr(w.ws), 1,
-              _lib.ptr(dsdf), _lib.ptr(acc.dws), _lib.ptr(acc.dbs), _lib.ptr(dH0), hid,
-              _code(dH0), int(has_feat), _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS,
-              SOFTPLUS_BETA, _lib.ptr(acc.db0), _s(X0))
-    dpre0 = dH0 if dH0.dtype == cdt else dH0.to(cdt)
+        # dpre0 = (dsdf ws) * softplus'(H0); dws, dbs; db0 = column sums of dpre0
+        _lib.call("na_linear_narrow_bwd", _lib.ptr(H0), hid, _code(H0), P, hid, _lib.ptr(w.ws), 1,
+                  _lib.ptr(dsdf), _lib.ptr(acc.dws), _lib.ptr(acc.dbs), _lib.ptr(dpre0), hid,
+                  _code(dpre0), 0, _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS,
+                  SOFTPLUS_BETA, _lib.ptr(acc.db0), _s(X0))
     linear_weight_grad(dpre0, X0, w.k0p, w.k0p, acc.dW0p)
     return dpre0
 
@@ -598,19 +601,19 @@ class RGBNetFunction(torch.autograd.Function):
         FB, normals_src, out, *Hs = ctx.saved_tensors
         w, n_lin = ctx.w, ctx.n_lin
         P, hid, cdt, dev = FB.shape[0], w.hidden, w.cdt, FB.device
-        dpre = dout.float()
+        dpre3 = dout.float()
         if ctx.final_act == NA_ACT_SIGMOID:
-            dpre = dpre * out * (1 - out)
-        dpre = dpre.contiguous()
+            dpre3 = dpre3 * out * (1 - out)
+        dpre3 = dpre3.contiguous()
         dWs = [None] * n_lin
         dbs = [torch.zeros(b.shape[0], device=dev) for b in w.b]
         dWs[-1] = torch.zeros(3, hid, device=dev)
-        dH = torch.zeros(P, hid, dtype=torch.float32, device=dev)
+        # last hidden layer's pre-activation gradient, written in the compute dtype
+        dpre = torch.empty(P, hid, dtype=cdt, device=dev)
         _lib.call("na_linear_narrow_bwd", _lib.ptr(Hs[-1]), hid, _code(Hs[-1]), P, hid,
-                  _lib.ptr(w.W_last), 3, _lib.ptr(dpre), _lib.ptr(dWs[-1]), _lib.ptr(dbs[-1]),
-                  _lib.ptr(dH), hid, _code(dH), 0, _lib.ptr(Hs[-1]), hid, _code(Hs[-1]),
+                  _lib.ptr(w.W_last), 3, _lib.ptr(dpre3), _lib.ptr(dWs[-1]), _lib.ptr(dbs[-1]),
+                  _lib.ptr(dpre), hid, _code(dpre), 0, _lib.ptr(Hs[-1]), hid, _code(Hs[-1]),
                   NA_ACT_RELU, SOFTPLUS_BETA, _lib.ptr(dbs[-2]), _s(FB))
-        dpre = dH if cdt == torch.float32 else dH.to(cdt)
         for li in range(n_lin - 2, 0, -1):
             X = Hs[li - 1]
             dWs[li] = torch.zeros(hid, hid, device=dev)
