@@ -8,8 +8,11 @@ of radius 3 (32x32 px, FOV 50 deg) x (64 coarse + 2 x 16 resampled) samples;
 (lr 1e-3, wd 1e-2) over all parameters.  Synthetic inputs, random targets.
 
 Prints one JSON line.  value = train rays/s of the BF16 path; the FP32 path
-is measured beside it.  roofline = the hash-grid encoder forward kernel
-(HBM-bound gather), timed with HIP events on torch's stream.
+is measured beside it.  roofline = the step's dominant kernel, the hash-grid
+backward over the centre point and its 4 taps (HBM/atomic-bound scatter),
+timed with HIP events on torch's stream around its launch inside the timed
+steps; traffic = its HBM bytes per launch from the committed rocprofv3
+counter summary.  hash_fwd = the encoder forward kernel (hash GB/s).
 """
 import argparse
 import json
@@ -82,16 +85,60 @@ def time_train(dtype, dev, n_rays, steps, warmup):
     c, v = camera_rays(n_cams)
     c, v = c[:n_rays].to(dev), v[:n_rays].to(dev)
     target = torch.rand(n_rays, 3, generator=torch.Generator().manual_seed(1)).to(dev)
+    from neuralangelo_amd import functional as Fn
     for _ in range(warmup):
         loss = train_step(model, opt, c, v, target)
     torch.cuda.synchronize()
+    Fn.LAUNCH_EVENTS = {ROOF_CALL: []}
     t0 = time.perf_counter()
     for _ in range(steps):
         loss = train_step(model, opt, c, v, target)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    pairs, Fn.LAUNCH_EVENTS = Fn.LAUNCH_EVENTS[ROOF_CALL], None
+    launch_ms = sum(a.elapsed_time(b) for a, b in pairs) / max(1, len(pairs))
     assert torch.isfinite(loss).item(), "non-finite loss"
-    return model, dt, float(loss.item())
+    return model, dt, float(loss.item()), launch_ms, len(pairs)
+
+
+ROOF_CALL = "na_sdf_taps_bwd"
+ROOF_KERNEL = "hashgrid_taps_bwd_kernel"
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hashgrid_taps_bwd_train_step_pmc.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor the same
+    counter reads for a copy of known size), or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def taps_bwd_roofline(model, n_points, launch_ms, n_launches):
+    """Roofline of the dominant kernel of the step: the hash-grid backward over the
+    centre point and its four taps (one launch per step, timed with HIP events on
+    torch's stream inside the timed steps).  Algorithmic bytes per point-eval of the
+    encoder backward: L*F*4 (dL/dy) + 12 (point) + L*8*F*4 (corner gradient RMW,
+    counted once); 5 evals (centre + 4 taps) per point."""
+    enc = model.neural_sdf.tcnn_encoding
+    L, F = enc.n_levels, enc.n_features_per_level
+    per_eval = L * F * 4 + 12 + L * 8 * F * 4
+    per_point = 5 * per_eval
+    achieved = per_point * n_points / (launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic()
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=round(traffic) if traffic is not None else None,
+                kernel=f"{ROOF_KERNEL} (centre + 4 taps, merged corner scatter)",
+                bytes_per_point=per_point, points_per_launch=n_points,
+                avg_launch_ms=round(launch_ms, 4), launches_timed=n_launches,
+                traffic_source=os.path.relpath(TRAFFIC_FILE, REPO) if traffic is not None else None)
 
 
 def encoder_roofline(model, dev, n_points=786432, iters=20):
@@ -280,13 +327,16 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    model, dt_bf16, loss_bf16 = time_train(torch.bfloat16, dev, args.rays, args.steps, args.warmup)
-    roof = encoder_roofline(model, dev)
+    model, dt_bf16, loss_bf16, taps_ms, n_taps = time_train(torch.bfloat16, dev, args.rays,
+                                                           args.steps, args.warmup)
+    roof = taps_bwd_roofline(model, args.rays * 96, taps_ms, n_taps)
+    hash_fwd = encoder_roofline(model, dev)
     del model
     torch.cuda.empty_cache()
     extra = {}
     if not args.no_fp32:
-        m32, dt_fp32, loss_fp32 = time_train(torch.float32, dev, args.rays, args.steps, args.warmup)
+        m32, dt_fp32, loss_fp32, _, _ = time_train(torch.float32, dev, args.rays, args.steps,
+                                                   args.warmup)
         extra["fp32_rays_per_s"] = round(args.rays / dt_fp32, 1)
         extra["fp32_ms_per_step"] = round(dt_fp32 * 1e3, 3)
         extra["bf16_over_fp32"] = round(dt_fp32 / dt_bf16, 3)
@@ -319,7 +369,8 @@ def main():
                    "rays": args.rays, "samples_per_ray": 96},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "hash_GBps": roof["achieved"],
+        "hash_GBps": hash_fwd["achieved"],
+        "hash_fwd": hash_fwd,
         "loss": round(loss_bf16, 5),
         **extra,
     }

@@ -29,6 +29,27 @@ SOFTPLUS_BETA = 100.0
 RGB_LD = 280  # 256 features + 3 + 16 + 3, padded to a multiple of 8
 
 
+# Optional HIP-event timing of named launches on the launching stream: a dict
+# name -> list that collects (start, end) torch.cuda.Event pairs while set.
+LAUNCH_EVENTS = None
+
+
+class _timed:
+    def __init__(self, name):
+        self.pair = None
+        if LAUNCH_EVENTS is not None and name in LAUNCH_EVENTS:
+            self.pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            LAUNCH_EVENTS[name].append(self.pair)
+
+    def __enter__(self):
+        if self.pair:
+            self.pair[0].record()
+
+    def __exit__(self, *exc):
+        if self.pair:
+            self.pair[1].record()
+
+
 def _pad8(k):
     return (k + 7) // 8 * 8
 
@@ -478,9 +499,10 @@ def _centre_taps_backward(ctx, pts, X0c, H0c, FB, X0t, H0t, dsdf, dfeat, ds_taps
         dXt = _encoder_grad(dpre_t, w)
         dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
         vmin, vmax = enc.vol_range
-        _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
-                  float(vmax), ctx.eps, _lib.ptr(dXc), _lib.ptr(dXt), _code(dXc), w.n_enc,
-                  _lib.ptr(dtable), int(enc.active_levels), _s(pts))
+        with _timed("na_sdf_taps_bwd"):
+            _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
+                      float(vmax), ctx.eps, _lib.ptr(dXc), _lib.ptr(dXt), _code(dXc), w.n_enc,
+                      _lib.ptr(dtable), int(enc.active_levels), _s(pts))
     dW0 = w.unpermute_dW0(acc.dW0p)
     return (None, dtable, dW0, acc.db0, acc.dW1, acc.db1, acc.dws.view(w.ws_shape),
             acc.dbs.view(w.bs_shape), None, None, None)
