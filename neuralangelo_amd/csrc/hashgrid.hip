@@ -392,9 +392,30 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
 // in one pass.  The taps lie less than one cell from the centre at every
 // active level, so the union of the five points' corners fits a 3x3x3 box of
 // lattice points; the five contributions to each corner are summed in
-// registers and each touched corner gets one atomic.  When a level's cells
-// spread wider (an epsilon larger than the finest active cell), the thread
-// falls back to one atomic per point and corner.
+// registers.  A thread owns one feature of one level for kTapRun consecutive
+// points (consecutive samples of a ray): while their boxes share one anchor
+// the sums keep accumulating, and each touched corner gets one atomic when the
+// anchor changes or the run ends.  At coarse levels, where neighbouring
+// samples share cells and atomics to one address would otherwise queue, this
+// removes most of the atomics.  When a level's cells spread wider than one cell
+// (an epsilon larger than the finest active cell), that point falls back to
+// one atomic per point and corner.
+constexpr int kTapRun = 8;
+
+template <int F>
+__device__ __attribute__((always_inline)) inline void taps_flush(float* gtab, const LevelParams& p,
+                                                                 const int (&blo)[3],
+                                                                 const float (&acc)[27],
+                                                                 uint32_t mask) {
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    if (mask & (1u << o)) {
+      const uint32_t id = corner_index((uint32_t)(blo[0] + o % 3), (uint32_t)(blo[1] + (o / 3) % 3),
+                                       (uint32_t)(blo[2] + o / 9), p);
+      atomic_add_f32(gtab + (size_t)id * F, acc[o]);
+    }
+  }
+}
 
 template <int F, typename TD>
 __global__ void __launch_bounds__(kBlock)
@@ -403,90 +424,113 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
                          const TD* __restrict__ dyt, int64_t ld, float* __restrict__ grad) {
   const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int l = blockIdx.y;
-  const int64_t i = tid / F;
+  const int64_t i0 = (tid / F) * kTapRun;
   const int f = (int)(tid % F);
-  if (i >= n) return;
+  if (i0 >= n) return;
   const int64_t col = (int64_t)l * F + f;
-  float g[5];
-  g[0] = scalar_io<TD>::load(dyc + i * ld + col);
-#pragma unroll
-  for (int t = 0; t < 4; ++t) g[1 + t] = scalar_io<TD>::load(dyt + ((int64_t)t * n + i) * ld + col);
-  if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && g[3] == 0.0f && g[4] == 0.0f) return;
   const LevelParams p = level_params(d, l);
-  int cell[5][3];
-  float fr[5][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float xc = x[i * 3 + k];
-    uint32_t c;
-    pos_fract(normalize_coord(xc, vmin, vrange), p.scale, c, fr[0][k]);
-    cell[0][k] = (int)c;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const float xt = xc + kTapDir[t][k] * eps;  // the tap point, as the forward built it
-      pos_fract(normalize_coord(xt, vmin, vrange), p.scale, c, fr[1 + t][k]);
-      cell[1 + t][k] = (int)c;
-    }
-  }
   float* gtab = grad + (size_t)p.off * F + f;
-  int lo[3];
-  bool box = true;
+  float acc[27];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    int a = cell[0][k], b = cell[0][k];
+  for (int o = 0; o < 27; ++o) acc[o] = 0.0f;
+  uint32_t mask = 0;
+  int blo[3] = {0, 0, 0};
+  const int64_t iend = i0 + kTapRun < n ? i0 + kTapRun : n;
+  for (int64_t i = i0; i < iend; ++i) {
+    float g[5];
+    g[0] = scalar_io<TD>::load(dyc + i * ld + col);
 #pragma unroll
-    for (int q = 1; q < 5; ++q) {
-      a = min(a, cell[q][k]);
-      b = max(b, cell[q][k]);
-    }
-    lo[k] = a;
-    box = box && (b - a <= 1);
-  }
-  if (!box) {
+    for (int t = 0; t < 4; ++t) g[1 + t] = scalar_io<TD>::load(dyt + ((int64_t)t * n + i) * ld + col);
+    if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && g[3] == 0.0f && g[4] == 0.0f) continue;
+    int cell[5][3];
+    float fr[5][3];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      if (g[q] == 0.0f) continue;
+    for (int k = 0; k < 3; ++k) {
+      const float xc = x[i * 3 + k];
+      uint32_t c;
+      pos_fract(normalize_coord(xc, vmin, vrange), p.scale, c, fr[0][k]);
+      cell[0][k] = (int)c;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const uint32_t id = corner_index((uint32_t)cell[q][0] + (c & 1),
-                                         (uint32_t)cell[q][1] + ((c >> 1) & 1),
-                                         (uint32_t)cell[q][2] + ((c >> 2) & 1), p);
-        atomic_add_f32(gtab + (size_t)id * F, corner_weight(c, fr[q]) * g[q]);
+      for (int t = 0; t < 4; ++t) {
+        const float xt = xc + kTapDir[t][k] * eps;  // the tap point, as the forward built it
+        pos_fract(normalize_coord(xt, vmin, vrange), p.scale, c, fr[1 + t][k]);
+        cell[1 + t][k] = (int)c;
       }
     }
-    return;
-  }
-  // per point and axis: offset of its cell in the box (0 or 1)
-  int off[5][3];
+    int lo[3];
+    bool box = true;
 #pragma unroll
-  for (int q = 0; q < 5; ++q)
+    for (int k = 0; k < 3; ++k) {
+      int a = cell[0][k], b = cell[0][k];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) off[q][k] = cell[q][k] - lo[k];
+      for (int q = 1; q < 5; ++q) {
+        a = min(a, cell[q][k]);
+        b = max(b, cell[q][k]);
+      }
+      lo[k] = a;
+      box = box && (b - a <= 1);
+    }
+    if (!box) {
 #pragma unroll
-  for (int oz = 0; oz < 3; ++oz)
+      for (int q = 0; q < 5; ++q) {
+        if (g[q] == 0.0f) continue;
 #pragma unroll
-    for (int oy = 0; oy < 3; ++oy)
+        for (int c = 0; c < 8; ++c) {
+          const uint32_t id = corner_index((uint32_t)cell[q][0] + (c & 1),
+                                           (uint32_t)cell[q][1] + ((c >> 1) & 1),
+                                           (uint32_t)cell[q][2] + ((c >> 2) & 1), p);
+          atomic_add_f32(gtab + (size_t)id * F, corner_weight(c, fr[q]) * g[q]);
+        }
+      }
+      continue;
+    }
+    // the point joins the running box when all its cells lie in [blo, blo + 1]
+    bool fits = mask != 0;
 #pragma unroll
-      for (int ox = 0; ox < 3; ++ox) {
-        float acc = 0.0f;
-        bool touched = false;
+    for (int k = 0; k < 3; ++k) {
+      int a = cell[0][k], b = cell[0][k];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          const int dx = ox - off[q][0], dy = oy - off[q][1], dz = oz - off[q][2];
-          if (g[q] != 0.0f && (unsigned)dx <= 1u && (unsigned)dy <= 1u && (unsigned)dz <= 1u) {
-            float w = dx ? fr[q][0] : 1.0f - fr[q][0];
-            w *= dy ? fr[q][1] : 1.0f - fr[q][1];
-            w *= dz ? fr[q][2] : 1.0f - fr[q][2];
-            acc += w * g[q];
-            touched = true;
+      for (int q = 1; q < 5; ++q) {
+        a = min(a, cell[q][k]);
+        b = max(b, cell[q][k]);
+      }
+      fits = fits && a >= blo[k] && b <= blo[k] + 1;
+    }
+    if (!fits) {
+      if (mask) taps_flush<F>(gtab, p, blo, acc, mask);
+#pragma unroll
+      for (int o = 0; o < 27; ++o) acc[o] = 0.0f;
+      mask = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) blo[k] = lo[k];
+    }
+    // per point and axis: offset of its cell in the running box (0 or 1)
+    int off[5][3];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) off[q][k] = cell[q][k] - blo[k];
+#pragma unroll
+    for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox) {
+          const int o = (oz * 3 + oy) * 3 + ox;
+#pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const int dx = ox - off[q][0], dy = oy - off[q][1], dz = oz - off[q][2];
+            if (g[q] != 0.0f && (unsigned)dx <= 1u && (unsigned)dy <= 1u && (unsigned)dz <= 1u) {
+              float w = dx ? fr[q][0] : 1.0f - fr[q][0];
+              w *= dy ? fr[q][1] : 1.0f - fr[q][1];
+              w *= dz ? fr[q][2] : 1.0f - fr[q][2];
+              acc[o] += w * g[q];
+              mask |= 1u << o;
+            }
           }
         }
-        if (touched) {
-          const uint32_t id = corner_index((uint32_t)(lo[0] + ox), (uint32_t)(lo[1] + oy),
-                                           (uint32_t)(lo[2] + oz), p);
-          atomic_add_f32(gtab + (size_t)id * F, acc);
-        }
-      }
+  }
+  if (mask) taps_flush<F>(gtab, p, blo, acc, mask);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,7 +630,7 @@ template <int F, typename TD>
 static int launch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, float vmin,
                            float vrange, float eps, const void* dyc, const void* dyt, int64_t ld,
                            float* grad, int active, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(n * F, kBlock), active);
+  dim3 grid((unsigned)ceil_div(ceil_div(n, (int64_t)kTapRun) * F, kBlock), active);
   hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD>), grid, dim3(kBlock), 0, s, *d, x, n, vmin,
                      vrange, eps, (const TD*)dyc, (const TD*)dyt, ld, grad);
   NA_CHECK_LAUNCH();

@@ -20,16 +20,35 @@ def _enc(log2_T, F, cuda):
                             per_level_scale=2 ** 0.4)).to(cuda)
 
 
-@pytest.mark.parametrize("eps,active,dt,F", [(1 / 2048 / 3 ** 0.5, 16, torch.float32, 8),
-                                             (1 / 2048 / 3 ** 0.5, 16, torch.bfloat16, 8),
-                                             (1 / 256 / 3 ** 0.5, 11, torch.float32, 4),
-                                             (0.05, 16, torch.float32, 2)])
-def test_merged_scatter_equals_five_scatters(cuda, eps, active, dt, F):
-    print("taps bwd", eps, active, dt, F, flush=True)
+def _ray_points(n, g, spacing):
+    """n points as 100-point runs along random rays (consecutive samples of a ray, as
+    the renderer orders them), `spacing` apart."""
+    rays = (n + 99) // 100
+    o = torch.rand(rays, 1, 3, generator=g) * 1.6 - 0.8
+    d = torch.nn.functional.normalize(torch.randn(rays, 1, 3, generator=g), dim=-1)
+    t = torch.arange(100).view(1, 100, 1) * spacing
+    return (o + d * t).reshape(-1, 3)[:n].contiguous()
+
+
+@pytest.mark.parametrize("eps,active,dt,F,layout", [
+    (1 / 2048 / 3 ** 0.5, 16, torch.float32, 8, "random"),
+    (1 / 2048 / 3 ** 0.5, 16, torch.bfloat16, 8, "random"),
+    (1 / 256 / 3 ** 0.5, 11, torch.float32, 4, "random"),
+    (0.05, 16, torch.float32, 2, "random"),
+    (1 / 2048 / 3 ** 0.5, 16, torch.float32, 8, "rays"),
+    (1 / 2048 / 3 ** 0.5, 16, torch.bfloat16, 8, "rays_dense"),
+    (0.05, 16, torch.float32, 4, "rays")])
+def test_merged_scatter_equals_five_scatters(cuda, eps, active, dt, F, layout):
+    """Random points, and runs of close samples along rays whose boxes coincide for
+    several consecutive points (sums carried across points before the atomics)."""
+    print("taps bwd", eps, active, dt, F, layout, flush=True)
     enc = _enc(16, F, cuda)
-    n, LF = 4000, 16 * F
+    n, LF = 4001, 16 * F
     g = torch.Generator().manual_seed(0)
-    pts = (torch.rand(n, 3, generator=g) * 3.8 - 1.9).to(cuda)
+    if layout == "random":
+        pts = (torch.rand(n, 3, generator=g) * 3.8 - 1.9).to(cuda)
+    else:
+        pts = _ray_points(n, g, 2e-3 if layout == "rays" else 1e-4).to(cuda)
     dyc = torch.randn(n, LF, generator=g).to(cuda).to(dt)
     dyt = torch.randn(4 * n, LF, generator=g).to(cuda).to(dt)
     dyc[::7] = 0  # zero rows are skipped
