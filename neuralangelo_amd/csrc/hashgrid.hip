@@ -400,7 +400,7 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
 // removes most of the atomics.  When a level's cells spread wider than one cell
 // (an epsilon larger than the finest active cell), that point falls back to
 // one atomic per point and corner.
-constexpr int kTapRun = 8;
+constexpr int kTapRun = 4;
 
 template <int F>
 __device__ __attribute__((always_inline)) inline void taps_flush(float* gtab, const LevelParams& p,
