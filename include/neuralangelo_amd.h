@@ -312,6 +312,38 @@ int na_weighted_sum_fwd(const float* q, const float* w, int64_t R, int32_t N, in
 int na_weighted_sum_bwd(const float* q, const float* w, int64_t R, int32_t N, int32_t K,
                         const float* dout, float* dq, float* dw, na_stream_t stream);
 
+/* Packed (ragged) rays: the same operators over rays with uneven sample counts.
+ * Ray r owns samples [offsets[r], offsets[r+1]) of every per-sample array
+ * (offsets int32[R+1], non-decreasing, offsets[0] = 0); per-ray arrays are [R, ...].
+ * They replace, for packed batches, alpha_compositing_weights / composite
+ * (projects/nerf/utils/render.py:87-112) and Model.compute_neus_alphas
+ * (projects/neuralangelo/model.py:275-298, the last sample of a ray uses its
+ * far distance as at model.py:170-171); the reference itself renders a fixed N
+ * per ray, which is the packed layout with equal counts.
+ * na_packed_ray_index writes the ray of every sample (int32[S]). */
+int na_packed_ray_index(const int32_t* offsets, int64_t R, int32_t* ray_idx, na_stream_t stream);
+int na_composite_packed_fwd(const float* alpha, const float* q, const int32_t* offsets, int64_t R,
+                            int32_t K, float* weights, float* out, na_stream_t stream);
+int na_composite_packed_bwd(const float* alpha, const float* q, const float* weights,
+                            const int32_t* offsets, int64_t R, int32_t K, const float* dout,
+                            const float* dweights, float* dalpha, float* dq, na_stream_t stream);
+int na_weighted_sum_packed_fwd(const float* q, const float* w, const int32_t* offsets, int64_t R,
+                               int32_t K, float* out, na_stream_t stream);
+int na_weighted_sum_packed_bwd(const float* q, const float* w, const int32_t* ray_idx, int64_t S,
+                               int32_t K, const float* dout, float* dq, float* dw,
+                               na_stream_t stream);
+int na_neus_alpha_packed_fwd(const float* ray_unit, const float* sdfs, const float* grads,
+                             const float* dists, const float* far, const uint8_t* outside,
+                             float outside_val, const float* s_var, float progress,
+                             float anneal_end, const int32_t* offsets, const int32_t* ray_idx,
+                             int64_t S, float* alpha, na_stream_t stream);
+int na_neus_alpha_packed_bwd(const float* ray_unit, const float* sdfs, const float* grads,
+                             const float* dists, const float* far, const uint8_t* outside,
+                             float outside_val, const float* s_var, float progress,
+                             float anneal_end, const int32_t* offsets, const int32_t* ray_idx,
+                             int64_t S, const float* dalpha, float* dsdf, float* dgrad,
+                             float* ds_var, na_stream_t stream);
+
 /* 4-tap numerical gradient (projects/neuralangelo/utils/modules.py:157-175).
  * taps: writes x + eps*k_i for k = (1,-1,-1), (-1,-1,1), (-1,1,-1), (1,1,1) as
  * [4, n, 3] (eps = float32 of normal_eps/sqrt(3)).  combine: s_taps [4, n],

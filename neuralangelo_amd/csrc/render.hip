@@ -201,19 +201,30 @@ struct NeusIn {
   float outside_val;
   const float* s_var;
   float anneal_ratio;
-  int64_t R;
-  int N;
+  int64_t R;  // rays (dense) or samples (packed)
+  int N;      // samples per ray (dense) or 1 (packed)
+  const int32_t* off;      // packed: ray r owns samples [off[r], off[r+1])
+  const int32_t* ray_idx;  // packed: ray of each sample
 };
 
 struct NeusTerms {
   float sdf, cosv, intv, iter_cos, est_prev, est_next, inv_s, pc, nc, ratio;
   bool outside;
+  int64_t r;
 };
 
 __device__ inline NeusTerms neus_terms(const NeusIn& p, int64_t t) {
   NeusTerms q;
-  int64_t r = t / p.N;
-  int i = (int)(t % p.N);
+  int64_t r;
+  bool has_next;
+  if (p.off) {
+    r = p.ray_idx[t];
+    has_next = t + 1 < (int64_t)p.off[r + 1];
+  } else {
+    r = t / p.N;
+    has_next = (int)(t % p.N) + 1 < p.N;
+  }
+  q.r = r;
   q.outside = p.outside && p.outside[r];
   q.sdf = q.outside ? p.outside_val : p.sdfs[t];
   float v0 = p.ray[r * 3], v1 = p.ray[r * 3 + 1], v2 = p.ray[r * 3 + 2];
@@ -225,7 +236,7 @@ __device__ inline NeusTerms neus_terms(const NeusIn& p, int64_t t) {
   float r1 = u1 > 0.0f ? u1 : 0.0f;
   float r2 = u2 > 0.0f ? u2 : 0.0f;
   q.iter_cos = -(r1 * (1.0f - a) + r2 * a);
-  float dnext = (i + 1 < p.N) ? p.dists[t + 1] : (p.far ? p.far[r] : 1e10f);
+  float dnext = has_next ? p.dists[t + 1] : (p.far ? p.far[r] : 1e10f);
   q.intv = dnext - p.dists[t];
   q.est_prev = q.sdf - q.iter_cos * q.intv * 0.5f;
   q.est_next = q.sdf + q.iter_cos * q.intv * 0.5f;
@@ -272,7 +283,7 @@ neus_alpha_bwd_kernel(NeusIn p, const float* __restrict__ dalpha, float* __restr
     float u1 = -q.cosv * 0.5f + 0.5f;
     float u2 = -q.cosv;
     float dcos = diter * (0.5f * (1.0f - a) * (u1 > 0.0f ? 1.0f : 0.0f) + a * (u2 > 0.0f ? 1.0f : 0.0f));
-    int64_t r = t / p.N;
+    const int64_t r = q.r;
     dgrad[t * 3] = dcos * p.ray[r * 3];
     dgrad[t * 3 + 1] = dcos * p.ray[r * 3 + 1];
     dgrad[t * 3 + 2] = dcos * p.ray[r * 3 + 2];
@@ -338,6 +349,100 @@ composite_bwd_kernel(const float* __restrict__ alpha, const float* __restrict__ 
     dalpha[r * N + j] = Tj * (g - U);
     U = g * a + (1.0f - a) * U;
   }
+}
+
+// Packed rays (ragged sample counts): ray r owns samples [off[r], off[r+1]) of
+// every per-sample array.  Same arithmetic and order as the dense kernels; the
+// backward keeps the transmittance of the forward pass in dalpha until the
+// reverse pass overwrites it, so a ray may hold any number of samples.
+__global__ void __launch_bounds__(kRayBlock)
+composite_packed_fwd_kernel(const float* __restrict__ alpha, const float* __restrict__ q,
+                            const int32_t* __restrict__ off, int64_t R, int K,
+                            float* __restrict__ weights, float* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  if (r >= R) return;
+  const int64_t b = off[r], e = off[r + 1];
+  float logT = 0.0f;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = b; i < e; ++i) {
+    float a = alpha[i];
+    float T = expf(logT);
+    float w = a * T;
+    weights[i] = w;
+    for (int k = 0; k < K; ++k) acc[k] = acc[k] + q[i * K + k] * w;
+    logT = logT + log1pf(-a);
+  }
+  for (int k = 0; k < K; ++k) out[r * K + k] = acc[k];
+}
+
+__global__ void __launch_bounds__(kRayBlock)
+composite_packed_bwd_kernel(const float* __restrict__ alpha, const float* __restrict__ q,
+                            const float* __restrict__ weights, const int32_t* __restrict__ off,
+                            int64_t R, int K, const float* __restrict__ dout,
+                            const float* __restrict__ dweights, float* __restrict__ dalpha,
+                            float* __restrict__ dq) {
+  const int64_t r = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  if (r >= R) return;
+  const int64_t b = off[r], e = off[r + 1];
+  float logT = 0.0f;
+  for (int64_t i = b; i < e; ++i) {
+    dalpha[i] = expf(logT);  // T_i, replaced by the gradient below
+    logT = logT + log1pf(-alpha[i]);
+  }
+  float d[4] = {0.f, 0.f, 0.f, 0.f};
+  if (dout)
+    for (int k = 0; k < K; ++k) d[k] = dout[r * K + k];
+  float U = 0.0f;
+  for (int64_t j = e - 1; j >= b; --j) {
+    float g = dweights ? dweights[j] : 0.0f;
+    for (int k = 0; k < K; ++k) g = g + d[k] * q[j * K + k];
+    if (dq) {
+      float w = weights[j];
+      for (int k = 0; k < K; ++k) dq[j * K + k] = w * d[k];
+    }
+    float a = alpha[j];
+    float Tj = dalpha[j];
+    dalpha[j] = Tj * (g - U);
+    U = g * a + (1.0f - a) * U;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+weighted_sum_packed_fwd_kernel(const float* __restrict__ q, const float* __restrict__ w,
+                               const int32_t* __restrict__ off, int64_t R, int K,
+                               float* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * K) return;
+  int64_t r = t / K;
+  int k = (int)(t % K);
+  float acc = 0.0f;
+  for (int64_t i = off[r]; i < off[r + 1]; ++i) acc = acc + q[i * K + k] * w[i];
+  out[t] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+weighted_sum_packed_bwd_kernel(const float* __restrict__ q, const float* __restrict__ w,
+                               const int32_t* __restrict__ ray_idx, int64_t S, int K,
+                               const float* __restrict__ dout, float* __restrict__ dq,
+                               float* __restrict__ dw) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S) return;
+  int64_t r = ray_idx[t];
+  float acc = 0.0f;
+  for (int k = 0; k < K; ++k) {
+    float d = dout[r * K + k];
+    if (dq) dq[t * K + k] = w[t] * d;
+    acc = acc + q[t * K + k] * d;
+  }
+  if (dw) dw[t] = acc;
+}
+
+// ray of every packed sample from the offsets (one thread per ray)
+__global__ void __launch_bounds__(256)
+packed_ray_index_kernel(const int32_t* __restrict__ off, int64_t R, int32_t* __restrict__ ray_idx) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  for (int64_t i = off[r]; i < off[r + 1]; ++i) ray_idx[i] = (int32_t)r;
 }
 
 __global__ void __launch_bounds__(256)
@@ -727,6 +832,7 @@ static NeusIn make_neus(const float* ray_unit, const float* sdfs, const float* g
   float ratio = progress / anneal_end;  // _get_iter_cos: min(progress / anneal_end, 1)
   p.anneal_ratio = ratio < 1.0f ? ratio : 1.0f;
   p.R = R; p.N = N;
+  p.off = nullptr; p.ray_idx = nullptr;
   return p;
 }
 
@@ -923,6 +1029,111 @@ extern "C" int na_taps_delta_bwd(int64_t n, float eps, float eps_sq, const float
   NA_CHECK_ARG(dds, "null pointer");
   hipLaunchKernelGGL(taps_delta_bwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, n, eps,
                      eps_sq, dgrad, dhess, dds, dsc, dsdf);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// packed (ragged) rays
+// ---------------------------------------------------------------------------
+extern "C" int na_packed_ray_index(const int32_t* offsets, int64_t R, int32_t* ray_idx,
+                                   na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(offsets && ray_idx, "null pointer");
+  hipLaunchKernelGGL(packed_ray_index_kernel, grid1d(R), dim3(256), 0, (hipStream_t)stream,
+                     offsets, R, ray_idx);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_composite_packed_fwd(const float* alpha, const float* q, const int32_t* offsets,
+                                       int64_t R, int32_t K, float* weights, float* out,
+                                       na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && K >= 0 && K <= 4, "bad sizes (K <= 4)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(offsets && alpha && weights && (K == 0 || (q && out)), "null pointer");
+  hipLaunchKernelGGL(composite_packed_fwd_kernel, grid1d(R, kRayBlock), dim3(kRayBlock), 0,
+                     (hipStream_t)stream, alpha, q, offsets, R, K, weights, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_composite_packed_bwd(const float* alpha, const float* q, const float* weights,
+                                       const int32_t* offsets, int64_t R, int32_t K,
+                                       const float* dout, const float* dweights, float* dalpha,
+                                       float* dq, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && K >= 0 && K <= 4, "bad sizes (K <= 4)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(offsets && alpha && weights && dalpha, "null pointer");
+  NA_CHECK_ARG(K == 0 || q, "q needed when K > 0");
+  hipLaunchKernelGGL(composite_packed_bwd_kernel, grid1d(R, kRayBlock), dim3(kRayBlock), 0,
+                     (hipStream_t)stream, alpha, q, weights, offsets, R, K, dout, dweights, dalpha,
+                     dq);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_weighted_sum_packed_fwd(const float* q, const float* w, const int32_t* offsets,
+                                          int64_t R, int32_t K, float* out, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && K >= 1, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(q && w && offsets && out, "null pointer");
+  hipLaunchKernelGGL(weighted_sum_packed_fwd_kernel, grid1d(R * K), dim3(256), 0,
+                     (hipStream_t)stream, q, w, offsets, R, K, out);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_weighted_sum_packed_bwd(const float* q, const float* w, const int32_t* ray_idx,
+                                          int64_t S, int32_t K, const float* dout, float* dq,
+                                          float* dw, na_stream_t stream) {
+  NA_CHECK_ARG(S >= 0 && K >= 1, "bad sizes");
+  if (S == 0) return NA_OK;
+  NA_CHECK_ARG(q && w && ray_idx && dout, "null pointer");
+  hipLaunchKernelGGL(weighted_sum_packed_bwd_kernel, grid1d(S), dim3(256), 0, (hipStream_t)stream,
+                     q, w, ray_idx, S, K, dout, dq, dw);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_neus_alpha_packed_fwd(const float* ray_unit, const float* sdfs,
+                                        const float* grads, const float* dists, const float* far,
+                                        const uint8_t* outside, float outside_val,
+                                        const float* s_var, float progress, float anneal_end,
+                                        const int32_t* offsets, const int32_t* ray_idx, int64_t S,
+                                        float* alpha, na_stream_t stream) {
+  NA_CHECK_ARG(S >= 0, "bad sizes");
+  if (S == 0) return NA_OK;
+  NA_CHECK_ARG(ray_unit && sdfs && grads && dists && s_var && alpha && offsets && ray_idx,
+               "null pointer");
+  NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
+  NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
+                       anneal_end, S, 1);
+  p.off = offsets; p.ray_idx = ray_idx;
+  hipLaunchKernelGGL(neus_alpha_fwd_kernel, grid1d(S), dim3(256), 0, (hipStream_t)stream, p, alpha);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_neus_alpha_packed_bwd(const float* ray_unit, const float* sdfs,
+                                        const float* grads, const float* dists, const float* far,
+                                        const uint8_t* outside, float outside_val,
+                                        const float* s_var, float progress, float anneal_end,
+                                        const int32_t* offsets, const int32_t* ray_idx, int64_t S,
+                                        const float* dalpha, float* dsdf, float* dgrad,
+                                        float* ds_var, na_stream_t stream) {
+  NA_CHECK_ARG(S >= 0, "bad sizes");
+  if (S == 0) return NA_OK;
+  NA_CHECK_ARG(ray_unit && sdfs && grads && dists && s_var && dalpha && dsdf && dgrad && ds_var &&
+                   offsets && ray_idx,
+               "null pointer");
+  NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
+  NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
+                       anneal_end, S, 1);
+  p.off = offsets; p.ray_idx = ray_idx;
+  hipLaunchKernelGGL(neus_alpha_bwd_kernel, grid1d(S), dim3(256), 0, (hipStream_t)stream, p,
+                     dalpha, dsdf, dgrad, ds_var);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -810,6 +810,130 @@ class WeightedSum(torch.autograd.Function):
 
 
 # ---------------------------------------------------------------------------
+# packed (ragged) rays: ray r owns samples offsets[r] .. offsets[r+1]-1
+# ---------------------------------------------------------------------------
+def packed_ray_index(offsets, S):
+    """offsets int32 [R+1] -> int32 [S], the ray of every packed sample."""
+    R = offsets.shape[0] - 1
+    ray_idx = torch.empty(S, dtype=torch.int32, device=offsets.device)
+    _lib.call("na_packed_ray_index", _lib.ptr(offsets), R, _lib.ptr(ray_idx), _s(offsets))
+    return ray_idx
+
+
+def _offsets(offsets):
+    if offsets.dtype != torch.int32:
+        raise RuntimeError("packed offsets must be int32")
+    return offsets.contiguous()
+
+
+class CompositePacked(torch.autograd.Function):
+    """alphas [S], q [S,K] (or None), offsets int32 [R+1] -> weights [S], out [R,K]."""
+
+    @staticmethod
+    def forward(ctx, alphas, q, offsets):
+        off = _offsets(offsets)
+        R = off.shape[0] - 1
+        alphas = alphas.contiguous().float()
+        K = 0 if q is None else q.shape[-1]
+        qc = q.contiguous().float() if q is not None else None
+        weights = torch.empty_like(alphas)
+        out = torch.empty(R, K, device=alphas.device)
+        _lib.call("na_composite_packed_fwd", _lib.ptr(alphas), _lib.ptr(qc), _lib.ptr(off), R, K,
+                  _lib.ptr(weights), _lib.ptr(out) if K else None, _s(alphas))
+        ctx.save_for_backward(alphas, qc if qc is not None else torch.empty(0, device=alphas.device),
+                              weights, off)
+        ctx.K = K
+        return weights, out
+
+    @staticmethod
+    def backward(ctx, dweights, dout):
+        alphas, qc, weights, off = ctx.saved_tensors
+        R, K = off.shape[0] - 1, ctx.K
+        dalpha = torch.empty_like(alphas)
+        dq = torch.empty(alphas.shape[0], K, device=alphas.device) \
+            if (K and ctx.needs_input_grad[1]) else None
+        _lib.call("na_composite_packed_bwd", _lib.ptr(alphas), _lib.ptr(qc) if K else None,
+                  _lib.ptr(weights), _lib.ptr(off), R, K,
+                  _lib.ptr(dout.contiguous().float()) if (dout is not None and K) else None,
+                  _lib.ptr(dweights.contiguous().float()) if dweights is not None else None,
+                  _lib.ptr(dalpha), _lib.ptr(dq), _s(alphas))
+        return dalpha, dq, None
+
+
+class WeightedSumPacked(torch.autograd.Function):
+    """quantities [S,K] x weights [S] per packed ray -> [R,K]."""
+
+    @staticmethod
+    def forward(ctx, q, w, offsets, ray_idx):
+        off = _offsets(offsets)
+        R = off.shape[0] - 1
+        qc = q.contiguous().float()
+        wc = w.reshape(-1).contiguous().float()
+        K = qc.shape[-1]
+        out = torch.empty(R, K, device=qc.device)
+        _lib.call("na_weighted_sum_packed_fwd", _lib.ptr(qc), _lib.ptr(wc), _lib.ptr(off), R, K,
+                  _lib.ptr(out), _s(qc))
+        ctx.save_for_backward(qc, wc, ray_idx)
+        ctx.wshape = w.shape
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        qc, wc, ray_idx = ctx.saved_tensors
+        S, K = qc.shape
+        dq = torch.empty_like(qc) if ctx.needs_input_grad[0] else None
+        dw = torch.empty_like(wc) if ctx.needs_input_grad[1] else None
+        _lib.call("na_weighted_sum_packed_bwd", _lib.ptr(qc), _lib.ptr(wc), _lib.ptr(ray_idx), S, K,
+                  _lib.ptr(dout.contiguous().float()), _lib.ptr(dq), _lib.ptr(dw), _s(qc))
+        return dq, (dw.view(ctx.wshape) if dw is not None else None), None, None
+
+
+class NeusAlphaPacked(torch.autograd.Function):
+    """Packed NeuS alphas: ray_unit [R,3], sdfs [S], gradients [S,3], dists [S], far [R]
+    (or None: 1e10), outside [R] (or None) -> alphas [S]."""
+
+    @staticmethod
+    def forward(ctx, ray_unit, sdfs, gradients, dists, far, outside, offsets, ray_idx, s_var,
+                progress, anneal_end, outside_val):
+        off = _offsets(offsets)
+        S = sdfs.shape[0]
+        ray_unit = ray_unit.contiguous().float()
+        sdfs = sdfs.reshape(S).contiguous().float()
+        gradients = gradients.reshape(S, 3).contiguous().float()
+        dists = dists.reshape(S).contiguous().float()
+        far = far.reshape(-1).contiguous().float() if far is not None else None
+        out_u8 = outside.contiguous().to(torch.uint8) if outside is not None else None
+        alpha = torch.empty(S, device=sdfs.device)
+        _lib.call("na_neus_alpha_packed_fwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
+                  _lib.ptr(gradients), _lib.ptr(dists), _lib.ptr(far), _lib.ptr(out_u8),
+                  float(outside_val), _lib.ptr(s_var), float(progress), float(anneal_end),
+                  _lib.ptr(off), _lib.ptr(ray_idx), S, _lib.ptr(alpha), _s(sdfs))
+        empty = torch.empty(0, device=sdfs.device)
+        ctx.save_for_backward(ray_unit, sdfs, gradients, dists,
+                              far if far is not None else empty,
+                              out_u8 if out_u8 is not None else empty, s_var, off, ray_idx)
+        ctx.has_far, ctx.has_out = far is not None, out_u8 is not None
+        ctx.progress, ctx.anneal_end, ctx.outside_val = progress, anneal_end, outside_val
+        return alpha
+
+    @staticmethod
+    def backward(ctx, dalpha):
+        ray_unit, sdfs, gradients, dists, far, out_u8, s_var, off, ray_idx = ctx.saved_tensors
+        S = sdfs.shape[0]
+        dsdf = torch.empty_like(sdfs)
+        dgrad = torch.empty_like(gradients)
+        ds_var = torch.zeros(1, device=sdfs.device)
+        _lib.call("na_neus_alpha_packed_bwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
+                  _lib.ptr(gradients), _lib.ptr(dists), _lib.ptr(far) if ctx.has_far else None,
+                  _lib.ptr(out_u8) if ctx.has_out else None, float(ctx.outside_val),
+                  _lib.ptr(s_var), float(ctx.progress), float(ctx.anneal_end), _lib.ptr(off),
+                  _lib.ptr(ray_idx), S, _lib.ptr(dalpha.contiguous().float()), _lib.ptr(dsdf),
+                  _lib.ptr(dgrad), _lib.ptr(ds_var), _s(sdfs))
+        return (None, dsdf, dgrad, None, None, None, None, None, ds_var.view_as(s_var), None,
+                None, None)
+
+
+# ---------------------------------------------------------------------------
 # eikonal + curvature losses
 # ---------------------------------------------------------------------------
 class SDFLosses(torch.autograd.Function):

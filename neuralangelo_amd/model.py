@@ -193,6 +193,55 @@ class Model(torch.nn.Module):
                     gradients=gradients.view(R, N, 3),
                     hessians=hessians.view(R, N, 3) if hessians is not None else None)
 
+    # -- packed (ragged) rays -----------------------------------------------------
+    def compute_neus_alphas_packed(self, ray_unit, sdfs, gradients, dists, offsets, dist_far=None,
+                                   ray_idx=None, outside=None, progress=1.):
+        """compute_neus_alphas (model.py:275-292) for packed rays: ray r owns samples
+        offsets[r] .. offsets[r+1]-1 of sdfs [S,1], gradients [S,3], dists [S,1]; ray_unit
+        [R,3], dist_far [R,1] or None -> alphas [S]."""
+        S = sdfs.shape[0]
+        if ray_idx is None:
+            ray_idx = Fn.packed_ray_index(offsets, S)
+        far = dist_far.reshape(-1) if dist_far is not None else None
+        return Fn.NeusAlphaPacked.apply(ray_unit, sdfs.reshape(S), gradients.reshape(S, 3),
+                                        dists.reshape(S), far, outside, offsets, ray_idx,
+                                        self.s_var, float(progress), float(self.anneal_end),
+                                        float(self.outside_val))
+
+    def render_rays_packed(self, center, ray_unit, dists, offsets):
+        """render_rays_object (model.py:158-190) over packed rays with uneven sample
+        counts: center, ray_unit [R,3]; dists [S] sorted within each ray; offsets int32
+        [R+1].  Every kernel of the dense path runs on the packed samples; per-ray
+        quantities are read through the sample's ray index.  Returns rgb [R,3] and the
+        per-sample tensors (gradients/hessians [S,1,3] and outside [S] for the losses)."""
+        c = center.reshape(-1, 3).contiguous().float()
+        v = ray_unit.reshape(-1, 3).contiguous().float()
+        dists = dists.reshape(-1).contiguous().float()
+        S = dists.shape[0]
+        with torch.no_grad():
+            near, far, outside = self.get_dist_bounds(c, v)
+        far = far.reshape(-1)
+        outside = outside.reshape(-1)
+        ray_idx = Fn.packed_ray_index(offsets, S)
+        ridx = ray_idx.long()
+        v_s = v.index_select(0, ridx)
+        points = c.index_select(0, ridx) + v_s * dists[:, None]
+        sdfs, feats, taps, delta = self.neural_sdf.forward_with_taps(points)
+        out_s = outside.index_select(0, ridx)
+        sdfs_filled = torch.where(out_s[:, None], torch.full_like(sdfs, self.outside_val), sdfs)
+        gradients, hessians = self.neural_sdf.gradients_from_taps(taps, sdfs, sdfs_filled,
+                                                                  self.training, delta)
+        rgbs = self.neural_rgb.forward(points, gradients, v_s, feats, None,
+                                       normals_are_gradients=True)  # [S,3]
+        alphas = self.compute_neus_alphas_packed(v, sdfs, gradients, dists, offsets, far,
+                                                 ray_idx=ray_idx, outside=outside,
+                                                 progress=self.progress)
+        weights, rgb = Fn.CompositePacked.apply(alphas, rgbs, offsets)
+        return dict(rgb=rgb, rgbs=rgbs, sdfs=sdfs_filled.view(S), dists=dists, alphas=alphas,
+                    weights=weights, ray_idx=ray_idx, outside=out_s,
+                    gradients=gradients.view(S, 1, 3),
+                    hessians=hessians.view(S, 1, 3) if hessians is not None else None)
+
     def forward(self, data):
         return self.render_rays(data["center"], data["ray_unit"], stratified=True,
                                 rands=data.get("rands"))

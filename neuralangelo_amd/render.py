@@ -36,3 +36,29 @@ def composite(quantities, weights):
         quantities = torch.full_like(weights, float(quantities))
     return Fn.WeightedSum.apply(quantities.expand(*weights.shape[:-1], quantities.shape[-1]),
                                 weights)
+
+
+# ---------------------------------------------------------------------------
+# packed (ragged) rays: ray r owns samples offsets[r] .. offsets[r+1]-1
+# ---------------------------------------------------------------------------
+def alpha_compositing_weights_packed(alphas, offsets):
+    """alphas [S] of packed rays (offsets int32 [R+1]) -> weights [S, 1]."""
+    weights, _ = Fn.CompositePacked.apply(alphas.reshape(-1), None, offsets)
+    return weights.view(-1, 1)
+
+
+def composite_from_alphas_packed(alphas, quantities, offsets):
+    """Fused packed alpha_compositing_weights + composite: (weights [S,1], out [R,K])."""
+    weights, out = Fn.CompositePacked.apply(alphas.reshape(-1), quantities.reshape(alphas.numel(), -1),
+                                            offsets)
+    return weights.view(-1, 1), out
+
+
+def composite_packed(quantities, weights, offsets, ray_idx=None):
+    """Per-ray sum of quantities [S,K] x weights [S,1] over packed rays -> [R,K]."""
+    S = weights.shape[0]
+    if not torch.is_tensor(quantities):
+        quantities = torch.full((S, 1), float(quantities), device=weights.device)
+    if ray_idx is None:
+        ray_idx = Fn.packed_ray_index(offsets, S)
+    return Fn.WeightedSumPacked.apply(quantities.reshape(S, -1), weights, offsets, ray_idx)

@@ -10,6 +10,8 @@ Restates:
   Model.compute_neus_alphas / _get_iter_cos projects/neuralangelo/model.py:275-298
   Model.render_rays_object / render_rays  projects/neuralangelo/model.py:124-190 (no background)
   eikonal_loss / curvature_loss         projects/neuralangelo/utils/misc.py:74-89
+  *_packed                              the same functions over packed (ragged) rays, through
+                                        the padded dense form (zero alpha / far distance)
   render loss (L1 * 3)                  projects/neuralangelo/trainer.py:43
   get_center_and_ray / points from dist projects/nerf/utils/camera.py:283-320
 
@@ -209,6 +211,95 @@ def compute_neus_alphas(ray_unit, sdfs, gradients, dists, dist_far, s_var, progr
     prev_cdf = (est_prev * inv_s).sigmoid()
     next_cdf = (est_next * inv_s).sigmoid()
     return ((prev_cdf - next_cdf) / (prev_cdf + eps)).clip(0.0, 1.0)
+
+
+# ---------------------------------------------------------------------------
+# packed (ragged) rays: ray r owns samples offsets[r] .. offsets[r+1]-1
+# ---------------------------------------------------------------------------
+# The reference renders N samples per ray.  A packed batch is restated through
+# that dense form by padding every ray to the longest one:
+#   compositing: padding samples carry alpha = 0, which leaves every real
+#     sample's transmittance and weight unchanged and adds zero weight;
+#   NeuS alphas: padding samples carry the ray's far distance, so the last real
+#     sample's interval ends at far exactly as model.py:170-171 sets it.
+# Real samples are then read back in packed order.
+def pad_packed(x, offsets, fill):
+    """[S, ...] packed -> ([R, Nmax, ...] padded with `fill`, mask [R, Nmax])."""
+    counts = (offsets[1:] - offsets[:-1]).long()
+    R, Nmax = counts.numel(), int(counts.max().item()) if counts.numel() else 0
+    Nmax = max(Nmax, 1)
+    mask = torch.arange(Nmax)[None, :] < counts[:, None]
+    out = torch.empty((R, Nmax) + tuple(x.shape[1:]), dtype=x.dtype)
+    if torch.is_tensor(fill):
+        out[...] = fill.reshape((R, 1) + (1,) * (x.dim() - 1)).to(x.dtype)
+    else:
+        out[...] = fill
+    out[mask] = x
+    return out, mask
+
+
+def alpha_compositing_weights_packed(alphas, offsets):
+    """alphas [S] -> weights [S, 1]."""
+    a, mask = pad_packed(alphas, offsets, 0.0)
+    return alpha_compositing_weights(a)[mask]
+
+
+def composite_packed(quantities, weights, offsets):
+    """quantities [S, K], weights [S, 1] -> [R, K]."""
+    q, _ = pad_packed(quantities, offsets, 0.0)
+    w, _ = pad_packed(weights, offsets, 0.0)
+    return composite(q, w)
+
+
+def compute_neus_alphas_packed(ray_unit, sdfs, gradients, dists, dist_far, offsets, s_var,
+                               progress, anneal_end, eps=1e-5):
+    """ray_unit [R,3], sdfs [S,1], gradients [S,3], dists [S,1], dist_far [R,1] -> alphas [S]."""
+    s, mask = pad_packed(sdfs, offsets, 0.0)
+    g, _ = pad_packed(gradients, offsets, 0.0)
+    d, _ = pad_packed(dists, offsets, dist_far.reshape(-1))
+    alphas = compute_neus_alphas(ray_unit, s, g, d, dist_far.reshape(-1, 1, 1), s_var, progress,
+                                 anneal_end, eps)
+    return alphas[mask]
+
+
+def ragged_batch(n_rays, seed=0, n_spheres=None, max_coarse=160, n_fine=32, per_unit=160.0):
+    """Ragged sample counts in the manner of the packed-batch configuration: rays from
+    cameras on a sphere of radius 3 looking at the origin pass through a random union of
+    1-8 spheres with radii U(0.1, 0.5); a ray gets floor(per_unit * length inside the
+    union) coarse samples (at most max_coarse) plus n_fine resampled ones when it has at
+    least 2, i.e. 0..192 samples.  Returns center [R,3], ray_unit [R,3], offsets int32
+    [R+1] and sorted dists [S] inside the ray's unit-sphere bounds."""
+    g = torch.Generator().manual_seed(seed)
+    k = n_spheres or int(torch.randint(1, 9, (1,), generator=g))
+    sc = (torch.rand(k, 3, generator=g, dtype=torch.float64) * 1.2 - 0.6)
+    sr = torch.rand(k, generator=g, dtype=torch.float64) * 0.4 + 0.1
+    c = torch.randn(n_rays, 3, generator=g, dtype=torch.float64)
+    c = c / c.norm(dim=-1, keepdim=True) * 3.0
+    # three rays in four aim at a sphere (jittered within its radius), the rest anywhere
+    pick = torch.randint(0, k, (n_rays,), generator=g)
+    jit = (torch.rand(n_rays, 3, generator=g, dtype=torch.float64) * 2 - 1) * sr[pick, None]
+    tgt = torch.where(torch.rand(n_rays, 1, generator=g) < 0.75, sc[pick] + jit,
+                      torch.rand(n_rays, 3, generator=g, dtype=torch.float64) * 1.6 - 0.8)
+    v = tgt - c
+    v = v / v.norm(dim=-1, keepdim=True)
+    # length of the ray inside the union, by dense stepping (test data, not timed)
+    steps = torch.linspace(1.5, 4.5, 601, dtype=torch.float64)
+    pts = c[:, None, :] + v[:, None, :] * steps[None, :, None]
+    inside = ((pts[:, :, None, :] - sc[None, None]).norm(dim=-1) < sr[None, None]).any(-1)
+    length = inside.double().sum(-1) * (steps[1] - steps[0])
+    n_c = (length * per_unit).floor().clamp(max=max_coarse).long()
+    counts = torch.where(n_c >= 2, n_c + n_fine, n_c)
+    offsets = torch.zeros(n_rays + 1, dtype=torch.int32)
+    offsets[1:] = counts.cumsum(0).to(torch.int32)
+    near, far, _ = get_dist_bounds(c.float(), v.float())
+    S = int(offsets[-1])
+    ridx = torch.repeat_interleave(torch.arange(n_rays), counts)
+    u = torch.rand(S, generator=g)
+    d = near[ridx, 0] + u * (far[ridx, 0] - near[ridx, 0])
+    # sort within each ray: key = ray index, then distance
+    order = torch.argsort(ridx.double() * 16.0 + (d.double() - near[ridx, 0].double()) /
+                          (far[ridx, 0] - near[ridx, 0] + 1e-9).double(), stable=True)
+    return c.float(), v.float(), offsets, d[order].contiguous()
 
 
 # ---------------------------------------------------------------------------
