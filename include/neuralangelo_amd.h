@@ -441,6 +441,14 @@ int na_mc_emit(const float* volume, int64_t nx, int64_t ny, int64_t nz, int64_t 
                double iso, void* workspace, int64_t ws_bytes, const float* axis_x,
                const float* axis_y, const float* axis_z, double intv, double* vertices,
                int64_t* faces, na_stream_t stream);
+/* Per-block ranges of the concatenated output, valid after na_mc_count on the same
+ * workspace: block_v / block_f [nblocks + 1] (device int64) receive the offset of each
+ * block's first vertex / face (x-major block order, projects/neuralangelo/utils/
+ * mesh.py:82-96) and the totals; the reference's per-block post-processing
+ * (mesh.py:119-133: sphere filter, largest component) acts on these ranges. */
+int64_t na_mc_num_blocks(int64_t nx, int64_t ny, int64_t nz, int64_t block_res);
+int na_mc_block_offsets(int64_t nx, int64_t ny, int64_t nz, int64_t block_res, void* workspace,
+                        int64_t ws_bytes, int64_t* block_v, int64_t* block_f, na_stream_t stream);
 
 #ifdef __cplusplus
 }

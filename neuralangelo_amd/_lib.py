@@ -122,6 +122,8 @@ _SIGNATURES = {
     "na_mc_workspace_size": (_I64, [_I64, _I64, _I64, _I64]),
     "na_mc_count": (ctypes.c_int, [_P, _I64, _I64, _I64, _I64, _F64, _P, _I64, _P, _P]),
     "na_mc_emit": (ctypes.c_int, [_P, _I64, _I64, _I64, _I64, _F64, _P, _I64, _P, _P, _P, _F64, _P, _P, _P]),
+    "na_mc_num_blocks": (_I64, [_I64, _I64, _I64, _I64]),
+    "na_mc_block_offsets": (ctypes.c_int, [_I64, _I64, _I64, _I64, _P, _I64, _P, _P, _P]),
 }
 
 _lib = None

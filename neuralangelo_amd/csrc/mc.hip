@@ -583,6 +583,28 @@ __global__ void mc_zero_totals_kernel(int64_t* totals) {
   if (threadIdx.x < 2) totals[threadIdx.x] = 0;
 }
 
+// Offsets of every block's vertices and faces in the concatenated output:
+// block b starts at tile b * tpb (tiles are block-major, tpb per block), and
+// entry nblocks holds the totals.
+__global__ void __launch_bounds__(256)
+mc_block_offsets_kernel(int64_t nblocks, int64_t tpb, int64_t ntiles,
+                        const int* __restrict__ tile_v, const int* __restrict__ tile_f,
+                        const int64_t* __restrict__ toff_v, const int64_t* __restrict__ toff_f,
+                        const int64_t* __restrict__ gsum_v, const int64_t* __restrict__ gsum_f,
+                        int64_t* __restrict__ block_v, int64_t* __restrict__ block_f) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nblocks) return;
+  if (b < nblocks) {
+    const int64_t t = b * tpb;
+    block_v[b] = toff_v[t] + gsum_v[t / kScanGroup];
+    block_f[b] = toff_f[t] + gsum_f[t / kScanGroup];
+  } else {
+    const int64_t t = ntiles - 1;
+    block_v[b] = toff_v[t] + gsum_v[t / kScanGroup] + tile_v[t];
+    block_f[b] = toff_f[t] + gsum_f[t / kScanGroup] + tile_f[t];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -718,6 +740,31 @@ extern "C" int na_mc_emit(const float* volume, int64_t nx, int64_t ny, int64_t n
   else
     hipLaunchKernelGGL(mc_emit_kernel<uint64_t>, grid, dim3(kMcBlock), 0, s, p.g, wt, o, p.ntiles,
                        w.tile_v, w.tile_f, w.toff_f, w.gsum_f, vertices, faces);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int64_t na_mc_num_blocks(int64_t nx, int64_t ny, int64_t nz, int64_t block_res) {
+  NA_CHECK_ARG(nx >= 0 && ny >= 0 && nz >= 0, "negative size");
+  const McPlan p = mc_plan(nullptr, nx, ny, nz, block_res, 0.0);
+  if (p.empty) return 0;
+  return p.g.nb[0] * p.g.nb[1] * p.g.nb[2];
+}
+
+extern "C" int na_mc_block_offsets(int64_t nx, int64_t ny, int64_t nz, int64_t block_res,
+                                   void* workspace, int64_t ws_bytes, int64_t* block_v,
+                                   int64_t* block_f, na_stream_t stream) {
+  NA_CHECK_ARG(nx >= 0 && ny >= 0 && nz >= 0, "negative size");
+  const McPlan p = mc_plan(nullptr, nx, ny, nz, block_res, 0.0);
+  if (p.empty) return NA_OK;
+  NA_CHECK_ARG(workspace && block_v && block_f, "null pointer");
+  NA_CHECK_ARG((int64_t)mc_ws_layout(p, nullptr, nullptr) <= ws_bytes, "workspace too small");
+  McWorkspace w;
+  mc_ws_layout(p, workspace, &w);
+  const int64_t nblocks = p.g.nb[0] * p.g.nb[1] * p.g.nb[2];
+  hipLaunchKernelGGL(mc_block_offsets_kernel, dim3((unsigned)ceil_div(nblocks + 1, (int64_t)256)),
+                     dim3(256), 0, (hipStream_t)stream, nblocks, p.g.tpb, p.ntiles, w.tile_v,
+                     w.tile_f, w.toff_v, w.toff_f, w.gsum_v, w.gsum_f, block_v, block_f);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -14,14 +14,15 @@ from . import _lib
 
 
 def lattice_marching_cubes(volume, block_res=0, isovalue=0.0, axes=None, intv=1.0,
-                           workspace=None):
+                           workspace=None, block_ranges=False):
     """Concatenated per-block meshes of a lattice volume (CUDA float32 [X,Y,Z]).
 
     block_res <= 0 treats the volume as one block.  With ``axes`` (three CUDA
     float32 lattice coordinate vectors) vertices are ``local*intv + axes[block origin]``
     (projects/neuralangelo/utils/mesh.py:123), else block-local index coordinates.
     Returns (V [nv,3] float64, F [nf,3] int64) CUDA tensors; F indexes the
-    concatenated vertex list.
+    concatenated vertex list.  block_ranges: also return the per-block offsets
+    (block_v, block_f: int64 [nblocks + 1] CUDA tensors, x-major block order).
     """
     _lib.require_cuda(volume)
     if volume.dim() != 3:
@@ -51,7 +52,16 @@ def lattice_marching_cubes(volume, block_res=0, isovalue=0.0, axes=None, intv=1.
         _lib.call("na_mc_emit", _lib.ptr(vol), nx, ny, nz, int(block_res), float(isovalue),
                   _lib.ptr(workspace), int(workspace.numel()), _lib.ptr(ax[0]), _lib.ptr(ax[1]),
                   _lib.ptr(ax[2]), float(intv), _lib.ptr(V), _lib.ptr(F), s)
-    return V, F
+    if not block_ranges:
+        return V, F
+    nb = lib.na_mc_num_blocks(nx, ny, nz, int(block_res))
+    _lib.check(nb, "na_mc_num_blocks")
+    bv = torch.zeros(int(nb) + 1, dtype=torch.int64, device=dev)
+    bf = torch.zeros(int(nb) + 1, dtype=torch.int64, device=dev)
+    if nb:
+        _lib.call("na_mc_block_offsets", nx, ny, nz, int(block_res), _lib.ptr(workspace),
+                  int(workspace.numel()), _lib.ptr(bv), _lib.ptr(bf), s)
+    return V, F, bv, bf
 
 
 def marching_cubes_gpu(volume, isovalue=0.0, workspace=None):

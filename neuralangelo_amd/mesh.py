@@ -1,22 +1,29 @@
-"""Mesh extraction on the GPU (projects/neuralangelo/utils/mesh.py:24-49,68-148).
+"""Mesh extraction on the GPU (projects/neuralangelo/utils/mesh.py:24-158,
+projects/neuralangelo/scripts/extract_mesh.py:80-101).
 
 The reference walks LatticeGrid blocks of block_res+1 points (overlapping by
 one point), evaluates the SDF of each block on the GPU, copies it to the host,
 runs marching cubes on the CPU, maps vertices to world space
-(V*intv + xyz[0,0,0]), drops faces with a vertex outside the unit sphere and
-concatenates the blocks without vertex welding.
+(V*intv + xyz[0,0,0]), colours them (optional), drops faces with a vertex
+outside the unit sphere, keeps each block's largest connected component
+(optional) and concatenates the blocks without vertex welding.
 
 Here the SDF is evaluated once over the whole lattice (in slabs of bounded
 size; every lattice point once, where the reference evaluates block seams
 twice), stays in HBM, and one lattice marching-cubes launch chain produces the
 concatenation of all per-block meshes, in block order, with the world
-transform fused.  The sphere filter runs on the device over the concatenated
-mesh; since it acts per vertex and per face and the order is the block order,
-the result equals filtering each block and concatenating.  Returns numpy
-(vertices float64 [nv,3], faces int64 [nf,3]).
+transform fused, plus each block's vertex / face range.  Colours come from the
+network at the vertices (extract_texture).  The sphere filter runs on the
+device over the concatenated mesh: it acts per vertex and per face, so it
+equals filtering each block.  The largest-component filter runs per block on
+the host (face components joined through shared edges, as trimesh's
+Trimesh.split; blocks share no vertices, so components never cross blocks).
+Returns numpy arrays (vertices float64 [nv,3], faces int64 [nf,3][, colours
+uint8 [nv,3]]).
 
-Not reproduced: trimesh's own processing (vertex merging at 1e-8) and the
-largest-connected-component / texture options (trimesh is not available).
+Not reproduced: trimesh's own processing on construction (merging vertices
+that coincide to 1e-8, which only happens when the SDF is exactly 0 at a
+lattice point).
 """
 import numpy as np
 import torch
@@ -68,15 +75,147 @@ def lattice_sdf(sdf_func, axes, device, max_points=1 << 24):
 
 
 @torch.no_grad()
-def extract_mesh(sdf_func, bounds, intv, block_res=64, device=None, max_points=1 << 24):
+def extract_texture(xyz, neural_rgb, neural_sdf, appear_embed=None, chunk=1 << 20):
+    """Per-vertex colours (mesh.py:53-65): the colour network at the vertex, with the
+    unit SDF gradient as normal and its negation as the viewing direction; rgb * 255
+    truncated to uint8.  xyz: [N,3] numpy or tensor -> uint8 numpy [N,3]."""
+    if appear_embed is not None:
+        raise NotImplementedError("appearance embeddings are outside the accelerated path")
+    dev = neural_sdf.tcnn_encoding.params.device
+    x = torch.as_tensor(np.asarray(xyz) if not torch.is_tensor(xyz) else xyz).to(
+        dev, torch.float32).reshape(-1, 3)
+    out = torch.empty(x.shape[0], 3, dtype=torch.uint8, device=dev)
+    for i in range(0, x.shape[0], chunk):
+        p = x[i:i + chunk].contiguous()
+        sdfs, feats = neural_sdf.forward(p)
+        gradients, _ = neural_sdf.compute_gradients(p, training=False, sdf=sdfs)
+        normals = torch.nn.functional.normalize(gradients, dim=-1)
+        rgbs = neural_rgb.forward(p, normals, -normals, feats, None)
+        out[i:i + chunk] = (rgbs.float() * 255).to(torch.uint8)
+    return out.cpu().numpy()
+
+
+def _face_areas(V, F):
+    e1 = V[F[:, 1]] - V[F[:, 0]]
+    e2 = V[F[:, 2]] - V[F[:, 0]]
+    return 0.5 * np.linalg.norm(np.cross(e1, e2), axis=-1)
+
+
+def largest_component_per_block(V, F, block_f, C=None):
+    """Keep, in every block's face range [block_f[b], block_f[b+1]), the connected
+    component of largest area (mesh.py:151-158 applied per block as at mesh.py:130):
+    faces are connected through shared edges (trimesh's face adjacency); the kept
+    faces keep their order, vertices are renumbered in their original order (trimesh
+    submesh).  Blocks whose range is empty stay empty."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    nf = F.shape[0]
+    if nf == 0:
+        return (V, F) if C is None else (V, F, C)
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    e.sort(axis=1)
+    fid = np.tile(np.arange(nf), 3)
+    key = e[:, 0].astype(np.int64) * (V.shape[0] + 1) + e[:, 1]
+    order = np.argsort(key, kind="stable")
+    ks, fs = key[order], fid[order]
+    same = ks[1:] == ks[:-1]
+    a, b = fs[:-1][same], fs[1:][same]
+    g = sp.coo_matrix((np.ones(a.shape[0], np.int8), (a, b)), shape=(nf, nf))
+    ncomp, labels = connected_components(g, directed=False)
+    area = np.bincount(labels, weights=_face_areas(V, F), minlength=ncomp)
+    block_of_face = np.searchsorted(block_f, np.arange(nf), side="right") - 1
+    first_face = np.full(ncomp, nf, np.int64)
+    np.minimum.at(first_face, labels, np.arange(nf))
+    comp_block = block_of_face[first_face]
+    # per block: the component of largest area, the first one on ties
+    best = {}
+    for c in np.argsort(first_face, kind="stable"):
+        blk = comp_block[c]
+        if blk not in best or area[c] > area[best[blk]]:
+            best[blk] = c
+    keep_comp = np.zeros(ncomp, bool)
+    keep_comp[list(best.values())] = True
+    Fk = F[keep_comp[labels]]
+    used = np.zeros(V.shape[0], bool)
+    used[Fk.reshape(-1)] = True
+    remap = np.cumsum(used) - 1
+    out = (V[used], remap[Fk])
+    return out if C is None else out + (C[used],)
+
+
+def nondegenerate_faces(V, F, height=1e-8):
+    """Mask of faces whose smallest height (2 area / longest edge) exceeds `height`
+    (trimesh's nondegenerate_faces with its merge tolerance, as used at
+    extract_mesh.py:99)."""
+    if F.shape[0] == 0:
+        return np.zeros(0, bool)
+    t = V[F]
+    longest = np.max(np.linalg.norm(t - np.roll(t, 1, axis=1), axis=-1), axis=-1)
+    h = 2.0 * _face_areas(V, F) / np.maximum(longest, 1e-300)
+    return h > height
+
+
+def write_ply(path, V, F, C=None):
+    """Binary little-endian PLY: vertices as doubles, optional RGBA uint8 colours
+    (alpha 255), faces as uint8-counted int32 index lists."""
+    V = np.ascontiguousarray(V, dtype="<f8")
+    F = np.ascontiguousarray(F, dtype="<i4")
+    header = ["ply", "format binary_little_endian 1.0", f"element vertex {V.shape[0]}",
+              "property double x", "property double y", "property double z"]
+    vdt = [("x", "<f8"), ("y", "<f8"), ("z", "<f8")]
+    if C is not None:
+        header += ["property uchar red", "property uchar green", "property uchar blue",
+                   "property uchar alpha"]
+        vdt += [("red", "u1"), ("green", "u1"), ("blue", "u1"), ("alpha", "u1")]
+    header += [f"element face {F.shape[0]}", "property list uchar int vertex_indices",
+               "end_header"]
+    verts = np.empty(V.shape[0], dtype=vdt)
+    verts["x"], verts["y"], verts["z"] = V[:, 0], V[:, 1], V[:, 2]
+    if C is not None:
+        C = np.asarray(C, dtype=np.uint8)
+        verts["red"], verts["green"], verts["blue"] = C[:, 0], C[:, 1], C[:, 2]
+        verts["alpha"] = C[:, 3] if C.shape[1] > 3 else 255
+    faces = np.empty(F.shape[0], dtype=[("n", "u1"), ("v", "<i4", (3,))])
+    faces["n"] = 3
+    faces["v"] = F
+    with open(path, "wb") as f:
+        f.write(("\n".join(header) + "\n").encode("ascii"))
+        f.write(verts.tobytes())
+        f.write(faces.tobytes())
+
+
+@torch.no_grad()
+def extract_mesh(sdf_func, bounds, intv, block_res=64, texture_func=None, filter_lcc=False,
+                 device=None, max_points=1 << 24):
     """sdf_func(xyz [...,3] on the GPU) -> [...,1]; the caller passes the negated SDF
-    (projects/neuralangelo/scripts/extract_mesh.py:85)."""
+    (projects/neuralangelo/scripts/extract_mesh.py:85).  texture_func(V [N,3]) -> uint8
+    colours [N,3] (e.g. functools.partial(extract_texture, neural_rgb=..., neural_sdf=...)).
+    Returns (V, F) or, with texture_func, (V, F, C)."""
     device = device or torch.device("cuda", torch.cuda.current_device())
     axes = lattice_axes(bounds, intv)
     vol = lattice_sdf(sdf_func, axes, device, max_points)
-    V, F = lattice_marching_cubes(vol, block_res, 0.0, axes=axes, intv=intv)
+    V, F, _, block_f = lattice_marching_cubes(vol, block_res, 0.0, axes=axes, intv=intv,
+                                              block_ranges=True)
     del vol
+    C = None
+    if texture_func is not None and V.shape[0] > 0:
+        C = np.asarray(texture_func(V.float()), dtype=np.uint8)
+    empty = (np.zeros((0, 3)), np.zeros((0, 3), np.int64)) + \
+        ((np.zeros((0, 3), np.uint8),) if texture_func is not None else ())
     if V.shape[0] == 0:
-        return np.zeros((0, 3)), np.zeros((0, 3), np.int64)
+        return empty
+    mask = V.norm(dim=-1) < 1.0
+    if not bool(mask.any()):
+        return empty
+    fmask = mask[F[:, 0]] & mask[F[:, 1]] & mask[F[:, 2]]
+    kept_before = torch.cumsum(fmask.to(torch.int64), 0)
+    block_f = torch.cat([torch.zeros(1, dtype=torch.int64, device=F.device), kept_before])[block_f]
     V, F = filter_outside_sphere(V, F)
-    return V.cpu().numpy(), F.cpu().numpy()
+    Vn, Fn = V.cpu().numpy(), F.cpu().numpy()
+    if C is not None:
+        C = C[mask.cpu().numpy()]
+    if filter_lcc:
+        res = largest_component_per_block(Vn, Fn, block_f.cpu().numpy(), C)
+        Vn, Fn = res[0], res[1]
+        C = res[2] if C is not None else None
+    return (Vn, Fn) if texture_func is None else (Vn, Fn, C)

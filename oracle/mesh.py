@@ -6,6 +6,9 @@ extract_mesh    -> projects/neuralangelo/utils/mesh.py:24-49 (LatticeGrid blocks
                    of block_res+1 points, :68-100), marching_cubes per block
                    (:119-133), sphere filter (:136-148), concatenation without
                    vertex welding (:44-46)
+largest_component -> filter_largest_cc (:151-158): trimesh's edge-connected split and
+                   the component of largest area (trimesh is absent; its split is
+                   restated from its documented behaviour)
 
 Pinned by (tests/test_oracle_mesh.py): every one of the 256 single-cube cases
 against the case table's crossed-edge sets and triangle counts; watertightness
@@ -58,21 +61,25 @@ def marching_cubes(volume, isovalue=0.0):
     return V, F
 
 
-def lattice_marching_cubes(vol, block_res, isovalue=0.0, axes=None, intv=1.0):
+def lattice_marching_cubes(vol, block_res, isovalue=0.0, axes=None, intv=1.0, counts=False):
     """The reference's per-block loop (mesh.py:24-49, :82-96 slicing, :121-123) on a
     stored lattice volume, without the sphere filter: blocks of block_res+1
     points, x-major; each block's V (index coords, or V*intv + axes[block
-    origin] when axes are given) and F + running vertex offset, concatenated."""
+    origin] when axes are given) and F + running vertex offset, concatenated.
+    counts: also return each block's vertex and face count (empty blocks included)."""
     vol = np.asarray(vol, np.float32)
     n = vol.shape
     nb = [int(np.ceil(d / block_res)) for d in n]
     Vs, Fs, off = [], [], 0
+    cv, cf = [], []
     for bx in range(nb[0]):
         for by in range(nb[1]):
             for bz in range(nb[2]):
                 xi, yi, zi = bx * block_res, by * block_res, bz * block_res
                 blk = vol[xi:xi + block_res + 1, yi:yi + block_res + 1, zi:zi + block_res + 1]
                 V, F = marching_cubes(blk, isovalue)
+                cv.append(V.shape[0])
+                cf.append(F.shape[0])
                 if V.shape[0] == 0:
                     continue
                 if axes is not None:
@@ -81,8 +88,59 @@ def lattice_marching_cubes(vol, block_res, isovalue=0.0, axes=None, intv=1.0):
                 Fs.append(F + off)
                 off += V.shape[0]
     if not Vs:
-        return np.zeros((0, 3)), np.zeros((0, 3), np.int64)
-    return np.concatenate(Vs), np.concatenate(Fs)
+        out = np.zeros((0, 3)), np.zeros((0, 3), np.int64)
+    else:
+        out = np.concatenate(Vs), np.concatenate(Fs)
+    return out + (np.array(cv, np.int64), np.array(cf, np.int64)) if counts else out
+
+
+def largest_component(V, F, C=None):
+    """filter_largest_cc (mesh.py:151-158) restated: trimesh's Trimesh.split joins faces
+    that share an edge; the component of largest total area is kept (the first on
+    ties), with its faces in their original order and its vertices (those it
+    references) in their original order.  Union-find over faces, sequential."""
+    nf = len(F)
+    if nf == 0 or len(V) == 0:
+        return (V[:0], F[:0]) + ((C[:0],) if C is not None else ())
+    parent = list(range(nf))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    owner = {}
+    for f in range(nf):
+        a, b, c = (int(x) for x in F[f])
+        for e in ((a, b), (b, c), (c, a)):
+            e = (min(e), max(e))
+            if e in owner:
+                ra, rb = find(owner[e]), find(f)
+                if ra != rb:
+                    parent[max(ra, rb)] = min(ra, rb)
+            else:
+                owner[e] = f
+    roots = [find(f) for f in range(nf)]
+    tri = V[F]
+    areas = 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=-1)
+    comp_area, order = {}, []
+    for f in range(nf):
+        r = roots[f]
+        if r not in comp_area:
+            comp_area[r] = 0.0
+            order.append(r)
+        comp_area[r] += areas[f]
+    best = order[0]
+    for r in order[1:]:
+        if comp_area[r] > comp_area[best]:
+            best = r
+    keep = np.array([r == best for r in roots])
+    Fk = F[keep]
+    used = np.zeros(len(V), bool)
+    used[Fk.reshape(-1)] = True
+    remap = np.cumsum(used) - 1
+    return (V[used], remap[Fk]) + ((C[used],) if C is not None else ())
 
 
 def lattice_blocks(bounds, intv, block_res):
@@ -104,33 +162,44 @@ def lattice_blocks(bounds, intv, block_res):
         yield idx, torch.stack([x, y, z], dim=-1)
 
 
-def filter_points_outside_bounding_sphere(V, F):
-    """mesh.py:136-148 on plain arrays."""
+def filter_points_outside_bounding_sphere(V, F, C=None):
+    """mesh.py:136-148 on plain arrays (colours C follow their vertices)."""
     mask = np.linalg.norm(V, axis=-1) < 1.0
     if not np.any(mask):
-        return np.zeros((0, 3)), np.zeros((0, 3), np.int64)
+        out = np.zeros((0, 3)), np.zeros((0, 3), np.int64)
+        return out + ((np.zeros((0, 3), np.uint8),) if C is not None else ())
     indices = np.ones(len(V), dtype=np.int64) * -1
     indices[mask] = np.arange(mask.sum())
     fmask = mask[F[:, 0]] & mask[F[:, 1]] & mask[F[:, 2]]
-    return V[mask], indices[F[fmask]]
+    out = V[mask], indices[F[fmask]]
+    return out + ((C[mask],) if C is not None else ())
 
 
-def extract_mesh(sdf_func, bounds, intv, block_res=64):
-    """mesh.py:24-49: per block, MC on sdf_func(xyz) (already negated by the caller,
-    extract_mesh.py:85), V*intv + xyz[0,0,0], sphere filter, concatenation."""
-    Vs, Fs, off = [], [], 0
+def extract_mesh(sdf_func, bounds, intv, block_res=64, texture_func=None, filter_lcc=False):
+    """mesh.py:24-49 with marching_cubes (:119-133): per block, MC on sdf_func(xyz)
+    (already negated by the caller, extract_mesh.py:85), V*intv + xyz[0,0,0], colours
+    texture_func(V), sphere filter, the block's largest component when filter_lcc;
+    the blocks concatenated.  Returns (V, F) or, with texture_func, (V, F, C)."""
+    Vs, Fs, Cs, off = [], [], [], 0
     for _, xyz in lattice_blocks(bounds, intv, block_res):
         sdf = sdf_func(xyz)[..., 0]
         V, F = marching_cubes(sdf.numpy(), 0.0)
         if V.shape[0] == 0:
             continue
         V = V * intv + xyz[0, 0, 0].numpy()
-        V, F = filter_points_outside_bounding_sphere(V, F)
+        C = np.asarray(texture_func(V), np.uint8) if texture_func is not None else None
+        res = filter_points_outside_bounding_sphere(V, F, C)
+        if filter_lcc and res[0].shape[0] > 0:
+            res = largest_component(*res)
+        V, F = res[0], res[1]
         if V.shape[0] == 0:
             continue
         Vs.append(V)
         Fs.append(F + off)
+        if texture_func is not None:
+            Cs.append(res[2])
         off += V.shape[0]
+    tail = (np.concatenate(Cs) if Cs else np.zeros((0, 3), np.uint8),) if texture_func else ()
     if not Vs:
-        return np.zeros((0, 3)), np.zeros((0, 3), np.int64)
-    return np.concatenate(Vs), np.concatenate(Fs)
+        return (np.zeros((0, 3)), np.zeros((0, 3), np.int64)) + tail
+    return (np.concatenate(Vs), np.concatenate(Fs)) + tail
