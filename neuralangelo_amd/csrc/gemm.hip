@@ -1109,6 +1109,41 @@ __device__ inline float softplus_delta_sel(float z0, float dz, float beta, float
   return lp * ib;
 }
 
+// log1p(t) for t >= 0 small or moderate: series below 1/32, else the hardware log
+__device__ inline float log1p_pos(float t) {
+  return t < 0.03125f ? t * (1.0f - t * (0.5f - t * (1.0f / 3.0f - t * 0.25f))) : __logf(1.0f + t);
+}
+
+// Increment h = softplus(z0 + dz) - softplus(z0) and value y1 = softplus(z0 + dz).
+// With t0 = exp(-|b z0|): softplus(z0) = max(z0, 0) + log1p(t0) / b and
+// sigmoid(b z0) = (z0 >= 0 ? 1 : t0) / (1 + t0).  For |b dz| < 1 (the tap
+// increments) h = log1p(sigmoid(b z0) expm1(b dz)) / b, free of cancellation
+// (1 + that argument stays above 0.36); larger increments take the direct
+// difference, which is accurate there.  The smooth softplus differs from the
+// thresholded one (linear for b z > 20) by at most log1p(exp(-20)) / b, far
+// below the float32 spacing of those values.
+__device__ inline void softplus_step(float z0, float dz, float beta, float ib, float& h, float& y1) {
+  const float a0 = z0 * beta;
+  const float t0 = __expf(-fabsf(a0));
+  const float sp0 = fmaxf(z0, 0.0f) + log1p_pos(t0) * ib;
+  const float u = beta * dz;
+  if (fabsf(u) < 1.0f) {
+    const float sig = (a0 >= 0.0f ? 1.0f : t0) * __frcp_rn(1.0f + t0);
+    const float em = fabsf(u) < 0.03125f ? u * (1.0f + u * (0.5f + u * (1.0f / 6.0f + u * (1.0f / 24.0f))))
+                                         : __expf(u) - 1.0f;
+    const float y = sig * em;
+    const float lp = fabsf(y) < 0.03125f
+                         ? y * (1.0f - y * (0.5f - y * (1.0f / 3.0f - y * (0.25f - y * 0.2f))))
+                         : __logf(1.0f + y);
+    h = lp * ib;
+  } else {
+    const float z1 = z0 + dz;
+    const float t1 = __expf(-fabsf(z1 * beta));
+    h = (fmaxf(z1, 0.0f) + log1p_pos(t1) * ib) - sp0;
+  }
+  y1 = sp0 + h;
+}
+
 // One 32 x 32 block of the wave, staged in eb: lane (rl, cg) handles columns
 // nb .. nb+7 of rows mrow0 + rl and mrow0 + 16 + rl.  cs collects column sums,
 // hs the head partial of each of the two rows.
@@ -1167,10 +1202,7 @@ __device__ __attribute__((always_inline)) inline void persist_rows(
       float z0[8];
       load_cols<FULL>(e.zc + mz * e.ld_zc + nb, nc, z0);
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        hv[c] = softplus_delta_sel(z0[c], x[c], e.beta, ib);
-        x[c] = softplus_sel(z0[c] + x[c], e.beta, ib);
-      }
+      for (int c = 0; c < 8; ++c) softplus_step(z0[c], x[c], e.beta, ib, hv[c], x[c]);
     }
     if (e.c_dtype == NA_BF16) store_cols<FULL>((bf16_t*)e.C + m * e.ldc + nb, nc, x);
     else store_cols<FULL>((float*)e.C + m * e.ldc + nb, nc, x);

@@ -232,3 +232,33 @@ def test_weight_grad_row_reduction(cuda, M, N, K):
     Fn.linear_weight_grad(dpre.to(cuda), X.to(cuda), K, K, dW)
     err = (dW.cpu().double() - ref).abs().max().item()
     assert err < 1e-3 * (M ** 0.5) * 0.01 + 1e-4, err
+
+
+def test_increment_epilogue_large_steps_and_linear_region(cuda):
+    """Increment mode where |beta * dz| >= 1 and beta * z0 lies in or near the linear
+    region (17..25), where sigmoid(beta z0) rounds to 1 in float32: values and
+    increments stay finite and match float64."""
+    bf = torch.bfloat16
+    M, N, K = 4096, 128, 16
+    v = torch.linspace(-1.0, 1.0, M)
+    A = torch.zeros(M, K)
+    A[:, 0] = v
+    B = torch.zeros(N, K)
+    B[:, 0] = 1.0
+    A, B = A.to(bf), B.to(bf)
+    acc = A.double() @ B.double().t()  # acc[m, n] = v_m (bf16-rounded)
+    rows = 512
+    z0 = torch.linspace(-0.3, 0.3, N)[None, :].repeat(rows, 1)
+    z0[:, ::4] = torch.linspace(0.17, 0.25, N // 4)[None, :]  # beta z0 in 17..25
+    z0d = z0.double()[torch.arange(M) % rows]
+    y1 = _act(z0d + acc, NA_ACT_SOFTPLUS)
+    dsp = y1 - _act(z0d, NA_ACT_SOFTPLUS)
+    C = torch.empty(M, N, dtype=torch.float32, device=cuda)
+    hw = torch.ones(N)
+    ho = torch.empty(Fn.head_parts(N), M, device=cuda)
+    Fn.gemm(A.to(cuda), K, 0, B.to(cuda), K, 0, M, N, K, C, N, act=NA_ACT_SOFTPLUS,
+            zc=z0.to(cuda), zc_rows=rows, head_w=hw.to(cuda), head_out=ho)
+    got = C.cpu().double()
+    assert torch.isfinite(got).all() and torch.isfinite(ho).all()
+    assert ((got - y1).abs() <= 1e-5 * y1.abs() + 1e-6).all()
+    assert (ho.cpu().double().sum(0) - dsp.sum(1)).abs().max() < 1e-4
