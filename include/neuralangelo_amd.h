@@ -344,6 +344,25 @@ int na_neus_alpha_packed_bwd(const float* ray_unit, const float* sdfs, const flo
                              int64_t S, const float* dalpha, float* dsdf, float* dgrad,
                              float* ds_var, na_stream_t stream);
 
+/* Background NeRF (projects/neuralangelo/utils/modules.py:232-311).
+ * na_bg_encode: points [n,3] -> rows [n, ld] of [q | Fourier(q)] with
+ *   q = [p/|p|, 1/|p|] (modules.py:290-293) and nerf_util.positional_encoding
+ *   (nerf_util.py:134-147, `levels` frequencies 2^l pi, sin block then cos block
+ *   per component), zero-padded to ld (>= 4 + 8 levels); f32 or bf16 out.
+ * na_bg_alpha_fwd: alpha = 1 - exp(-softplus(o) (d_{i+1} - d_i)), d_N = 1e10
+ *   (render.volume_rendering_alphas_dist, render.py:68-84, with the density
+ *   activation softplus of modules.py:270-272); o = column 0 of mlp_out rows
+ *   (stride ld), dists [R, N]; density optional output [R, N].
+ * na_bg_alpha_bwd: d(o) from dalpha [R,N] and/or ddensity [R,N], written to column 0
+ *   of dmlp_out rows (stride ld_d). */
+int na_bg_encode(const float* points, int64_t n, int32_t levels, void* out, int32_t out_dtype,
+                 int64_t ld, na_stream_t stream);
+int na_bg_alpha_fwd(const float* mlp_out, int64_t ld, const float* dists, int64_t R, int32_t N,
+                    float* density, float* alpha, na_stream_t stream);
+int na_bg_alpha_bwd(const float* mlp_out, int64_t ld, const float* dists, int64_t R, int32_t N,
+                    const float* dalpha, const float* ddensity, float* dmlp_out, int64_t ld_d,
+                    na_stream_t stream);
+
 /* 4-tap numerical gradient (projects/neuralangelo/utils/modules.py:157-175).
  * taps: writes x + eps*k_i for k = (1,-1,-1), (-1,-1,1), (-1,1,-1), (1,1,1) as
  * [4, n, 3] (eps = float32 of normal_eps/sqrt(3)).  combine: s_taps [4, n],

@@ -29,16 +29,18 @@ def _strip(key):
     return key
 
 
-def network_state(state):
+def network_state(state, known=()):
     """Reference checkpoint dict (or its ``model`` entry) -> network keys without the
     wrapper prefixes, and the list of keys this package does not model (background
-    NeRF, appearance embedding, EMA copies)."""
+    NeRF weights when the model has no background, appearance embedding, EMA
+    copies).  known: network keys of the target model, which are always kept."""
     if "model" in state and isinstance(state["model"], dict):
         state = state["model"]
+    known = set(known)
     out, skipped = {}, []
     for k, v in state.items():
         name = _strip(k)
-        if any(name.startswith(u) for u in _UNSUPPORTED):
+        if name not in known and any(name.startswith(u) for u in _UNSUPPORTED):
             skipped.append(k)
             continue
         out[name] = v
@@ -53,8 +55,8 @@ def load_reference_checkpoint(model, path_or_state, strict=True):
     if not isinstance(state, dict):
         state = torch.load(path_or_state, map_location="cpu", weights_only=True)
     iteration = state.get("iteration") if isinstance(state.get("iteration"), int) else None
-    net, skipped = network_state(state)
     own = model.state_dict()
+    net, skipped = network_state(state, known=own.keys())
     missing = [k for k in own if k not in net]
     unexpected = [k for k in net if k not in own]
     if strict and (missing or unexpected):

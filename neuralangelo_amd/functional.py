@@ -810,6 +810,96 @@ class WeightedSum(torch.autograd.Function):
 
 
 # ---------------------------------------------------------------------------
+# generic dense layer and the background NeRF pieces
+# ---------------------------------------------------------------------------
+class LinearAct(torch.autograd.Function):
+    """Y = act(X W^T + b) on the GEMM kernels (bias and activation in the epilogue).
+    X [P, K] compute dtype (K a multiple of 8), W [N, K] float32, b [N]; Y [P, N] in
+    out_dtype.  The output width is padded to a multiple of 8 inside (zero rows of W),
+    so every operand of the backward products has aligned rows.  Backward: act'(Y)
+    with the bias gradient in one pass, the weight gradient on the row-reduction
+    kernel, the input gradient on the NN GEMM."""
+
+    @staticmethod
+    def forward(ctx, X, W, b, act, out_dtype):
+        P, K = X.shape
+        N = W.shape[0]
+        N8 = _pad8(N)
+        Wc = W.to(X.dtype)
+        bc = b.float()
+        if N8 != N:
+            Wc = torch_F.pad(Wc, (0, 0, 0, N8 - N))
+            bc = torch_F.pad(bc, (0, N8 - N))
+        Wc, bc = Wc.contiguous(), bc.contiguous()
+        Y = torch.empty(P, N8, dtype=out_dtype, device=X.device)
+        gemm(X, K, 0, Wc, K, 0, P, N8, K, Y, N8, bias=bc, act=act)
+        ctx.save_for_backward(X, Wc, Y)
+        ctx.act, ctx.N = act, N
+        return Y[:, :N] if N8 != N else Y
+
+    @staticmethod
+    def backward(ctx, dY):
+        X, Wc, Y = ctx.saved_tensors
+        P, K = X.shape
+        N, N8 = ctx.N, Wc.shape[0]
+        if N8 != N:
+            d8 = torch.zeros(P, N8, dtype=dY.dtype, device=dY.device)
+            d8[:, :N] = dY
+            dY = d8
+        db = torch.zeros(N8, device=X.device)
+        if ctx.act == NA_ACT_NONE:
+            dpre = dY.to(X.dtype).contiguous()
+            db += dY.float().sum(0)
+        else:
+            dpre = act_grad(dY.contiguous(), Y, ctx.act, X.dtype, colsum=db, n_cols=N8)
+        dW = torch.zeros(N8, K, device=X.device)
+        linear_weight_grad(dpre, X, K, K, dW)
+        dX = None
+        if ctx.needs_input_grad[0]:
+            dX = torch.empty(P, K, dtype=X.dtype, device=X.device)
+            gemm(dpre, N8, 0, _t(Wc), N8, 0, P, K, N8, dX, K)
+        return dX, dW[:N], db[:N], None, None
+
+
+def bg_encode(points, levels, cdt, width):
+    """Background MLP input rows [P, width] (compute dtype): [q | Fourier(q) | 0]."""
+    P = points.shape[0]
+    out = torch.empty(P, width, dtype=cdt, device=points.device)
+    _lib.call("na_bg_encode", _lib.ptr(points.contiguous().float()), P, int(levels), _lib.ptr(out),
+              _code(out), width, _s(points))
+    return out
+
+
+class BgAlpha(torch.autograd.Function):
+    """MLP output rows [R*N, ld] (column 0 = raw density), dists [R, N] -> alphas [R, N]
+    and densities [R, N] (softplus of column 0)."""
+
+    @staticmethod
+    def forward(ctx, out, dists, R, N):
+        o = out.float()
+        dists = dists.contiguous().float()
+        alpha = torch.empty(R, N, device=out.device)
+        density = torch.empty(R, N, device=out.device)
+        _lib.call("na_bg_alpha_fwd", _lib.ptr(o), o.stride(0), _lib.ptr(dists), R, N,
+                  _lib.ptr(density), _lib.ptr(alpha), _s(o))
+        ctx.save_for_backward(o, dists)
+        ctx.RN = (R, N)
+        ctx.shape, ctx.dtype = out.shape, out.dtype
+        return alpha, density
+
+    @staticmethod
+    def backward(ctx, dalpha, ddensity):
+        o, dists = ctx.saved_tensors
+        R, N = ctx.RN
+        dout = torch.zeros(ctx.shape, device=o.device)
+        _lib.call("na_bg_alpha_bwd", _lib.ptr(o), o.stride(0), _lib.ptr(dists), R, N,
+                  _lib.ptr(dalpha.contiguous().float()) if dalpha is not None else None,
+                  _lib.ptr(ddensity.contiguous().float()) if ddensity is not None else None,
+                  _lib.ptr(dout), dout.stride(0), _s(o))
+        return dout.to(ctx.dtype), None, None, None
+
+
+# ---------------------------------------------------------------------------
 # packed (ragged) rays: ray r owns samples offsets[r] .. offsets[r+1]-1
 # ---------------------------------------------------------------------------
 def packed_ray_index(offsets, S):

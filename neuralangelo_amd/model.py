@@ -15,11 +15,22 @@ import torch
 
 from . import functional as Fn
 from . import nerf_util
-from .modules import NeuralRGB, NeuralSDF, _get
+from .modules import BackgroundNeRF, NeuralRGB, NeuralSDF, _get
 
 
-def default_config(log2_T=22, F=8, levels=16, n_coarse=64, n_fine=16, n_hier=2):
-    """The BASELINE hot-path configuration (base.yaml:38-115 with background off)."""
+def background_config():
+    """The reference's default background NeRF (base.yaml:84-104)."""
+    return dict(enabled=True, white=False,
+                mlp=dict(num_layers=8, hidden_dim=256, skip=[4], num_layers_rgb=2,
+                         hidden_dim_rgb=128, skip_rgb=[], activ="relu", activ_params={},
+                         activ_density="softplus", activ_density_params={}),
+                view_dep=True, encoding=dict(type="fourier", levels=10),
+                encoding_view=dict(type="spherical", levels=3))
+
+
+def default_config(log2_T=22, F=8, levels=16, n_coarse=64, n_fine=16, n_hier=2, background=False,
+                   n_background=32):
+    """The hot-path configuration of base.yaml:38-115 (background off unless asked)."""
     return dict(
         object=dict(
             sdf=dict(
@@ -35,8 +46,9 @@ def default_config(log2_T=22, F=8, levels=16, n_coarse=64, n_fine=16, n_hier=2):
                               weight_norm=True),
                      mode="idr", encoding_view=dict(type="spherical", levels=3)),
             s_var=dict(init_val=3.0, anneal_end=0.1)),
-        background=dict(enabled=False, white=False),
-        render=dict(rand_rays=512, num_samples=dict(coarse=n_coarse, fine=n_fine, background=32),
+        background=background_config() if background else dict(enabled=False, white=False),
+        render=dict(rand_rays=512, num_samples=dict(coarse=n_coarse, fine=n_fine,
+                                                    background=n_background),
                     num_sample_hierarchy=n_hier, stratified=True),
         appear_embed=dict(enabled=False, dim=8),
     )
@@ -48,10 +60,8 @@ class Model(torch.nn.Module):
         super().__init__()
         self.cfg_render = _get(cfg_model, "render")
         bg = _get(cfg_model, "background")
-        if _get(bg, "enabled", False):
-            raise NotImplementedError("the background NeRF is outside the accelerated path")
         self.white_background = _get(bg, "white", False)
-        self.with_background = False
+        self.with_background = bool(_get(bg, "enabled", False))
         obj = _get(cfg_model, "object")
         self.anneal_end = _get(_get(obj, "s_var"), "anneal_end")
         self.outside_val = 1000. * (-1 if _get(_get(_get(obj, "sdf"), "mlp"), "inside_out") else 1)
@@ -63,6 +73,9 @@ class Model(torch.nn.Module):
                                     compute_dtype=compute_dtype)
         self.s_var = torch.nn.Parameter(torch.tensor(_get(_get(obj, "s_var"), "init_val"),
                                                      dtype=torch.float32))
+        self.background_nerf = (BackgroundNeRF(bg, _get(cfg_model, "appear_embed"),
+                                               compute_dtype=compute_dtype)
+                                if self.with_background else None)
         self.progress = 1.0
 
     # -- sampling --------------------------------------------------------------
@@ -129,11 +142,12 @@ class Model(torch.nn.Module):
 
     # -- rendering ---------------------------------------------------------------
     def render_rays(self, center, ray_unit, sample_idx=None, stratified=False, rands=None,
-                    dists=None):
+                    dists=None, rands_bg=None):
         """center, ray_unit [R,3] (or [B,R,3]) -> output dict (model.py:124-156).
 
         rands: optional [R, Nc] stratification offsets; dists: optional [R, N]
-        samples to render instead of sampling (parity testing)."""
+        samples to render instead of sampling (parity testing); rands_bg: optional
+        [R, Nb] stratification offsets of the background samples."""
         lead = center.shape[:-1]
         c = center.reshape(-1, 3).contiguous().float()
         v = ray_unit.reshape(-1, 3).contiguous().float()
@@ -143,18 +157,26 @@ class Model(torch.nn.Module):
         with torch.no_grad():
             near, far, outside, coarse = nerf_util.bounds_and_coarse(c, v, n_coarse, rands)
         out = self.render_rays_object(c, v, near, far, outside, coarse, dists=dists)
-        rgb = out["rgb"]
+        rgb, weights, dists_all = out["rgb"], out["weights"], out["dists"]
+        if self.with_background:
+            # object and background samples composited together (model.py:129-141)
+            bg = self.render_rays_background(c, v, far, stratified=stratified, rands=rands_bg)
+            alphas = torch.cat([out["alphas"], bg["alphas"]], dim=1)
+            rgbs = torch.cat([out["rgbs"].view(c.shape[0], -1, 3), bg["rgbs"]], dim=1)
+            weights, rgb = Fn.Composite.apply(alphas, rgbs)
+            dists_all = torch.cat([out["dists"], bg["dists"]], dim=1)
         if self.white_background:
-            rgb = rgb + (1 - out["weights"].sum(dim=1, keepdim=False)[..., None])
+            rgb = rgb + (1 - weights.sum(dim=1, keepdim=False)[..., None])
         R = c.shape[0]
         N = out["dists"].shape[1]
+        Nw = weights.shape[1]
 
         def shp(t, tail):
             return t.view(*lead, *tail)
         res = dict(
             rgb=shp(rgb, (3,)), opacity=shp(out["opacity"], (1,)) if out["opacity"] is not None else None,
-            outside=shp(outside, (1,)), dists=shp(out["dists"], (N, 1)),
-            weights=shp(out["weights"], (N, 1)),
+            outside=shp(outside, (1,)), dists=shp(dists_all, (Nw, 1)),
+            weights=shp(weights, (Nw, 1)),
             gradient=shp(out["gradient"], (3,)) if out["gradient"] is not None else None,
             gradients=shp(out["gradients"], (N, 3)),
             hessians=shp(out["hessians"], (N, 3)) if out["hessians"] is not None else None,
@@ -162,6 +184,31 @@ class Model(torch.nn.Module):
             near=shp(near, (1,)), far=shp(far, (1,)))
         del R
         return res
+
+    def sample_dists_background(self, ray_unit, far, stratified=False, eps=1e-5, rands=None):
+        """model.py:269-273: inverse distances (i + u)/Nb from 1 down to 0, dists =
+        far / (inv + eps) -> [R, Nb] (u = 0.5, or U(0,1) when stratified)."""
+        nb = _get(_get(self.cfg_render, "num_samples"), "background")
+        R = ray_unit.reshape(-1, 3).shape[0]
+        if rands is None:
+            rands = (torch.rand(R, nb, device=ray_unit.device) if stratified else
+                     torch.full((R, nb), 0.5, device=ray_unit.device))
+        i = torch.arange(nb, device=ray_unit.device, dtype=torch.float32)
+        inv = (i[None, :] + rands) / nb * (0.0 - 1.0) + 1.0
+        return far.reshape(R, 1) / (inv + eps)
+
+    def render_rays_background(self, center, ray_unit, far, stratified=False, rands=None):
+        """model.py:192-205: background samples beyond the unit sphere, the background
+        NeRF, and volume-rendering alphas (render.py:68-84)."""
+        with torch.no_grad():
+            dists = self.sample_dists_background(ray_unit, far, stratified=stratified, rands=rands)
+        R, Nb = dists.shape
+        points = (center[:, None, :] + ray_unit[:, None, :] * dists[..., None]).reshape(-1, 3)
+        rays = ray_unit[:, None, :].expand(R, Nb, 3).reshape(-1, 3)
+        out = self.background_nerf.mlp_out(points)
+        rgbs = self.background_nerf.rgb_from(out, rays).view(R, Nb, 3)
+        alphas, _ = Fn.BgAlpha.apply(out, dists, R, Nb)
+        return dict(rgbs=rgbs, dists=dists, alphas=alphas)
 
     def render_rays_object(self, center, ray_unit, near, far, outside, coarse, dists=None):
         if dists is None:

@@ -14,7 +14,7 @@ import torch
 from . import functional as Fn
 from .encoding import Encoding
 from .mlp import MLPforNeuralSDF, MLPwithSkipConnection
-from ._lib import NA_ACT_SIGMOID
+from ._lib import NA_ACT_NONE, NA_ACT_RELU, NA_ACT_SIGMOID
 from .spherical_harmonics import get_spherical_harmonics
 
 
@@ -260,3 +260,112 @@ def _is_buffer_view(t):
 
 def _flat_view(t, P):
     return torch.as_strided(t, (P, t.shape[-1]), (Fn.RGB_LD, 1))
+
+
+class _LinearStack(torch.nn.Module):
+    """nerf_util.MLPwithSkipConnection's parameters (nerf_util.py:158-184): plain
+    Linear layers (default init, last bias zero) named linears.<i>.weight / bias."""
+
+    def __init__(self, layer_dims, skip_connection):
+        super().__init__()
+        self.skip_connection = list(skip_connection)
+        self.linears = torch.nn.ModuleList()
+        pairs = list(zip(layer_dims[:-1], layer_dims[1:]))
+        for li, (k_in, k_out) in enumerate(pairs):
+            if li in self.skip_connection:
+                k_in += layer_dims[0]
+            lin = torch.nn.Linear(k_in, k_out)
+            if li == len(pairs) - 1:
+                with torch.no_grad():
+                    lin.bias.zero_()
+            self.linears.append(lin)
+
+
+class BackgroundNeRF(torch.nn.Module):
+    """Background NeRF (modules.py:232-311) on the GEMM kernels.
+
+    Points are reparametrised on the inverted sphere, q = [p/|p|, 1/|p|], and
+    Fourier-encoded in one kernel into a zero-padded input row; the feature MLP
+    (ReLU, skip connection re-reading that row) ends in [density | feature]; the
+    colour MLP reads [relu(feature) | SH16(view)] and ends in a sigmoid.  Density
+    activation: softplus (beta 1).  Parameters keep the reference's names
+    (mlp_feat.linears.<i>, mlp_rgb.linears.<i>)."""
+
+    def __init__(self, cfg_background, appear_embed, compute_dtype=torch.float32):
+        super().__init__()
+        self.cfg_background = cfg_background
+        if _get(appear_embed, "enabled", False):
+            raise NotImplementedError("appearance embeddings are outside the accelerated path")
+        enc = _get(cfg_background, "encoding")
+        encv = _get(cfg_background, "encoding_view")
+        mlp = _get(cfg_background, "mlp")
+        if _get(enc, "type") != "fourier":
+            raise NotImplementedError("background encoding must be fourier")
+        if _get(encv, "type") != "spherical" or _get(encv, "levels") != 3:
+            raise NotImplementedError("background view encoding must be spherical, degree 3")
+        if not _get(cfg_background, "view_dep", True):
+            raise NotImplementedError("only the view-dependent background is implemented")
+        if _get(mlp, "activ") not in ("relu", "relu_") or _get(mlp, "activ_density") != "softplus":
+            raise NotImplementedError("background activations must be relu / softplus")
+        if _get(mlp, "activ_density_params", {}) or _get(mlp, "skip_rgb", []):
+            raise NotImplementedError("softplus parameters and colour skips are not supported")
+        self.compute_dtype = compute_dtype
+        self.levels = _get(enc, "levels")
+        self.input_dim = 4 + 8 * self.levels
+        self.width = (self.input_dim + 7) // 8 * 8
+        hid = _get(mlp, "hidden_dim")
+        self.hidden = hid
+        layer_dims = [self.input_dim] + [hid] * (_get(mlp, "num_layers") - 1) + [hid + 1]
+        self.mlp_feat = _LinearStack(layer_dims, _get(mlp, "skip", []))
+        layer_dims_rgb = [hid + 16] + [_get(mlp, "hidden_dim_rgb")] * (_get(mlp, "num_layers_rgb") - 1) + [3]
+        self.mlp_rgb = _LinearStack(layer_dims_rgb, [])
+        self.spherical_harmonic_encoding = partial(get_spherical_harmonics, levels=3)
+
+    def _kernel_weight(self, li, lin, last):
+        """Input columns in the kernel layout (the encoded block padded to self.width);
+        the 1 + hidden outputs of the last layer padded to a multiple of 8."""
+        W, b = lin.weight, lin.bias
+        pad_in = self.width - self.input_dim
+        if pad_in and (li == 0 or li in self.mlp_feat.skip_connection):
+            W = torch.nn.functional.pad(W, (0, pad_in))  # the encoded block is last
+        if last:
+            n_out = (W.shape[0] + 7) // 8 * 8
+            W = torch.nn.functional.pad(W, (0, 0, 0, n_out - W.shape[0]))
+            b = torch.nn.functional.pad(b, (0, n_out - b.shape[0]))
+        return W, b
+
+    def mlp_out(self, points):
+        """points [P,3] -> raw feature-MLP output [P, hidden + 1 padded] float32."""
+        X = Fn.bg_encode(points.reshape(-1, 3), self.levels, self.compute_dtype, self.width)
+        h = X
+        n = len(self.mlp_feat.linears)
+        for li, lin in enumerate(self.mlp_feat.linears):
+            if li in self.mlp_feat.skip_connection:
+                h = torch.cat([h, X], dim=-1)
+            last = li == n - 1
+            W, b = self._kernel_weight(li, lin, last)
+            h = Fn.LinearAct.apply(h, W, b, NA_ACT_NONE if last else NA_ACT_RELU,
+                                   torch.float32 if last else self.compute_dtype)
+        return h
+
+    def rgb_from(self, out, rays_unit):
+        """Colour MLP on relu(feature) and SH16 of the view direction per point."""
+        feat = torch.relu(out[:, 1:1 + self.hidden])
+        view = self.spherical_harmonic_encoding(rays_unit.reshape(-1, 3).float())
+        h = torch.cat([feat, view], dim=-1).to(self.compute_dtype).contiguous()
+        n = len(self.mlp_rgb.linears)
+        for li, lin in enumerate(self.mlp_rgb.linears):
+            last = li == n - 1
+            h = Fn.LinearAct.apply(h, lin.weight, lin.bias, NA_ACT_SIGMOID if last else NA_ACT_RELU,
+                                   torch.float32 if last else self.compute_dtype)
+        return h
+
+    def forward(self, points_3D, rays_unit, app_outside):
+        """(rgb [..., 3], density [...]) as modules.py:264-280."""
+        if app_outside is not None:
+            raise NotImplementedError("appearance embeddings are outside the accelerated path")
+        shape = points_3D.shape[:-1]
+        out = self.mlp_out(points_3D.reshape(-1, 3).contiguous().float())
+        density = torch.nn.functional.softplus(out[:, 0])
+        rgb = self.rgb_from(out, rays_unit.expand_as(points_3D))
+        return rgb.view(*shape, 3), density.view(shape)

@@ -207,3 +207,39 @@ def neural_rgb(points, normals, rays_unit, feats, P):
     view_enc = spherical_harmonics3(rays_unit)
     inp = torch.cat([points, view_enc, normals, feats], dim=-1)
     return torch.sigmoid(rgb_mlp(inp, P))
+
+
+# ---------------------------------------------------------------------------
+# background NeRF (projects/neuralangelo/utils/modules.py:232-311)
+# ---------------------------------------------------------------------------
+def positional_encoding(x, num_freq_bases):
+    """nerf_util.py:134-147: per component, sin then cos of x * 2^l * pi, l < L."""
+    freq = 2.0 ** torch.arange(num_freq_bases, dtype=x.dtype) * np.pi
+    spectrum = x[..., None] * freq
+    return torch.stack([spectrum.sin(), spectrum.cos()], dim=-2).view(*x.shape[:-1], -1)
+
+
+def background_nerf(points, rays_unit, Wf, bf, Wr, br, levels=10, skip=(4,)):
+    """BackgroundNeRF.forward (modules.py:264-280) with encode (:282-296): inverted
+    sphere q = [p/|p|, 1/|p|], input [q, PE(q)], feature MLP with skip concat
+    (nerf_util.py:186-196, ReLU), density = softplus(out[0]) (beta 1), feature =
+    relu(out[1:]), colour MLP on [feature, SH16(view)] ending in a sigmoid.
+    Wf, bf / Wr, br: the layers' weights and biases (reference shapes)."""
+    n = points.norm(dim=-1, keepdim=True)
+    q = torch.cat([points / n, 1.0 / n], dim=-1)
+    inp = torch.cat([q, positional_encoding(q, levels)], dim=-1)
+    h = inp
+    for li, (W, b) in enumerate(zip(Wf, bf)):
+        if li in skip:
+            h = torch.cat([h, inp], dim=-1)
+        h = h @ W.t() + b
+        if li != len(Wf) - 1:
+            h = h.relu()
+    density = torch.nn.functional.softplus(h[..., 0])
+    feat = h[..., 1:].relu()
+    x = torch.cat([feat, spherical_harmonics3(rays_unit)], dim=-1)
+    for li, (W, b) in enumerate(zip(Wr, br)):
+        x = x @ W.t() + b
+        if li != len(Wr) - 1:
+            x = x.relu()
+    return x.sigmoid(), density

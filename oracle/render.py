@@ -214,6 +214,27 @@ def compute_neus_alphas(ray_unit, sdfs, gradients, dists, dist_far, s_var, progr
 
 
 # ---------------------------------------------------------------------------
+# background samples and volume-rendering alphas
+# ---------------------------------------------------------------------------
+def sample_dists_background(far, n, rands=None, eps=1e-5):
+    """model.py:269-273: inverse distances sample_dists over (1, 0), dists = far / (inv + eps);
+    far [R, 1] -> [R, n]."""
+    R = far.shape[0]
+    u = torch.full((R, n), 0.5, dtype=far.dtype) if rands is None else rands.to(far.dtype)
+    inv = (torch.arange(n, dtype=far.dtype)[None, :] + u) / n * (0.0 - 1.0) + 1.0
+    return far.reshape(R, 1) / (inv + eps)
+
+
+def volume_rendering_alphas_dist(densities, dists, dist_far=None):
+    """render.py:68-84: densities [..., N], dists [..., N, 1] -> alphas [..., N]."""
+    if dist_far is None:
+        dist_far = torch.empty_like(dists[..., :1, :]).fill_(1e10)
+    d = torch.cat([dists, dist_far], dim=-2)
+    intv = d[..., 1:, 0] - d[..., :-1, 0]
+    return 1 - (-(densities * intv)).exp()
+
+
+# ---------------------------------------------------------------------------
 # packed (ragged) rays: ray r owns samples offsets[r] .. offsets[r+1]-1
 # ---------------------------------------------------------------------------
 # The reference renders N samples per ray.  A packed batch is restated through

@@ -60,3 +60,18 @@ def test_network_state_strips_prefixes_only():
     net, skipped = network_state({"model": {"module.module.s_var": torch.tensor(1.0),
                                             "module.appear_embed.weight": torch.zeros(2)}})
     assert list(net) == ["s_var"] and skipped == ["module.appear_embed.weight"]
+
+
+def test_background_weights_load_when_the_model_has_them(tmp_path):
+    torch.manual_seed(5)
+    src = Model(default_config(log2_T=12, F=4, background=True))
+    torch.manual_seed(6)
+    dst = Model(default_config(log2_T=12, F=4, background=True))
+    path = tmp_path / "bg.pt"
+    torch.save(reference_checkpoint(src, iteration=7), path)
+    it, skipped = load_reference_checkpoint(dst, str(path))
+    assert it == 7 and skipped == []
+    k = "background_nerf.mlp_feat.linears.4.weight"
+    assert tuple(dst.state_dict()[k].shape) == (256, 340)
+    for name, v in src.state_dict().items():
+        assert torch.equal(dst.state_dict()[name], v), name
