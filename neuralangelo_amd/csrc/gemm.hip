@@ -1002,11 +1002,12 @@ typedef __attribute__((ext_vector_type(8))) float f32x8_t;
 // M (or N) read zeros; K-tail chunks get an offset past the record count.
 constexpr uint32_t kOobOffset = 0x80000000u;
 
-__device__ inline __amdgpu_buffer_rsrc_t tile_rsrc(const bf16_t* X, int64_t ld, int64_t rows,
-                                                   int64_t r0, int64_t k0) {
-  const int64_t bytes = (rows - r0) * ld * 2 - k0 * 2;
+// Buffer of one stage: the tile's first row starts at `base` and `ext` bytes of
+// the matrix follow it; the stage starts kb bytes further along the rows.
+__device__ inline __amdgpu_buffer_rsrc_t stage_rsrc(const char* base, int64_t ext, int64_t kb) {
+  const int64_t bytes = ext - kb;
   const int n = bytes > 0x7fffffffLL ? 0x7fffffff : (bytes > 0 ? (int)bytes : 0);
-  return __builtin_amdgcn_make_buffer_rsrc((void*)(X + r0 * ld + k0), 0, n, 0x00020000);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(base + kb), 0, n, 0x00020000);
 }
 
 template <int WN>
@@ -1091,57 +1092,37 @@ __device__ inline float softplus_sel(float v, float beta, float ib) {
   return z > 20.0f ? v : r;
 }
 
-// softplus_delta with the small-argument series chosen by select; only the
-// linear-region case (either argument above the threshold) branches.
-__device__ inline float softplus_delta_sel(float z0, float dz, float beta, float ib) {
-  const float z1 = z0 + dz;
-  const float a0 = z0 * beta, a1 = z1 * beta;
-  if (a0 > 20.0f || a1 > 20.0f)
-    return (a0 > 20.0f && a1 > 20.0f) ? z1 - z0 : softplus_sel(z1, beta, ib) - softplus_sel(z0, beta, ib);
-  const float sig = __frcp_rn(1.0f + __expf(-a0));
-  const float u = beta * dz;
-  const float em = fabsf(u) < 0.03125f ? u * (1.0f + u * (0.5f + u * (1.0f / 6.0f + u * (1.0f / 24.0f))))
-                                       : __expf(u) - 1.0f;
-  const float y = sig * em;
-  const float lp = fabsf(y) < 0.03125f
-                       ? y * (1.0f - y * (0.5f - y * (1.0f / 3.0f - y * (0.25f - y * 0.2f))))
-                       : __logf(1.0f + y);
-  return lp * ib;
-}
-
-// log1p(t) for t >= 0 small or moderate: series below 1/32, else the hardware log
-__device__ inline float log1p_pos(float t) {
-  return t < 0.03125f ? t * (1.0f - t * (0.5f - t * (1.0f / 3.0f - t * 0.25f))) : __logf(1.0f + t);
-}
-
-// Increment h = softplus(z0 + dz) - softplus(z0) and value y1 = softplus(z0 + dz).
-// With t0 = exp(-|b z0|): softplus(z0) = max(z0, 0) + log1p(t0) / b and
-// sigmoid(b z0) = (z0 >= 0 ? 1 : t0) / (1 + t0).  For |b dz| < 1 (the tap
-// increments) h = log1p(sigmoid(b z0) expm1(b dz)) / b, free of cancellation
-// (1 + that argument stays above 0.36); larger increments take the direct
-// difference, which is accurate there.  The smooth softplus differs from the
-// thresholded one (linear for b z > 20) by at most log1p(exp(-20)) / b, far
-// below the float32 spacing of those values.
+// Increment h = softplus(z0 + dz) - softplus(z0) and value y1 = softplus(z0 + dz)
+// for softplus(z) = ln(1 + exp(b z)) / b.  With a0 = b z0, u = b dz and the two
+// factors t0 = exp(-|a0|), e = exp(-|u|) in (0, 1], taking the larger exponent
+// out of ln(1 + exp(a0 + u)) in each sign case gives
+//   b y1 = max(a0, 0) + max(u, 0) + ln P,   P = 1 + t0 e  (z0, dz of one sign)
+//                                           P = t0 + e    (opposite signs)
+//   b h  = max(u, 0) + ln P - ln(1 + t0).
+// P lies in [max(t0, e), 2], so nothing overflows, and the difference of the
+// two logarithms (both of numbers near 1 when dz is small) leaves h an absolute
+// error of a few float32 ulps of 1 / b at any step size: four transcendentals
+// and no data-dependent branch.  P can underflow only when both |a0| and |u|
+// exceed 64; those elements take the direct difference of the two values.  The
+// smooth softplus differs from the thresholded one (linear for b z > 20) by at
+// most log1p(exp(-20)) / b, far below the float32 spacing of those values.
 __device__ inline void softplus_step(float z0, float dz, float beta, float ib, float& h, float& y1) {
-  const float a0 = z0 * beta;
-  const float t0 = __expf(-fabsf(a0));
-  const float sp0 = fmaxf(z0, 0.0f) + log1p_pos(t0) * ib;
-  const float u = beta * dz;
-  if (fabsf(u) < 1.0f) {
-    const float sig = (a0 >= 0.0f ? 1.0f : t0) * __frcp_rn(1.0f + t0);
-    const float em = fabsf(u) < 0.03125f ? u * (1.0f + u * (0.5f + u * (1.0f / 6.0f + u * (1.0f / 24.0f))))
-                                         : __expf(u) - 1.0f;
-    const float y = sig * em;
-    const float lp = fabsf(y) < 0.03125f
-                         ? y * (1.0f - y * (0.5f - y * (1.0f / 3.0f - y * (0.25f - y * 0.2f))))
-                         : __logf(1.0f + y);
-    h = lp * ib;
-  } else {
+  constexpr float kLog2e = 1.4426950408889634f, kLn2 = 0.6931471805599453f;
+  const float a0 = z0 * beta, u = dz * beta;
+  const float t0 = __builtin_amdgcn_exp2f(-fabsf(a0) * kLog2e);
+  const float e = __builtin_amdgcn_exp2f(-fabsf(u) * kLog2e);
+  const bool same = (z0 >= 0.0f) == (dz >= 0.0f);
+  const float P = same ? fmaf(t0, e, 1.0f) : t0 + e;
+  const float lP = __builtin_amdgcn_logf(P), l0 = __builtin_amdgcn_logf(1.0f + t0);  // log2
+  const float s = kLn2 * ib, mu = fmaxf(dz, 0.0f);
+  h = fmaf(lP - l0, s, mu);
+  y1 = fmaf(lP, s, fmaxf(z0, 0.0f) + mu);
+  if (__builtin_expect(fminf(fabsf(a0), fabsf(u)) > 64.0f, 0)) {
     const float z1 = z0 + dz;
-    const float t1 = __expf(-fabsf(z1 * beta));
-    h = (fmaxf(z1, 0.0f) + log1p_pos(t1) * ib) - sp0;
+    const float t1 = __builtin_amdgcn_exp2f(-fabsf(z1 * beta) * kLog2e);
+    y1 = fmaf(__builtin_amdgcn_logf(1.0f + t1), s, fmaxf(z1, 0.0f));
+    h = y1 - fmaf(l0, s, fmaxf(z0, 0.0f));
   }
-  y1 = sp0 + h;
 }
 
 // One 32 x 32 block of the wave, staged in eb: lane (rl, cg) handles columns
@@ -1336,7 +1317,9 @@ gemm_bf16_nn_persist_kernel(GemmParams p) {
   using PT = PersistTile<WN>;
   constexpr int NS = PT::NS;
   __shared__ __attribute__((aligned(16))) unsigned char smem[PT::LDS_BYTES];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index is uniform: held in a scalar register, the stage buffers and
+  // LDS destinations chosen from it stay scalar too
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave / WN, wn = wave % WN;
   bf16_t* st0 = reinterpret_cast<bf16_t*>(smem);
   float* eb = reinterpret_cast<float*>(smem + NS * PT::STAGE_EL * 2) + wave * kPEpiFloats;
@@ -1368,19 +1351,28 @@ gemm_bf16_nn_persist_kernel(GemmParams p) {
   int64_t issued = 0;
   uint32_t is_tile = blockIdx.x;
   int is_t = 0, is_slot = 0;
-  int64_t is_m0 = (int64_t)(is_tile % n_mt) * GBM, is_n0 = (int64_t)(is_tile / n_mt) * PT::BN;
+  // per-tile buffer bases (first row of the tile) and the bytes that follow them
+  const char *is_a = nullptr, *is_b = nullptr;
+  int64_t is_ae = 0, is_be = 0;
+  auto set_issue_tile = [&]() {
+    const int64_t m0 = (int64_t)(is_tile % n_mt) * GBM, n0 = (int64_t)(is_tile / n_mt) * PT::BN;
+    is_a = reinterpret_cast<const char*>(A + m0 * p.lda);
+    is_b = reinterpret_cast<const char*>(B + n0 * p.ldb);
+    is_ae = (p.M - m0) * p.lda * 2;
+    is_be = (p.N - n0) * p.ldb * 2;
+  };
+  set_issue_tile();
   auto issue_next = [&]() {
     const int64_t k0 = (int64_t)is_t * kNNBK;
     const int64_t kr = p.K - k0;
-    persist_issue<WN>(tile_rsrc(A, p.lda, p.M, is_m0, k0), tile_rsrc(B, p.ldb, p.N, is_n0, k0), voff,
-                      kofs, kr < kNNBK ? (int)kr : kNNBK, st0 + is_slot * PT::STAGE_EL, wave);
+    persist_issue<WN>(stage_rsrc(is_a, is_ae, k0 * 2), stage_rsrc(is_b, is_be, k0 * 2), voff, kofs,
+                      kr < kNNBK ? (int)kr : kNNBK, st0 + is_slot * PT::STAGE_EL, wave);
     ++issued;
     is_slot = is_slot + 1 == NS ? 0 : is_slot + 1;
     if (++is_t == nk) {
       is_t = 0;
       is_tile += gridDim.x;
-      is_m0 = (int64_t)(is_tile % n_mt) * GBM;
-      is_n0 = (int64_t)(is_tile / n_mt) * PT::BN;
+      set_issue_tile();
     }
   };
 #pragma unroll
