@@ -1203,7 +1203,145 @@ __device__ __attribute__((always_inline)) inline void persist_stage(float* eb, c
   for (int r = 0; r < 16; ++r) eb[(8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * kPEP + (lane & 31)] = a[r];
 }
 
-// The four 32 x 32 blocks go through one (rolled) loop so that each kind
+// Epilogue of a wave tile that lies wholly inside C (64 x 64).  Phase 1 asks
+// for every per-element operand of the four 32 x 32 blocks (zc rows, bf16 or
+// fp32 activations, r1_a) and the column vectors, then stages, reads and
+// transforms the blocks into registers; phase 2 issues all stores.  Vector
+// memory operations complete in issue order, so a staging read placed after one
+// of this wave's stores would wait for that store: here none is.  Lane (rl, cg)
+// owns rows mw + 32 i + 16 hh + rl and columns nw + 32 j + 8 cg .. + 7.
+template <int WN, int KIND>
+__device__ __attribute__((always_inline)) inline void persist_epilogue_full(
+    const GemmParams& p, float* eb, f32x16_t (&acc)[2][2], int64_t mw, int64_t nw, int lane,
+    float (&csum)[2][8], int64_t zr0, float ib, int part) {
+  const EpiParams& e = p.e;
+  const int rl = lane & 15, cg = lane >> 4;
+  constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA;
+  constexpr bool kOpd = KIND == PK_DELTA || KIND == PK_DACT;
+  float opd[kOpd ? 4 : 1][2][8];
+  float ra[2][2];
+  if constexpr (KIND == PK_DELTA) {
+#pragma unroll
+    for (int blk = 0; blk < 4; ++blk)
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int64_t off = zr0 + (blk >> 1) * 32 + hh * 16 + rl;
+        const int64_t m = mw + (blk >> 1) * 32 + hh * 16 + rl;
+        const int64_t mz = e.zc_rows >= GBM ? (off >= e.zc_rows ? off - e.zc_rows : off)
+                                            : (int64_t)((uint32_t)m % (uint32_t)e.zc_rows);
+        vload8(e.zc + mz * e.ld_zc + nw + (blk & 1) * 32 + cg * 8, opd[blk][hh]);
+      }
+  } else if constexpr (KIND == PK_DACT) {
+#pragma unroll
+    for (int blk = 0; blk < 4; ++blk)
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int64_t m = mw + (blk >> 1) * 32 + hh * 16 + rl;
+        const int64_t o = m * e.ld_dact + nw + (blk & 1) * 32 + cg * 8;
+        if (e.dact_dtype == NA_BF16) vload8((const bf16_t*)e.dact_y + o, opd[blk][hh]);
+        else vload8((const float*)e.dact_y + o, opd[blk][hh]);
+      }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) ra[i][hh] = e.r1_a ? e.r1_a[mw + i * 32 + hh * 16 + rl] : 0.0f;
+  }
+  float cb[2][8], hw[2][8];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t nb = nw + j * 32 + cg * 8;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) cb[j][c] = hw[j][c] = 0.0f;
+    if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) vload8(e.bias + nb, cb[j]);
+    if constexpr (KIND == PK_DACT) { if (e.r1_a) vload8(e.r1_b + nb, cb[j]); }
+    if constexpr (kHead) { if (e.head_w) vload8(e.head_w + nb, hw[j]); }
+  }
+  float oc[4][2][8];
+  float oz[KIND == PK_BIAS_SOFTPLUS ? 4 : 1][2][8];
+  float hs[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+#pragma unroll
+  for (int blk = 0; blk < 4; ++blk) {
+    const int i = blk >> 1, j = blk & 1;
+    persist_stage(eb, acc[i][j], lane);
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int rr = hh * 16 + rl;
+      float x[8];
+      {
+        const float4 a = *reinterpret_cast<const float4*>(eb + rr * kPEP + cg * 8);
+        const float4 b = *reinterpret_cast<const float4*>(eb + rr * kPEP + cg * 8 + 4);
+        x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+      }
+      float hv[8];
+      if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) x[c] += cb[j][c];
+        if constexpr (KIND == PK_BIAS_SOFTPLUS) {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) oz[blk][hh][c] = x[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          x[c] = KIND == PK_BIAS_RELU ? fmaxf(x[c], 0.0f) : softplus_sel(x[c], e.beta, ib);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) hv[c] = x[c];
+      } else if constexpr (KIND == PK_DACT) {
+        if (e.r1_a) {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) x[c] = fmaf(ra[i][hh], cb[j][c], x[c]);
+        }
+        if (e.dact == NA_ACT_RELU) {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) x[c] = opd[blk][hh][c] > 0.0f ? x[c] : 0.0f;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) x[c] *= act_deriv_from_y(NA_ACT_SOFTPLUS, opd[blk][hh][c], e.beta);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) csum[j][c] += x[c];
+      } else if constexpr (KIND == PK_DELTA) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) softplus_step(opd[blk][hh][c], x[c], e.beta, ib, hv[c], x[c]);
+      }
+      if constexpr (kHead) {
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s += hw[j][c] * hv[c];
+        hs[i][hh] += s;
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) oc[blk][hh][c] = x[c];
+    }
+  }
+  // phase 2: stores
+#pragma unroll
+  for (int blk = 0; blk < 4; ++blk)
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int64_t m = mw + (blk >> 1) * 32 + hh * 16 + rl;
+      const int64_t nb = nw + (blk & 1) * 32 + cg * 8;
+      if (e.c_dtype == NA_BF16) vstore8((bf16_t*)e.C + m * e.ldc + nb, oc[blk][hh]);
+      else vstore8((float*)e.C + m * e.ldc + nb, oc[blk][hh]);
+      if constexpr (KIND == PK_BIAS_SOFTPLUS) {
+        if (e.z_out) vstore8(e.z_out + m * e.ld_z + nb, oz[blk][hh]);
+      }
+    }
+  if constexpr (kHead) {
+    if (e.head_w) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+          float s = hs[i][hh];
+          s += __shfl_xor(s, 16);
+          s += __shfl_xor(s, 32);
+          if (cg == 0) e.head_out[(int64_t)part * p.M + mw + i * 32 + hh * 16 + rl] = s;
+        }
+    }
+  }
+}
+
+// Edge tiles: the four 32 x 32 blocks go through one (rolled) loop so that each kind
 // carries a single copy of its element math per bounds mode.
 template <int WN, int KIND>
 __device__ __attribute__((always_inline)) inline void persist_epilogue(
@@ -1215,6 +1353,11 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue(
   const int64_t mw = m0 + wm * 64, nw = n0 + wn * 64;
   const bool full = mw + 64 <= p.M && nw + 64 <= p.N;  // wave-uniform
   constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA;
+  if (full) {
+    persist_epilogue_full<WN, KIND>(p, eb, acc, mw, nw, lane, csum, zbase + wm * 64, ib,
+                                    (int)(n0 / 64 + wn));
+    return;
+  }
   float hs[2] = {0.0f, 0.0f};
 #pragma unroll 1
   for (int blk = 0; blk < 4; ++blk) {
@@ -1230,19 +1373,12 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue(
 #pragma unroll
     for (int c = 0; c < 8; ++c) cb[c] = hw8[c] = cs[c] = 0.0f;
     const int64_t zr = zbase + wm * 64 + i * 32;
-    if (full) {
-      if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) load_cols<true>(e.bias + nb, 8, cb);
-      if constexpr (KIND == PK_DACT) { if (e.r1_a) load_cols<true>(e.r1_b + nb, 8, cb); }
-      if constexpr (kHead) { if (e.head_w) load_cols<true>(e.head_w + nb, 8, hw8); }
-      persist_rows<KIND, true>(p, eb, mw + i * 32, nb, 8, rl, cg, cb, hw8, hs, cs, zr, ib);
-    } else {
-      if (nc > 0) {
-        if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) load_cols<false>(e.bias + nb, nc, cb);
-        if constexpr (KIND == PK_DACT) { if (e.r1_a) load_cols<false>(e.r1_b + nb, nc, cb); }
-        if constexpr (kHead) { if (e.head_w) load_cols<false>(e.head_w + nb, nc, hw8); }
-      }
-      persist_rows<KIND, false>(p, eb, mw + i * 32, nb, nc, rl, cg, cb, hw8, hs, cs, zr, ib);
+    if (nc > 0) {
+      if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) load_cols<false>(e.bias + nb, nc, cb);
+      if constexpr (KIND == PK_DACT) { if (e.r1_a) load_cols<false>(e.r1_b + nb, nc, cb); }
+      if constexpr (kHead) { if (e.head_w) load_cols<false>(e.head_w + nb, nc, hw8); }
     }
+    persist_rows<KIND, false>(p, eb, mw + i * 32, nb, nc, rl, cg, cb, hw8, hs, cs, zr, ib);
     if constexpr (KIND == PK_DACT) {
       if (j == 0) {
 #pragma unroll
@@ -1312,7 +1448,7 @@ __device__ inline void persist_flush_colsum(float* colsum, int64_t N, int64_t nc
 }
 
 template <int WN, int KIND>
-__global__ void __launch_bounds__(128 * WN)
+__global__ void __launch_bounds__(128 * WN, WN == 2 ? 2 : 1)  // WN = 2: two workgroups per CU
 gemm_bf16_nn_persist_kernel(GemmParams p) {
   using PT = PersistTile<WN>;
   constexpr int NS = PT::NS;
