@@ -16,17 +16,32 @@ __device__ inline float lerp_torch(float a, float b, float w) {
   return fabsf(w) < 0.5f ? a + w * (b - a) : b - (b - a) * (1.0f - w);
 }
 
+// Streaming accesses: every byte is touched once per step, so loads and stores
+// carry the non-temporal hint.
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int nt_u2 __attribute__((ext_vector_type(2)));
+__device__ inline float4 ld_nt(const float* p) {
+  const nt_f4 r = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  return make_float4(r.x, r.y, r.z, r.w);
+}
+__device__ inline void st_nt(float* p, const float4& v) {
+  const nt_f4 r = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(r, reinterpret_cast<nt_f4*>(p));
+}
+
+// One chunk of 4 parameters per thread (no grid-stride loop: the launch holds
+// n / 4 threads, which keeps the most loads in flight).
 template <bool LOWP>
 __global__ void __launch_bounds__(256)
 adamw_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
              float* __restrict__ v, int64_t n, AdamArgs a, bf16_t* __restrict__ lowp) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i < n; i += stride) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i < n) {
     if (i + 4 <= n) {
-      float4 pp = *reinterpret_cast<float4*>(p + i);
-      float4 gg = *reinterpret_cast<const float4*>(g + i);
-      float4 mm = *reinterpret_cast<float4*>(m + i);
-      float4 vv = *reinterpret_cast<float4*>(v + i);
+      float4 pp = ld_nt(p + i);
+      float4 gg = ld_nt(g + i);
+      float4 mm = ld_nt(m + i);
+      float4 vv = ld_nt(v + i);
       float* P = &pp.x; const float* G = &gg.x; float* Mv = &mm.x; float* V = &vv.x;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -37,14 +52,13 @@ adamw_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restri
         pe = pe - a.step_size * me / den;
         P[e] = pe; Mv[e] = me; V[e] = ve;
       }
-      *reinterpret_cast<float4*>(p + i) = pp;
-      *reinterpret_cast<float4*>(m + i) = mm;
-      *reinterpret_cast<float4*>(v + i) = vv;
+      st_nt(p + i, pp);
+      st_nt(m + i, mm);
+      st_nt(v + i, vv);
       if (LOWP) {
-        uint2 w;
-        w.x = (uint32_t)f32_to_bf16(pp.x) | ((uint32_t)f32_to_bf16(pp.y) << 16);
-        w.y = (uint32_t)f32_to_bf16(pp.z) | ((uint32_t)f32_to_bf16(pp.w) << 16);
-        *reinterpret_cast<uint2*>(lowp + i) = w;
+        const nt_u2 w = {(uint32_t)f32_to_bf16(pp.x) | ((uint32_t)f32_to_bf16(pp.y) << 16),
+                         (uint32_t)f32_to_bf16(pp.z) | ((uint32_t)f32_to_bf16(pp.w) << 16)};
+        __builtin_nontemporal_store(w, reinterpret_cast<nt_u2*>(lowp + i));
       }
     } else {
       for (int64_t j = i; j < n; ++j) {
@@ -82,9 +96,9 @@ extern "C" int na_adamw_step(float* param, const float* grad, float* exp_avg, fl
   const double bc2 = 1.0 - pow((double)beta2, (double)step);
   a.step_size = (float)((double)lr / bc1);
   a.inv_bc2_sqrt = (float)(1.0 / sqrt(bc2));
-  int64_t threads = ceil_div(n, 4);
-  int64_t blocks = ceil_div(threads, 256);
-  if (blocks > 8192) blocks = 8192;
+  const int64_t threads = ceil_div(n, 4);
+  const int64_t blocks = ceil_div(threads, 256);
+  NA_CHECK_ARG(blocks <= 0x7fffffffLL, "too many parameters for one launch");
   hipStream_t s = (hipStream_t)stream;
   if (lowp)
     hipLaunchKernelGGL(adamw_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, param, grad,

@@ -647,7 +647,9 @@ class RGBNetFunction(torch.autograd.Function):
         dW0p = torch.zeros(hid, RGB_LD, device=dev)
         linear_weight_grad(dpre, FB, RGB_LD, RGB_LD, dW0p)
         dWs[0] = w.unpermute_dW0(dW0p)
-        dFB = torch.empty(P, RGB_LD, dtype=torch.float32, device=dev)
+        # input gradient in the compute dtype: the feature columns go back to the SDF
+        # node as a view, in the dtype of its output, with no cast or copy
+        dFB = torch.empty(P, RGB_LD, dtype=cdt, device=dev)
         gemm(dpre, hid, 0, _t(w.W[0]), hid, 0, P, RGB_LD, hid, dFB, RGB_LD)
         dnormals = None
         if ctx.build:

@@ -189,10 +189,10 @@ class NeuralSDF(torch.nn.Module):
 
 
 def _keep_view(feat, shape):
-    """Reshape the [P,256] buffer view to [..., 256] without copying."""
-    return torch.as_strided(feat, (*shape, feat.shape[-1]),
-                            tuple(int(np.prod(shape[i + 1:])) * feat.stride(0)
-                                  for i in range(len(shape))) + (1,))
+    """Reshape the [P,256] buffer view to [..., 256] without copying (splitting the
+    row dimension keeps the buffer's row stride; a plain view keeps autograd from
+    materialising a buffer-shaped gradient)."""
+    return feat.view(*shape, feat.shape[-1])
 
 
 class NeuralRGB(torch.nn.Module):
@@ -259,7 +259,8 @@ def _is_buffer_view(t):
 
 
 def _flat_view(t, P):
-    return torch.as_strided(t, (P, t.shape[-1]), (Fn.RGB_LD, 1))
+    """[..., C] buffer view -> [P, C] view (the leading dimensions merge exactly)."""
+    return t.view(P, t.shape[-1])
 
 
 class _LinearStack(torch.nn.Module):
