@@ -63,6 +63,48 @@ __device__ inline float corner_weight(int c, const float t[3]) {
   return w;
 }
 
+// The eight corner indices of one cell, as corner_index computes them.  The
+// corners differ from corner 0 by fixed per-axis terms (uint32 arithmetic
+// wraps the same way), so a hashed level costs two xors and a mask per corner
+// and a dense level one add; both are formed and one is selected, since the
+// lanes of a wave hold different levels.  The dense wrap (index % size) only
+// changes corners past the grid's upper face or of points outside the unit
+// cube, so it runs only when some corner of the cell needs it.
+__device__ inline void cell_corners(const uint32_t (&cell)[3], const LevelParams& p,
+                                    uint32_t (&id)[8]) {
+  constexpr uint32_t kPy = 2654435761u, kPz = 805459861u;
+  const uint32_t hx[2] = {cell[0], cell[0] + 1u};
+  const uint32_t hy[2] = {cell[1] * kPy, cell[1] * kPy + kPy};
+  const uint32_t hz[2] = {cell[2] * kPz, cell[2] * kPz + kPz};
+  const uint32_t r1 = p.res, r2 = p.res * p.res;
+  const uint32_t d0 = cell[0] + cell[1] * r1 + cell[2] * r2;
+  const uint32_t dd[4] = {d0, d0 + r1, d0 + r2, d0 + r2 + r1};
+  const uint32_t mask = p.size - 1u;
+  bool wrap = false;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t h = (hx[c & 1] ^ hy[(c >> 1) & 1] ^ hz[c >> 2]) & mask;
+    const uint32_t d = dd[c >> 1] + (uint32_t)(c & 1);
+    wrap = wrap || d >= p.size;
+    id[c] = p.hashed ? h : d;
+  }
+  if (__builtin_expect(!p.hashed && wrap, 0)) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) id[c] = (dd[c >> 1] + (uint32_t)(c & 1)) % p.size;
+  }
+}
+
+// Trilinear weights of the eight corners, multiplied in corner_weight's order.
+__device__ inline void cell_weights(const float (&t)[3], float (&w)[8]) {
+  const float wx[2] = {1.0f - t[0], t[0]}, wy[2] = {1.0f - t[1], t[1]};
+  const float wz[2] = {1.0f - t[2], t[2]};
+  float wxy[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) wxy[a] = wx[a & 1] * wy[a >> 1];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) w[c] = wxy[c & 3] * wz[c >> 2];
+}
+
 // Load F table values of one entry as floats (vectorised).
 template <int F, typename TT> struct EntryLoad;
 template <int F> struct EntryLoad<F, float> {
@@ -168,10 +210,39 @@ hashgrid_corners_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
   }
 }
 
+// acc = sum over the eight corners of the cell of weight * entry (corner order
+// 0..7, as the per-corner loops accumulate).
+template <int F, typename TT>
+__device__ inline void gather_cell(const TT* __restrict__ tab, const LevelParams& p,
+                                   const uint32_t (&cell)[3], const float (&t)[3], float (&acc)[F]) {
+  uint32_t id[8];
+  float w[8];
+  cell_corners(cell, p, id);
+  cell_weights(t, w);
+#pragma unroll
+  for (int f = 0; f < F; ++f) acc[f] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float v[F];
+    EntryLoad<F, TT>::load(tab + (size_t)id[c] * F, v);
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = fmaf(w[c], v[f], acc[f]);
+  }
+}
+
 // Input point -> unit-cube coordinate: u = (x - vmin) / vrange
 // (projects/neuralangelo/utils/modules.py:82-83; vmin = 0, vrange = 1 is the
 // tcnn convention of inputs already in [0,1]).
 __device__ inline float normalize_coord(float x, float vmin, float vrange) {
+  return (x - vmin) / vrange;
+}
+
+// The same coordinate for the gather kernels, where the division is a visible
+// share of the work: a positive normal power-of-two range divides exactly by
+// an exponent shift.
+__device__ inline float normalize_coord_gather(float x, float vmin, float vrange) {
+  const uint32_t b = __float_as_uint(vrange);
+  if ((b & 0x807fffffu) == 0u && (b >> 23) != 0u) return ldexpf(x - vmin, 127 - (int)(b >> 23));
   return (x - vmin) / vrange;
 }
 
@@ -219,18 +290,8 @@ hashgrid_fwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
     float t[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      pos_fract(normalize_coord(x[i * 3 + k], vmin, vrange), p.scale, cell[k], t[k]);
-    const TT* tab = table + (size_t)p.off * F;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
-                                 cell[2] + ((c >> 2) & 1), p);
-      float w = corner_weight(c, t);
-      float v[F];
-      EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
-#pragma unroll
-      for (int f = 0; f < F; ++f) acc[f] = fmaf(w, v[f], acc[f]);
-    }
+      pos_fract(normalize_coord_gather(x[i * 3 + k], vmin, vrange), p.scale, cell[k], t[k]);
+    gather_cell<F, TT>(table + (size_t)p.off * F, p, cell, t, acc);
   }
   RowStore<F, TO>::store(o, acc);
 }
@@ -256,13 +317,12 @@ hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
   for (int k = 0; k < 3; ++k)
     pos_fract(normalize_coord(x[i * 3 + k], vmin, vrange), p.scale, cell[k], t[k]);
   float* gtab = grad + (size_t)p.off * F + f;
+  uint32_t id[8];
+  float w[8];
+  cell_corners(cell, p, id);
+  cell_weights(t, w);
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
-                               cell[2] + ((c >> 2) & 1), p);
-    float w = corner_weight(c, t);
-    atomic_add_f32(gtab + (size_t)id * F, w * g);
-  }
+  for (int c = 0; c < 8; ++c) atomic_add_f32(gtab + (size_t)id[c] * F, w[c] * g);
 }
 
 constexpr float kTapDir[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
@@ -343,19 +403,8 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
     float fr[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      pos_fract(normalize_coord(xc[k], vmin, vrange), p.scale, cell[k], fr[k]);
-#pragma unroll
-    for (int f = 0; f < F; ++f) ac[f] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
-                                       cell[2] + ((c >> 2) & 1), p);
-      const float w = corner_weight(c, fr);
-      float v[F];
-      EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
-#pragma unroll
-      for (int f = 0; f < F; ++f) ac[f] = fmaf(w, v[f], ac[f]);
-    }
+      pos_fract(normalize_coord_gather(xc[k], vmin, vrange), p.scale, cell[k], fr[k]);
+    gather_cell<F, TT>(tab, p, cell, fr, ac);
     RowStore<F, bf16_t>::store(X0c + i * ld + l * F, ac);
   }
 #pragma unroll
@@ -364,20 +413,9 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
     float fr[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      pos_fract(normalize_coord(xt[t][k], vmin, vrange), p.scale, cell[k], fr[k]);
+      pos_fract(normalize_coord_gather(xt[t][k], vmin, vrange), p.scale, cell[k], fr[k]);
     float at[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) at[f] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const uint32_t id = corner_index(cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
-                                       cell[2] + ((c >> 2) & 1), p);
-      const float w = corner_weight(c, fr);
-      float v[F];
-      EntryLoad<F, TT>::load(tab + (size_t)id * F, v);
-#pragma unroll
-      for (int f = 0; f < F; ++f) at[f] = fmaf(w, v[f], at[f]);
-    }
+    gather_cell<F, TT>(tab, p, cell, fr, at);
     const int64_t r = (int64_t)t * n + i;
     float dv[F];
 #pragma unroll

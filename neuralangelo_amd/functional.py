@@ -97,6 +97,24 @@ def gemm(A, lda, trans_a, B, ldb, trans_b, M, N, K, C, ldc, epilogue=NA_EPI_STOR
     _lib.call("na_gemm", ctypes.byref(a), _s(A))
 
 
+def _zero_block(dev, *shapes):
+    """Zero-filled fp32 tensors of the given shapes carved from one buffer: one fill
+    launch for a group of gradient accumulators.  Each tensor starts on a 256-byte
+    boundary (sizes rounded up to 64 floats) for the vectorised kernels."""
+    sizes = []
+    for s in shapes:
+        n = 1
+        for d in s:
+            n *= int(d)
+        sizes.append(n)
+    buf = torch.zeros(sum((n + 63) // 64 * 64 for n in sizes), device=dev)
+    out, o = [], 0
+    for s, n in zip(shapes, sizes):
+        out.append(buf[o:o + n].view(*s))
+        o += (n + 63) // 64 * 64
+    return out
+
+
 def head_parts(N):
     """Number of partial sums per row written by a fused GEMM head over N columns."""
     return 2 * ((N + 127) // 128)
@@ -163,8 +181,7 @@ class WeightNorm(torch.autograd.Function):
     def backward(ctx, dW):
         v, gf = ctx.saved_tensors
         N, K = v.shape
-        dv = torch.zeros_like(v)
-        dg = torch.zeros_like(gf)
+        dv, dg = _zero_block(v.device, v.shape, gf.shape)
         _lib.call("na_weight_norm_bwd", _lib.ptr(v), _lib.ptr(gf), _lib.ptr(dW.contiguous()), N, K,
                   _lib.ptr(dv), _lib.ptr(dg), _s(v))
         return dv, dg.view(ctx.g_shape)
@@ -294,12 +311,10 @@ class _SDFGrads:
 
     def __init__(self, w, dev, with_feat):
         hid = w.hidden
-        self.dW0p = torch.zeros(hid, w.k0p, device=dev)
-        self.db0 = torch.zeros(hid, device=dev)
-        self.dws = torch.zeros(1, hid, device=dev)
-        self.dbs = torch.zeros(1, device=dev)
-        self.dW1 = torch.zeros(hid, hid, device=dev) if with_feat else None
-        self.db1 = torch.zeros(hid, device=dev) if with_feat else None
+        shapes = [(hid, w.k0p), (hid,), (1, hid), (1,)] + ([(hid, hid), (hid,)] if with_feat else [])
+        z = _zero_block(dev, *shapes)
+        self.dW0p, self.db0, self.dws, self.dbs = z[:4]
+        self.dW1, self.db1 = (z[4], z[5]) if with_feat else (None, None)
 
 
 def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc):
@@ -628,8 +643,12 @@ class RGBNetFunction(torch.autograd.Function):
             dpre3 = dpre3 * out * (1 - out)
         dpre3 = dpre3.contiguous()
         dWs = [None] * n_lin
-        dbs = [torch.zeros(b.shape[0], device=dev) for b in w.b]
-        dWs[-1] = torch.zeros(3, hid, device=dev)
+        # every weight/bias gradient of the colour MLP in one zeroed block
+        z = _zero_block(dev, *[(b.shape[0],) for b in w.b], (3, hid),
+                        *[(hid, hid) for _ in range(n_lin - 2)], (hid, RGB_LD))
+        dbs = z[:n_lin]
+        dWs[-1] = z[n_lin]
+        dW_mid, dW0p = z[n_lin + 1:-1], z[-1]
         # last hidden layer's pre-activation gradient, written in the compute dtype
         dpre = torch.empty(P, hid, dtype=cdt, device=dev)
         _lib.call("na_linear_narrow_bwd", _lib.ptr(Hs[-1]), hid, _code(Hs[-1]), P, hid,
@@ -638,13 +657,12 @@ class RGBNetFunction(torch.autograd.Function):
                   NA_ACT_RELU, SOFTPLUS_BETA, _lib.ptr(dbs[-2]), _s(FB))
         for li in range(n_lin - 2, 0, -1):
             X = Hs[li - 1]
-            dWs[li] = torch.zeros(hid, hid, device=dev)
+            dWs[li] = dW_mid[li - 1]
             linear_weight_grad(dpre, X, hid, hid, dWs[li])
             dprev = torch.empty(P, hid, dtype=cdt, device=dev)
             gemm(dpre, hid, 0, _t(w.W[li]), hid, 0, P, hid, hid, dprev, hid, dact=NA_ACT_RELU,
                  dact_y=X, ld_dact=hid, colsum=dbs[li - 1])
             dpre = dprev
-        dW0p = torch.zeros(hid, RGB_LD, device=dev)
         linear_weight_grad(dpre, FB, RGB_LD, RGB_LD, dW0p)
         dWs[0] = w.unpermute_dW0(dW0p)
         # input gradient in the compute dtype: the feature columns go back to the SDF
