@@ -1572,6 +1572,172 @@ gemm_bf16_nn_persist_kernel(GemmParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Weights-stationary GEMM for the 256-wide MLP layers: C = epi(A . B^T) with
+// N = 256, M a multiple of 128 and NK 32-wide K stages.  Eight waves each own
+// 32 output columns and hold those columns' weights over the whole K range in
+// registers (2 NK fragments of 8 bf16 per lane), so only activation rows
+// stream: 128 x 32 bf16 stages (8 KB, one buffer-to-LDS load per wave) in
+// kWsNS LDS slots, up to kWsNS - 1 of them in flight, and no weight tile is
+// re-read per row tile.  The products take the weights as the first MFMA
+// operand, which leaves one output row per lane (four 4-column groups); a
+// permlane32 swap between the two halves of the wave turns them into two
+// 8-column groups per lane, so the epilogue stores 16-byte row segments from
+// registers with no LDS staging.
+constexpr int kWsNS = 15;
+constexpr int kWsStage = GBM * kNNBK;
+
+// vmcnt(ahead) for ahead = 0 .. kWsNS - 2 (one load per wave per stage)
+__device__ inline void ws_wait(int ahead) {
+  switch (ahead) {
+    case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    case 1: __builtin_amdgcn_s_waitcnt(0xF71); break;
+    case 2: __builtin_amdgcn_s_waitcnt(0xF72); break;
+    case 3: __builtin_amdgcn_s_waitcnt(0xF73); break;
+    case 4: __builtin_amdgcn_s_waitcnt(0xF74); break;
+    case 5: __builtin_amdgcn_s_waitcnt(0xF75); break;
+    case 6: __builtin_amdgcn_s_waitcnt(0xF76); break;
+    case 7: __builtin_amdgcn_s_waitcnt(0xF77); break;
+    case 8: __builtin_amdgcn_s_waitcnt(0xF78); break;
+    case 9: __builtin_amdgcn_s_waitcnt(0xF79); break;
+    case 10: __builtin_amdgcn_s_waitcnt(0xF7A); break;
+    case 11: __builtin_amdgcn_s_waitcnt(0xF7B); break;
+    case 12: __builtin_amdgcn_s_waitcnt(0xF7C); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF7D); break;
+  }
+}
+
+__device__ inline void ws_issue(__amdgpu_buffer_rsrc_t r, uint32_t off, bf16_t* dst) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t*)dst, 16, off, 0, 0, 0);
+}
+
+template <int NK, int KIND>
+__global__ void __launch_bounds__(512) gemm_bf16_ws_kernel(GemmParams p) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[kWsNS * kWsStage * 2];
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int h = lane >> 5;
+  bf16_t* st0 = reinterpret_cast<bf16_t*>(smem);
+  const bf16_t* A = (const bf16_t*)p.A;
+  const EpiParams& e = p.e;
+  const uint32_t n_mt = (uint32_t)(p.M / GBM);
+  const int64_t my_tiles = blockIdx.x < n_mt ? (n_mt - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  const int64_t total = my_tiles * NK;
+  const int nw = wave * 32;
+
+  // weights of row nw + lane % 32 of B: 8 k-values (offset 8 h) of each 16-wide step
+  bf16x8_t bw[2 * NK];
+  {
+    const bf16_t* brow = (const bf16_t*)p.B + (int64_t)(nw + (lane & 31)) * p.ldb;
+#pragma unroll
+    for (int s = 0; s < 2 * NK; ++s) {
+      const int64_t k = 16 * s + 8 * h;
+      const uint4 u = k < p.K ? *reinterpret_cast<const uint4*>(brow + k) : make_uint4(0u, 0u, 0u, 0u);
+      bw[s] = __builtin_bit_cast(bf16x8_t, u);
+    }
+  }
+  // bias of the lane's columns nw + 16 g + 8 h + c
+  float bb[2][8];
+#pragma unroll
+  for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) bb[gq][c] = 0.0f;
+  if constexpr (KIND == PK_BIAS_RELU || KIND == PK_BIAS_SOFTPLUS) {
+#pragma unroll
+    for (int gq = 0; gq < 2; ++gq) vload8(e.bias + nw + 16 * gq + 8 * h, bb[gq]);
+  }
+
+  // stage loads: wave w fills rows 16 w .. 16 w + 15; lane l lands at slot l & 3
+  // of row l >> 2 and fetches the chunk that belongs there
+  const int arow = wave * 16 + (lane >> 2);
+  const int achunk = (lane & 3) ^ ((arow >> 2) & 3);
+  const uint32_t avoff = (uint32_t)(arow * p.lda * 2 + achunk * 16);
+  const int akofs = achunk * 8;
+  int64_t issued = 0;
+  uint32_t is_tile = blockIdx.x;
+  int is_t = 0, is_slot = 0;
+  const char* is_a = reinterpret_cast<const char*>(A + (int64_t)is_tile * GBM * p.lda);
+  int64_t is_ae = (p.M - (int64_t)is_tile * GBM) * p.lda * 2;
+  auto issue_next = [&]() {
+    const int64_t k0 = (int64_t)is_t * kNNBK;
+    ws_issue(stage_rsrc(is_a, is_ae, k0 * 2), akofs < p.K - k0 ? avoff : kOobOffset,
+             st0 + is_slot * kWsStage + wave * 16 * kNNBK);
+    ++issued;
+    is_slot = is_slot + 1 == kWsNS ? 0 : is_slot + 1;
+    if (++is_t == NK) {
+      is_t = 0;
+      is_tile += gridDim.x;
+      is_a = reinterpret_cast<const char*>(A + (int64_t)is_tile * GBM * p.lda);
+      is_ae = (p.M - (int64_t)is_tile * GBM) * p.lda * 2;
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < kWsNS - 1; ++s)
+    if (issued < total) issue_next();
+
+  const float ib = 1.0f / e.beta;
+  int slot = 0;
+  int64_t g = 0;
+  for (int64_t jt = 0; jt < my_tiles; ++jt) {
+    const int64_t m0 = (int64_t)(blockIdx.x + (uint32_t)jt * gridDim.x) * GBM;
+    f32x16_t acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][r] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NK; ++t, ++g) {
+      ws_wait((int)(issued - g - 1));
+      __builtin_amdgcn_s_barrier();  // stage g landed in every wave; stage g-1 consumed
+      if (issued < total) issue_next();
+      const bf16_t* sa = st0 + slot * kWsStage;
+      slot = slot + 1 == kWsNS ? 0 : slot + 1;
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        const int ck = 2 * kk + h;
+        bf16x8_t a[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int ra = i * 32 + (lane & 31);
+          a[i] = *reinterpret_cast<const bf16x8_t*>(sa + ra * kNNBK + ((ck ^ ((ra >> 2) & 3)) << 3));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw[2 * t + kk], a[i], acc[i], 0, 0, 0);
+      }
+    }
+    // epilogue: lane holds row m0 + 32 i + lane % 32; register 8 g + 4 k2 + q holds
+    // column nw + 16 g + 8 k2 + 4 h + q; after the swap, o[g][0..7] holds the
+    // columns nw + 16 g + 8 h + 0 .. 7
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t m = m0 + i * 32 + (lane & 31);
+      float o[2][8];
+#pragma unroll
+      for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[i][8 * gq + q]),
+                                                          __float_as_uint(acc[i][8 * gq + 4 + q]), false, false);
+          o[gq][q] = __uint_as_float(r[0]);
+          o[gq][4 + q] = __uint_as_float(r[1]);
+        }
+#pragma unroll
+      for (int gq = 0; gq < 2; ++gq) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          float x = o[gq][c] + bb[gq][c];
+          if constexpr (KIND == PK_BIAS_RELU) x = fmaxf(x, 0.0f);
+          if constexpr (KIND == PK_BIAS_SOFTPLUS) x = softplus_sel(x, e.beta, ib);
+          o[gq][c] = x;
+        }
+        const int64_t col = nw + 16 * gq + 8 * h;
+        if (e.c_dtype == NA_BF16) vstore8((bf16_t*)e.C + m * e.ldc + col, o[gq]);
+        else vstore8((float*)e.C + m * e.ldc + col, o[gq]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Weight-gradient GEMM: C[n, k] += sum_m A[m, n] B[m, k] (bf16 in, fp32 out),
 // A = [M, lda] and B = [M, ldb] row-major, M (the points) much larger than N
 // and K (the layer widths).  Each workgroup owns a C tile of 256 x 64*KB in
@@ -1797,6 +1963,38 @@ static void launch_persist(int kind, int grid, const GemmParams& p, hipStream_t 
   }
 }
 
+// Weights-stationary path: kinds without head, pre-activation output or
+// per-element operands, N = 256, whole 128-row tiles, K of 5, 8 or 9 stages.
+static int ws_stages(const na_gemm_args* a, int kind) {
+  if (kind != PK_PLAIN && kind != PK_BIAS_RELU && kind != PK_BIAS_SOFTPLUS) return 0;
+  if (a->z_out || a->head_w) return 0;
+  if (a->N != 256 || a->M % GBM != 0 || a->K % 8 != 0 || a->lda % 8 || a->ldb % 8) return 0;
+  if (!aligned16(a->A) || !aligned16(a->B)) return 0;
+  const int64_t nk = ceil_div(a->K, (int64_t)kNNBK);
+  return (nk == 5 || nk == 8 || nk == 9) ? (int)nk : 0;
+}
+
+static int ws_grid(int64_t n_mt) {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cus[dev] == 0 && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return (int)(n_mt < cus[dev] ? n_mt : cus[dev]);
+}
+
+static void launch_ws(int nk, int kind, int grid, const GemmParams& p, hipStream_t s) {
+  const dim3 g(grid), b(512);
+#define NA_WS(NK)                                                                                       \
+  switch (kind) {                                                                                       \
+    case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NK, PK_PLAIN>), g, b, 0, s, p); break;        \
+    case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NK, PK_BIAS_RELU>), g, b, 0, s, p); break;\
+    default: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NK, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;      \
+  }
+  if (nk == 5) { NA_WS(5) } else if (nk == 8) { NA_WS(8) } else { NA_WS(9) }
+#undef NA_WS
+}
+
 // Weight-gradient products (both operands transposed, bf16, atomic fp32
 // output, no epilogue terms) run on the row-range reduction kernel.
 static bool wgrad_ok(const na_gemm_args* a) {
@@ -1907,6 +2105,15 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   hipStream_t s = (hipStream_t)stream;
   if (wgrad_ok(a)) return launch_wgrad(a, s);
   const int kind = persist_kind(a, (int)nsplit);
+  if (kind >= 0) {
+    const int nk = ws_stages(a, kind);
+    const int wg = nk ? ws_grid(a->M / GBM) : 0;
+    if (wg > 0) {
+      launch_ws(nk, kind, wg, p, s);
+      NA_CHECK_LAUNCH();
+      return NA_OK;
+    }
+  }
   if (kind >= 0) {
     const int wn = a->N > 128 ? 4 : 2;
     const int64_t n_tiles = ceil_div(a->M, GBM) * ceil_div(a->N, (int64_t)(64 * wn));

@@ -167,6 +167,33 @@ def test_gemm_rejects_misaligned(cuda):
                 C, 10, split_k=2)
 
 
+@pytest.mark.parametrize("M,K,lda", [(128 * 300, 144, 144), (128 * 517, 256, 256),
+                                     (128 * 211, 280, 280), (128 * 3, 256, 264), (128, 136, 136)])
+def test_gemm_weights_stationary_layers(cuda, M, K, lda):
+    """256-wide layers on whole 128-row tiles (the weights-stationary path: weights
+    in registers, activation stages only, register epilogue): plain, bias + ReLU
+    and bias + softplus against float64 on the same bf16 inputs, bf16 and fp32 C.
+    K = 136, 144 and 280 end in a partly filled 32-wide stage; lda = 264 pads rows."""
+    bf = torch.bfloat16
+    N = 256
+    A = _rand((M, lda), 31).to(bf)
+    B = (_rand((N, K), 32) * 0.1).to(bf)
+    b = _rand((N,), 33, 0.5)
+    z = A[:, :K].double() @ B.double().t()
+    for act in (0, NA_ACT_RELU, NA_ACT_SOFTPLUS):
+        ref = z if act == 0 else _act(z + b.double(), act)
+        for cdt in (bf, torch.float32):
+            C = torch.full((M, N), float("nan"), dtype=cdt, device=cuda)
+            if act == 0:
+                Fn.gemm(A.to(cuda), lda, 0, B.to(cuda), K, 0, M, N, K, C, N)
+            else:
+                Fn.gemm(A.to(cuda), lda, 0, B.to(cuda), K, 0, M, N, K, C, N, bias=b.to(cuda), act=act)
+            got = C.cpu().double()
+            tol = (2 ** -7 if cdt == bf else 1e-5) * ref.abs() + 1e-4
+            assert torch.isfinite(got).all(), (act, cdt)
+            assert ((got - ref).abs() <= tol).all(), (act, cdt, (got - ref).abs().max().item())
+
+
 @pytest.mark.parametrize("M,N,K", [(70001, 256, 144), (140001, 128, 256), (3001, 200, 280),
                                    (999, 24, 136), (1000, 300, 256)])
 def test_gemm_nn_bf16_fused_epilogues(cuda, M, N, K):
