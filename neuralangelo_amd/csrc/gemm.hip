@@ -1583,10 +1583,15 @@ gemm_bf16_nn_persist_kernel(GemmParams p) {
 // permlane32 swap between the two halves of the wave turns them into two
 // 8-column groups per lane, so the epilogue stores 16-byte row segments from
 // registers with no LDS staging.
-constexpr int kWsNS = 15;
 constexpr int kWsStage = GBM * kNNBK;
+// NWV waves own 32 columns each (N = 32 NWV); eight waves: one workgroup per CU
+// with 15 stages; four waves: two workgroups per CU with 9 stages each.
+template <int NWV> struct WsCfg {
+  static constexpr int NS = NWV == 8 ? 15 : 9;
+  static constexpr int ROWS = GBM / NWV;  // stage rows loaded per wave
+};
 
-// vmcnt(ahead) for ahead = 0 .. kWsNS - 2 (one load per wave per stage)
+// vmcnt(n) for n = 0 .. 15 (larger counts wait for 15, which is also safe)
 __device__ inline void ws_wait(int ahead) {
   switch (ahead) {
     case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
@@ -1602,7 +1607,9 @@ __device__ inline void ws_wait(int ahead) {
     case 10: __builtin_amdgcn_s_waitcnt(0xF7A); break;
     case 11: __builtin_amdgcn_s_waitcnt(0xF7B); break;
     case 12: __builtin_amdgcn_s_waitcnt(0xF7C); break;
-    default: __builtin_amdgcn_s_waitcnt(0xF7D); break;
+    case 13: __builtin_amdgcn_s_waitcnt(0xF7D); break;
+    case 14: __builtin_amdgcn_s_waitcnt(0xF7E); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF7F); break;
   }
 }
 
@@ -1610,8 +1617,10 @@ __device__ inline void ws_issue(__amdgpu_buffer_rsrc_t r, uint32_t off, bf16_t* 
   __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t*)dst, 16, off, 0, 0, 0);
 }
 
-template <int NK, int KIND>
-__global__ void __launch_bounds__(512) gemm_bf16_ws_kernel(GemmParams p) {
+template <int NWV, int NK, int KIND>
+__global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kernel(GemmParams p) {
+  constexpr int kWsNS = WsCfg<NWV>::NS;
+  constexpr int LOADS = WsCfg<NWV>::ROWS / 16;
   __shared__ __attribute__((aligned(16))) unsigned char smem[kWsNS * kWsStage * 2];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int h = lane >> 5;
@@ -1645,11 +1654,13 @@ __global__ void __launch_bounds__(512) gemm_bf16_ws_kernel(GemmParams p) {
     for (int gq = 0; gq < 2; ++gq) vload8(e.bias + nw + 16 * gq + 8 * h, bb[gq]);
   }
 
-  // stage loads: wave w fills rows 16 w .. 16 w + 15; lane l lands at slot l & 3
-  // of row l >> 2 and fetches the chunk that belongs there
-  const int arow = wave * 16 + (lane >> 2);
+  // stage loads: wave w fills the 16-row blocks LOADS w .. LOADS w + LOADS - 1;
+  // lane l lands at slot l & 3 of row l >> 2 of a block and fetches the chunk
+  // that belongs there (the chunk index is the same for every block)
+  const int arow = wave * 16 * LOADS + (lane >> 2);
   const int achunk = (lane & 3) ^ ((arow >> 2) & 3);
   const uint32_t avoff = (uint32_t)(arow * p.lda * 2 + achunk * 16);
+  const uint32_t ablk = (uint32_t)(16 * p.lda * 2);  // byte step between a wave's blocks
   const int akofs = achunk * 8;
   int64_t issued = 0;
   uint32_t is_tile = blockIdx.x;
@@ -1658,8 +1669,12 @@ __global__ void __launch_bounds__(512) gemm_bf16_ws_kernel(GemmParams p) {
   int64_t is_ae = (p.M - (int64_t)is_tile * GBM) * p.lda * 2;
   auto issue_next = [&]() {
     const int64_t k0 = (int64_t)is_t * kNNBK;
-    ws_issue(stage_rsrc(is_a, is_ae, k0 * 2), akofs < p.K - k0 ? avoff : kOobOffset,
-             st0 + is_slot * kWsStage + wave * 16 * kNNBK);
+    const __amdgpu_buffer_rsrc_t rs = stage_rsrc(is_a, is_ae, k0 * 2);
+    const bool in_k = akofs < p.K - k0;
+#pragma unroll
+    for (int u = 0; u < LOADS; ++u)
+      ws_issue(rs, in_k ? avoff + u * ablk : kOobOffset,
+               st0 + is_slot * kWsStage + (wave * LOADS + u) * 16 * kNNBK);
     ++issued;
     is_slot = is_slot + 1 == kWsNS ? 0 : is_slot + 1;
     if (++is_t == NK) {
@@ -1685,7 +1700,7 @@ __global__ void __launch_bounds__(512) gemm_bf16_ws_kernel(GemmParams p) {
       for (int r = 0; r < 16; ++r) acc[i][r] = 0.0f;
 #pragma unroll
     for (int t = 0; t < NK; ++t, ++g) {
-      ws_wait((int)(issued - g - 1));
+      ws_wait((int)(issued - g - 1) * LOADS);
       __builtin_amdgcn_s_barrier();  // stage g landed in every wave; stage g-1 consumed
       if (issued < total) issue_next();
       const bf16_t* sa = st0 + slot * kWsStage;
@@ -1968,28 +1983,32 @@ static void launch_persist(int kind, int grid, const GemmParams& p, hipStream_t 
 static int ws_stages(const na_gemm_args* a, int kind) {
   if (kind != PK_PLAIN && kind != PK_BIAS_RELU && kind != PK_BIAS_SOFTPLUS) return 0;
   if (a->z_out || a->head_w) return 0;
-  if (a->N != 256 || a->M % GBM != 0 || a->K % 8 != 0 || a->lda % 8 || a->ldb % 8) return 0;
+  if ((a->N != 256 && a->N != 128) || a->M % GBM != 0 || a->K % 8 != 0 || a->lda % 8 || a->ldb % 8)
+    return 0;
   if (!aligned16(a->A) || !aligned16(a->B)) return 0;
   const int64_t nk = ceil_div(a->K, (int64_t)kNNBK);
   return (nk == 5 || nk == 8 || nk == 9) ? (int)nk : 0;
 }
 
-static int ws_grid(int64_t n_mt) {
+// one workgroup per CU (eight waves) or two (four waves)
+static int ws_grid(int64_t n_mt, int nwv) {
   static int cus[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
   if (cus[dev] == 0 && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
-  return (int)(n_mt < cus[dev] ? n_mt : cus[dev]);
+  const int64_t per = (int64_t)cus[dev] * (nwv == 8 ? 1 : 2);
+  return (int)(n_mt < per ? n_mt : per);
 }
 
+template <int NWV>
 static void launch_ws(int nk, int kind, int grid, const GemmParams& p, hipStream_t s) {
-  const dim3 g(grid), b(512);
-#define NA_WS(NK)                                                                                       \
-  switch (kind) {                                                                                       \
-    case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NK, PK_PLAIN>), g, b, 0, s, p); break;        \
-    case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NK, PK_BIAS_RELU>), g, b, 0, s, p); break;\
-    default: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NK, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;      \
+  const dim3 g(grid), b(64 * NWV);
+#define NA_WS(NK)                                                                                             \
+  switch (kind) {                                                                                             \
+    case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_PLAIN>), g, b, 0, s, p); break;         \
+    case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_RELU>), g, b, 0, s, p); break; \
+    default: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;       \
   }
   if (nk == 5) { NA_WS(5) } else if (nk == 8) { NA_WS(8) } else { NA_WS(9) }
 #undef NA_WS
@@ -2107,9 +2126,11 @@ extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   const int kind = persist_kind(a, (int)nsplit);
   if (kind >= 0) {
     const int nk = ws_stages(a, kind);
-    const int wg = nk ? ws_grid(a->M / GBM) : 0;
+    const int nwv = a->N == 256 ? 8 : 4;
+    const int wg = nk ? ws_grid(a->M / GBM, nwv) : 0;
     if (wg > 0) {
-      launch_ws(nk, kind, wg, p, s);
+      if (nwv == 8) launch_ws<8>(nk, kind, wg, p, s);
+      else launch_ws<4>(nk, kind, wg, p, s);
       NA_CHECK_LAUNCH();
       return NA_OK;
     }

@@ -167,15 +167,17 @@ def test_gemm_rejects_misaligned(cuda):
                 C, 10, split_k=2)
 
 
-@pytest.mark.parametrize("M,K,lda", [(128 * 300, 144, 144), (128 * 517, 256, 256),
-                                     (128 * 211, 280, 280), (128 * 3, 256, 264), (128, 136, 136)])
-def test_gemm_weights_stationary_layers(cuda, M, K, lda):
-    """256-wide layers on whole 128-row tiles (the weights-stationary path: weights
-    in registers, activation stages only, register epilogue): plain, bias + ReLU
-    and bias + softplus against float64 on the same bf16 inputs, bf16 and fp32 C.
-    K = 136, 144 and 280 end in a partly filled 32-wide stage; lda = 264 pads rows."""
+@pytest.mark.parametrize("M,N,K,lda", [(128 * 300, 256, 144, 144), (128 * 517, 256, 256, 256),
+                                       (128 * 211, 256, 280, 280), (128 * 3, 256, 256, 264),
+                                       (128, 256, 136, 136), (128 * 401, 128, 256, 256),
+                                       (128 * 7, 128, 144, 152)])
+def test_gemm_weights_stationary_layers(cuda, M, N, K, lda):
+    """256- and 128-wide layers on whole 128-row tiles (the weights-stationary path:
+    weights in registers, activation stages only, register epilogue): plain,
+    bias + ReLU and bias + softplus against float64 on the same bf16 inputs, bf16
+    and fp32 C.  K = 136, 144 and 280 end in a partly filled 32-wide stage;
+    lda = 264 and 152 pad rows."""
     bf = torch.bfloat16
-    N = 256
     A = _rand((M, lda), 31).to(bf)
     B = (_rand((N, K), 32) * 0.1).to(bf)
     b = _rand((N,), 33, 0.5)
