@@ -1638,8 +1638,11 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
     const bf16_t* brow = (const bf16_t*)p.B + (int64_t)(nw + (lane & 31)) * p.ldb;
 #pragma unroll
     for (int s = 0; s < 2 * NK; ++s) {
+      // unconditional load (the row's last chunk stands in past K) and a select,
+      // so the loads carry no branch and the compiler counts them exactly
       const int64_t k = 16 * s + 8 * h;
-      const uint4 u = k < p.K ? *reinterpret_cast<const uint4*>(brow + k) : make_uint4(0u, 0u, 0u, 0u);
+      uint4 u = *reinterpret_cast<const uint4*>(brow + (k < p.K ? k : p.K - 8));
+      if (k >= p.K) u = make_uint4(0u, 0u, 0u, 0u);
       bw[s] = __builtin_bit_cast(bf16x8_t, u);
     }
   }
@@ -1684,15 +1687,46 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
       is_ae = (p.M - (int64_t)is_tile * GBM) * p.lda * 2;
     }
   };
+  // Stages are issued unconditionally: those past the last tile read nothing
+  // (their buffer covers no bytes) and land in slots no wave reads, and the
+  // workgroup waits for them before it ends.  With a fixed count of loads per
+  // iteration the compiler's waits for the per-tile operand registers below
+  // count exactly the stage loads issued after them.
+  (void)total;
 #pragma unroll
-  for (int s = 0; s < kWsNS - 1; ++s)
-    if (issued < total) issue_next();
+  for (int s = 0; s < kWsNS - 1; ++s) issue_next();
 
   const float ib = 1.0f / e.beta;
+  // derivative kind: rank-1 coefficients of the lane's columns and column sums
+  float rb[2][8], cs[2][8];
+#pragma unroll
+  for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) rb[gq][c] = cs[gq][c] = 0.0f;
+  if constexpr (KIND == PK_DACT) {
+    if (e.r1_a) {
+#pragma unroll
+      for (int gq = 0; gq < 2; ++gq) vload8(e.r1_b + nw + 16 * gq + 8 * h, rb[gq]);
+    }
+  }
   int slot = 0;
   int64_t g = 0;
   for (int64_t jt = 0; jt < my_tiles; ++jt) {
     const int64_t m0 = (int64_t)(blockIdx.x + (uint32_t)jt * gridDim.x) * GBM;
+    // derivative kind: the tile's stored activations (bf16) and rank-1 row terms,
+    // requested before this tile's stage loads
+    uint4 yv[KIND == PK_DACT ? 4 : 1][2];
+    float r1v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (KIND == PK_DACT) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t m = m0 + i * 32 + (lane & 31);
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq)
+          yv[i][gq] = *reinterpret_cast<const uint4*>((const bf16_t*)e.dact_y + m * e.ld_dact + nw + 16 * gq + 8 * h);
+        if (e.r1_a) r1v[i] = e.r1_a[m];
+      }
+    }
     f32x16_t acc[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -1702,7 +1736,7 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
     for (int t = 0; t < NK; ++t, ++g) {
       ws_wait((int)(issued - g - 1) * LOADS);
       __builtin_amdgcn_s_barrier();  // stage g landed in every wave; stage g-1 consumed
-      if (issued < total) issue_next();
+      issue_next();
       const bf16_t* sa = st0 + slot * kWsStage;
       slot = slot + 1 == kWsNS ? 0 : slot + 1;
 #pragma unroll
@@ -1737,12 +1771,30 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
         }
 #pragma unroll
       for (int gq = 0; gq < 2; ++gq) {
+        if constexpr (KIND == PK_DACT) {
+          const uint32_t yw[4] = {yv[i][gq].x, yv[i][gq].y, yv[i][gq].z, yv[i][gq].w};
+          float y[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          float x = o[gq][c] + bb[gq][c];
-          if constexpr (KIND == PK_BIAS_RELU) x = fmaxf(x, 0.0f);
-          if constexpr (KIND == PK_BIAS_SOFTPLUS) x = softplus_sel(x, e.beta, ib);
-          o[gq][c] = x;
+          for (int c = 0; c < 4; ++c) {
+            y[2 * c] = __uint_as_float(yw[c] << 16);
+            y[2 * c + 1] = __uint_as_float(yw[c] & 0xffff0000u);
+          }
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            float x = fmaf(r1v[i], rb[gq][c], o[gq][c]);
+            x = e.dact == NA_ACT_RELU ? (y[c] > 0.0f ? x : 0.0f)
+                                      : x * act_deriv_from_y(NA_ACT_SOFTPLUS, y[c], e.beta);
+            cs[gq][c] += x;
+            o[gq][c] = x;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            float x = o[gq][c] + bb[gq][c];
+            if constexpr (KIND == PK_BIAS_RELU) x = fmaxf(x, 0.0f);
+            if constexpr (KIND == PK_BIAS_SOFTPLUS) x = softplus_sel(x, e.beta, ib);
+            o[gq][c] = x;
+          }
         }
         const int64_t col = nw + 16 * gq + 8 * h;
         if (e.c_dtype == NA_BF16) vstore8((bf16_t*)e.C + m * e.ldc + col, o[gq]);
@@ -1750,6 +1802,30 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
       }
     }
   }
+  if constexpr (KIND == PK_DACT) {
+    if (e.colsum) {
+      // sums over the 32 rows of the lanes that share h, then one atomic per column
+#pragma unroll
+      for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          float s = cs[gq][c];
+          s += __shfl_xor(s, 1);
+          s += __shfl_xor(s, 2);
+          s += __shfl_xor(s, 4);
+          s += __shfl_xor(s, 8);
+          s += __shfl_xor(s, 16);
+          cs[gq][c] = s;
+        }
+      if ((lane & 31) == 0) {
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) atomic_add_f32(e.colsum + nw + 16 * gq + 8 * h + c, cs[gq][c]);
+      }
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0xF70);  // no buffer-to-LDS load outlives the workgroup
 }
 
 // ---------------------------------------------------------------------------
@@ -1981,8 +2057,9 @@ static void launch_persist(int kind, int grid, const GemmParams& p, hipStream_t 
 // Weights-stationary path: kinds without head, pre-activation output or
 // per-element operands, N = 256, whole 128-row tiles, K of 5, 8 or 9 stages.
 static int ws_stages(const na_gemm_args* a, int kind) {
-  if (kind != PK_PLAIN && kind != PK_BIAS_RELU && kind != PK_BIAS_SOFTPLUS) return 0;
+  if (kind != PK_PLAIN && kind != PK_BIAS_RELU && kind != PK_BIAS_SOFTPLUS && kind != PK_DACT) return 0;
   if (a->z_out || a->head_w) return 0;
+  if (kind == PK_DACT && (a->dact_dtype != NA_BF16 || a->ld_dact % 8 || !aligned16(a->dact_y))) return 0;
   if ((a->N != 256 && a->N != 128) || a->M % GBM != 0 || a->K % 8 != 0 || a->lda % 8 || a->ldb % 8)
     return 0;
   if (!aligned16(a->A) || !aligned16(a->B)) return 0;
@@ -2008,6 +2085,7 @@ static void launch_ws(int nk, int kind, int grid, const GemmParams& p, hipStream
   switch (kind) {                                                                                             \
     case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_PLAIN>), g, b, 0, s, p); break;         \
     case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_RELU>), g, b, 0, s, p); break; \
+    case PK_DACT: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_DACT>), g, b, 0, s, p); break;           \
     default: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;       \
   }
   if (nk == 5) { NA_WS(5) } else if (nk == 8) { NA_WS(8) } else { NA_WS(9) }

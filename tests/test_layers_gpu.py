@@ -196,6 +196,36 @@ def test_gemm_weights_stationary_layers(cuda, M, N, K, lda):
             assert ((got - ref).abs() <= tol).all(), (act, cdt, (got - ref).abs().max().item())
 
 
+@pytest.mark.parametrize("M", [128 * 613, 128 * 2])
+def test_gemm_weights_stationary_derivative(cuda, M):
+    """Backward layers on the weights-stationary path: (dY . W + r1_a r1_b) * act'(Y)
+    from a bf16 activation Y, ReLU and softplus, with and without the rank-1 term,
+    column sums accumulated over row tiles, bf16 and fp32 C, against float64."""
+    bf = torch.bfloat16
+    N = K = 256
+    A = _rand((M, K), 41, 0.3).to(bf)
+    B = (_rand((N, K), 42) * 0.1).to(bf)
+    r1a, r1b = _rand((M,), 43), _rand((N,), 44)
+    z = A.double() @ B.double().t()
+    for dact in (NA_ACT_RELU, NA_ACT_SOFTPLUS):
+        pre = _rand((M, N), 45, 0.02 if dact == NA_ACT_SOFTPLUS else 1.0)
+        Y = (_act(pre, dact)).to(bf)
+        d = _dact(Y.double(), dact)
+        for rank1 in (False, True):
+            ref = (z + (r1a.double()[:, None] * r1b.double()[None] if rank1 else 0.0)) * d
+            for cdt in (bf, torch.float32):
+                C = torch.full((M, N), float("nan"), dtype=cdt, device=cuda)
+                cs = torch.zeros(N, device=cuda)
+                kw = dict(r1_a=r1a.to(cuda), r1_b=r1b.to(cuda)) if rank1 else {}
+                Fn.gemm(A.to(cuda), K, 0, B.to(cuda), K, 0, M, N, K, C, N, dact=dact,
+                        dact_y=Y.to(cuda), ld_dact=N, colsum=cs, **kw)
+                got = C.cpu().double()
+                tol = (2 ** -7 if cdt == bf else 1e-5) * ref.abs() + 1e-4
+                assert ((got - ref).abs() <= tol).all(), (dact, rank1, cdt)
+                # column sums of the fp32 values before the output cast
+                assert (cs.cpu().double() - ref.sum(0)).abs().max() < 1e-4 * M ** 0.5 + 1e-3
+
+
 @pytest.mark.parametrize("M,N,K", [(70001, 256, 144), (140001, 128, 256), (3001, 200, 280),
                                    (999, 24, 136), (1000, 300, 256)])
 def test_gemm_nn_bf16_fused_epilogues(cuda, M, N, K):
