@@ -147,8 +147,8 @@ typedef struct na_gemm_args {
    *   head_w:   a narrow head fused into the epilogue: head_out[part * M + m] =
    *             sum over the tile's columns n of head_w[n] * h, h = the
    *             activated output, or act(z0 + dz) - act(z0) in delta mode
-   *             (evaluated without cancellation); part = 2 * (n / 128) +
-   *             (n % 128) / 64, i.e. 2 * ceil(N / 128) partial sums per row. */
+   *             (evaluated without cancellation); part = n / 32, i.e.
+   *             4 * ceil(N / 128) partial sums per row (zero past N). */
   float* z_out;
   int64_t ld_z;
   const float* zc;

@@ -316,7 +316,7 @@ __device__ __attribute__((always_inline)) inline void gemm_epilogue(
   const bool ncol = n < p.N;
   const bool head = e.head_w != nullptr;
   const bool rowop = PIPE && (e.zc || e.dact);
-  const int64_t part = 2 * (int64_t)blockIdx.y + wn;
+  const int64_t part = 2 * (2 * (int64_t)blockIdx.y + wn);  // first of the wave's two 32-column parts
   // per-column operands, loaded once
   const float hw = (head && ncol) ? e.head_w[n] : 0.0f;
   const float bias_n = (e.bias && ncol) ? e.bias[n] : 0.0f;
@@ -382,11 +382,12 @@ __device__ __attribute__((always_inline)) inline void gemm_epilogue(
     }
     if (head) {
       __syncthreads();
-      // lanes 0..31 each reduce one row of the wave's 64 columns (fixed order)
-      if (lane < 32 && mb + lane < p.M) {
+      // lane l reduces row l % 32 over the 32 columns of half l / 32 (fixed order)
+      const int rr = lane & 31, half = lane >> 5;
+      if (mb + rr < p.M) {
         float sum = 0.0f;
-        for (int c = 0; c < 64; ++c) sum += Cw[lane * CP + c];
-        e.head_out[part * p.M + mb + lane] = sum;
+        for (int c = 0; c < 32; ++c) sum += Cw[rr * CP + 32 * half + c];
+        e.head_out[(part + half) * p.M + mb + rr] = sum;
       }
     }
     __syncthreads();
@@ -1258,7 +1259,7 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue_full(
   }
   float oc[4][2][8];
   float oz[KIND == PK_BIAS_SOFTPLUS ? 4 : 1][2][8];
-  float hs[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+  float hs[2][2][2] = {};  // [row block][column block][half row] head partials
 #pragma unroll
   for (int blk = 0; blk < 4; ++blk) {
     const int i = blk >> 1, j = blk & 1;
@@ -1307,7 +1308,7 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue_full(
         float s = 0.0f;
 #pragma unroll
         for (int c = 0; c < 8; ++c) s += hw[j][c] * hv[c];
-        hs[i][hh] += s;
+        hs[i][j][hh] += s;
       }
 #pragma unroll
       for (int c = 0; c < 8; ++c) oc[blk][hh][c] = x[c];
@@ -1331,12 +1332,14 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue_full(
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int hh = 0; hh < 2; ++hh) {
-          float s = hs[i][hh];
-          s += __shfl_xor(s, 16);
-          s += __shfl_xor(s, 32);
-          if (cg == 0) e.head_out[(int64_t)part * p.M + mw + i * 32 + hh * 16 + rl] = s;
-        }
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int hh = 0; hh < 2; ++hh) {
+            float s = hs[i][j][hh];
+            s += __shfl_xor(s, 16);
+            s += __shfl_xor(s, 32);
+            if (cg == 0) e.head_out[(int64_t)(part + j) * p.M + mw + i * 32 + hh * 16 + rl] = s;
+          }
     }
   }
 }
@@ -1355,7 +1358,7 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue(
   constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA;
   if (full) {
     persist_epilogue_full<WN, KIND>(p, eb, acc, mw, nw, lane, csum, zbase + wm * 64, ib,
-                                    (int)(n0 / 64 + wn));
+                                    (int)(nw / 32));
     return;
   }
   float hs[2] = {0.0f, 0.0f};
@@ -1389,15 +1392,15 @@ __device__ __attribute__((always_inline)) inline void persist_epilogue(
       }
     }
     if constexpr (kHead) {
-      if (e.head_w && j == 1) {
+      if (e.head_w) {
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
           float s = hs[hh];
           s += __shfl_xor(s, 16);
           s += __shfl_xor(s, 32);
-          // parts = one per 64 columns, 2 * ceil(N / 128) of them (zero where past N)
-          const int64_t part = n0 / 64 + wn, m = mw + i * 32 + hh * 16 + rl;
-          if (cg == 0 && m < p.M && part < 2 * ((p.N + 127) / 128)) e.head_out[part * p.M + m] = s;
+          // parts = one per 32 columns, 4 * ceil(N / 128) of them (zero where past N)
+          const int64_t part = (nw + 32 * j) / 32, m = mw + i * 32 + hh * 16 + rl;
+          if (cg == 0 && m < p.M && part < 4 * ((p.N + 127) / 128)) e.head_out[part * p.M + m] = s;
           hs[hh] = 0.0f;
         }
       }
@@ -1698,11 +1701,18 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
 
   const float ib = 1.0f / e.beta;
   // derivative kind: rank-1 coefficients of the lane's columns and column sums
-  float rb[2][8], cs[2][8];
+  constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA;
+  float rb[2][8], cs[2][8], hw[2][8];
 #pragma unroll
   for (int gq = 0; gq < 2; ++gq)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) rb[gq][c] = cs[gq][c] = 0.0f;
+    for (int c = 0; c < 8; ++c) rb[gq][c] = cs[gq][c] = hw[gq][c] = 0.0f;
+  if constexpr (kHead) {
+    if (e.head_w) {
+#pragma unroll
+      for (int gq = 0; gq < 2; ++gq) vload8(e.head_w + nw + 16 * gq + 8 * h, hw[gq]);
+    }
+  }
   if constexpr (KIND == PK_DACT) {
     if (e.r1_a) {
 #pragma unroll
@@ -1717,6 +1727,17 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
     // requested before this tile's stage loads
     uint4 yv[KIND == PK_DACT ? 4 : 1][2];
     float r1v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // increment kind: the stored pre-activation rows (m mod zc_rows; zc_rows is a
+    // multiple of 128, so a tile's rows do not wrap)
+    float zv[KIND == PK_DELTA ? 4 : 1][2][8];
+    if constexpr (KIND == PK_DELTA) {
+      const int64_t z0 = m0 % e.zc_rows;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq)
+          vload8(e.zc + (z0 + i * 32 + (lane & 31)) * e.ld_zc + nw + 16 * gq + 8 * h, zv[i][gq]);
+    }
     if constexpr (KIND == PK_DACT) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -1760,6 +1781,7 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
     for (int i = 0; i < 4; ++i) {
       const int64_t m = m0 + i * 32 + (lane & 31);
       float o[2][8];
+      float hsum = 0.0f;
 #pragma unroll
       for (int gq = 0; gq < 2; ++gq)
 #pragma unroll
@@ -1787,18 +1809,42 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
             cs[gq][c] += x;
             o[gq][c] = x;
           }
-        } else {
+        } else if constexpr (KIND == PK_DELTA) {
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
-            float x = o[gq][c] + bb[gq][c];
+            float hv, y1;
+            softplus_step(zv[i][gq][c], o[gq][c], e.beta, ib, hv, y1);
+            hsum = fmaf(hw[gq][c], hv, hsum);
+            o[gq][c] = y1;
+          }
+        } else {
+          const int64_t col = nw + 16 * gq + 8 * h;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) o[gq][c] += bb[gq][c];
+          if constexpr (KIND == PK_BIAS_SOFTPLUS) {
+            if (e.z_out) vstore8(e.z_out + m * e.ld_z + col, o[gq]);
+          }
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            float x = o[gq][c];
             if constexpr (KIND == PK_BIAS_RELU) x = fmaxf(x, 0.0f);
-            if constexpr (KIND == PK_BIAS_SOFTPLUS) x = softplus_sel(x, e.beta, ib);
+            if constexpr (KIND == PK_BIAS_SOFTPLUS) {
+              x = softplus_sel(x, e.beta, ib);
+              hsum = fmaf(hw[gq][c], x, hsum);
+            }
             o[gq][c] = x;
           }
         }
         const int64_t col = nw + 16 * gq + 8 * h;
         if (e.c_dtype == NA_BF16) vstore8((bf16_t*)e.C + m * e.ldc + col, o[gq]);
         else vstore8((float*)e.C + m * e.ldc + col, o[gq]);
+      }
+      if constexpr (kHead) {
+        // row m over the wave's 32 columns: this lane's 16 and its partner's 16
+        if (e.head_w) {
+          hsum += __shfl_xor(hsum, 32);
+          if (h == 0) e.head_out[(int64_t)wave * p.M + m] = hsum;
+        }
       }
     }
   }
@@ -2054,16 +2100,19 @@ static void launch_persist(int kind, int grid, const GemmParams& p, hipStream_t 
   }
 }
 
-// Weights-stationary path: kinds without head, pre-activation output or
-// per-element operands, N = 256, whole 128-row tiles, K of 5, 8 or 9 stages.
+// Weights-stationary path: N = 256 or 128, whole 128-row tiles, K of 5, 8 or 9
+// stages; bf16 activations for the derivative kind, stored rows in whole tiles
+// for the increment kind.
 static int ws_stages(const na_gemm_args* a, int kind) {
-  if (kind != PK_PLAIN && kind != PK_BIAS_RELU && kind != PK_BIAS_SOFTPLUS && kind != PK_DACT) return 0;
-  if (a->z_out || a->head_w) return 0;
   if (kind == PK_DACT && (a->dact_dtype != NA_BF16 || a->ld_dact % 8 || !aligned16(a->dact_y))) return 0;
+  if (kind == PK_DELTA && (a->zc_rows % GBM || a->ld_zc % 4)) return 0;
+  if ((a->z_out || a->head_w) && kind != PK_BIAS_SOFTPLUS && kind != PK_DELTA) return 0;
+  if (a->z_out && a->ld_z % 4) return 0;
   if ((a->N != 256 && a->N != 128) || a->M % GBM != 0 || a->K % 8 != 0 || a->lda % 8 || a->ldb % 8)
     return 0;
   if (!aligned16(a->A) || !aligned16(a->B)) return 0;
   const int64_t nk = ceil_div(a->K, (int64_t)kNNBK);
+  if (kind == PK_DELTA && nk != 5) return 0;  // the increment layer's width (K <= 160)
   return (nk == 5 || nk == 8 || nk == 9) ? (int)nk : 0;
 }
 
@@ -2086,6 +2135,7 @@ static void launch_ws(int nk, int kind, int grid, const GemmParams& p, hipStream
     case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_PLAIN>), g, b, 0, s, p); break;         \
     case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_RELU>), g, b, 0, s, p); break; \
     case PK_DACT: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_DACT>), g, b, 0, s, p); break;           \
+    case PK_DELTA: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_DELTA>), g, b, 0, s, p); break;         \
     default: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;       \
   }
   if (nk == 5) { NA_WS(5) } else if (nk == 8) { NA_WS(8) } else { NA_WS(9) }

@@ -116,8 +116,9 @@ def _zero_block(dev, *shapes):
 
 
 def head_parts(N):
-    """Number of partial sums per row written by a fused GEMM head over N columns."""
-    return 2 * ((N + 127) // 128)
+    """Number of partial sums per row written by a fused GEMM head over N columns
+    (one per 32 columns of the 128-column tiles)."""
+    return 4 * ((N + 127) // 128)
 
 
 def head_sum(parts, bias=None):

@@ -227,10 +227,14 @@ def test_gemm_weights_stationary_derivative(cuda, M):
 
 
 @pytest.mark.parametrize("M,N,K", [(70001, 256, 144), (140001, 128, 256), (3001, 200, 280),
-                                   (999, 24, 136), (1000, 300, 256)])
+                                   (999, 24, 136), (1000, 300, 256), (128 * 500, 256, 144),
+                                   (128 * 300, 256, 256), (128 * 8, 128, 144)])
 def test_gemm_nn_bf16_fused_epilogues(cuda, M, N, K):
     """bf16 NN products with every store-epilogue term the MLPs use, at row counts that
-    give each workgroup several tiles, ragged M and N, and N below / above one tile."""
+    give each workgroup several tiles, ragged M and N, and N below / above one tile;
+    the whole-tile shapes (M a multiple of 128, N = 256 or 128) take the
+    weights-stationary path, increment mode included when the stored rows fill
+    whole tiles."""
     bf = torch.bfloat16
     A = _rand((M, K), 20, 0.1).to(bf)
     B = _rand((N, K), 21, 0.1).to(bf)
