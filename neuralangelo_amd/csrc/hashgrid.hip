@@ -230,6 +230,32 @@ __device__ inline void gather_cell(const TT* __restrict__ tab, const LevelParams
   }
 }
 
+// The eight raw corner entries of a cell (F = 8 bf16: 16 bytes each) and their
+// weighted sum, split so that a tap in the centre's cell can reuse the centre's
+// entries instead of fetching them again.
+__device__ inline void load_cell8(const bf16_t* __restrict__ tab, const LevelParams& p,
+                                  const uint32_t (&cell)[3], uint4 (&v)[8]) {
+  uint32_t id[8];
+  cell_corners(cell, p, id);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = *reinterpret_cast<const uint4*>(tab + (size_t)id[c] * 8);
+}
+__device__ inline void sum_cell8(const uint4 (&v)[8], const float (&t)[3], float (&acc)[8]) {
+  float w[8];
+  cell_weights(t, w);
+#pragma unroll
+  for (int f = 0; f < 8; ++f) acc[f] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t u[4] = {v[c].x, v[c].y, v[c].z, v[c].w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[2 * k] = fmaf(w[c], __uint_as_float(u[k] << 16), acc[2 * k]);
+      acc[2 * k + 1] = fmaf(w[c], __uint_as_float(u[k] & 0xffff0000u), acc[2 * k + 1]);
+    }
+  }
+}
+
 // Input point -> unit-cube coordinate: u = (x - vmin) / vrange
 // (projects/neuralangelo/utils/modules.py:82-83; vmin = 0, vrange = 1 is the
 // tcnn convention of inputs already in [0,1]).
@@ -396,18 +422,28 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
   }
   const LevelParams p = level_params(d, l);
   const TT* tab = table + (size_t)p.off * F;
-  // centre first, then one tap at a time (two F-vectors live at once)
+  // centre first, then one tap at a time (two F-vectors live at once); with a
+  // bf16 table of 8 features the centre's raw corner entries stay in registers
+  // and a tap in the same cell reuses them (the fetch runs only for the lanes
+  // whose tap crossed a cell boundary)
+  constexpr bool kReuse = F == 8 && sizeof(TT) == 2;
   float ac[F];
+  uint32_t cc[3];
+  uint4 vc[kReuse ? 8 : 1];
   {
-    uint32_t cell[3];
     float fr[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      pos_fract(normalize_coord_gather(xc[k], vmin, vrange), p.scale, cell[k], fr[k]);
-    gather_cell<F, TT>(tab, p, cell, fr, ac);
+      pos_fract(normalize_coord_gather(xc[k], vmin, vrange), p.scale, cc[k], fr[k]);
+    if constexpr (kReuse) {
+      load_cell8((const bf16_t*)tab, p, cc, vc);
+      sum_cell8(vc, fr, ac);
+    } else {
+      gather_cell<F, TT>(tab, p, cc, fr, ac);
+    }
     RowStore<F, bf16_t>::store(X0c + i * ld + l * F, ac);
   }
-#pragma unroll
+#pragma unroll 1
   for (int t = 0; t < 4; ++t) {
     uint32_t cell[3];
     float fr[3];
@@ -415,7 +451,18 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
     for (int k = 0; k < 3; ++k)
       pos_fract(normalize_coord_gather(xt[t][k], vmin, vrange), p.scale, cell[k], fr[k]);
     float at[F];
-    gather_cell<F, TT>(tab, p, cell, fr, at);
+    if constexpr (kReuse) {
+      uint4 vt[8];
+      if (cell[0] == cc[0] && cell[1] == cc[1] && cell[2] == cc[2]) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) vt[c] = vc[c];
+      } else {
+        load_cell8((const bf16_t*)tab, p, cell, vt);
+      }
+      sum_cell8(vt, fr, at);
+    } else {
+      gather_cell<F, TT>(tab, p, cell, fr, at);
+    }
     const int64_t r = (int64_t)t * n + i;
     float dv[F];
 #pragma unroll
