@@ -589,31 +589,36 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
 #pragma unroll
       for (int k = 0; k < 3; ++k) blo[k] = lo[k];
     }
-    // per point and axis: offset of its cell in the running box (0 or 1)
-    int off[5][3];
+    // The weights are separable: point q adds g_q wx[ox] wy[oy] wz[oz] to box
+    // corner (ox, oy, oz), each factor the linear weight of that coordinate in the
+    // point's cell (offset 0 or 1 in the box) and zero outside it.  Corners a
+    // point touches are marked with the eight-corner pattern shifted by its
+    // offset; a point with zero gradient adds nothing and marks nothing.
 #pragma unroll
-    for (int q = 0; q < 5; ++q)
+    for (int q = 0; q < 5; ++q) {
+      float wv[3][3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) off[q][k] = cell[q][k] - blo[k];
+      for (int k = 0; k < 3; ++k) {
+        const bool o0 = cell[q][k] == blo[k];
+        const float f = fr[q][k], f1 = 1.0f - f;
+        wv[k][0] = o0 ? f1 : 0.0f;
+        wv[k][1] = o0 ? f : f1;
+        wv[k][2] = o0 ? 0.0f : f;
+      }
 #pragma unroll
-    for (int oz = 0; oz < 3; ++oz)
+      for (int oz = 0; oz < 3; ++oz)
 #pragma unroll
-      for (int oy = 0; oy < 3; ++oy)
+        for (int oy = 0; oy < 3; ++oy) {
+          const float gyz = (g[q] * wv[1][oy]) * wv[2][oz];
 #pragma unroll
-        for (int ox = 0; ox < 3; ++ox) {
-          const int o = (oz * 3 + oy) * 3 + ox;
-#pragma unroll
-          for (int q = 0; q < 5; ++q) {
-            const int dx = ox - off[q][0], dy = oy - off[q][1], dz = oz - off[q][2];
-            if (g[q] != 0.0f && (unsigned)dx <= 1u && (unsigned)dy <= 1u && (unsigned)dz <= 1u) {
-              float w = dx ? fr[q][0] : 1.0f - fr[q][0];
-              w *= dy ? fr[q][1] : 1.0f - fr[q][1];
-              w *= dz ? fr[q][2] : 1.0f - fr[q][2];
-              acc[o] += w * g[q];
-              mask |= 1u << o;
-            }
+          for (int ox = 0; ox < 3; ++ox) {
+            const int o = (oz * 3 + oy) * 3 + ox;
+            acc[o] = fmaf(wv[0][ox], gyz, acc[o]);
           }
         }
+      const int shift = (cell[q][0] - blo[0]) + 3 * (cell[q][1] - blo[1]) + 9 * (cell[q][2] - blo[2]);
+      if (g[q] != 0.0f) mask |= 0x361Bu << shift;  // corners {0,1}^3 of a 3x3x3 box
+    }
   }
   if (mask) taps_flush<F>(gtab, p, blo, acc, mask);
 }
