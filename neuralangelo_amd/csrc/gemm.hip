@@ -2203,6 +2203,50 @@ static int launch_gemm(const na_gemm_args* a, const GemmParams& p, dim3 grid, hi
 }
 
 
+// Vectorised act_grad for rows of whole 16-byte chunks (N a multiple of 8 with
+// N / 8 dividing 256): thread (r, c) of a block handles columns 8 c .. 8 c + 7
+// of rows r, r + R, ... (R = 256 / (N / 8) row lanes) over the block's row
+// range; column sums are reduced over the row lanes in LDS and leave the block
+// as one atomic per column.
+template <typename TD, typename TY, typename TO>
+__global__ void __launch_bounds__(256)
+act_grad_vec_kernel(const TD* __restrict__ dy, int64_t ld_dy, const TY* __restrict__ y, int64_t ld_y,
+                    int64_t M, int64_t N, int act, float beta, TO* __restrict__ out, int64_t ld_out,
+                    float* __restrict__ colsum, int64_t rows_per_block) {
+  __shared__ float red[256 * 8];
+  const int cpr = (int)(N / 8);
+  const int c = threadIdx.x % cpr, r = threadIdx.x / cpr, lanes = 256 / cpr;
+  const int64_t m0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t m1 = m0 + rows_per_block < M ? m0 + rows_per_block : M;
+  float cs[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cs[k] = 0.0f;
+  for (int64_t m = m0 + r; m < m1; m += lanes) {
+    float g[8], v[8];
+    vload8(dy + m * ld_dy + 8 * c, g);
+    vload8(y + m * ld_y + 8 * c, v);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      g[k] *= act_deriv_from_y(act, v[k], beta);
+      cs[k] += g[k];
+    }
+    vstore8(out + m * ld_out + 8 * c, g);
+  }
+  if (colsum) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[k * 256 + threadIdx.x] = cs[k];
+    __syncthreads();
+    if (r == 0) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        float s = 0.0f;
+        for (int q = 0; q < lanes; ++q) s += red[k * 256 + q * cpr + c];
+        atomic_add_f32(colsum + 8 * c + k, s);
+      }
+    }
+  }
+}
+
 }  // namespace na
 
 using namespace na;
@@ -2364,12 +2408,37 @@ extern "C" int na_act_grad(const void* dy, int64_t ld_dy, int32_t dy_dtype, cons
   NA_CHECK_ARG(M >= 0 && N >= 0, "bad sizes");
   if (M == 0 || N == 0) return NA_OK;
   NA_CHECK_ARG(dy && y && out, "null pointer");
-  dim3 grid((unsigned)ceil_div(M, 256), (unsigned)ceil_div(N, 64));
   hipStream_t s = (hipStream_t)stream;
+  const bool d16 = dy_dtype == NA_BF16, y16 = y_dtype == NA_BF16, o16 = out_dtype == NA_BF16;
+  // whole 16-byte chunks: the vectorised kernel, about two blocks per CU
+  const bool vec = N % 8 == 0 && 256 % (N / 8) == 0 && ld_dy % 8 == 0 && ld_y % 8 == 0 &&
+                   ld_out % 8 == 0 && ((uintptr_t)dy | (uintptr_t)y | (uintptr_t)out) % 16 == 0 &&
+                   (d16 || ld_dy % 4 == 0);
+  if (vec && d16 && y16) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    const int64_t lanes = 256 / (N / 8);
+    int64_t blocks = 2 * (int64_t)cus;
+    int64_t rpb = ceil_div(ceil_div(M, blocks), lanes) * lanes;
+    if (rpb < 64) rpb = 64;
+    blocks = ceil_div(M, rpb);
+    if (o16)
+      hipLaunchKernelGGL((act_grad_vec_kernel<bf16_t, bf16_t, bf16_t>), dim3((unsigned)blocks), dim3(256), 0,
+                         s, (const bf16_t*)dy, ld_dy, (const bf16_t*)y, ld_y, M, N, act, act_beta,
+                         (bf16_t*)out, ld_out, colsum, rpb);
+    else
+      hipLaunchKernelGGL((act_grad_vec_kernel<bf16_t, bf16_t, float>), dim3((unsigned)blocks), dim3(256), 0,
+                         s, (const bf16_t*)dy, ld_dy, (const bf16_t*)y, ld_y, M, N, act, act_beta,
+                         (float*)out, ld_out, colsum, rpb);
+    NA_CHECK_LAUNCH();
+    return NA_OK;
+  }
+  dim3 grid((unsigned)ceil_div(M, 256), (unsigned)ceil_div(N, 64));
 #define NA_ACTG(TD, TY, TO)                                                                   \
   hipLaunchKernelGGL((act_grad_kernel<TD, TY, TO>), grid, dim3(256), 0, s, (const TD*)dy, ld_dy, \
                      (const TY*)y, ld_y, M, N, act, act_beta, (TO*)out, ld_out, colsum)
-  const bool d16 = dy_dtype == NA_BF16, y16 = y_dtype == NA_BF16, o16 = out_dtype == NA_BF16;
   if (!d16 && !y16 && !o16) NA_ACTG(float, float, float);
   else if (!d16 && !y16 && o16) NA_ACTG(float, float, bf16_t);
   else if (!d16 && y16 && !o16) NA_ACTG(float, bf16_t, float);

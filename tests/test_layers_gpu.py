@@ -157,6 +157,22 @@ def test_act_grad(cuda):
     assert (cs.cpu().double() - ref.sum(0)).abs().max() < 1e-3
 
 
+@pytest.mark.parametrize("M", [3001, 128 * 900])
+def test_act_grad_bf16_rows(cuda, M):
+    """bf16 gradient and activation rows with a wider row stride (the vectorised path):
+    bf16 and fp32 output, column sums over ragged and large row counts."""
+    N = 256
+    dy = _rand((M, 280), 26).to(torch.bfloat16)
+    y = torch.nn.functional.softplus(_rand((M, 280), 27, 0.02), beta=BETA).to(torch.bfloat16)
+    ref = dy[:, :N].double() * _dact(y[:, :N].double(), NA_ACT_SOFTPLUS)
+    for odt in (torch.bfloat16, torch.float32):
+        cs = torch.zeros(N, device=cuda)
+        out = Fn.act_grad(dy.to(cuda), y.to(cuda), NA_ACT_SOFTPLUS, odt, colsum=cs, n_cols=N)
+        tol = (2 ** -7 if odt == torch.bfloat16 else 1e-6) * ref.abs() + 1e-6
+        assert ((out.cpu().double() - ref).abs() <= tol).all(), odt
+        assert (cs.cpu().double() - ref.sum(0)).abs().max() < 1e-4 * M ** 0.5 + 1e-3
+
+
 def test_gemm_rejects_misaligned(cuda):
     A = torch.zeros(10, 6, device=cuda)
     C = torch.zeros(10, 10, device=cuda)
