@@ -18,6 +18,8 @@
 // workgroups launched far apart).  Backward: one thread per (point, feature) so the F
 // atomics of one corner are adjacent lanes of one instruction and coalesce
 // into a single L2 atomic request.
+#include <cstring>
+
 #include "common.h"
 
 namespace na {
@@ -502,11 +504,17 @@ __device__ __attribute__((always_inline)) inline void taps_flush(float* gtab, co
   }
 }
 
-template <int F, typename TD>
+// P2: the volume range is a positive normal power of two, 2^(-vshift) of the
+// unit cube, and coordinates normalise by an exact exponent shift (no division
+// code in that instantiation).
+template <int F, typename TD, bool P2>
 __global__ void __launch_bounds__(kBlock)
 hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
-                         float vrange, float eps, const TD* __restrict__ dyc,
+                         float vrange, int vshift, float eps, const TD* __restrict__ dyc,
                          const TD* __restrict__ dyt, int64_t ld, float* __restrict__ grad) {
+  auto unit = [&](float v) -> float {
+    return P2 ? ldexpf(v - vmin, vshift) : normalize_coord(v, vmin, vrange);
+  };
   const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int l = blockIdx.y;
   const int64_t i0 = (tid / F) * kTapRun;
@@ -533,12 +541,12 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
     for (int k = 0; k < 3; ++k) {
       const float xc = x[i * 3 + k];
       uint32_t c;
-      pos_fract(normalize_coord(xc, vmin, vrange), p.scale, c, fr[0][k]);
+      pos_fract(unit(xc), p.scale, c, fr[0][k]);
       cell[0][k] = (int)c;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const float xt = xc + kTapDir[t][k] * eps;  // the tap point, as the forward built it
-        pos_fract(normalize_coord(xt, vmin, vrange), p.scale, c, fr[1 + t][k]);
+        pos_fract(unit(xt), p.scale, c, fr[1 + t][k]);
         cell[1 + t][k] = (int)c;
       }
     }
@@ -721,8 +729,15 @@ static int launch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, flo
                            float vrange, float eps, const void* dyc, const void* dyt, int64_t ld,
                            float* grad, int active, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(ceil_div(n, (int64_t)kTapRun) * F, kBlock), active);
-  hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD>), grid, dim3(kBlock), 0, s, *d, x, n, vmin,
-                     vrange, eps, (const TD*)dyc, (const TD*)dyt, ld, grad);
+  uint32_t vb;
+  memcpy(&vb, &vrange, 4);
+  if ((vb & 0x807fffffu) == 0u && (vb >> 23) != 0u)  // positive normal power of two
+    hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, true>), grid, dim3(kBlock), 0, s, *d, x, n,
+                       vmin, vrange, 127 - (int)(vb >> 23), eps, (const TD*)dyc, (const TD*)dyt,
+                       ld, grad);
+  else
+    hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, false>), grid, dim3(kBlock), 0, s, *d, x, n,
+                       vmin, vrange, 0, eps, (const TD*)dyc, (const TD*)dyt, ld, grad);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
