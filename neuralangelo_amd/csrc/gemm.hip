@@ -596,7 +596,52 @@ narrow_fwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
 // Workgroup size of the narrow backward (waves).
 template <int NM> struct NarrowBwdCfg { static constexpr int WPG = 8; };
 
-template <typename TX, typename TDX, int KC, int NM>
+// Eight consecutive elements of a row held as raw 16-byte chunks (one for
+// bf16, two for fp32) until they are used.
+// The rows stream through once, so loads and stores carry the non-temporal hint.
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
+template <typename T> struct Chunk8 { uint4 q[sizeof(T) / 2]; };
+template <typename T> __device__ inline void chunk_load(const T* p, Chunk8<T>& c) {
+  const u32x4_t* s = reinterpret_cast<const u32x4_t*>(p);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 2); ++i) {
+    const u32x4_t r = __builtin_nontemporal_load(s + i);
+    c.q[i] = make_uint4(r.x, r.y, r.z, r.w);
+  }
+}
+__device__ inline void store8v_nt(float* p, const float (&v)[8]) {
+  typedef __attribute__((ext_vector_type(4))) float f4_t;
+  const f4_t a = {v[0], v[1], v[2], v[3]}, b = {v[4], v[5], v[6], v[7]};
+  __builtin_nontemporal_store(a, reinterpret_cast<f4_t*>(p));
+  __builtin_nontemporal_store(b, reinterpret_cast<f4_t*>(p + 4));
+}
+__device__ inline void store8v_nt(bf16_t* p, const float (&v)[8]) {
+  typedef __attribute__((ext_vector_type(8))) float f8_t;
+  f8_t x;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) x[c] = v[c];
+  const bf16x8_t b = __builtin_convertvector(x, bf16x8_t);
+  __builtin_nontemporal_store(__builtin_bit_cast(u32x4_t, b), reinterpret_cast<u32x4_t*>(p));
+}
+__device__ inline void chunk_f32(const Chunk8<bf16_t>& c, float v[8]) {
+  const uint32_t w[4] = {c.q[0].x, c.q[0].y, c.q[0].z, c.q[0].w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[2 * i] = bf16_to_f32((bf16_t)(w[i] & 0xffffu));
+    v[2 * i + 1] = bf16_to_f32((bf16_t)(w[i] >> 16));
+  }
+}
+__device__ inline void chunk_f32(const Chunk8<float>& c, float v[8]) {
+  const uint32_t w[8] = {c.q[0].x, c.q[0].y, c.q[0].z, c.q[0].w,
+                         c.q[1].x, c.q[1].y, c.q[1].z, c.q[1].w};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = __uint_as_float(w[i]);
+}
+
+// X_ONLY: the only row operand is X (no dX accumulation, and the activation
+// derivative, if any, is taken from X itself), so a batch holds just the raw
+// X chunks and more rows fit in flight.
+template <typename TX, typename TDX, int KC, int NM, bool X_ONLY>
 __global__ void __launch_bounds__(64 * NarrowBwdCfg<NM>::WPG)
 narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
                       const float* __restrict__ W, int N, const float* __restrict__ dpre,
@@ -625,10 +670,62 @@ narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
   // Rows go in batches of NB per half-wave: all loads of a batch are issued
   // before its arithmetic.  When Y is X (the layer's own stored activation) the
   // derivative reads the loaded X values.
-  constexpr int NB = (KC > 1 && NM > 1) ? 1 : 2;  // keeps the widest variant within registers
+  // (rows per batch: what the registers of each variant hold)
+  constexpr int NB = X_ONLY ? (NM == 1 ? 4 : (KC == 1 ? 2 : 1))
+                            : ((KC > 1 && NM > 1) ? 1 : 2);
   const int64_t stride = 2 * nwaves;
   const bool y_is_x = dact && (const void*)X == Y && ldy == ldx &&
                       y_dtype == (sizeof(TX) == 2 ? NA_BF16 : NA_F32);
+  if constexpr (X_ONLY) {
+    for (int64_t mb = 2 * wave + half; mb < M; mb += stride * NB) {
+      float d[NB][NM];
+      Chunk8<TX> xr[NB][KC];
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int64_t m = mb + r * stride;
+        const bool ok = m < M;
+#pragma unroll
+        for (int n = 0; n < NM; ++n) d[r][n] = (ok && n < N) ? dpre[m * N + n] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int64_t k = 8 * hl + 256 * c;
+          if (ok && k < K) chunk_load(X + m * ldx + k, xr[r][c]);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int64_t m = mb + r * stride;
+        if (m >= M) continue;
+#pragma unroll
+        for (int n = 0; n < NM; ++n) gb[n] += d[r][n];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int64_t k = 8 * hl + 256 * c;
+          if (k >= K) continue;
+          float xv[8], dx[8];
+          chunk_f32(xr[r][c], xv);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            dx[e] = 0.0f;
+#pragma unroll
+            for (int n = 0; n < NM; ++n) {
+              gw[n][c][e] = fmaf(d[r][n], xv[e], gw[n][c][e]);
+              dx[e] = fmaf(d[r][n], w[n][c][e], dx[e]);
+            }
+          }
+          if (dX) {
+            if (dact) {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) dx[e] *= act_deriv_from_y(dact, xv[e], beta);
+            }
+            store8v_nt(dX + m * lddx + k, dx);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) cs[c][e] += dx[e];
+          }
+        }
+      }
+    }
+  } else {
   for (int64_t mb = 2 * wave + half; mb < M; mb += stride * NB) {
     float d[NB][NM], xv[NB][KC][8], yv[NB][KC][8], ov[NB][KC][8];
 #pragma unroll
@@ -685,6 +782,7 @@ narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
         }
       }
     }
+  }
   }
   // combine the two half-waves (same k ranges), then the waves of the
   // workgroup through LDS; one atomic per element leaves the workgroup
@@ -2373,16 +2471,21 @@ extern "C" int na_linear_narrow_bwd(const void* X, int64_t ldx, int32_t x_dtype,
   if (waves > 4096) waves = 4096;
   dim3 grid8((unsigned)ceil_div(waves, 8));
   hipStream_t s = (hipStream_t)stream;
+  // X is the only row operand: no dX accumulation, and the derivative (if
+  // any) is read from X itself
+  const bool x_only = !add_dx && (!dX || !dact || (Y == X && ldy == ldx && y_dtype == x_dtype));
+#define NA_NBWD_L(TX, TDX, KC, NM, XO)                                                         \
+  hipLaunchKernelGGL((narrow_bwd_vec_kernel<TX, TDX, KC, NM, XO>), grid8, dim3(512), 0, s,     \
+                     (const TX*)X, ldx, M, K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, Y,   \
+                     ldy, y_dtype, dact, act_beta, dx_colsum)
 #define NA_NBWD(TX, TDX, KC)                                                                   \
   do {                                                                                         \
-    if (N == 1)                                                                                \
-      hipLaunchKernelGGL((narrow_bwd_vec_kernel<TX, TDX, KC, 1>), grid8, dim3(512), 0, s,      \
-                         (const TX*)X, ldx, M, K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, Y, \
-                         ldy, y_dtype, dact, act_beta, dx_colsum);                              \
-    else                                                                                       \
-      hipLaunchKernelGGL((narrow_bwd_vec_kernel<TX, TDX, KC, kNarrowMax>), grid8, dim3(512), 0,\
-                         s, (const TX*)X, ldx, M, K, W, N, dpre, dW, db, (TDX*)dX, lddx, add_dx, \
-                         Y, ldy, y_dtype, dact, act_beta, dx_colsum);                           \
+    if (N == 1) {                                                                              \
+      if (x_only) NA_NBWD_L(TX, TDX, KC, 1, true); else NA_NBWD_L(TX, TDX, KC, 1, false);      \
+    } else {                                                                                   \
+      if (x_only) NA_NBWD_L(TX, TDX, KC, kNarrowMax, true);                                    \
+      else NA_NBWD_L(TX, TDX, KC, kNarrowMax, false);                                          \
+    }                                                                                          \
   } while (0)
   const bool x16 = x_dtype == NA_BF16, d16 = dx_dtype == NA_BF16;
   if (K <= 256) {
@@ -2397,6 +2500,7 @@ extern "C" int na_linear_narrow_bwd(const void* X, int64_t ldx, int32_t x_dtype,
     else NA_NBWD(float, float, 2);
   }
 #undef NA_NBWD
+#undef NA_NBWD_L
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -127,6 +127,56 @@ def test_narrow_forward_backward(cuda):
     assert (cs.cpu().double() - refdX.sum(0)).abs().max() < 1e-3
 
 
+@pytest.mark.parametrize("M,K,N,dt", [(3001, 256, 1, torch.bfloat16),
+                                      (128 * 900, 256, 1, torch.bfloat16),
+                                      (2999, 384, 1, torch.bfloat16),
+                                      (4097, 256, 3, torch.bfloat16),
+                                      (1000, 136, 4, torch.float32)])
+def test_narrow_backward_x_only(cuda, M, K, N, dt):
+    """Rows whose only operand is X (softplus derivative read from X itself, no dX
+    accumulation; the SDF head backward): against the general path, taken when Y is
+    a separate copy of X, and float64.  Rows visit the same lanes in the same order
+    on both paths, so dX agrees bitwise; the dW-only call (dX null) is covered too."""
+    X = torch.nn.functional.softplus(_rand((M, K), 30, 0.02), beta=BETA).to(dt)
+    W = _rand((N, K), 31, 0.1)
+    dpre = _rand((M, N), 32)
+    Xd, Wd, dpre_d = X.to(cuda), W.to(cuda), dpre.to(cuda)
+    Ycopy = Xd.clone()
+    code = _lib.dtype_code(dt)
+    outs = []
+    for Y in (Xd, Ycopy):
+        dW = torch.zeros(N, K, device=cuda)
+        db = torch.zeros(N, device=cuda)
+        cs = torch.zeros(K, device=cuda)
+        dX = torch.full((M, K), float("nan"), dtype=dt, device=cuda)
+        _lib.call("na_linear_narrow_bwd", _lib.ptr(Xd), K, code, M, K, _lib.ptr(Wd), N,
+                  _lib.ptr(dpre_d), _lib.ptr(dW), _lib.ptr(db), _lib.ptr(dX), K, code, 0,
+                  _lib.ptr(Y), K, code, NA_ACT_SOFTPLUS, BETA, _lib.ptr(cs), _lib.stream_of(dW))
+        outs.append((dW.cpu(), db.cpu(), dX.cpu(), cs.cpu()))
+    (dW, db, dX, cs), (dW_g, db_g, dX_g, cs_g) = outs
+    assert torch.equal(dX.view(torch.int16 if dt == torch.bfloat16 else torch.int32),
+                       dX_g.view(torch.int16 if dt == torch.bfloat16 else torch.int32))
+    Xr = X.double()
+    refW = dpre.double().t() @ Xr
+    refdX = (dpre.double() @ W.double()) * _dact(Xr, NA_ACT_SOFTPLUS)
+    tol_w = 1e-5 * M ** 0.5 + 1e-4
+    assert (dW.double() - refW).abs().max() < tol_w
+    assert (dW.double() - dW_g.double()).abs().max() < tol_w
+    assert (db.double() - dpre.double().sum(0)).abs().max() < tol_w
+    tol_x = (2 ** -7 if dt == torch.bfloat16 else 1e-6) * refdX.abs() + 1e-6
+    assert ((dX.double() - refdX).abs() <= tol_x).all()
+    assert (cs.double() - refdX.sum(0)).abs().max() < tol_w
+    assert (cs.double() - cs_g.double()).abs().max() < tol_w
+    # weight gradients only (no input gradient, no derivative)
+    dW2 = torch.zeros(N, K, device=cuda)
+    db2 = torch.zeros(N, device=cuda)
+    _lib.call("na_linear_narrow_bwd", _lib.ptr(Xd), K, code, M, K, _lib.ptr(Wd), N,
+              _lib.ptr(dpre_d), _lib.ptr(dW2), _lib.ptr(db2), None, K, code, 0,
+              None, K, code, 0, BETA, None, _lib.stream_of(dW2))
+    assert (dW2.cpu().double() - refW).abs().max() < tol_w
+    assert (db2.cpu().double() - dpre.double().sum(0)).abs().max() < tol_w
+
+
 def test_weight_norm_matches_torch(cuda):
     v = _rand((256, 131), 18)
     g = _rand((256, 1), 19).abs() + 0.5
