@@ -531,71 +531,6 @@ __device__ inline void load8(const float* __restrict__ p, float v[8]) {
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 
-// Row-parallel narrow layer: a half-wave (32 lanes) owns one row; lane l covers
-// k = 8*l + 256*c (c < KC) with 16/32-byte vector loads; each wave keeps two
-// rows per iteration and unrolls two iterations so four rows are in flight.
-template <typename TX, int KC>
-__global__ void __launch_bounds__(256)
-narrow_fwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
-                      const float* __restrict__ W, const float* __restrict__ b, int N, int act,
-                      float* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const int hl = lane & 31, half = lane >> 5;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  float w[kNarrowMax][KC][8];
-#pragma unroll
-  for (int n = 0; n < kNarrowMax; ++n)
-#pragma unroll
-    for (int c = 0; c < KC; ++c)
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int64_t k = 8 * hl + 256 * c + e;
-        w[n][c][e] = (n < N && k < K) ? W[n * K + k] : 0.0f;
-      }
-  const int64_t pairs = (M + 1) / 2;
-  for (int64_t pr = wave; pr < pairs; pr += 2 * nwaves) {
-    float s[2][kNarrowMax];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int64_t m = 2 * (pr + u * nwaves) + half;
-#pragma unroll
-      for (int n = 0; n < kNarrowMax; ++n) s[u][n] = 0.0f;
-      if (m < M) {
-#pragma unroll
-        for (int c = 0; c < KC; ++c) {
-          const int64_t k = 8 * hl + 256 * c;
-          if (k < K) {
-            float x[8];
-            load8(X + m * ldx + k, x);
-#pragma unroll
-            for (int n = 0; n < kNarrowMax; ++n)
-#pragma unroll
-              for (int e = 0; e < 8; ++e) s[u][n] = fmaf(x[e], w[n][c][e], s[u][n]);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-      for (int n = 0; n < kNarrowMax; ++n)
-#pragma unroll
-        for (int o = 16; o > 0; o >>= 1) s[u][n] += __shfl_xor(s[u][n], o);
-    if (hl == 0) {
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int64_t m = 2 * (pr + u * nwaves) + half;
-        if (m < M)
-          for (int n = 0; n < N; ++n) out[m * N + n] = act_forward(act, s[u][n] + b[n], 1.0f);
-      }
-    }
-  }
-}
-
-// Workgroup size of the narrow backward (waves).
-template <int NM> struct NarrowBwdCfg { static constexpr int WPG = 8; };
-
 // Eight consecutive elements of a row held as raw 16-byte chunks (one for
 // bf16, two for fp32) until they are used.
 // The rows stream through once, so loads and stores carry the non-temporal hint.
@@ -623,6 +558,85 @@ __device__ inline void store8v_nt(bf16_t* p, const float (&v)[8]) {
   const bf16x8_t b = __builtin_convertvector(x, bf16x8_t);
   __builtin_nontemporal_store(__builtin_bit_cast(u32x4_t, b), reinterpret_cast<u32x4_t*>(p));
 }
+
+// Row-parallel narrow layer: a half-wave (32 lanes) owns one row; lane l covers
+// k = 8*l + 256*c (c < KC) with 16/32-byte vector loads; each wave takes two
+// rows per step and issues the loads of U steps before their arithmetic.
+template <typename TX, int KC>
+__global__ void __launch_bounds__(256)
+narrow_fwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t K,
+                      const float* __restrict__ W, const float* __restrict__ b, int N, int act,
+                      float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int hl = lane & 31, half = lane >> 5;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float w[kNarrowMax][KC][8];
+#pragma unroll
+  for (int n = 0; n < kNarrowMax; ++n)
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t k = 8 * hl + 256 * c + e;
+        w[n][c][e] = (n < N && k < K) ? W[n * K + k] : 0.0f;
+      }
+  constexpr int U = KC == 1 ? 4 : 2;
+  const int64_t pairs = (M + 1) / 2;
+  for (int64_t pr = wave; pr < pairs; pr += U * nwaves) {
+    Chunk8<TX> xr[U][KC];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t m = 2 * (pr + u * nwaves) + half;
+      if (m < M) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int64_t k = 8 * hl + 256 * c;
+          if (k < K) chunk_load(X + m * ldx + k, xr[u][c]);
+        }
+      }
+    }
+    float s[U][kNarrowMax];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t m = 2 * (pr + u * nwaves) + half;
+#pragma unroll
+      for (int n = 0; n < kNarrowMax; ++n) s[u][n] = 0.0f;
+      if (m < M) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int64_t k = 8 * hl + 256 * c;
+          if (k < K) {
+            float x[8];
+            chunk_f32(xr[u][c], x);
+#pragma unroll
+            for (int n = 0; n < kNarrowMax; ++n)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) s[u][n] = fmaf(x[e], w[n][c][e], s[u][n]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int n = 0; n < kNarrowMax; ++n)
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) s[u][n] += __shfl_xor(s[u][n], o);
+    if (hl == 0) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t m = 2 * (pr + u * nwaves) + half;
+        if (m < M)
+          for (int n = 0; n < N; ++n) out[m * N + n] = act_forward(act, s[u][n] + b[n], 1.0f);
+      }
+    }
+  }
+}
+
+// Workgroup size of the narrow backward (waves).
+template <int NM> struct NarrowBwdCfg { static constexpr int WPG = 8; };
+
 __device__ inline void chunk_f32(const Chunk8<bf16_t>& c, float v[8]) {
   const uint32_t w[4] = {c.q[0].x, c.q[0].y, c.q[0].z, c.q[0].w};
 #pragma unroll
@@ -2319,16 +2333,34 @@ act_grad_vec_kernel(const TD* __restrict__ dy, int64_t ld_dy, const TY* __restri
   float cs[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) cs[k] = 0.0f;
-  for (int64_t m = m0 + r; m < m1; m += lanes) {
-    float g[8], v[8];
-    vload8(dy + m * ld_dy + 8 * c, g);
-    vload8(y + m * ld_y + 8 * c, v);
+  // four rows per thread in flight (raw 16-byte chunks, non-temporal: every
+  // row streams through once); each thread still visits its rows in order
+  constexpr int U = 4;
+  for (int64_t mb = m0 + r; mb < m1; mb += U * lanes) {
+    Chunk8<TD> gr[U];
+    Chunk8<TY> vr[U];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      g[k] *= act_deriv_from_y(act, v[k], beta);
-      cs[k] += g[k];
+    for (int u = 0; u < U; ++u) {
+      const int64_t m = mb + u * lanes;
+      if (m < m1) {
+        chunk_load(dy + m * ld_dy + 8 * c, gr[u]);
+        chunk_load(y + m * ld_y + 8 * c, vr[u]);
+      }
     }
-    vstore8(out + m * ld_out + 8 * c, g);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t m = mb + u * lanes;
+      if (m >= m1) continue;
+      float g[8], v[8];
+      chunk_f32(gr[u], g);
+      chunk_f32(vr[u], v);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        g[k] *= act_deriv_from_y(act, v[k], beta);
+        cs[k] += g[k];
+      }
+      store8v_nt(out + m * ld_out + 8 * c, g);
+    }
   }
   if (colsum) {
 #pragma unroll
