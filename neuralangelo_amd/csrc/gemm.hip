@@ -2026,6 +2026,10 @@ __device__ inline bf16x8_t tr_pair(unsigned char* p0, unsigned char* p1) {
   return __builtin_bit_cast(bf16x8_t, v);
 }
 
+// Cache policy of a buffer-to-LDS load whose rows are read once (the
+// non-temporal bit of the instruction's cache-policy field).
+constexpr int kStreamAux = 2;
+
 // One 32-row slice of A and B into a stage; rows from `me` on read zeros.
 template <int KB>
 __device__ inline void wgrad_issue(const bf16_t* A, int64_t lda, const bf16_t* B, int64_t ldb,
@@ -2042,7 +2046,7 @@ __device__ inline void wgrad_issue(const bf16_t* A, int64_t lda, const bf16_t* B
   for (int u = 0; u < WT::LOADS; ++u)
     __builtin_amdgcn_raw_ptr_buffer_load_lds(isa[u] ? ra : rb,
                                              (lds_void_t*)(st + (wave * WT::LOADS + u) * 1024), 16,
-                                             voff[u], 0, 0, 0);
+                                             voff[u], 0, 0, kStreamAux);
 }
 
 template <int KB>
