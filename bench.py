@@ -120,15 +120,16 @@ def measured_traffic():
     return None
 
 
-def taps_bwd_roofline(model, n_points, launch_ms, n_launches):
+def taps_bwd_roofline(model, n_points, launch_ms, n_launches, dy_bytes):
     """Roofline of the dominant kernel of the step: the hash-grid backward over the
     centre point and its four taps (one launch per step, timed with HIP events on
     torch's stream inside the timed steps).  Algorithmic bytes per point-eval of the
-    encoder backward: L*F*4 (dL/dy) + 12 (point) + L*8*F*4 (corner gradient RMW,
-    counted once); 5 evals (centre + 4 taps) per point."""
+    encoder backward: L*F*dy_bytes (dL/dy rows in the compute dtype) + 12 (point)
+    + L*8*F*4 (fp32 corner gradient RMW, counted once); 5 evals (centre + 4 taps)
+    per point."""
     enc = model.neural_sdf.tcnn_encoding
     L, F = enc.n_levels, enc.n_features_per_level
-    per_eval = L * F * 4 + 12 + L * 8 * F * 4
+    per_eval = L * F * dy_bytes + 12 + L * 8 * F * 4
     per_point = 5 * per_eval
     achieved = per_point * n_points / (launch_ms * 1e-3) / 1e9
     traffic = measured_traffic()
@@ -329,7 +330,7 @@ def main():
         del src, dst
     model, dt_bf16, loss_bf16, taps_ms, n_taps = time_train(torch.bfloat16, dev, args.rays,
                                                            args.steps, args.warmup)
-    roof = taps_bwd_roofline(model, args.rays * 96, taps_ms, n_taps)
+    roof = taps_bwd_roofline(model, args.rays * 96, taps_ms, n_taps, dy_bytes=2)
     hash_fwd = encoder_roofline(model, dev)
     del model
     torch.cuda.empty_cache()

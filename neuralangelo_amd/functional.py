@@ -347,9 +347,10 @@ def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc):
 
 
 def _encoder_grad(dpre0, w):
-    """dX (encoder columns) = dpre0 . W0p[:, :n_enc]  ->  [P, n_enc] fp32."""
+    """dX (encoder columns) = dpre0 . W0p[:, :n_enc]  ->  [P, n_enc] in the compute dtype
+    (the hash-grid scatter reads bf16 or fp32 rows and accumulates the table in fp32)."""
     P = dpre0.shape[0]
-    dX = torch.empty(P, w.n_enc, dtype=torch.float32, device=dpre0.device)
+    dX = torch.empty(P, w.n_enc, dtype=w.cdt, device=dpre0.device)
     gemm(dpre0, w.hidden, 0, _t(w.W0p), w.hidden, 0, P, w.n_enc, w.hidden, dX, w.n_enc)
     return dX
 
