@@ -156,9 +156,20 @@ typedef struct na_gemm_args {
   int64_t zc_rows;
   const float* head_w;
   float* head_out;
+  /* optional workspace of the weight-gradient path (bf16, trans_a = trans_b = 1,
+   * atomic epilogue, no epilogue terms): each row split stores its partial
+   * product here and one more launch adds the splits into C in split order
+   * (no atomics; the result does not depend on scheduling).  wg_partials_len
+   * floats, at least na_gemm_wgrad_workspace(args); NULL or shorter = atomics. */
+  float* wg_partials;
+  int64_t wg_partials_len;
 } na_gemm_args;
 
 int na_gemm(const na_gemm_args* args, na_stream_t stream);
+
+/* Floats of wg_partials that na_gemm's weight-gradient path uses for these
+ * arguments (0 when another path runs them; -2 on bad arguments). */
+int64_t na_gemm_wgrad_workspace(const na_gemm_args* args);
 
 /* out[n] += sum_m X[m*ld + n] (fp32 accumulation, X of x_dtype): the bias gradient
  * of a linear layer from its pre-activation gradient (torch.nn.Linear backward,

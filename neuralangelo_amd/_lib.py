@@ -60,6 +60,7 @@ class GemmArgs(ctypes.Structure):
         ("z_out", ctypes.c_void_p), ("ld_z", ctypes.c_int64),
         ("zc", ctypes.c_void_p), ("ld_zc", ctypes.c_int64), ("zc_rows", ctypes.c_int64),
         ("head_w", ctypes.c_void_p), ("head_out", ctypes.c_void_p),
+        ("wg_partials", ctypes.c_void_p), ("wg_partials_len", ctypes.c_int64),
     ]
 
 
@@ -81,6 +82,7 @@ _SIGNATURES = {
     "na_hashgrid_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _I32, _P]),
     "na_hashgrid_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _P]),
     "na_gemm": (ctypes.c_int, [_PGEMM, _P]),
+    "na_gemm_wgrad_workspace": (ctypes.c_int64, [_PGEMM]),
     "na_colsum": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P]),
     "na_head_sum": (ctypes.c_int, [_P, _I32, _I64, _P, _P, _P]),
     "na_linear_narrow_fwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P, _I32, _I32, _P, _P]),

@@ -2053,7 +2053,7 @@ template <int KB>
 __global__ void __launch_bounds__(512)
 gemm_bf16_wgrad_kernel(const bf16_t* __restrict__ A, int64_t lda, const bf16_t* __restrict__ B,
                        int64_t ldb, int64_t M, int64_t N, int64_t K, int64_t rows_per,
-                       float* __restrict__ C, int64_t ldc) {
+                       float* __restrict__ C, int64_t ldc, float* __restrict__ part) {
   using WT = WgTile<KB>;
   constexpr int NS = WT::NS, PB = WT::PB;
   __shared__ __attribute__((aligned(16))) unsigned char smem[NS * WT::STAGE_BYTES];
@@ -2134,8 +2134,25 @@ gemm_bf16_wgrad_kernel(const bf16_t* __restrict__ A, int64_t lda, const bf16_t* 
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  if (nsl == 0) return;
   // acc[i][j][r]: n = n0 + wn*64 + i*32 + 8*(r>>2) + 4*(lane>>5) + (r&3), k = k0 + wk*32*KB + j*32 + (lane&31)
+  if (part) {
+    // this split's partial product, [split][n][k] (zeros for a split without rows)
+    float* pp = part + (int64_t)blockIdx.x * N * K;
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const int64_t k = k0 + wk * 32 * KB + j * 32 + (lane & 31);
+      if (k >= K) continue;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t n = n0 + wn * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          if (n < N) __builtin_nontemporal_store(acc[i][j][r], pp + n * K + k);
+        }
+    }
+    return;
+  }
+  if (nsl == 0) return;
 #pragma unroll
   for (int j = 0; j < KB; ++j) {
     const int64_t k = k0 + wk * 32 * KB + j * 32 + (lane & 31);
@@ -2147,6 +2164,38 @@ gemm_bf16_wgrad_kernel(const bf16_t* __restrict__ A, int64_t lda, const bf16_t* 
         const int64_t n = n0 + wn * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         if (n < N) atomic_add_f32(C + n * ldc + k, acc[i][j][r]);
       }
+  }
+}
+
+// C[n, k] += sum over splits s (four interleaved chains, each in split order) of
+// part[s][n][k].  A workgroup of 4 waves takes 64 consecutive outputs; wave w
+// sums splits w, w + 4, ...; the four sums combine as (0 + 1) + (2 + 3).
+__global__ void __launch_bounds__(256)
+wgrad_reduce_kernel(const float* __restrict__ part, int64_t splits, int64_t N, int64_t K,
+                    float* __restrict__ C, int64_t ldc) {
+  __shared__ float red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t total = N * K;
+  const int64_t o = (int64_t)blockIdx.x * 64 + lane;
+  float s = 0.0f;
+  if (o < total) {
+    const float* p = part + o;
+    int64_t sp = w;
+    for (; sp + 28 < splits; sp += 32) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(p + (sp + 4 * u) * total);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; sp < splits; sp += 4) s += __builtin_nontemporal_load(p + sp * total);
+  }
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && o < total) {
+    const float v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    const int64_t n = o / K, k = o - n * K;
+    C[n * ldc + k] += v;
   }
 }
 
@@ -2267,35 +2316,56 @@ static bool wgrad_ok(const na_gemm_args* a) {
          a->ldb * 2 * 32 < 0x7fffffffLL;
 }
 
-static int launch_wgrad(const na_gemm_args* a, hipStream_t s) {
+// Launch plan of the weight-gradient kernel: output [a->M, a->N] = sum over
+// a->K rows, split into `splits` row ranges of rows_per rows (kernel names:
+// n = output row, k = output column).
+struct WgradPlan { int64_t No, Ko, Mr, splits, rows_per; int kb; };
+
+static WgradPlan wgrad_plan(const na_gemm_args* a) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
-  // output [a->M, a->N] = sum over a->K rows; kernel names: n = output row, k = output column
-  const int64_t No = a->M, Ko = a->N, Mr = a->K;
-  const int kb = Ko <= 128 ? 2 : (Ko <= 192 ? 3 : (Ko <= 256 ? 4 : 5));
-  const int64_t bk = 64 * kb;
-  const int64_t tiles = ceil_div(No, (int64_t)256) * ceil_div(Ko, bk);
+  WgradPlan w;
+  w.No = a->M; w.Ko = a->N; w.Mr = a->K;
+  w.kb = w.Ko <= 128 ? 2 : (w.Ko <= 192 ? 3 : (w.Ko <= 256 ? 4 : 5));
+  const int64_t tiles = ceil_div(w.No, (int64_t)256) * ceil_div(w.Ko, (int64_t)(64 * w.kb));
   int64_t splits = cus / tiles;
   if (splits < 1) splits = 1;
-  int64_t rows_per = ceil_div(ceil_div(Mr, splits), (int64_t)32) * 32;
+  int64_t rows_per = ceil_div(ceil_div(w.Mr, splits), (int64_t)32) * 32;
   if (rows_per < 128) rows_per = 128;
-  splits = ceil_div(Mr, rows_per);
-  NA_CHECK_ARG(splits < 0x7fffffffLL && ceil_div(No, (int64_t)256) < 65536 && ceil_div(Ko, bk) < 65536,
+  w.rows_per = rows_per;
+  w.splits = ceil_div(w.Mr, rows_per);
+  return w;
+}
+
+static int launch_wgrad(const na_gemm_args* a, hipStream_t s) {
+  const WgradPlan w = wgrad_plan(a);
+  const int64_t bk = 64 * w.kb;
+  NA_CHECK_ARG(w.splits < 0x7fffffffLL && ceil_div(w.No, (int64_t)256) < 65536 &&
+                   ceil_div(w.Ko, bk) < 65536,
                "weight-gradient grid too large");
-  const dim3 grid((unsigned)splits, (unsigned)ceil_div(No, (int64_t)256), (unsigned)ceil_div(Ko, bk));
+  const bool use_part = a->wg_partials && a->wg_partials_len >= w.splits * w.No * w.Ko;
+  float* part = use_part ? a->wg_partials : nullptr;
+  const dim3 grid((unsigned)w.splits, (unsigned)ceil_div(w.No, (int64_t)256), (unsigned)ceil_div(w.Ko, bk));
   const bf16_t* A = (const bf16_t*)a->A;
   const bf16_t* B = (const bf16_t*)a->B;
   float* C = (float*)a->C;
 #define NA_WG(KB) hipLaunchKernelGGL((gemm_bf16_wgrad_kernel<KB>), grid, dim3(512), 0, s, A, a->lda, B, \
-                                     a->ldb, Mr, No, Ko, rows_per, C, a->ldc)
-  if (kb == 2) NA_WG(2);
-  else if (kb == 3) NA_WG(3);
-  else if (kb == 4) NA_WG(4);
+                                     a->ldb, w.Mr, w.No, w.Ko, w.rows_per, C, a->ldc, part)
+  if (w.kb == 2) NA_WG(2);
+  else if (w.kb == 3) NA_WG(3);
+  else if (w.kb == 4) NA_WG(4);
   else NA_WG(5);
 #undef NA_WG
   NA_CHECK_LAUNCH();
+  if (use_part) {
+    const int64_t blocks = ceil_div(w.No * w.Ko, (int64_t)64);
+    NA_CHECK_ARG(blocks < 0x7fffffffLL, "weight-gradient output too large");
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, part, w.splits,
+                       w.No, w.Ko, C, a->ldc);
+    NA_CHECK_LAUNCH();
+  }
   return NA_OK;
 }
 
@@ -2384,6 +2454,13 @@ act_grad_vec_kernel(const TD* __restrict__ dy, int64_t ld_dy, const TY* __restri
 }  // namespace na
 
 using namespace na;
+
+extern "C" int64_t na_gemm_wgrad_workspace(const na_gemm_args* a) {
+  NA_CHECK_ARG(a != nullptr, "args is null");
+  if (a->M <= 0 || a->N <= 0 || a->K <= 0 || !wgrad_ok(a)) return 0;
+  const WgradPlan w = wgrad_plan(a);
+  return w.splits * w.No * w.Ko;
+}
 
 extern "C" int na_gemm(const na_gemm_args* a, na_stream_t stream) {
   NA_CHECK_ARG(a != nullptr, "args is null");

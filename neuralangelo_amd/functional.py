@@ -68,8 +68,10 @@ def _s(t):
 def gemm(A, lda, trans_a, B, ldb, trans_b, M, N, K, C, ldc, epilogue=NA_EPI_STORE, bias=None,
          act=NA_ACT_NONE, beta=SOFTPLUS_BETA, dact=NA_ACT_NONE, dact_y=None, ld_dact=0,
          colsum=None, split_k=1, r1_a=None, r1_b=None, z_out=None, zc=None, zc_rows=0,
-         head_w=None, head_out=None):
-    """C[m,n] = epi(sum_k A(m,k) B(n,k)) via na_gemm (see include/neuralangelo_amd.h)."""
+         head_w=None, head_out=None, wgrad_partials=True):
+    """C[m,n] = epi(sum_k A(m,k) B(n,k)) via na_gemm (see include/neuralangelo_amd.h).
+    When the weight-gradient path runs the call, wgrad_partials gives it a workspace
+    for per-split partial products (split-ordered sums instead of atomics)."""
     if A.dtype != B.dtype:
         raise RuntimeError(f"gemm operands must share a dtype ({A.dtype} vs {B.dtype})")
     a = _lib.GemmArgs()
@@ -94,6 +96,12 @@ def gemm(A, lda, trans_a, B, ldb, trans_b, M, N, K, C, ldc, epilogue=NA_EPI_STOR
         a.zc, a.ld_zc, a.zc_rows = zc.data_ptr(), int(zc.stride(0)), int(zc_rows)
     if head_w is not None:
         a.head_w, a.head_out = head_w.data_ptr(), head_out.data_ptr()
+    ws = None
+    if wgrad_partials and epilogue == NA_EPI_ATOMIC:
+        n = _lib.call("na_gemm_wgrad_workspace", ctypes.byref(a))
+        if n > 0:
+            ws = torch.empty(n, dtype=torch.float32, device=C.device)
+            a.wg_partials, a.wg_partials_len = ws.data_ptr(), n
     _lib.call("na_gemm", ctypes.byref(a), _s(A))
 
 

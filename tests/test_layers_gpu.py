@@ -1,4 +1,6 @@
 """Dense-layer kernels (GEMM + epilogues, narrow layers, weight norm) vs float64 torch on the CPU."""
+import ctypes
+
 import pytest
 import torch
 
@@ -96,6 +98,40 @@ def test_weight_grad_split_k(cuda, dt):
     Fn.linear_weight_grad(dpre.to(cuda), X.to(cuda), K, K, dW)
     err = (dW.cpu().double() - ref).abs().max().item()
     assert err < 1e-3 * (M ** 0.5) * 0.01 + 1e-3, err
+
+
+@pytest.mark.parametrize("M,N,K", [(20000, 256, 136), (3001, 256, 280), (100, 24, 256),
+                                   (128 * 900 + 17, 256, 256), (5000, 280, 96)])
+def test_weight_grad_split_partials(cuda, M, N, K):
+    """Weight gradient with the partial-product workspace (split-ordered sums, no
+    atomics): float64 reference, the atomic path, bitwise-equal repeats, and
+    accumulation into a non-zero dW."""
+    dpre = _rand((M, N), 40, 0.1).to(torch.bfloat16)
+    X = _rand((M, K), 41, 0.1).to(torch.bfloat16)
+    init = _rand((N, K), 42, 0.01)
+    ref = init.double() + dpre.double().t() @ X.double()
+    dd, Xd = dpre.to(cuda), X.to(cuda)
+    outs = []
+    for partials in (True, True, False):
+        dW = init.to(cuda)
+        Fn.gemm(dd, N, 1, Xd, K, 1, N, K, M, dW, K, epilogue=Fn.NA_EPI_ATOMIC,
+                wgrad_partials=partials)
+        outs.append(dW.cpu())
+    tol = 1e-5 * M ** 0.5 + 1e-5
+    assert torch.equal(outs[0], outs[1])
+    for o in outs:
+        assert (o.double() - ref).abs().max().item() < tol
+    assert _lib.call("na_gemm_wgrad_workspace", ctypes.byref(_wgrad_args(dd, Xd, M, N, K))) > 0
+
+
+def _wgrad_args(dpre, X, M, N, K):
+    a = _lib.GemmArgs()
+    a.M, a.N, a.K = N, K, M
+    a.A, a.lda, a.trans_a = dpre.data_ptr(), N, 1
+    a.B, a.ldb, a.trans_b = X.data_ptr(), K, 1
+    a.in_dtype, a.epilogue, a.c_dtype = _lib.NA_BF16, Fn.NA_EPI_ATOMIC, _lib.NA_F32
+    a.split_k = 1
+    return a
 
 
 def test_narrow_forward_backward(cuda):
