@@ -439,6 +439,23 @@ int na_adamw_step(float* param, const float* grad, float* exp_avg, float* exp_av
                   float lr, float beta1, float beta2, float eps, float weight_decay,
                   int64_t step, void* lowp, int32_t lowp_dtype, na_stream_t stream);
 
+/* The same update for up to NA_ADAMW_MULTI_MAX tensors that share the
+ * hyper-parameters and the step count, in one launch (the optimizer's loop
+ * over its parameter group, get_trainer.py:106-150, for the small tensors).
+ * `tensors` is a host array (copied into the launch); entries with n = 0 are
+ * skipped; no bf16 copy.  Results equal na_adamw_step's bit for bit. */
+#define NA_ADAMW_MULTI_MAX 32
+typedef struct na_adamw_tensor {
+  float* param;
+  const float* grad;
+  float* exp_avg;
+  float* exp_avg_sq;
+  int64_t n;
+} na_adamw_tensor;
+int na_adamw_step_multi(const na_adamw_tensor* tensors, int32_t count, float lr, float beta1,
+                        float beta2, float eps, float weight_decay, int64_t step,
+                        na_stream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Marching cubes over a block-partitioned lattice.  Replaces the per-block
  * loop of projects/neuralangelo/utils/mesh.py:24-49 (LatticeGrid blocks of

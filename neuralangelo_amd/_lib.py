@@ -64,6 +64,15 @@ class GemmArgs(ctypes.Structure):
     ]
 
 
+NA_ADAMW_MULTI_MAX = 32
+
+
+class AdamwTensor(ctypes.Structure):
+    _fields_ = [("param", ctypes.c_void_p), ("grad", ctypes.c_void_p),
+                ("exp_avg", ctypes.c_void_p), ("exp_avg_sq", ctypes.c_void_p),
+                ("n", ctypes.c_int64)]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int32
@@ -124,6 +133,8 @@ _SIGNATURES = {
     "na_sdf_losses_fwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P]),
     "na_sdf_losses_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P, _I32, _P, _P]),
     "na_adamw_step": (ctypes.c_int, [_P, _P, _P, _P, _I64, _F32, _F32, _F32, _F32, _F32, _I64, _P, _I32, _P]),
+    "na_adamw_step_multi": (ctypes.c_int, [ctypes.POINTER(AdamwTensor), _I32, _F32, _F32, _F32,
+                                            _F32, _F32, _I64, _P]),
     "na_mc_workspace_size": (_I64, [_I64, _I64, _I64, _I64]),
     "na_mc_count": (ctypes.c_int, [_P, _I64, _I64, _I64, _I64, _F64, _P, _I64, _P, _P]),
     "na_mc_emit": (ctypes.c_int, [_P, _I64, _I64, _I64, _I64, _F64, _P, _I64, _P, _P, _P, _F64, _P, _P, _P]),

@@ -29,54 +29,92 @@ __device__ inline void st_nt(float* p, const float4& v) {
   __builtin_nontemporal_store(r, reinterpret_cast<nt_f4*>(p));
 }
 
+// The update of the 4-parameter chunk at i (element-wise tail past n - 4).
+template <bool LOWP>
+__device__ inline void adam_chunk(float* __restrict__ p, const float* __restrict__ g,
+                                  float* __restrict__ m, float* __restrict__ v, int64_t n,
+                                  int64_t i, const AdamArgs& a, bf16_t* __restrict__ lowp) {
+  if (i >= n) return;
+  if (i + 4 <= n) {
+    float4 pp = ld_nt(p + i);
+    float4 gg = ld_nt(g + i);
+    float4 mm = ld_nt(m + i);
+    float4 vv = ld_nt(v + i);
+    float* P = &pp.x; const float* G = &gg.x; float* Mv = &mm.x; float* V = &vv.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float pe = P[e] * a.decay;
+      float me = lerp_torch(Mv[e], G[e], 1.0f - a.beta1);
+      float ve = V[e] * a.beta2 + (1.0f - a.beta2) * G[e] * G[e];
+      float den = sqrtf(ve) * a.inv_bc2_sqrt + a.eps;
+      pe = pe - a.step_size * me / den;
+      P[e] = pe; Mv[e] = me; V[e] = ve;
+    }
+    st_nt(p + i, pp);
+    st_nt(m + i, mm);
+    st_nt(v + i, vv);
+    if (LOWP) {
+      const nt_u2 w = {(uint32_t)f32_to_bf16(pp.x) | ((uint32_t)f32_to_bf16(pp.y) << 16),
+                       (uint32_t)f32_to_bf16(pp.z) | ((uint32_t)f32_to_bf16(pp.w) << 16)};
+      __builtin_nontemporal_store(w, reinterpret_cast<nt_u2*>(lowp + i));
+    }
+  } else {
+    for (int64_t j = i; j < n; ++j) {
+      float pe = p[j] * a.decay;
+      float me = lerp_torch(m[j], g[j], 1.0f - a.beta1);
+      float ve = v[j] * a.beta2 + (1.0f - a.beta2) * g[j] * g[j];
+      float den = sqrtf(ve) * a.inv_bc2_sqrt + a.eps;
+      pe = pe - a.step_size * me / den;
+      p[j] = pe; m[j] = me; v[j] = ve;
+      if (LOWP) lowp[j] = f32_to_bf16(pe);
+    }
+  }
+}
+
 // One chunk of 4 parameters per thread (no grid-stride loop: the launch holds
 // n / 4 threads, which keeps the most loads in flight).
 template <bool LOWP>
 __global__ void __launch_bounds__(256)
 adamw_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
              float* __restrict__ v, int64_t n, AdamArgs a, bf16_t* __restrict__ lowp) {
-  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (i < n) {
-    if (i + 4 <= n) {
-      float4 pp = ld_nt(p + i);
-      float4 gg = ld_nt(g + i);
-      float4 mm = ld_nt(m + i);
-      float4 vv = ld_nt(v + i);
-      float* P = &pp.x; const float* G = &gg.x; float* Mv = &mm.x; float* V = &vv.x;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float pe = P[e] * a.decay;
-        float me = lerp_torch(Mv[e], G[e], 1.0f - a.beta1);
-        float ve = V[e] * a.beta2 + (1.0f - a.beta2) * G[e] * G[e];
-        float den = sqrtf(ve) * a.inv_bc2_sqrt + a.eps;
-        pe = pe - a.step_size * me / den;
-        P[e] = pe; Mv[e] = me; V[e] = ve;
-      }
-      st_nt(p + i, pp);
-      st_nt(m + i, mm);
-      st_nt(v + i, vv);
-      if (LOWP) {
-        const nt_u2 w = {(uint32_t)f32_to_bf16(pp.x) | ((uint32_t)f32_to_bf16(pp.y) << 16),
-                         (uint32_t)f32_to_bf16(pp.z) | ((uint32_t)f32_to_bf16(pp.w) << 16)};
-        __builtin_nontemporal_store(w, reinterpret_cast<nt_u2*>(lowp + i));
-      }
-    } else {
-      for (int64_t j = i; j < n; ++j) {
-        float pe = p[j] * a.decay;
-        float me = lerp_torch(m[j], g[j], 1.0f - a.beta1);
-        float ve = v[j] * a.beta2 + (1.0f - a.beta2) * g[j] * g[j];
-        float den = sqrtf(ve) * a.inv_bc2_sqrt + a.eps;
-        pe = pe - a.step_size * me / den;
-        p[j] = pe; m[j] = me; v[j] = ve;
-        if (LOWP) lowp[j] = f32_to_bf16(pe);
-      }
-    }
-  }
+  adam_chunk<LOWP>(p, g, m, v, n, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4, a, lowp);
+}
+
+// Several tensors in one launch: the list travels in the kernel arguments;
+// tensor t owns workgroups [first_block[t], first_block[t + 1]).
+struct AdamMultiList {
+  float* p[NA_ADAMW_MULTI_MAX];
+  const float* g[NA_ADAMW_MULTI_MAX];
+  float* m[NA_ADAMW_MULTI_MAX];
+  float* v[NA_ADAMW_MULTI_MAX];
+  int64_t n[NA_ADAMW_MULTI_MAX];
+  int64_t first_block[NA_ADAMW_MULTI_MAX + 1];
+  int count;
+};
+
+__global__ void __launch_bounds__(256) adamw_multi_kernel(AdamMultiList L, AdamArgs a) {
+  const int64_t b = blockIdx.x;
+  int t = 0;
+  while (t + 1 < L.count && L.first_block[t + 1] <= b) ++t;
+  const int64_t i = ((b - L.first_block[t]) * blockDim.x + threadIdx.x) * 4;
+  adam_chunk<false>(L.p[t], L.g[t], L.m[t], L.v[t], L.n[t], i, a, nullptr);
 }
 
 }  // namespace na
 
 using namespace na;
+
+static AdamArgs adam_args(float lr, float beta1, float beta2, float eps, float weight_decay,
+                           int64_t step) {
+  AdamArgs a;
+  a.lr = lr; a.beta1 = beta1; a.beta2 = beta2; a.eps = eps;
+  a.decay = (float)(1.0 - (double)lr * (double)weight_decay);
+  const double bc1 = 1.0 - pow((double)beta1, (double)step);
+  const double bc2 = 1.0 - pow((double)beta2, (double)step);
+  a.step_size = (float)((double)lr / bc1);
+  a.inv_bc2_sqrt = (float)(1.0 / sqrt(bc2));
+  return a;
+}
 
 extern "C" int na_adamw_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
                              int64_t n, float lr, float beta1, float beta2, float eps,
@@ -89,13 +127,7 @@ extern "C" int na_adamw_step(float* param, const float* grad, float* exp_avg, fl
                "buffers must be 16-byte aligned");
   NA_CHECK_ARG(!lowp || lowp_dtype == NA_BF16, "lowp copy must be bf16");
   NA_CHECK_ARG(!lowp || (uintptr_t)lowp % 8 == 0, "lowp copy must be 8-byte aligned");
-  AdamArgs a;
-  a.lr = lr; a.beta1 = beta1; a.beta2 = beta2; a.eps = eps;
-  a.decay = (float)(1.0 - (double)lr * (double)weight_decay);
-  const double bc1 = 1.0 - pow((double)beta1, (double)step);
-  const double bc2 = 1.0 - pow((double)beta2, (double)step);
-  a.step_size = (float)((double)lr / bc1);
-  a.inv_bc2_sqrt = (float)(1.0 / sqrt(bc2));
+  const AdamArgs a = adam_args(lr, beta1, beta2, eps, weight_decay, step);
   const int64_t threads = ceil_div(n, 4);
   const int64_t blocks = ceil_div(threads, 256);
   NA_CHECK_ARG(blocks <= 0x7fffffffLL, "too many parameters for one launch");
@@ -106,6 +138,38 @@ extern "C" int na_adamw_step(float* param, const float* grad, float* exp_avg, fl
   else
     hipLaunchKernelGGL(adamw_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, param, grad,
                        exp_avg, exp_avg_sq, n, a, (bf16_t*)nullptr);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_adamw_step_multi(const na_adamw_tensor* tensors, int32_t count, float lr,
+                                   float beta1, float beta2, float eps, float weight_decay,
+                                   int64_t step, na_stream_t stream) {
+  NA_CHECK_ARG(count >= 0 && count <= NA_ADAMW_MULTI_MAX, "count must be in [0, NA_ADAMW_MULTI_MAX]");
+  NA_CHECK_ARG(step >= 1, "bad step");
+  NA_CHECK_ARG(count == 0 || tensors, "null tensor list");
+  AdamMultiList L;
+  L.count = 0;
+  int64_t blocks = 0;
+  for (int i = 0; i < count; ++i) {
+    const na_adamw_tensor& t = tensors[i];
+    NA_CHECK_ARG(t.n >= 0, "bad size");
+    if (t.n == 0) continue;
+    NA_CHECK_ARG(t.param && t.grad && t.exp_avg && t.exp_avg_sq, "null pointer");
+    NA_CHECK_ARG(((uintptr_t)t.param | (uintptr_t)t.grad | (uintptr_t)t.exp_avg |
+                  (uintptr_t)t.exp_avg_sq) % 16 == 0,
+                 "buffers must be 16-byte aligned");
+    const int c = L.count++;
+    L.p[c] = t.param; L.g[c] = t.grad; L.m[c] = t.exp_avg; L.v[c] = t.exp_avg_sq; L.n[c] = t.n;
+    L.first_block[c] = blocks;
+    blocks += ceil_div(ceil_div(t.n, (int64_t)4), (int64_t)256);
+  }
+  if (L.count == 0) return NA_OK;
+  L.first_block[L.count] = blocks;
+  NA_CHECK_ARG(blocks <= 0x7fffffffLL, "too many parameters for one launch");
+  const AdamArgs a = adam_args(lr, beta1, beta2, eps, weight_decay, step);
+  hipLaunchKernelGGL(adamw_multi_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     L, a);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

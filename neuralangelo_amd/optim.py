@@ -6,6 +6,8 @@ projects/neuralangelo/configs/base.yaml:117-121).  Parameters registered with
 ``set_lowp_shadow`` (the hash table when it is gathered in bf16) get their
 bf16 copy refreshed in the same pass.
 """
+import ctypes
+
 import torch
 
 from . import _lib
@@ -26,6 +28,9 @@ class AdamW(torch.optim.Optimizer):
         loss = closure() if closure is not None else None
         for group in self.param_groups:
             b1, b2 = group["betas"]
+            hyper = (float(group["lr"]), float(b1), float(b2), float(group["eps"]),
+                     float(group["weight_decay"]))
+            batch = {}  # (step, device) -> [(p, g, state)]: tensors without a bf16 copy
             for p in group["params"]:
                 if p.grad is None:
                     continue
@@ -42,14 +47,32 @@ class AdamW(torch.optim.Optimizer):
                     st["exp_avg_sq"] = torch.zeros_like(p)
                 st["step"] += 1
                 holder = self._shadows.get(id(p))
-                lowp = holder.lowp_shadow() if holder is not None else None
+                if holder is None:
+                    batch.setdefault((int(st["step"]), p.device), []).append((p, g, st))
+                    continue
+                lowp = holder.lowp_shadow()
                 _lib.call("na_adamw_step", _lib.ptr(p), _lib.ptr(g), _lib.ptr(st["exp_avg"]),
-                          _lib.ptr(st["exp_avg_sq"]), p.numel(), float(group["lr"]), float(b1),
-                          float(b2), float(group["eps"]), float(group["weight_decay"]),
-                          int(st["step"]), _lib.ptr(lowp), _lib.NA_BF16, _lib.stream_of(p))
+                          _lib.ptr(st["exp_avg_sq"]), p.numel(), *hyper, int(st["step"]),
+                          _lib.ptr(lowp), _lib.NA_BF16, _lib.stream_of(p))
                 # the kernel wrote through a raw pointer: bump the version counter so
                 # caches keyed on it (the encoder's bf16 copy) see the update
                 torch.autograd.graph.increment_version(p)
-                if holder is not None:
-                    holder.mark_shadow_fresh()
+                holder.mark_shadow_fresh()
+            for (step, _), items in batch.items():
+                self._step_multi(items, hyper, step)
         return loss
+
+    @staticmethod
+    def _step_multi(items, hyper, step):
+        """One launch per NA_ADAMW_MULTI_MAX tensors (na_adamw_step_multi)."""
+        for k in range(0, len(items), _lib.NA_ADAMW_MULTI_MAX):
+            chunk = items[k:k + _lib.NA_ADAMW_MULTI_MAX]
+            arr = (_lib.AdamwTensor * len(chunk))()
+            for e, (p, g, st) in zip(arr, chunk):
+                e.param, e.grad = p.data_ptr(), g.data_ptr()
+                e.exp_avg, e.exp_avg_sq = st["exp_avg"].data_ptr(), st["exp_avg_sq"].data_ptr()
+                e.n = p.numel()
+            _lib.call("na_adamw_step_multi", arr, len(chunk), *hyper, step,
+                      _lib.stream_of(chunk[0][0]))
+            for p, _, _ in chunk:
+                torch.autograd.graph.increment_version(p)

@@ -43,3 +43,29 @@ def test_adamw_refreshes_bf16_table(cuda):
     with torch.no_grad():
         enc.params.add_(1.0)
     assert torch.equal(enc.table_for_forward(), enc.params.detach().to(torch.bfloat16))
+
+
+def test_adamw_multi_tensor_equals_single(cuda):
+    """40 tensors of ragged sizes (one empty) through the optimizer's batched launches
+    (32 + 8 tensors) against na_adamw_step per tensor: bit-identical parameters and
+    moments over three steps."""
+    from neuralangelo_amd import _lib
+    g = torch.Generator().manual_seed(1)
+    sizes = [1, 3, 4, 5, 7, 64, 1023, 1024, 1025, 0, 100003] + [257 * (i + 1) for i in range(29)]
+    params = [torch.nn.Parameter(torch.randn(n, generator=g).to(cuda)) for n in sizes]
+    single = [p.detach().clone() for p in params]
+    m1 = [torch.zeros_like(p) for p in single]
+    v1 = [torch.zeros_like(p) for p in single]
+    opt = AdamW(params, lr=1e-3, weight_decay=1e-2)
+    for step in range(1, 4):
+        grads = [torch.randn(n, generator=g).to(cuda) * (10.0 if step % 2 else 0.1) for n in sizes]
+        for p, gr in zip(params, grads):
+            p.grad = gr.clone()
+        opt.step()
+        for p, gr, m, v in zip(single, grads, m1, v1):
+            _lib.call("na_adamw_step", _lib.ptr(p), _lib.ptr(gr), _lib.ptr(m), _lib.ptr(v), p.numel(),
+                      1e-3, 0.9, 0.999, 1e-8, 1e-2, step, None, _lib.NA_BF16, _lib.stream_of(p))
+    for p, s, m, v in zip(params, single, m1, v1):
+        st = opt.state[p]
+        assert torch.equal(p.detach(), s)
+        assert torch.equal(st["exp_avg"], m) and torch.equal(st["exp_avg_sq"], v)
