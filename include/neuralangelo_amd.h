@@ -208,8 +208,8 @@ int na_act_grad(const void* dy, int64_t ld_dy, int32_t dy_dtype, const void* y, 
                 int64_t ld_out, int32_t out_dtype, float* colsum, na_stream_t stream);
 
 /* Weight normalisation W = g * v / ||v||_row (v [N,K], g [N]); optionally also
- * writes a bf16 copy Wb.  Backward: dg = sum_k dW*v/||v||,
- * dv = g/||v|| * (dW - dg * v/||v||). */
+ * writes a bf16 copy Wb.  Backward writes (does not accumulate)
+ * dg = sum_k dW*v/||v|| and dv = g/||v|| * (dW - dg * v/||v||). */
 int na_weight_norm_fwd(const float* v, const float* g, int64_t N, int64_t K, float* W,
                        void* Wlow, int32_t low_dtype, int64_t ld_low, na_stream_t stream);
 int na_weight_norm_bwd(const float* v, const float* g, const float* dW, int64_t N, int64_t K,

@@ -685,7 +685,7 @@ narrow_bwd_vec_kernel(const TX* __restrict__ X, int64_t ldx, int64_t M, int64_t 
   // before its arithmetic.  When Y is X (the layer's own stored activation) the
   // derivative reads the loaded X values.
   // (rows per batch: what the registers of each variant hold)
-  constexpr int NB = X_ONLY ? (NM == 1 ? 4 : (KC == 1 ? 2 : 1))
+  constexpr int NB = X_ONLY ? (KC == 1 ? 4 : (NM == 1 ? 4 : 1))
                             : ((KC > 1 && NM > 1) ? 1 : 2);
   const int64_t stride = 2 * nwaves;
   const bool y_is_x = dact && (const void*)X == Y && ldy == ldx &&
@@ -962,10 +962,10 @@ weight_norm_bwd_kernel(const float* __restrict__ v, const float* __restrict__ g,
   float norm = sqrtf(block_sum_256(ss, red));
   float d = block_sum_256(dot, red);
   float dgn = d / norm;
-  if (threadIdx.x == 0) dg[n] += dgn;
+  if (threadIdx.x == 0) dg[n] = dgn;
   float s = g[n] / norm;
   for (int64_t k = threadIdx.x; k < K; k += blockDim.x) {
-    dv[n * K + k] += s * (dW[n * K + k] - dgn * v[n * K + k] / norm);
+    dv[n * K + k] = s * (dW[n * K + k] - dgn * v[n * K + k] / norm);
   }
 }
 

@@ -190,7 +190,7 @@ class WeightNorm(torch.autograd.Function):
     def backward(ctx, dW):
         v, gf = ctx.saved_tensors
         N, K = v.shape
-        dv, dg = _zero_block(v.device, v.shape, gf.shape)
+        dv, dg = torch.empty_like(v), torch.empty_like(gf)  # written, not accumulated
         _lib.call("na_weight_norm_bwd", _lib.ptr(v), _lib.ptr(gf), _lib.ptr(dW.contiguous()), N, K,
                   _lib.ptr(dv), _lib.ptr(dg), _s(v))
         return dv, dg.view(ctx.g_shape)
