@@ -73,6 +73,7 @@ class NeuralSDF(torch.nn.Module):
                                    out_bias=_get(cfg_mlp, "out_bias", 0.5),
                                    invert=_get(cfg_mlp, "inside_out", False),
                                    compute_dtype=self.compute_dtype)
+        self._nograd_w = None  # (key, SDFWeights) of the sampling passes
 
     # -- reference API -------------------------------------------------------
     def forward(self, points_3D, with_sdf=True, with_feat=True):
@@ -134,9 +135,18 @@ class NeuralSDF(torch.nn.Module):
     def sdf_nograd(self, points_3D):
         """SDF values without autograd bookkeeping (sampling passes)."""
         shape = points_3D.shape[:-1]
-        W0, b0, W1, b1, ws, bs = [t.detach() for t in self.mlp.weights()]
-        w = Fn.SDFWeights(W0, b0, None, None, ws, bs, self.tcnn_encoding.n_output_dims,
-                          self.compute_dtype)
+        # the kernel-layout weights are rebuilt only when a parameter changed (its
+        # version counter moves on every in-place update, the optimizer's included;
+        # the storage pointer catches a swapped .data): the sampling passes of one
+        # step share one build
+        key = (self.compute_dtype,) + tuple((id(p), p._version, p.data_ptr())
+                                            for p in self.mlp.parameters())
+        if self._nograd_w is None or self._nograd_w[0] != key:
+            W0, b0, W1, b1, ws, bs = [t.detach() for t in self.mlp.weights()]
+            self._nograd_w = (key, Fn.SDFWeights(W0, b0, None, None, ws, bs,
+                                                 self.tcnn_encoding.n_output_dims,
+                                                 self.compute_dtype))
+        w = self._nograd_w[1]
         self.tcnn_encoding.active_levels = self.active_levels
         return Fn.sdf_forward_nograd(self.tcnn_encoding, w, points_3D.reshape(-1, 3).contiguous()
                                      .float()).view(*shape, 1)

@@ -119,3 +119,24 @@ def test_reference_signature_operators(setup, cuda):
     ref = orr.sample_dists_hierarchical(dists, sdfs, 64.0, 16)
     assert fine.shape == (B, R, 16, 1)
     assert (fine.cpu() - ref).abs().max() < 1e-5
+
+
+def test_sampling_weights_follow_parameter_updates(cuda):
+    """The sampling passes reuse their kernel-layout weights between calls; an in-place
+    parameter update (as the optimizer makes) must reach the next call."""
+    torch.manual_seed(0)
+    m = Model(default_config(log2_T=14), compute_dtype=torch.float32)
+    randomize_model(m, 5)
+    m = m.to(cuda)
+    sdf = m.neural_sdf
+    x = torch.rand(1000, 3, device=cuda) * 1.6 - 0.8
+    s0 = sdf.sdf_nograd(x)
+    assert torch.equal(sdf.sdf_nograd(x), s0)
+    bias = [p for n, p in sdf.mlp.named_parameters() if n.endswith("bias")][-1]
+    with torch.no_grad():
+        bias.add_(0.25)
+    s1 = sdf.sdf_nograd(x)
+    with torch.no_grad():
+        ref = sdf(x, with_feat=False)[0].reshape(s1.shape)
+    assert (s1 - ref).abs().max().item() < 1e-5
+    assert (s1 - s0).abs().max().item() > 1e-3
