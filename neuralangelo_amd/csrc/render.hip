@@ -260,9 +260,11 @@ __global__ void __launch_bounds__(256)
 neus_alpha_bwd_kernel(NeusIn p, const float* __restrict__ dalpha, float* __restrict__ dsdf,
                       float* __restrict__ dgrad, float* __restrict__ ds_var) {
   __shared__ float red[4];
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // grid-stride over the samples: a few hundred workgroups, so that few atomics
+  // meet on the one ds_var
   float dsv = 0.0f;
-  if (t < p.R * p.N) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p.R * p.N;
+       t += (int64_t)gridDim.x * blockDim.x) {
     NeusTerms q = neus_terms(p, t);
     float da = dalpha[t];
     // clamp backward: pass-through where 0 <= ratio <= 1
@@ -276,7 +278,7 @@ neus_alpha_bwd_kernel(NeusIn p, const float* __restrict__ dalpha, float* __restr
     float dzn = dnc * q.nc * (1.0f - q.nc);
     float dep = dzp * q.inv_s, den_ = dzn * q.inv_s;
     float dinv = dzp * q.est_prev + dzn * q.est_next;
-    dsv = dinv * q.inv_s;
+    dsv += dinv * q.inv_s;
     dsdf[t] = q.outside ? 0.0f : (dep + den_);
     float diter = (-dep + den_) * q.intv * 0.5f;
     float a = p.anneal_ratio;
@@ -697,19 +699,21 @@ sdf_losses_fwd_kernel(const float* __restrict__ grad, const float* __restrict__ 
                       const uint8_t* __restrict__ outside, int64_t n, int64_t n_per_ray,
                       float* __restrict__ sums) {
   __shared__ float red[2][4];
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // grid-stride: a few hundred workgroups, so that few atomics meet on the two sums
+  // (NaN / Inf terms count as 0, as before the sum)
   float e = 0.0f, c = 0.0f;
-  if (p < n) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
     float m = (outside && outside[p / n_per_ray]) ? 0.0f : 1.0f;
     if (grad) {
       float g0 = grad[p * 3], g1 = grad[p * 3 + 1], g2 = grad[p * 3 + 2];
       float nrm = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
       float ev = (nrm - 1.0f) * (nrm - 1.0f);
-      e = (finitef_(ev) ? ev : 0.0f) * m;
+      e += (finitef_(ev) ? ev : 0.0f) * m;
     }
     if (hess) {
       float lap = fabsf((hess[p * 3] + hess[p * 3 + 1]) + hess[p * 3 + 2]);
-      c = (finitef_(lap) ? lap : 0.0f) * m;
+      c += (finitef_(lap) ? lap : 0.0f) * m;
     }
   }
 #pragma unroll
@@ -765,6 +769,12 @@ sdf_losses_bwd_kernel(const float* __restrict__ grad, const float* __restrict__ 
 }
 
 static inline dim3 grid1d(int64_t n, int block = 256) { return dim3((unsigned)ceil_div(n, block)); }
+// Grid of a grid-stride kernel that ends in one atomic per workgroup on a
+// single address: at most 512 workgroups of 256 threads.
+static inline dim3 grid_reduce(int64_t n) {
+  const int64_t b = ceil_div(n, (int64_t)256);
+  return dim3((unsigned)(b < 512 ? b : 512));
+}
 
 }  // namespace na
 
@@ -864,8 +874,8 @@ extern "C" int na_neus_alpha_bwd(const float* ray_unit, const float* sdfs, const
   NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
   NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
                        anneal_end, R, N);
-  hipLaunchKernelGGL(neus_alpha_bwd_kernel, grid1d(R * N), dim3(256), 0, (hipStream_t)stream, p,
-                     dalpha, dsdf, dgrad, ds_var);
+  hipLaunchKernelGGL(neus_alpha_bwd_kernel, grid_reduce(R * N), dim3(256), 0, (hipStream_t)stream,
+                     p, dalpha, dsdf, dgrad, ds_var);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
@@ -991,8 +1001,8 @@ extern "C" int na_sdf_losses_fwd(const float* grad, const float* hess, const uin
   NA_CHECK_ARG(n >= 0 && n_per_ray >= 1, "bad sizes");
   if (n == 0) return NA_OK;
   NA_CHECK_ARG(sums, "null pointer");
-  hipLaunchKernelGGL(sdf_losses_fwd_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, grad,
-                     hess, outside, n, n_per_ray, sums);
+  hipLaunchKernelGGL(sdf_losses_fwd_kernel, grid_reduce(n), dim3(256), 0, (hipStream_t)stream,
+                     grad, hess, outside, n, n_per_ray, sums);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
@@ -1132,7 +1142,7 @@ extern "C" int na_neus_alpha_packed_bwd(const float* ray_unit, const float* sdfs
   NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
                        anneal_end, S, 1);
   p.off = offsets; p.ray_idx = ray_idx;
-  hipLaunchKernelGGL(neus_alpha_bwd_kernel, grid1d(S), dim3(256), 0, (hipStream_t)stream, p,
+  hipLaunchKernelGGL(neus_alpha_bwd_kernel, grid_reduce(S), dim3(256), 0, (hipStream_t)stream, p,
                      dalpha, dsdf, dgrad, ds_var);
   NA_CHECK_LAUNCH();
   return NA_OK;

@@ -98,9 +98,9 @@ def _neus_inputs(R, N, seed):
     return c, v, near, far, outside, dists, sdfs, grads
 
 
-@pytest.mark.parametrize("progress", [0.02, 1.0])
-def test_neus_alpha_fwd_bwd(cuda, progress):
-    R, N = 512, 96
+@pytest.mark.parametrize("progress,R", [(0.02, 512), (1.0, 512), (0.3, 3000)])
+def test_neus_alpha_fwd_bwd(cuda, progress, R):
+    N = 96
     c, v, near, far, outside, dists, sdfs, grads = _neus_inputs(R, N, 5)
     outside[:7] = True
     s_var = torch.tensor(0.3 * 10)
@@ -203,3 +203,22 @@ def test_taps_and_losses(cuda):
     fin = ~rg.isnan()
     assert (got[fin] - rg[fin]).abs().max() < 1e-8
     assert (hd.grad.cpu().double() - rh).abs().max() < 1e-9
+
+
+def test_sdf_losses_many_samples(cuda):
+    """Eikonal and curvature sums over 300,000 samples (several grid-stride passes per
+    thread before the per-workgroup atomics), with masked rays and non-finite rows,
+    against the float64 oracle."""
+    R, n = 3000, 300000
+    g = torch.Generator().manual_seed(9)
+    outside = torch.rand(R, generator=g) < 0.1
+    gg = torch.randn(n, 3, generator=g)
+    hh = torch.randn(n, 3, generator=g)
+    gg[12345] = float("inf")
+    hh[54321] = float("nan")
+    e_ref = orr.eikonal_loss(gg.double().view(R, -1, 3), outside[:, None])
+    c_ref = orr.curvature_loss(hh.double().view(R, -1, 3), outside[:, None])
+    e, cc = misc.sdf_losses(gg.to(cuda).view(R, -1, 3), hh.to(cuda).view(R, -1, 3),
+                            outside.to(cuda))
+    assert abs(e.item() - e_ref.item()) < 1e-5 * abs(e_ref.item()) + 1e-6
+    assert abs(cc.item() - c_ref.item()) < 1e-5 * abs(c_ref.item()) + 1e-6
