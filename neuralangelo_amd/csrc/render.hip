@@ -13,6 +13,7 @@
 namespace na {
 
 constexpr int kRayBlock = 64;  // rays per workgroup for the per-ray scan kernels
+constexpr int kCompU = 8;      // samples whose loads a compositing thread issues together
 
 __device__ inline float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
 
@@ -308,13 +309,29 @@ composite_fwd_kernel(const float* __restrict__ alpha, const float* __restrict__ 
   if (r >= R) return;
   float logT = 0.0f;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int i = 0; i < N; ++i) {
-    float a = alpha[r * N + i];
-    float T = expf(logT);
-    float w = a * T;
-    weights[r * N + i] = w;
-    for (int k = 0; k < K; ++k) acc[k] = acc[k] + q[(r * N + i) * K + k] * w;
-    logT = logT + log1pf(-a);
+  // the loads of kCompU samples go out before their (sequential) arithmetic
+  for (int i0 = 0; i0 < N; i0 += kCompU) {
+    float av[kCompU], qv[kCompU][4];
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int i = i0 + u;
+      av[u] = i < N ? alpha[r * N + i] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) qv[u][k] = (i < N && k < K) ? q[(r * N + i) * K + k] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int i = i0 + u;
+      if (i >= N) break;
+      float a = av[u];
+      float T = expf(logT);
+      float w = a * T;
+      weights[r * N + i] = w;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < K) acc[k] = acc[k] + qv[u][k] * w;
+      logT = logT + log1pf(-a);
+    }
   }
   for (int k = 0; k < K; ++k) out[r * K + k] = acc[k];
 }
@@ -330,26 +347,50 @@ composite_bwd_kernel(const float* __restrict__ alpha, const float* __restrict__ 
   if (r >= R) return;
   float* T = s_T + l;
   float logT = 0.0f;
-  for (int i = 0; i < N; ++i) {
-    T[i * kRayBlock] = expf(logT);
-    logT = logT + log1pf(-alpha[r * N + i]);
+  // (the loads of kCompU samples go out before their sequential arithmetic)
+  for (int i0 = 0; i0 < N; i0 += kCompU) {
+    float av[kCompU];
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) av[u] = i0 + u < N ? alpha[r * N + i0 + u] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      if (i0 + u >= N) break;
+      T[(i0 + u) * kRayBlock] = expf(logT);
+      logT = logT + log1pf(-av[u]);
+    }
   }
   float d[4] = {0.f, 0.f, 0.f, 0.f};
   if (dout)
     for (int k = 0; k < K; ++k) d[k] = dout[r * K + k];
   // U_j = sum_{i>j} g_i a_i prod_{j<k<i}(1-a_k);  dalpha_j = T_j (g_j - U_j)
   float U = 0.0f;
-  for (int j = N - 1; j >= 0; --j) {
-    float g = dweights ? dweights[r * N + j] : 0.0f;
-    for (int k = 0; k < K; ++k) g = g + d[k] * q[(r * N + j) * K + k];
-    if (dq) {
-      float w = weights[r * N + j];
-      for (int k = 0; k < K; ++k) dq[(r * N + j) * K + k] = w * d[k];
+  for (int j0 = N - 1; j0 >= 0; j0 -= kCompU) {
+    float gv[kCompU], av[kCompU], wv[kCompU], qv[kCompU][4];
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int j = j0 - u;
+      const bool ok = j >= 0;
+      gv[u] = (ok && dweights) ? dweights[r * N + j] : 0.0f;
+      av[u] = ok ? alpha[r * N + j] : 0.0f;
+      wv[u] = (ok && dq) ? weights[r * N + j] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) qv[u][k] = (ok && k < K) ? q[(r * N + j) * K + k] : 0.0f;
     }
-    float a = alpha[r * N + j];
-    float Tj = T[j * kRayBlock];
-    dalpha[r * N + j] = Tj * (g - U);
-    U = g * a + (1.0f - a) * U;
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int j = j0 - u;
+      if (j < 0) break;
+      float g = gv[u];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < K) g = g + d[k] * qv[u][k];
+      if (dq)
+        for (int k = 0; k < K; ++k) dq[(r * N + j) * K + k] = wv[u] * d[k];
+      float a = av[u];
+      float Tj = T[j * kRayBlock];
+      dalpha[r * N + j] = Tj * (g - U);
+      U = g * a + (1.0f - a) * U;
+    }
   }
 }
 
