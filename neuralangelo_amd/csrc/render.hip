@@ -130,24 +130,39 @@ hier_resample_kernel(const float* __restrict__ dists, const float* __restrict__ 
   float T = 1.0f;
   float prev_cos = 0.0f;
   float l1 = 0.0f;
-  for (int i = 0; i < N - 1; ++i) {
-    float sp = s[i], sn = s[i + 1];
-    float dp = d[i], dn = d[i + 1];
-    float mid = (sp + sn) * 0.5f;
-    float cosv = (sn - sp) / (dn - dp + 1e-5f);
-    float cr = tmin(prev_cos, cosv);
-    prev_cos = cosv;
-    float intv = dn - dp;
-    float est_prev = mid - cr * intv * 0.5f;
-    float est_next = mid + cr * intv * 0.5f;
-    float pc = sigmoidf_(est_prev * inv_s);
-    float nc = sigmoidf_(est_next * inv_s);
-    float a = (pc - nc) / (pc + 1e-5f);
-    a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
-    float w = a * T;
-    T = T * (1.0f - a);
-    cdf[(i + 1) * kRayBlock] = w;
-    l1 = l1 + fabsf(w);
+  // (the loads of kCompU intervals go out before their sequential arithmetic)
+  float sp = s[0], dp = d[0];
+  for (int i0 = 0; i0 < N - 1; i0 += kCompU) {
+    float sv[kCompU], dv[kCompU];
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int i = i0 + u + 1;
+      sv[u] = i < N ? s[i] : 0.0f;
+      dv[u] = i < N ? d[i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int i = i0 + u;
+      if (i >= N - 1) break;
+      const float sn = sv[u], dn = dv[u];
+      float mid = (sp + sn) * 0.5f;
+      float cosv = (sn - sp) / (dn - dp + 1e-5f);
+      float cr = tmin(prev_cos, cosv);
+      prev_cos = cosv;
+      float intv = dn - dp;
+      float est_prev = mid - cr * intv * 0.5f;
+      float est_next = mid + cr * intv * 0.5f;
+      float pc = sigmoidf_(est_prev * inv_s);
+      float nc = sigmoidf_(est_next * inv_s);
+      float a = (pc - nc) / (pc + 1e-5f);
+      a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
+      float w = a * T;
+      T = T * (1.0f - a);
+      cdf[(i + 1) * kRayBlock] = w;
+      l1 = l1 + fabsf(w);
+      sp = sn;
+      dp = dn;
+    }
   }
   // pass 2: pdf -> cdf in place (nerf_util.py:50-53)
   float denom = l1 > 1e-12f ? l1 : 1e-12f;
