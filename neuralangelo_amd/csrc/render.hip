@@ -684,11 +684,11 @@ template <typename TO>
 __global__ void __launch_bounds__(256)
 rgb_input_fwd_kernel(const float* __restrict__ x, const float* __restrict__ ray,
                      int64_t n_per_ray, const float* __restrict__ grad, int normalize, int64_t n,
-                     int col0, TO* __restrict__ out, int64_t ld) {
+                     int col0, TO* __restrict__ out, int64_t ld, int vec24) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   int64_t r = p / n_per_ray;
-  float row[22];
+  float row[24];
   row[0] = x[p * 3];
   row[1] = x[p * 3 + 1];
   row[2] = x[p * 3 + 2];
@@ -704,7 +704,26 @@ rgb_input_fwd_kernel(const float* __restrict__ x, const float* __restrict__ ray,
   row[19] = g0;
   row[20] = g1;
   row[21] = g2;
+  row[22] = row[23] = 0.0f;
   TO* o = out + p * ld + col0;
+  if (vec24) {
+    // the row's tail is exactly these 24 columns, 16-byte aligned: whole
+    // 16-byte stores (same rounding as the element stores)
+    if constexpr (sizeof(TO) == 2) {
+      uint32_t w[12];
+#pragma unroll
+      for (int i = 0; i < 12; ++i)
+        w[i] = (uint32_t)f32_to_bf16(row[2 * i]) | ((uint32_t)f32_to_bf16(row[2 * i + 1]) << 16);
+      uint4* q = reinterpret_cast<uint4*>(o);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+    } else {
+      float4* q = reinterpret_cast<float4*>(o);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) q[i] = make_float4(row[4 * i], row[4 * i + 1], row[4 * i + 2], row[4 * i + 3]);
+    }
+    return;
+  }
   for (int c = 0; c < 22; ++c) scalar_io<TO>::store(o + c, row[c]);
   for (int64_t c = col0 + 22; c < ld; ++c) scalar_io<TO>::store(out + p * ld + c, 0.0f);
 }
@@ -1026,12 +1045,16 @@ extern "C" int na_rgb_input_fwd(const float* x, const float* ray_unit, int64_t n
   NA_CHECK_ARG(n >= 0 && n_per_ray >= 1 && col0 >= 0 && ld >= col0 + 22, "bad sizes");
   if (n == 0) return NA_OK;
   NA_CHECK_ARG(x && ray_unit && grad && out, "null pointer");
+  // a tail of exactly 24 columns on 16-byte boundaries goes out as vector stores
+  const int64_t eb = out_dtype == NA_BF16 ? 2 : 4;
+  const int vec24 = ld - col0 == 24 && ((uintptr_t)out % 16) == 0 && (ld * eb) % 16 == 0 &&
+                    (col0 * eb) % 16 == 0;
   if (out_dtype == NA_BF16)
     hipLaunchKernelGGL(rgb_input_fwd_kernel<bf16_t>, grid1d(n), dim3(256), 0, (hipStream_t)stream,
-                       x, ray_unit, n_per_ray, grad, normalize, n, col0, (bf16_t*)out, ld);
+                       x, ray_unit, n_per_ray, grad, normalize, n, col0, (bf16_t*)out, ld, vec24);
   else
     hipLaunchKernelGGL(rgb_input_fwd_kernel<float>, grid1d(n), dim3(256), 0, (hipStream_t)stream,
-                       x, ray_unit, n_per_ray, grad, normalize, n, col0, (float*)out, ld);
+                       x, ray_unit, n_per_ray, grad, normalize, n, col0, (float*)out, ld, vec24);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
