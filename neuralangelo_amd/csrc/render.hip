@@ -422,13 +422,28 @@ composite_packed_fwd_kernel(const float* __restrict__ alpha, const float* __rest
   const int64_t b = off[r], e = off[r + 1];
   float logT = 0.0f;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int64_t i = b; i < e; ++i) {
-    float a = alpha[i];
-    float T = expf(logT);
-    float w = a * T;
-    weights[i] = w;
-    for (int k = 0; k < K; ++k) acc[k] = acc[k] + q[i * K + k] * w;
-    logT = logT + log1pf(-a);
+  for (int64_t i0 = b; i0 < e; i0 += kCompU) {
+    float av[kCompU], qv[kCompU][4];
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int64_t i = i0 + u;
+      av[u] = i < e ? alpha[i] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) qv[u][k] = (i < e && k < K) ? q[i * K + k] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int64_t i = i0 + u;
+      if (i >= e) break;
+      float a = av[u];
+      float T = expf(logT);
+      float w = a * T;
+      weights[i] = w;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < K) acc[k] = acc[k] + qv[u][k] * w;
+      logT = logT + log1pf(-a);
+    }
   }
   for (int k = 0; k < K; ++k) out[r * K + k] = acc[k];
 }
@@ -443,25 +458,49 @@ composite_packed_bwd_kernel(const float* __restrict__ alpha, const float* __rest
   if (r >= R) return;
   const int64_t b = off[r], e = off[r + 1];
   float logT = 0.0f;
-  for (int64_t i = b; i < e; ++i) {
-    dalpha[i] = expf(logT);  // T_i, replaced by the gradient below
-    logT = logT + log1pf(-alpha[i]);
+  for (int64_t i0 = b; i0 < e; i0 += kCompU) {
+    float av[kCompU];
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) av[u] = i0 + u < e ? alpha[i0 + u] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      if (i0 + u >= e) break;
+      dalpha[i0 + u] = expf(logT);  // T_i, replaced by the gradient below
+      logT = logT + log1pf(-av[u]);
+    }
   }
   float d[4] = {0.f, 0.f, 0.f, 0.f};
   if (dout)
     for (int k = 0; k < K; ++k) d[k] = dout[r * K + k];
   float U = 0.0f;
-  for (int64_t j = e - 1; j >= b; --j) {
-    float g = dweights ? dweights[j] : 0.0f;
-    for (int k = 0; k < K; ++k) g = g + d[k] * q[j * K + k];
-    if (dq) {
-      float w = weights[j];
-      for (int k = 0; k < K; ++k) dq[j * K + k] = w * d[k];
+  for (int64_t j0 = e - 1; j0 >= b; j0 -= kCompU) {
+    // each T_j is read here before this block overwrites it
+    float gv[kCompU], av[kCompU], wv[kCompU], tv[kCompU], qv[kCompU][4];
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int64_t j = j0 - u;
+      const bool ok = j >= b;
+      gv[u] = (ok && dweights) ? dweights[j] : 0.0f;
+      av[u] = ok ? alpha[j] : 0.0f;
+      wv[u] = (ok && dq) ? weights[j] : 0.0f;
+      tv[u] = ok ? dalpha[j] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) qv[u][k] = (ok && k < K) ? q[j * K + k] : 0.0f;
     }
-    float a = alpha[j];
-    float Tj = dalpha[j];
-    dalpha[j] = Tj * (g - U);
-    U = g * a + (1.0f - a) * U;
+#pragma unroll
+    for (int u = 0; u < kCompU; ++u) {
+      const int64_t j = j0 - u;
+      if (j < b) break;
+      float g = gv[u];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < K) g = g + d[k] * qv[u][k];
+      if (dq)
+        for (int k = 0; k < K; ++k) dq[j * K + k] = wv[u] * d[k];
+      float a = av[u];
+      dalpha[j] = tv[u] * (g - U);
+      U = g * a + (1.0f - a) * U;
+    }
   }
 }
 
