@@ -1,7 +1,8 @@
 """Drop-in Model for the object render path (projects/neuralangelo/model.py:24-298).
 
-Covers the hot path with the background model disabled (the BASELINE
-configurations); every per-sample operator runs on the HIP kernels:
+Covers the hot path with the background model disabled (the object-only
+training and extraction configurations); every per-sample operator runs on the
+HIP kernels:
   get_dist_bounds + sample_dists      -> one fused bounds/coarse kernel
   sample_dists_all / _hierarchical    -> SDF passes + one resample/merge kernel per round
   neural_sdf.forward / compute_grads  -> fused encoder + SDF MLP (GEMM kernels), 4-tap combine

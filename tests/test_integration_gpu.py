@@ -1,4 +1,4 @@
-"""The ctypes binding shown in INTEGRATION.md runs as written and agrees with the modules."""
+"""The ctypes binding shown in the integration guide runs as written and agrees with the modules."""
 import os
 import re
 
