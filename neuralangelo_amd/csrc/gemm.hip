@@ -492,7 +492,15 @@ head_sum_kernel(const float* __restrict__ parts, int nparts, int64_t M,
   const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (m >= M) return;
   float v = bias ? bias[0] : 0.0f;
-  for (int q = 0; q < nparts; ++q) v += parts[q * M + m];
+  int q = 0;
+  for (; q + 8 <= nparts; q += 8) {  // eight loads in flight, summed in order
+    float t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = parts[(q + u) * M + m];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += t[u];
+  }
+  for (; q < nparts; ++q) v += parts[q * M + m];
   out[m] = v;
 }
 
