@@ -168,7 +168,7 @@ typedef struct na_gemm_args {
 int na_gemm(const na_gemm_args* args, na_stream_t stream);
 
 /* Floats of wg_partials that na_gemm's weight-gradient path uses for these
- * arguments (0 when another path runs them; -2 on bad arguments). */
+ * arguments (0 when another path runs them; NA_ERR_ARGUMENT for a NULL args). */
 int64_t na_gemm_wgrad_workspace(const na_gemm_args* args);
 
 /* out[n] += sum_m X[m*ld + n] (fp32 accumulation, X of x_dtype): the bias gradient

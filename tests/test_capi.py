@@ -46,6 +46,28 @@ def test_version_and_error_string():
     assert "n_features" in _lib.last_error()
 
 
+def test_adamw_multi_and_wgrad_workspace_argument_checks():
+    """Argument checks of the multi-tensor AdamW and the weight-gradient workspace
+    query: they return before any GPU call."""
+    lib = _lib.load()
+    arr = (_lib.AdamwTensor * 1)()
+    st = lib.na_adamw_step_multi(arr, _lib.NA_ADAMW_MULTI_MAX + 1, 1e-3, 0.9, 0.999, 1e-8, 1e-2,
+                                 1, None)
+    assert st == _lib.NA_ERR_ARGUMENT and "NA_ADAMW_MULTI_MAX" in _lib.last_error()
+    st = lib.na_adamw_step_multi(arr, 1, 1e-3, 0.9, 0.999, 1e-8, 1e-2, 0, None)
+    assert st == _lib.NA_ERR_ARGUMENT and "step" in _lib.last_error()
+    arr[0].n = -1
+    st = lib.na_adamw_step_multi(arr, 1, 1e-3, 0.9, 0.999, 1e-8, 1e-2, 1, None)
+    assert st == _lib.NA_ERR_ARGUMENT and "size" in _lib.last_error()
+    assert lib.na_adamw_step_multi(None, 0, 1e-3, 0.9, 0.999, 1e-8, 1e-2, 1, None) == 0
+    assert lib.na_gemm_wgrad_workspace(None) == _lib.NA_ERR_ARGUMENT
+    a = _lib.GemmArgs()
+    a.M, a.N, a.K = 256, 256, 4096
+    a.trans_a, a.trans_b, a.in_dtype = 1, 1, _lib.NA_F32  # fp32 operands: not the bf16 path
+    a.epilogue, a.c_dtype = _lib.NA_EPI_ATOMIC, _lib.NA_F32
+    assert lib.na_gemm_wgrad_workspace(ctypes.byref(a)) == 0
+
+
 @pytest.mark.parametrize("log2_T,F", [(22, 8), (19, 4), (12, 2), (16, 1)])
 def test_layout_matches_oracle(log2_T, F):
     g = growth_rate(5, 11, 16)
