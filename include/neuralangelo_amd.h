@@ -270,6 +270,34 @@ int na_ray_bounds_sample(const float* center, const float* ray_unit, const float
                          int64_t R, int32_t N, float* near, float* far, uint8_t* outside,
                          float* dists, na_stream_t stream);
 
+/* Sphere crossings (projects/nerf/utils/nerf_util.py:199-205, intersect_with_sphere):
+ * near/far = -<c,v> -/+ sqrt(<c,v>^2 - (<c,c> - radius^2)) per ray, unclamped,
+ * NaN where the ray misses.  center, ray_unit [R,3] -> near, far [R]. */
+int na_sphere_crossings(const float* center, const float* ray_unit, int64_t R, float radius,
+                        float* near, float* far, na_stream_t stream);
+
+/* Pixel-centre camera rays (projects/nerf/utils/camera.py:283-311, get_center_and_ray,
+ * with the pixel selection of projects/nerf/utils/nerf_util.py:127-131, slice_by_ray_idx):
+ * pose [B,3,4] world-to-camera [R|t], intr [B,3,3]; pixel p = ray_idx[b,r] (int64,
+ * in [0, H*W)) or p = r when ray_idx is NULL (then R must be H*W) sits at
+ * (p mod W + 0.5, p div W + 0.5).  center [B,R,3] = -R^T t, ray [B,R,3] =
+ * R^T K^-1 [x,y,1]^T (depth 1), ray_unit [B,R,3] = ray/max(|ray|,1e-12) (may be NULL). */
+int na_camera_rays(const float* pose, const float* intr, int64_t B, int64_t H, int64_t W,
+                   const int64_t* ray_idx, int64_t R, float* center, float* ray, float* ray_unit,
+                   na_stream_t stream);
+
+/* Stratified distances (projects/nerf/utils/nerf_util.py:20-38, sample_dists):
+ * dists[r,i] = (i + u) / N * (hi - lo) + lo with u = rands[r,i] (stratified) or
+ * 0.5 when rands == NULL; lo / hi are per-ray [R] arrays, or the constants
+ * lo_const / hi_const when the pointer is NULL.  -> dists [R, N]. */
+int na_sample_dists(const float* lo, const float* hi, float lo_const, float hi_const,
+                    const float* rands, int64_t R, int32_t N, float* dists, na_stream_t stream);
+
+/* Real spherical harmonics of unit directions
+ * (projects/neuralangelo/utils/spherical_harmonics.py:47-71, get_spherical_harmonics):
+ * dirs [n,3] -> out [n, (levels+1)^2], degree 0..3 (NA_ERR_UNSUPPORTED above 3). */
+int na_sh_encode(const float* dirs, int64_t n, int32_t levels, float* out, na_stream_t stream);
+
 /* Inverse-CDF resampling (projects/nerf/utils/nerf_util.py:41-68):
  * pdf = w / max(sum|w|, 1e-12); cdf = [0, serial fp32 prefix of pdf];
  * u_j = linspace midpoints; idx = searchsorted(cdf, u, right=True); lerp.

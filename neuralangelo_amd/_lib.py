@@ -104,6 +104,10 @@ _SIGNATURES = {
     "na_sdf_taps_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _P, _I32, _I64, _P, _I32, _P]),
     "na_sdf_taps_input_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _I32, _I32, _P, _P, _P, _I64, _P]),
     "na_ray_bounds_sample": (ctypes.c_int, [_P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P]),
+    "na_sphere_crossings": (ctypes.c_int, [_P, _P, _I64, _F32, _P, _P, _P]),
+    "na_camera_rays": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _P]),
+    "na_sample_dists": (ctypes.c_int, [_P, _P, _F32, _F32, _P, _I64, _I32, _P, _P]),
+    "na_sh_encode": (ctypes.c_int, [_P, _I64, _I32, _P, _P]),
     "na_sample_pdf": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
     "na_hier_resample": (ctypes.c_int, [_P, _P, _I64, _I32, _F32, _I32, _P, _P, _P, _P, _P]),
     "na_scatter_merge": (ctypes.c_int, [_P, _P, _I32, _P, _P, _I32, _I64, _P, _P]),

@@ -57,6 +57,91 @@ ray_bounds_sample_kernel(const float* __restrict__ center, const float* __restri
   dists[t] = d;
 }
 
+// Raw sphere crossings, no clamping: t = -<c,v> -/+ sqrt(<c,v>^2 - (<c,c> - r^2)),
+// NaN for a ray that misses.  One thread per ray.
+__global__ void __launch_bounds__(256)
+sphere_crossings_kernel(const float* __restrict__ center, const float* __restrict__ ray, int64_t R,
+                        float r2, float* __restrict__ near_out, float* __restrict__ far_out) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  float c0 = center[r * 3], c1 = center[r * 3 + 1], c2 = center[r * 3 + 2];
+  float v0 = ray[r * 3], v1 = ray[r * 3 + 1], v2 = ray[r * 3 + 2];
+  float ctc = c0 * c0 + c1 * c1 + c2 * c2;
+  float ctv = c0 * v0 + c1 * v1 + c2 * v2;
+  float sq = sqrtf(ctv * ctv - (ctc - r2));
+  near_out[r] = -ctv - sq;
+  far_out[r] = -ctv + sq;
+}
+
+// Pixel-centre camera rays.  Pixel p of an H x W image sits at
+// (x, y) = (p mod W + 0.5, p div W + 0.5); its camera-space point is
+// K^-1 [x, y, 1]^T, and the inverse of the world-to-camera pose [R | t] maps it
+// to the world: X_w = R^T X_c - R^T t.  center = -R^T t, ray = X_w - center
+// (depth 1, not unit), ray_unit = ray / max(|ray|, 1e-12).  K^-1 comes from the
+// adjugate in double precision.  One thread per (camera, ray); ray_idx [B, R]
+// picks the pixels (NULL: pixel j for ray j).
+__global__ void __launch_bounds__(256)
+camera_rays_kernel(const float* __restrict__ pose, const float* __restrict__ intr, int64_t B,
+                   int64_t W, const int64_t* __restrict__ ray_idx, int64_t R,
+                   float* __restrict__ center, float* __restrict__ ray,
+                   float* __restrict__ ray_unit) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * R) return;
+  const int64_t b = t / R;
+  const int64_t p = ray_idx ? ray_idx[t] : t - b * R;
+  const float* K = intr + b * 9;
+  const double k0 = K[0], k1 = K[1], k2 = K[2], k3 = K[3], k4 = K[4], k5 = K[5], k6 = K[6],
+               k7 = K[7], k8 = K[8];
+  const double c00 = k4 * k8 - k5 * k7, c01 = k5 * k6 - k3 * k8, c02 = k3 * k7 - k4 * k6;
+  const double rdet = 1.0 / (k0 * c00 + k1 * c01 + k2 * c02);
+  float Ki[9];
+  Ki[0] = (float)(c00 * rdet);
+  Ki[1] = (float)((k2 * k7 - k1 * k8) * rdet);
+  Ki[2] = (float)((k1 * k5 - k2 * k4) * rdet);
+  Ki[3] = (float)(c01 * rdet);
+  Ki[4] = (float)((k0 * k8 - k2 * k6) * rdet);
+  Ki[5] = (float)((k2 * k3 - k0 * k5) * rdet);
+  Ki[6] = (float)(c02 * rdet);
+  Ki[7] = (float)((k1 * k6 - k0 * k7) * rdet);
+  Ki[8] = (float)((k0 * k4 - k1 * k3) * rdet);
+  const float px = (float)(p % W) + 0.5f, py = (float)(p / W) + 0.5f;
+  float xc[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) xc[k] = Ki[k * 3] * px + Ki[k * 3 + 1] * py + Ki[k * 3 + 2];
+  const float* P = pose + b * 12;  // row-major [3, 4]: P[r*4 + c]
+  float cw[3], rw[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    cw[k] = -(P[k] * P[3] + P[4 + k] * P[7] + P[8 + k] * P[11]);
+    const float xw = P[k] * xc[0] + P[4 + k] * xc[1] + P[8 + k] * xc[2] + cw[k];
+    rw[k] = xw - cw[k];
+  }
+  float nrm = sqrtf(rw[0] * rw[0] + rw[1] * rw[1] + rw[2] * rw[2]);
+  nrm = nrm > 1e-12f ? nrm : 1e-12f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    center[t * 3 + k] = cw[k];
+    ray[t * 3 + k] = rw[k];
+    if (ray_unit) ray_unit[t * 3 + k] = rw[k] / nrm;
+  }
+}
+
+// Evenly stratified distances between per-ray bounds (a NULL bound array means
+// the constant): d[r,i] = (i + u) / N * (hi - lo) + lo, u = rands[r,i] or 0.5.
+__global__ void __launch_bounds__(256)
+sample_dists_kernel(const float* __restrict__ lo, const float* __restrict__ hi, float lo_c,
+                    float hi_c, const float* __restrict__ rands, int64_t R, int N,
+                    float* __restrict__ dists) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * N) return;
+  int64_t r = t / N;
+  int i = (int)(t % N);
+  float a = lo ? lo[r] : lo_c;
+  float b = hi ? hi[r] : hi_c;
+  float u = rands ? rands[t] : 0.5f;
+  dists[t] = (u + (float)i) / (float)N * (b - a) + a;
+}
+
 // ---------------------------------------------------------------------------
 // inverse-CDF sampling (shared by na_sample_pdf and na_hier_resample)
 // cdf: LDS row of N floats (cdf[0] = 0, cdf[i] = prefix of pdf[0..i-1])
@@ -767,6 +852,17 @@ rgb_input_fwd_kernel(const float* __restrict__ x, const float* __restrict__ ray,
   for (int64_t c = col0 + 22; c < ld; ++c) scalar_io<TO>::store(out + p * ld + c, 0.0f);
 }
 
+// Standalone SH encoding of unit directions: the first (levels+1)^2 basis
+// values per direction (degree <= 3), the same polynomials the colour input uses.
+__global__ void __launch_bounds__(256)
+sh_encode_kernel(const float* __restrict__ dirs, int64_t n, int nb, float* __restrict__ out) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float v[16];
+  sh3(dirs[p * 3], dirs[p * 3 + 1], dirs[p * 3 + 2], v);
+  for (int k = 0; k < nb; ++k) out[p * nb + k] = v[k];
+}
+
 template <typename TD>
 __global__ void __launch_bounds__(256)
 rgb_input_bwd_kernel(const float* __restrict__ grad, int normalize, int64_t n, int col0,
@@ -902,6 +998,57 @@ extern "C" int na_ray_bounds_sample(const float* center, const float* ray_unit, 
   NA_CHECK_ARG(center && ray_unit && near && far && outside && dists, "null pointer");
   hipLaunchKernelGGL(ray_bounds_sample_kernel, grid1d(R * N), dim3(256), 0, (hipStream_t)stream,
                      center, ray_unit, rands, R, N, near, far, outside, dists);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_sphere_crossings(const float* center, const float* ray_unit, int64_t R,
+                                   float radius, float* near, float* far, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(center && ray_unit && near && far, "null pointer");
+  hipLaunchKernelGGL(sphere_crossings_kernel, grid1d(R), dim3(256), 0, (hipStream_t)stream,
+                     center, ray_unit, R, radius * radius, near, far);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_camera_rays(const float* pose, const float* intr, int64_t B, int64_t H,
+                              int64_t W, const int64_t* ray_idx, int64_t R, float* center,
+                              float* ray, float* ray_unit, na_stream_t stream) {
+  NA_CHECK_ARG(B >= 0 && H >= 1 && W >= 1 && R >= 0, "bad sizes");
+  NA_CHECK_ARG(ray_idx || R == H * W, "without ray_idx every pixel is a ray (R == H*W)");
+  if (B == 0 || R == 0) return NA_OK;
+  NA_CHECK_ARG(pose && intr && center && ray, "null pointer");
+  hipLaunchKernelGGL(camera_rays_kernel, grid1d(B * R), dim3(256), 0, (hipStream_t)stream, pose,
+                     intr, B, W, ray_idx, R, center, ray, ray_unit);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_sample_dists(const float* lo, const float* hi, float lo_const, float hi_const,
+                               const float* rands, int64_t R, int32_t N, float* dists,
+                               na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && N >= 1, "bad sizes");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(dists, "null pointer");
+  hipLaunchKernelGGL(sample_dists_kernel, grid1d(R * N), dim3(256), 0, (hipStream_t)stream, lo, hi,
+                     lo_const, hi_const, rands, R, N, dists);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_sh_encode(const float* dirs, int64_t n, int32_t levels, float* out,
+                            na_stream_t stream) {
+  NA_CHECK_ARG(n >= 0, "bad sizes");
+  if (levels < 0 || levels > 3) {
+    set_error("na_sh_encode: degree must be 0..3");
+    return NA_ERR_UNSUPPORTED;
+  }
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(dirs && out, "null pointer");
+  hipLaunchKernelGGL(sh_encode_kernel, grid1d(n), dim3(256), 0, (hipStream_t)stream, dirs, n,
+                     (levels + 1) * (levels + 1), out);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -30,3 +30,12 @@ def sdf_losses(gradients, hessians, outside):
                               hessians.reshape(-1, 3) if hessians is not None else None,
                               outside.reshape(-1) if outside is not None else None,
                               _n_per_ray(gradients, outside))
+
+
+def to_full_image(image, image_size=None, from_vec=True):
+    """misc.py:110-117: [B,HW,...,K] (from_vec) or [B,H,W,...,K] -> [B,K,H,W,...]."""
+    if from_vec:
+        if image_size is None:
+            raise ValueError("image_size is needed to unflatten the pixel axis")
+        image = image.unflatten(1, tuple(image_size))
+    return image.movedim(-1, 1)

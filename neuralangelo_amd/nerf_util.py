@@ -6,17 +6,37 @@ import torch
 from . import _lib
 
 
+def _per_ray(v, batch_size, num_rays, device):
+    """A distance bound as (pointer tensor or None, constant): tensors broadcast to [B*R]."""
+    if not torch.is_tensor(v):
+        return None, float(v)
+    t = v.to(device=device, dtype=torch.float32)
+    while t.dim() > 2 and t.shape[-1] == 1:  # [B,R,1,1] -> [B,R]
+        t = t.squeeze(-1)
+    return torch.broadcast_to(t, (batch_size, num_rays)).contiguous().view(-1), 0.0
+
+
 def sample_dists(ray_size, dist_range, intvs, stratified, device="cuda", rands=None):
-    """[B,R,intvs,1] distances (nerf_util.py:20-38); `rands` optionally supplies the u's."""
+    """[B,R,intvs,1] distances (nerf_util.py:20-38) from ``na_sample_dists``.
+
+    dist_range = (dist_min, dist_max), each a number or a tensor broadcastable to
+    [B,R,1,1]; `rands` optionally supplies the stratification offsets [B,R,intvs(,1)]."""
     batch_size, num_rays = ray_size
     dist_min, dist_max = dist_range
-    if rands is None:
-        if stratified:
-            rands = torch.rand(batch_size, num_rays, intvs, 1, device=device)
-        else:
-            rands = torch.empty(batch_size, num_rays, intvs, 1, device=device).fill_(0.5)
-    rands = rands + torch.arange(intvs, dtype=torch.float, device=device)[None, None, :, None]
-    return rands / intvs * (dist_max - dist_min) + dist_min
+    dev = torch.device(device)
+    if dev.type == "cuda" and dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    lo, lo_c = _per_ray(dist_min, batch_size, num_rays, dev)
+    hi, hi_c = _per_ray(dist_max, batch_size, num_rays, dev)
+    R = batch_size * num_rays
+    if rands is None and stratified:
+        rands = torch.rand(batch_size, num_rays, intvs, 1, device=dev)
+    r = rands.reshape(R, intvs).contiguous().float() if rands is not None else None
+    out = torch.empty(R, intvs, device=dev)
+    _lib.require_cuda(out)
+    _lib.call("na_sample_dists", _lib.ptr(lo), _lib.ptr(hi), lo_c, hi_c, _lib.ptr(r), R,
+              int(intvs), _lib.ptr(out), _lib.stream_of(out))
+    return out.view(batch_size, num_rays, intvs, 1)
 
 
 def sample_dists_from_pdf(bin, weights, intvs_fine):
@@ -36,12 +56,18 @@ def sample_dists_from_pdf(bin, weights, intvs_fine):
 
 
 def intersect_with_sphere(center, ray_unit, radius=1.0):
-    ctc = (center * center).sum(dim=-1, keepdim=True)
-    ctv = (center * ray_unit).sum(dim=-1, keepdim=True)
-    b2_minus_4ac = ctv ** 2 - (ctc - radius ** 2)
-    dist_near = -ctv - b2_minus_4ac.sqrt()
-    dist_far = -ctv + b2_minus_4ac.sqrt()
-    return dist_near, dist_far
+    """Sphere crossings (nerf_util.py:199-205) from ``na_sphere_crossings``:
+    center, ray_unit [..., 3] -> dist_near, dist_far [..., 1] (NaN where a ray misses)."""
+    _lib.require_cuda(center, ray_unit)
+    c = center.reshape(-1, 3).contiguous().float()
+    v = torch.broadcast_to(ray_unit, center.shape).reshape(-1, 3).contiguous().float()
+    R = c.shape[0]
+    near = torch.empty(R, device=c.device)
+    far = torch.empty(R, device=c.device)
+    _lib.call("na_sphere_crossings", _lib.ptr(c), _lib.ptr(v), R, float(radius), _lib.ptr(near),
+              _lib.ptr(far), _lib.stream_of(c))
+    lead = center.shape[:-1]
+    return near.view(*lead, 1), far.view(*lead, 1)
 
 
 def bounds_and_coarse(center, ray_unit, n_coarse, rands=None):

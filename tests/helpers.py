@@ -42,3 +42,26 @@ def randomize_model(model, seed=0, table_scale=0.05):
         l0.weight_g.copy_(v.norm(dim=1, keepdim=True))
         for lin in model.neural_sdf.mlp.linears:
             lin.bias.copy_(torch.randn(lin.bias.shape, generator=g) * 0.01)
+
+
+def sphere_cameras(n, H=32, W=32, fov_deg=50.0, radius=3.0, seed=0):
+    """Look-at cameras uniform on a sphere of `radius` aimed at the origin, in the
+    reference's world-to-camera convention (x right, y down, z forward):
+    pose [n,3,4] = [R | t] with t = -R c, intr [n,3,3] pinhole with the principal
+    point at the image centre and focal length from the horizontal field of view."""
+    import math
+    g = torch.Generator().manual_seed(seed)
+    f = 0.5 * W / math.tan(math.radians(fov_deg) / 2)
+    poses, intrs = [], []
+    for _ in range(n):
+        c = torch.randn(3, generator=g, dtype=torch.float64)
+        c = c / c.norm() * radius
+        fwd = -c / c.norm()
+        right = torch.linalg.cross(fwd, torch.tensor([0.0, 0.0, 1.0], dtype=torch.float64))
+        right = right / right.norm()
+        down = torch.linalg.cross(fwd, right)
+        R = torch.stack([right, down, fwd], dim=0)   # rows: camera axes in world coords
+        poses.append(torch.cat([R, (-R @ c)[:, None]], dim=1))
+        intrs.append(torch.tensor([[f, 0.0, W / 2], [0.0, f, H / 2], [0.0, 0.0, 1.0]],
+                                  dtype=torch.float64))
+    return torch.stack(poses).float(), torch.stack(intrs).float()

@@ -38,10 +38,11 @@ def test_sampling_matches_oracle(setup, cuda):
     c, v = _rays(R, 2)
     g = torch.Generator().manual_seed(3)
     u = torch.rand(R, 64, generator=g)
-    with torch.no_grad():
-        near, far, outside, coarse = __import__("neuralangelo_amd.nerf_util", fromlist=["x"]) \
-            .bounds_and_coarse(c.to(cuda), v.to(cuda), 64, u.to(cuda))
-        dists = m.sample_dists_all(c.to(cuda), v.to(cuda), coarse).cpu()
+    near, far, outside = m.get_dist_bounds(c.to(cuda), v.to(cuda))
+    # the reference signature (model.py:233): center, ray_unit, near, far, stratified
+    dists = m.sample_dists_all(c.to(cuda), v.to(cuda), near, far, stratified=True,
+                               rands=u.to(cuda))[..., 0].cpu()
+    near, far = near[:, 0], far[:, 0]
     P = oracle_params(m, torch.float32)
     fld, _ = oracle_field(m, P, dtype=torch.float32)
     on, of, oo = orr.get_dist_bounds(c, v)

@@ -3,6 +3,7 @@ library routines (CPU only)."""
 import math
 
 import numpy as np
+import pytest
 import torch
 
 from oracle import render as orr
@@ -198,3 +199,102 @@ def test_stable_merge_keeps_every_sample_in_order():
             merged.remove(x)
     # fine samples concentrate around the sphere hit (d = 2.5 along these rays)
     assert ((d - 2.5).abs() < 0.1).sum(-1).min() >= 16
+
+
+def _plane_field(seed=0, hidden=256, L=4, F=2, b=(0.3, -1.2, 2.0)):
+    """A field whose SDF is the plane s(x) = a.x + c0 and whose colour is the constant
+    sigmoid(b): zero hash table; first-layer biases of 2 with |W0 x| < 1 inside the
+    unit ball, so beta z > 20 and the thresholded softplus is the identity; colour
+    layers with weight_g = 0 (zero weights) and last bias b."""
+    from oracle import field as of
+    from oracle import hashgrid as hg
+    g = torch.Generator().manual_seed(seed)
+    lt = hg.level_table(L, F, 10, 4, 1.5)
+    n_enc = L * F
+    W0 = torch.zeros(hidden, 3 + n_enc, dtype=D)
+    W0[:, :3] = (torch.rand(hidden, 3, generator=g, dtype=D) * 2 - 1) * 0.3
+    W0[:, 3:] = torch.randn(hidden, n_enc, generator=g, dtype=D) * 0.1
+    b0 = torch.full((hidden,), 2.0, dtype=D)
+    ws = torch.randn(1, hidden, generator=g, dtype=D) * 0.2
+    a = (ws @ W0[:, :3])[0]
+    c0 = -0.15
+    bs = torch.tensor([c0], dtype=D) - (ws @ b0)
+    t = dict(W0v=W0, W0g=W0.norm(dim=1, keepdim=True), b0=b0,
+             W1v=torch.randn(hidden, hidden, generator=g, dtype=D),
+             W1g=torch.ones(hidden, 1, dtype=D), b1=torch.zeros(hidden, dtype=D), ws=ws, bs=bs)
+    dims = [3 + 16 + 3 + hidden] + [hidden] * 4 + [3]
+    for i in range(5):
+        t[f"R{i}v"] = torch.randn(dims[i + 1], dims[i], generator=g, dtype=D)
+        t[f"R{i}g"] = torch.zeros(dims[i + 1], 1, dtype=D)
+        t[f"R{i}b"] = torch.zeros(dims[i + 1], dtype=D)
+    t["R4b"] = torch.tensor(b, dtype=D)
+    P = of.FieldParams(t)
+    table = torch.zeros(lt["n_params"], dtype=D)
+    return of.SDFField(lt, table, P, dtype=D), P, a, c0, torch.tensor(b, dtype=D)
+
+
+@pytest.mark.parametrize("training", [False, True])
+def test_render_rays_closed_form_on_a_plane(training):
+    """The composed oracle render (model.py:124-190) on a plane SDF with a constant colour
+    network, against closed forms that use only the plane, near/far and the NeuS ratio:
+    4-tap gradient = a, Hessian = 0, sdfs = a.(c + d v) + c0 (1000 outside).  With the
+    64 uniform midpoint samples (spacing h, no resampling) and progress 1, a ray heading
+    into the plane has 1 - alpha_i = (Phi(s(d_i + h/2)) + e) / (Phi(s(d_i - h/2)) + e)
+    (Phi = sigmoid(inv_s .), e = 1e-5), so the transmittance telescopes up to the last
+    section, which ends at far (half a spacing after the last sample):
+    T = (Phi(s(far - h)) + e) / (Phi(s(near)) + e) * (Phi(s(far - h/4)) + e) /
+    (Phi(s(far - 3h/4)) + e); a ray heading away keeps T = 1.  Then rgb = sigmoid(b)(1 - T),
+    the composited gradient is a (1 - T) and depth = sum_i d_i w_i with the telescoped
+    weights w_i = (Phi(s(d_i - h/2)) - Phi(s(d_i + h/2))) / (Phi(s(near)) + e) before the
+    last sample."""
+    fld, P, a, c0, b = _plane_field()
+    g = torch.Generator().manual_seed(3)
+    R, N = 400, 64
+    c = torch.randn(R, 3, generator=g, dtype=D)
+    c = c / c.norm(dim=-1, keepdim=True) * 2.5
+    v = -c / 2.5 + 0.45 * torch.randn(R, 3, generator=g, dtype=D)
+    v = v / v.norm(dim=-1, keepdim=True)
+    inv_s = 30.0
+    s_var = torch.tensor(math.log(inv_s), dtype=D)
+    o = orr.render_rays(fld, P, c, v, s_var, 1.0, 1.0 / 64, training, n_coarse=N, n_hier=0)
+    near, far, outside = orr.get_dist_bounds(c, v)
+    inside = ~outside[:, 0]
+    assert inside.any() and outside.any()
+    # SDF values, gradients and Hessian
+    pts = c[:, None, :] + v[:, None, :] * o["dists"]
+    plane = (pts * a).sum(-1) + c0
+    assert torch.allclose(o["sdfs"][inside], plane[inside], rtol=0, atol=1e-12)
+    assert (o["sdfs"][~inside] == 1000.0).all()
+    assert torch.allclose(o["gradients"], a.expand_as(o["gradients"]), rtol=0, atol=1e-9)
+    if training:
+        # inside rays: 0; outside rays see the filled value (model.py:163, :166) in the
+        # centre term, as in the reference
+        assert o["hessians"][inside].abs().max().item() < 1e-6
+        assert (o["hessians"][~inside].abs() > 1e6).all()
+    # closed-form transmittance
+    h = (far - near)[:, 0] / N
+    sig = lambda x: torch.sigmoid(inv_s * x)  # noqa: E731
+    s_of = lambda d: (c * a).sum(-1) + d * (v * a).sum(-1) + c0  # noqa: E731
+    cos = (v * a).sum(-1)
+    fr = far[:, 0]
+    T = ((sig(s_of(fr - h)) + 1e-5) / (sig(s_of(near[:, 0])) + 1e-5)
+         * (sig(s_of(fr - h / 4)) + 1e-5) / (sig(s_of(fr - 3 * h / 4)) + 1e-5))
+    T = torch.where(cos < 0, T, torch.ones_like(T))
+    opacity = 1 - T
+    w = o["weights"][..., 0]
+    assert torch.allclose(w.sum(-1)[inside], opacity[inside], rtol=0, atol=1e-9)
+    rgb = torch.sigmoid(b) * opacity[:, None]
+    assert torch.allclose(o["rgb"][inside], rgb[inside], rtol=0, atol=1e-9)
+    assert (cos[inside] < 0).any() and (cos[inside] > 0).any()
+    if not training:
+        assert torch.allclose(o["opacity"][inside, 0], opacity[inside], rtol=0, atol=1e-9)
+        assert torch.allclose(o["gradient"][inside], a * opacity[inside, None], rtol=0, atol=1e-9)
+        d = o["dists"][..., 0]
+        p0 = sig(s_of(near[:, 0])) + 1e-5
+        s2 = lambda t: (c * a).sum(-1)[:, None] + t * cos[:, None] + c0  # noqa: E731
+        wi = (sig(s2(d - h[:, None] / 2)) - sig(s2(d + h[:, None] / 2))) / p0[:, None]
+        pl, nl = sig(s_of(fr - 3 * h / 4)), sig(s_of(fr - h / 4))
+        wi[:, -1] = (pl - nl) / (pl + 1e-5) * (sig(s_of(fr - h)) + 1e-5) / p0
+        depth = (d * wi).sum(-1)
+        sel = inside & (cos < 0)
+        assert torch.allclose(o["depth"][sel, 0], depth[sel], rtol=0, atol=1e-9)

@@ -37,6 +37,57 @@ def test_bounds_and_coarse_samples(cuda):
     assert torch.equal(d05.cpu(), orr.sample_dists(near.cpu()[:, None], far.cpu()[:, None], N)[..., 0])
 
 
+def test_sphere_crossings_and_sample_dists(cuda):
+    """intersect_with_sphere (nerf_util.py:199-205) and sample_dists (nerf_util.py:20-38)
+    through their reference signatures, against the oracle's restatement."""
+    B, R, N = 2, 700, 48
+    c, v = _rays(B * R, 4, spread=0.6)
+    c, v = c.view(B, R, 3), v.view(B, R, 3)
+    near, far = nerf_util.intersect_with_sphere(c.to(cuda), v.to(cuda), radius=1.0)
+    on, of = orr.intersect_with_sphere(c, v, 1.0)
+    assert near.shape == (B, R, 1) and far.shape == (B, R, 1)
+    miss = on.isnan()
+    assert miss.any() and (~miss).any()
+    assert torch.equal(near.cpu().isnan(), miss) and torch.equal(far.cpu().isnan(), of.isnan())
+    ok = ~miss
+    assert torch.allclose(near.cpu()[ok], on[ok], rtol=1e-6, atol=1e-6)
+    assert torch.allclose(far.cpu()[ok], of[ok], rtol=1e-6, atol=1e-6)
+    # closed form: a ray through the centre from distance 3 crosses at 2 and 4
+    n1, f1 = nerf_util.intersect_with_sphere(torch.tensor([[0., 0., 3.]], device=cuda),
+                                             torch.tensor([[0., 0., -1.]], device=cuda))
+    assert n1.item() == 2.0 and f1.item() == 4.0
+    # per-ray tensor bounds [B,R,1,1], stratified offsets: same op order as the oracle
+    lo = torch.rand(B, R, 1, 1, generator=torch.Generator().manual_seed(5))
+    hi = lo + 1.5
+    u = torch.rand(B, R, N, 1, generator=torch.Generator().manual_seed(6))
+    got = nerf_util.sample_dists((B, R), (lo.to(cuda), hi.to(cuda)), N, True, rands=u.to(cuda))
+    ref = orr.sample_dists(lo[..., 0], hi[..., 0], N, u[..., 0])
+    assert got.shape == (B, R, N, 1) and torch.equal(got.cpu(), ref)
+    # scalar bounds (the background's inverse distances, model.py:269-273), midpoints
+    got = nerf_util.sample_dists((B, R), (1, 0), 32, False).cpu()
+    ref = orr.sample_dists(torch.ones(B, R, 1), torch.zeros(B, R, 1), 32)
+    assert torch.equal(got, ref)
+    assert torch.allclose(got[0, 0, :, 0], 1 - (torch.arange(32) + 0.5) / 32, atol=1e-7)
+
+
+def test_spherical_harmonics_matches_oracle(cuda):
+    """get_spherical_harmonics (spherical_harmonics.py:47-71) on the GPU vs the oracle's
+    closed-form basis, for every degree 0..3 (prefix of the degree-3 basis)."""
+    from neuralangelo_amd.spherical_harmonics import get_spherical_harmonics
+    from oracle import field as of
+    g = torch.Generator().manual_seed(7)
+    d = torch.randn(3, 500, 3, generator=g)
+    d = d / d.norm(dim=-1, keepdim=True)
+    ref = of.spherical_harmonics3(d.double())
+    for levels in range(4):
+        got = get_spherical_harmonics(d.to(cuda), levels)
+        nb = (levels + 1) ** 2
+        assert got.shape == (3, 500, nb) and got.dtype == torch.float32
+        assert torch.allclose(got.cpu().double(), ref[..., :nb], rtol=0, atol=2e-6)
+    with pytest.raises(NotImplementedError):
+        get_spherical_harmonics(d.to(cuda), 4)
+
+
 def _weights_cases(R, N, seed):
     g = torch.Generator().manual_seed(seed)
     w = torch.rand(R, N - 1, generator=g) ** 4
