@@ -15,7 +15,11 @@ same names and shapes as this package's Model.  Files are read with
 import torch
 
 _WRAPPER_PREFIXES = ("module.",)
-_UNSUPPORTED = ("background_nerf.", "appear_embed.", "averaged_model.")
+_UNSUPPORTED = ("background_nerf.", "appear_embed.")
+# the EMA wrapper (imaginaire/models/utils/model_average.py:53-72) keeps the averaged
+# copy under "_averaged_model." and a step counter buffer next to the live network
+_EMA_PREFIX = "_averaged_model."
+_EMA_COUNTER = "num_updates_tracked"
 
 
 def _strip(key):
@@ -29,17 +33,24 @@ def _strip(key):
     return key
 
 
-def network_state(state, known=()):
+def network_state(state, known=(), averaged=False):
     """Reference checkpoint dict (or its ``model`` entry) -> network keys without the
     wrapper prefixes, and the list of keys this package does not model (background
-    NeRF weights when the model has no background, appearance embedding, EMA
-    copies).  known: network keys of the target model, which are always kept."""
+    NeRF weights when the model has no background, appearance embedding, the EMA
+    copy and its counter).  known: network keys of the target model, which are always
+    kept.  averaged: take the network from the EMA copy instead of the live weights."""
     if "model" in state and isinstance(state["model"], dict):
         state = state["model"]
     known = set(known)
     out, skipped = {}, []
     for k, v in state.items():
         name = _strip(k)
+        ema = name.startswith(_EMA_PREFIX)
+        if ema:
+            name = _strip(name[len(_EMA_PREFIX):])
+        if name == _EMA_COUNTER or ema != averaged:
+            skipped.append(k)
+            continue
         if name not in known and any(name.startswith(u) for u in _UNSUPPORTED):
             skipped.append(k)
             continue
@@ -47,16 +58,17 @@ def network_state(state, known=()):
     return out, skipped
 
 
-def load_reference_checkpoint(model, path_or_state, strict=True):
+def load_reference_checkpoint(model, path_or_state, strict=True, averaged=False):
     """Load a reference checkpoint (path or already-loaded dict) into ``model``.
     Returns (iteration or None, skipped keys).  strict: every network key of the model
-    must be present, with its shape (a table of another size raises)."""
+    must be present, with its shape (a table of another size raises).  averaged: load
+    the EMA copy of an EMA-wrapped checkpoint instead of the live weights."""
     state = path_or_state
     if not isinstance(state, dict):
         state = torch.load(path_or_state, map_location="cpu", weights_only=True)
     iteration = state.get("iteration") if isinstance(state.get("iteration"), int) else None
     own = model.state_dict()
-    net, skipped = network_state(state, known=own.keys())
+    net, skipped = network_state(state, known=own.keys(), averaged=averaged)
     missing = [k for k in own if k not in net]
     unexpected = [k for k in net if k not in own]
     if strict and (missing or unexpected):

@@ -932,8 +932,26 @@ class BgAlpha(torch.autograd.Function):
 # ---------------------------------------------------------------------------
 # packed (ragged) rays: ray r owns samples offsets[r] .. offsets[r+1]-1
 # ---------------------------------------------------------------------------
-def packed_ray_index(offsets, S):
-    """offsets int32 [R+1] -> int32 [S], the ray of every packed sample."""
+def check_offsets(offsets, S):
+    """Validate packed-ray CSR offsets against the sample count S: int32 [R+1] on the
+    GPU, offsets[0] == 0, offsets[-1] == S, non-decreasing.  The kernels index samples
+    and rays through these values, so a violation raises here (one device-to-host
+    read of three flags)."""
+    if offsets.dtype != torch.int32 or offsets.dim() != 1 or offsets.shape[0] < 1:
+        raise RuntimeError("packed offsets must be a 1-D int32 tensor [R+1]")
+    ok = torch.stack([offsets[0] == 0, offsets[-1] == S,
+                      (offsets[1:] >= offsets[:-1]).all()]).cpu()
+    if not bool(ok.all()):
+        raise RuntimeError(f"packed offsets must start at 0, end at the sample count {S} and "
+                           f"not decrease (first/last/monotone checks: {ok.tolist()})")
+    _lib.require_cuda(offsets)
+
+
+def packed_ray_index(offsets, S, validate=True):
+    """offsets int32 [R+1] -> int32 [S], the ray of every packed sample.  validate:
+    check the offsets against S first (check_offsets)."""
+    if validate:
+        check_offsets(offsets, S)
     R = offsets.shape[0] - 1
     ray_idx = torch.empty(S, dtype=torch.int32, device=offsets.device)
     _lib.call("na_packed_ray_index", _lib.ptr(offsets), R, _lib.ptr(ray_idx), _s(offsets))

@@ -354,12 +354,13 @@ class Model(torch.nn.Module):
                                         self.s_var, float(progress), float(self.anneal_end),
                                         float(self.outside_val))
 
-    def render_rays_packed(self, center, ray_unit, dists, offsets):
+    def render_rays_packed(self, center, ray_unit, dists, offsets, validate=True):
         """render_rays_object (model.py:158-190) over packed rays with uneven sample
         counts: center, ray_unit [R,3]; dists [S] sorted within each ray; offsets int32
         [R+1].  Every kernel of the dense path runs on the packed samples; per-ray
         quantities are read through the sample's ray index.  Returns rgb [R,3] and the
-        per-sample tensors (gradients/hessians [S,1,3] and outside [S] for the losses)."""
+        per-sample tensors (gradients/hessians [S,1,3] and outside [S] for the losses).
+        validate: check the offsets against the sample count first (one host read)."""
         c = center.reshape(-1, 3).contiguous().float()
         v = ray_unit.reshape(-1, 3).contiguous().float()
         dists = dists.reshape(-1).contiguous().float()
@@ -368,7 +369,7 @@ class Model(torch.nn.Module):
             near, far, outside = self.get_dist_bounds(c, v)
         far = far.reshape(-1)
         outside = outside.reshape(-1)
-        ray_idx = Fn.packed_ray_index(offsets, S)
+        ray_idx = Fn.packed_ray_index(offsets, S, validate=validate)
         ridx = ray_idx.long()
         v_s = v.index_select(0, ridx)
         points = c.index_select(0, ridx) + v_s * dists[:, None]

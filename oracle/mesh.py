@@ -61,32 +61,44 @@ def marching_cubes(volume, isovalue=0.0):
     return V, F
 
 
-def lattice_marching_cubes(vol, block_res, isovalue=0.0, axes=None, intv=1.0, counts=False):
+def lattice_marching_cubes(vol, block_res, isovalue=0.0, axes=None, intv=1.0, counts=False,
+                           threads=1):
     """The reference's per-block loop (mesh.py:24-49, :82-96 slicing, :121-123) on a
     stored lattice volume, without the sphere filter: blocks of block_res+1
     points, x-major; each block's V (index coords, or V*intv + axes[block
     origin] when axes are given) and F + running vertex offset, concatenated.
-    counts: also return each block's vertex and face count (empty blocks included)."""
+    counts: also return each block's vertex and face count (empty blocks included).
+    threads: blocks extracted concurrently (the C routine keeps no shared state);
+    the concatenation keeps the sequential block order."""
     vol = np.asarray(vol, np.float32)
     n = vol.shape
     nb = [int(np.ceil(d / block_res)) for d in n]
+    origins = [(bx * block_res, by * block_res, bz * block_res)
+               for bx in range(nb[0]) for by in range(nb[1]) for bz in range(nb[2])]
+
+    def one(o):
+        xi, yi, zi = o
+        blk = vol[xi:xi + block_res + 1, yi:yi + block_res + 1, zi:zi + block_res + 1]
+        return marching_cubes(blk, isovalue)
+
+    if threads > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            results = list(ex.map(one, origins))
+    else:
+        results = [one(o) for o in origins]
     Vs, Fs, off = [], [], 0
     cv, cf = [], []
-    for bx in range(nb[0]):
-        for by in range(nb[1]):
-            for bz in range(nb[2]):
-                xi, yi, zi = bx * block_res, by * block_res, bz * block_res
-                blk = vol[xi:xi + block_res + 1, yi:yi + block_res + 1, zi:zi + block_res + 1]
-                V, F = marching_cubes(blk, isovalue)
-                cv.append(V.shape[0])
-                cf.append(F.shape[0])
-                if V.shape[0] == 0:
-                    continue
-                if axes is not None:
-                    V = V * intv + np.array([axes[0][xi], axes[1][yi], axes[2][zi]], np.float32)
-                Vs.append(V)
-                Fs.append(F + off)
-                off += V.shape[0]
+    for (xi, yi, zi), (V, F) in zip(origins, results):
+        cv.append(V.shape[0])
+        cf.append(F.shape[0])
+        if V.shape[0] == 0:
+            continue
+        if axes is not None:
+            V = V * intv + np.array([axes[0][xi], axes[1][yi], axes[2][zi]], np.float32)
+        Vs.append(V)
+        Fs.append(F + off)
+        off += V.shape[0]
     if not Vs:
         out = np.zeros((0, 3)), np.zeros((0, 3), np.int64)
     else:

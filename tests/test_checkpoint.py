@@ -75,3 +75,25 @@ def test_background_weights_load_when_the_model_has_them(tmp_path):
     assert tuple(dst.state_dict()[k].shape) == (256, 340)
     for name, v in src.state_dict().items():
         assert torch.equal(dst.state_dict()[name], v), name
+
+
+def test_ema_wrapped_checkpoint():
+    """An EMA-wrapped reference checkpoint (model_average.py:53-72): live weights under
+    module.module.*, the averaged copy under module._averaged_model.*, and the
+    module.num_updates_tracked counter.  The live weights load by default, the averaged
+    copy with averaged=True; the counter is skipped either way."""
+    live, avg, dst = _model(5), _model(6), _model(7)
+    state = {"model": {}, "iteration": 1234}
+    for k, v in live.state_dict().items():
+        state["model"]["module.module." + k] = v.clone()
+    for k, v in avg.state_dict().items():
+        state["model"]["module._averaged_model." + k] = v.clone()
+    state["model"]["module.num_updates_tracked"] = torch.tensor(99)
+    it, skipped = load_reference_checkpoint(dst, state)
+    assert it == 1234 and "module.num_updates_tracked" in skipped
+    for k, v in dst.state_dict().items():
+        assert torch.equal(v, live.state_dict()[k]), k
+    _, skipped = load_reference_checkpoint(dst, state, averaged=True)
+    for k, v in dst.state_dict().items():
+        assert torch.equal(v, avg.state_dict()[k]), k
+    assert all("_averaged_model." not in k for k in skipped if k != "module.num_updates_tracked")

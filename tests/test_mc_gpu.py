@@ -195,3 +195,36 @@ def test_extract_texture_formula():
         ref = (m.neural_rgb.forward(x, n, -n, feat, None).cpu().numpy() * 255).astype(np.uint8)
     assert C.shape == (5000, 3) and C.dtype == np.uint8
     assert np.abs(C.astype(int) - ref.astype(int)).max() <= 1
+
+
+def test_full_lattice_1024_network_sdf():
+    """Mesh extraction at the 1024^3 configuration: the FP32 2^22 x 16 x 8 network SDF
+    (seeded initialisation: a sphere of radius 0.5 under a small hash-table field) over
+    the [-1,1)^3 lattice in 128^3 blocks; vertices and faces of every block bit-exact
+    against the oracle's block loop on the same stored volume (world coordinates)."""
+    from neuralangelo_amd.model import Model, default_config
+    dev = torch.device("cuda", 0)
+    torch.manual_seed(0)
+    m = Model(default_config(log2_T=22, F=8), compute_dtype=torch.float32).to(dev)
+    m.neural_sdf.active_levels = m.neural_sdf.anneal_levels = 16
+    res, block_res = 1024, 128
+    intv = 2.0 / res
+    axes = gmesh.lattice_axes(((-1.0, 1.0),) * 3, intv)
+    assert [len(a) for a in axes] == [res] * 3
+    with torch.no_grad():
+        vol = gmesh.lattice_sdf(lambda x: -m.neural_sdf.sdf_nograd(x), axes, dev,
+                                max_points=1 << 23)
+    del m
+    V, F = mcubes.lattice_marching_cubes(vol, block_res, 0.0, axes=[a.to(dev) for a in axes],
+                                         intv=intv)
+    V, F = V.cpu().numpy(), F.cpu().numpy()
+    host = vol.cpu().numpy()
+    del vol
+    torch.cuda.empty_cache()
+    print(f"1024^3 lattice: {V.shape[0]} vertices, {F.shape[0]} faces", flush=True)
+    Vr, Fr = omesh.lattice_marching_cubes(host, block_res, 0.0, [a.numpy() for a in axes], intv,
+                                          threads=16)
+    assert V.shape == Vr.shape and F.shape == Fr.shape
+    assert np.array_equal(F, Fr)
+    assert np.array_equal(V, Vr)
+    assert F.shape[0] > 1_000_000

@@ -163,3 +163,13 @@ def test_lattice_loop_equals_blockwise_extraction():
     V3, F3 = omesh.lattice_marching_cubes(vol, 1000)
     V4, F4 = omesh.marching_cubes(vol)
     assert np.array_equal(V3, V4) and np.array_equal(F3, F4)
+
+
+def test_block_loop_threads_keep_the_sequential_order():
+    """Blocks extracted concurrently concatenate exactly as the sequential loop."""
+    rng = np.random.default_rng(5)
+    vol = rng.normal(size=(41, 30, 37)).astype(np.float32)
+    V1, F1, c1, f1 = omesh.lattice_marching_cubes(vol, 8, counts=True)
+    V4, F4, c4, f4 = omesh.lattice_marching_cubes(vol, 8, counts=True, threads=4)
+    assert np.array_equal(V1, V4) and np.array_equal(F1, F4)
+    assert np.array_equal(c1, c4) and np.array_equal(f1, f4)
