@@ -167,6 +167,24 @@ typedef struct na_gemm_args {
 
 int na_gemm(const na_gemm_args* args, na_stream_t stream);
 
+/* Maximum partial-product floats of na_gemm_wgrad_static (80 MiB per device):
+ * every weight gradient of the train step fits (the largest, the 280-wide colour
+ * input layer over 256 row splits, needs 18.35 M floats). */
+#define NA_WGRAD_STATIC_MAX_FLOATS 20971520
+
+/* No counterpart in the reference.  na_gemm's deterministic weight-gradient path
+ * (the split-ordered partial-product reduction that a wg_partials workspace
+ * selects) without any allocation or workspace argument: the partial products
+ * live in a file-scope device array of NA_WGRAD_STATIC_MAX_FLOATS floats, one per
+ * device, held until the process ends.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments as na_gemm for a weight-gradient product (bf16, trans_a = trans_b = 1,
+ * NA_EPI_ATOMIC, fp32 C, no epilogue terms); args->wg_partials is ignored.
+ * Returns NA_ERR_ARGUMENT for other products or when the plan needs more than
+ * NA_WGRAD_STATIC_MAX_FLOATS floats.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (concurrent streams or threads) race. */
+int na_gemm_wgrad_static(const na_gemm_args* args, na_stream_t stream);
+
 /* Floats of wg_partials that na_gemm's weight-gradient path uses for these
  * arguments (0 when another path runs them; NA_ERR_ARGUMENT for a NULL args). */
 int64_t na_gemm_wgrad_workspace(const na_gemm_args* args);

@@ -65,6 +65,7 @@ class GemmArgs(ctypes.Structure):
 
 
 NA_ADAMW_MULTI_MAX = 32
+NA_WGRAD_STATIC_MAX_FLOATS = 20971520
 
 
 class AdamwTensor(ctypes.Structure):
@@ -92,6 +93,7 @@ _SIGNATURES = {
     "na_hashgrid_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _P]),
     "na_gemm": (ctypes.c_int, [_PGEMM, _P]),
     "na_gemm_wgrad_workspace": (ctypes.c_int64, [_PGEMM]),
+    "na_gemm_wgrad_static": (ctypes.c_int, [_PGEMM, _P]),
     "na_colsum": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P]),
     "na_head_sum": (ctypes.c_int, [_P, _I32, _I64, _P, _P, _P]),
     "na_linear_narrow_fwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P, _I32, _I32, _P, _P]),

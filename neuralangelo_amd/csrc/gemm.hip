@@ -15,6 +15,8 @@
 // small kernels.
 #include "common.h"
 
+#include <mutex>
+
 namespace na {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
@@ -2462,6 +2464,50 @@ act_grad_vec_kernel(const TD* __restrict__ dy, int64_t ld_dy, const TY* __restri
 }  // namespace na
 
 using namespace na;
+
+// Partial products of na_gemm_wgrad_static: a file-scope device array (one copy
+// per device, held until the process ends).  Its name is a host-side stand-in;
+// kernels receive the device address from hipGetSymbolAddress, looked up once per
+// device with that device current.
+static __device__ __align__(256) float scratch[NA_WGRAD_STATIC_MAX_FLOATS];
+static float* g_scratch_ptr[64];
+static std::mutex g_scratch_mu;
+
+static float* scratch_address(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    float* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error(std::string("na_gemm_wgrad_static: hipGetSymbolAddress failed: ") +
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+extern "C" int na_gemm_wgrad_static(const na_gemm_args* a, na_stream_t stream) {
+  NA_CHECK_ARG(a != nullptr, "args is null");
+  NA_CHECK_ARG(wgrad_ok(a), "weight-gradient products only (bf16, trans_a = trans_b = 1, atomic "
+                            "fp32 epilogue, no epilogue terms, K >= 1, rows addressable)");
+  if (a->M == 0 || a->N == 0) return NA_OK;
+  const WgradPlan w = wgrad_plan(a);
+  const int64_t need = w.splits * w.No * w.Ko;
+  NA_CHECK_ARG(need <= NA_WGRAD_STATIC_MAX_FLOATS,
+               "needs " + std::to_string(need) + " floats of partial products; the static "
+               "scratch holds " + std::to_string((int64_t)NA_WGRAD_STATIC_MAX_FLOATS));
+  int dev = 0;
+  NA_CHECK_HIP(hipGetDevice(&dev));
+  NA_CHECK_ARG(dev >= 0 && dev < 64, "device index out of range");
+  float* ptr = scratch_address(dev);
+  if (!ptr) return NA_ERR_HIP;
+  na_gemm_args b = *a;
+  b.wg_partials = ptr;
+  b.wg_partials_len = NA_WGRAD_STATIC_MAX_FLOATS;
+  return na_gemm(&b, stream);
+}
 
 extern "C" int64_t na_gemm_wgrad_workspace(const na_gemm_args* a) {
   NA_CHECK_ARG(a != nullptr, "args is null");

@@ -87,3 +87,19 @@ def test_reference_resolutions_for_epsilon():
     g = growth_rate(5, 11, 16)
     r = reference_resolutions(5, 16, g)
     assert r == [33, 43, 56, 74, 98, 128, 169, 223, 295, 389, 512, 676, 892, 1177, 1553, 2048]
+
+
+def test_wgrad_static_argument_checks_and_limit():
+    """na_gemm_wgrad_static returns before any GPU call for a NULL or non-weight-gradient
+    product; its size limit is the header's constant."""
+    lib = _lib.load()
+    assert lib.na_gemm_wgrad_static(None, None) == _lib.NA_ERR_ARGUMENT
+    a = _lib.GemmArgs()
+    a.M, a.N, a.K = 256, 256, 4096
+    a.trans_a, a.trans_b, a.in_dtype = 1, 1, _lib.NA_BF16
+    a.epilogue, a.c_dtype = _lib.NA_EPI_STORE, _lib.NA_F32  # store epilogue: not this path
+    assert lib.na_gemm_wgrad_static(ctypes.byref(a), None) == _lib.NA_ERR_ARGUMENT
+    assert "weight-gradient" in _lib.last_error()
+    text = open(os.path.join(REPO, "include", "neuralangelo_amd.h")).read()
+    m = re.search(r"#define NA_WGRAD_STATIC_MAX_FLOATS (\d+)", text)
+    assert m and int(m.group(1)) == _lib.NA_WGRAD_STATIC_MAX_FLOATS
