@@ -167,6 +167,43 @@ typedef struct na_gemm_args {
 
 int na_gemm(const na_gemm_args* args, na_stream_t stream);
 
+/* Tap increments of the BF16 4-tap numerical gradient
+ * (projects/neuralangelo/utils/modules.py:157-175 with the SDF MLP of
+ * projects/neuralangelo/utils/mlp.py:55-69), on the weights-stationary kernel.
+ * Rows are point-major: row m is tap m % 4 of point m / 4 (M = 4 * points, a
+ * multiple of 128).  D [M, ld_d] bf16 holds the first layer's input increments
+ * X(x + eps k_t) - X(x) (na_sdf_taps_increment_input), W [N, ldw] bf16 the
+ * matching first-layer weights, zc [M/4, ld_zc] fp32 the centre pre-activations
+ * z = W0 X(x) + b0, ws [N] fp32 the SDF head; N = 128 or 256, K in 129..160 or
+ * 225..288.  With dz = D W^T and softplus sp (beta):
+ *   fwd: head_out[w * M + m] = sum over wave w's 32 columns n of
+ *        ws[n] (sp(z + dz) - sp(z)) (evaluated without cancellation; N / 32 parts);
+ *   bwd: with dds [M] the gradients of the four increments of each point, a = their
+ *        mean (the part every tap shares) and b_t = dds - a, and dsp'_t =
+ *        sp'(z + dz_t) - sp'(z):
+ *          dpre[m, n] = ws[n] (b_t sp'(z + dz_t) + a dsp'_t)  (bf16; pairs with D_t),
+ *          q[p, n] = a ws[n] sp'(z)                           (bf16; pairs with sum_t D_t),
+ *          centre[p, n] += sum_t dds_t ws[n] dsp'_t  (bf16 read-modify-write: centre
+ *                          holds the centre's own pre-activation gradient),
+ *          db0[n] += column sums of that addition, dws[n] += sum_m dds[m] (sp(z + dz) -
+ *          sp(z)).
+ *        Together these are the gradient of the four sp(z + dz_t) - sp(z) split into
+ *        terms of their natural size.  Increments and their derivatives are formed
+ *        in fp32 from z and dz, so the second-order sums (the Hessian trace and its
+ *        gradient) keep fp32 precision relative to their own size.  The backward
+ *        recomputes dz = D W^T (fp32) into work [M, N], a caller-owned 16-byte
+ *        aligned workspace of M * N floats, and then forms the terms elementwise;
+ *        dds [M] is 16-byte aligned (four taps per point). */
+int na_taps_increment_fwd(const void* D, int64_t ld_d, int64_t M, int64_t K, const void* W,
+                          int64_t ldw, int64_t N, const float* zc, int64_t ld_zc,
+                          const float* ws, float beta, float* head_out, na_stream_t stream);
+int na_taps_increment_bwd(const void* D, int64_t ld_d, int64_t M, int64_t K, const void* W,
+                          int64_t ldw, int64_t N, const float* zc, int64_t ld_zc,
+                          const float* ws, float beta, const float* dds, void* dpre,
+                          int64_t ld_dpre, void* centre, int64_t ld_centre, void* q,
+                          int64_t ld_q, float* db0, float* dws, float* work,
+                          na_stream_t stream);
+
 /* Maximum partial-product floats of na_gemm_wgrad_static (80 MiB per device):
  * every weight gradient of the train step fits (the largest, the 280-wide colour
  * input layer over 256 row splits, needs 18.35 M floats). */
@@ -256,24 +293,62 @@ int na_sdf_input_bwd(const na_grid_desc* desc, const float* points, int64_t n, f
  * tetrahedral taps x + eps*k_i, k = (1,-1,-1), (-1,-1,1), (-1,1,-1), (1,1,1)
  * (modules.py:157-175) -- five tcnn backward calls in the reference.
  * points: centre points [n,3] (world); dy_centre [n, ld] and dy_taps [4n, ld]
- * (tap-major) hold the encoder gradient in columns 0..L*F-1.  Contributions of
- * the five points to a shared corner are summed before one atomic add into
- * grad_table (accumulated; the caller zeroes it). */
+ * hold the encoder gradient in columns 0..L*F-1.  increments = 0: dy_taps is
+ * tap-major (row t*n + i) and each point's gradient is scattered with its own
+ * trilinear weights (dy_shared must be NULL).  increments = 1 (the increment
+ * formulation): dy_taps is point-major (row 4i + t) and is the gradient of the
+ * input increment D_t = X(x + eps k_t) - X(x), dy_shared [n, ld] (or NULL) the
+ * gradient of sum_t D_t, shared by the four taps, and dy_centre that of the
+ * centre input alone: tap t's corners receive g_t = dy_taps + dy_shared and the
+ * centre's dy_centre - sum_t g_t (formed in fp32).  Contributions of the five
+ * points to a shared corner are summed before one atomic add into grad_table
+ * (accumulated; the caller zeroes it). */
 int na_sdf_taps_bwd(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
                     float vmax, float eps, const void* dy_centre, const void* dy_taps,
-                    int32_t dy_dtype, int64_t ld, float* grad_table, int32_t active_levels,
-                    na_stream_t stream);
+                    const void* dy_shared, int32_t dy_dtype, int64_t ld, float* grad_table,
+                    int32_t active_levels, int32_t increments, na_stream_t stream);
+
+/* Deterministic (sorted) variants of the three table-gradient scatters above:
+ * na_hashgrid_bwd (tcnn's backward, modules.py:84-86), na_sdf_input_bwd and
+ * na_sdf_taps_bwd, same arguments plus a workspace.  The atomic scatters add
+ * each contribution as it arrives, so a table entry's float sum order, and its
+ * last bits, vary between runs (tcnn's atomic backward behaves the same way).
+ * Here, per level, contribution id = (i * E + e) * 8 + c (point i, evaluation e
+ * of E, corner c; E = 1, or 5 for the centre and four taps) is sorted stably by
+ * table entry, and each entry's contributions are summed in id order in fp32
+ * and added into grad_table: the result depends only on the inputs.  The
+ * workspace is caller-owned, 256-byte aligned, of na_hashgrid_sorted_workspace
+ * (desc, n, E) bytes (16 n E 8 bytes plus the radix sort's temporary storage);
+ * n * E * 8 must stay below 2^31.  Returns the size (>= 0) or an error code. */
+int64_t na_hashgrid_sorted_workspace(const na_grid_desc* desc, int64_t n, int32_t evals);
+int na_hashgrid_bwd_sorted(const na_grid_desc* desc, const float* x, int64_t n, const void* dy,
+                           int32_t dy_dtype, float* grad_table, int32_t active_levels,
+                           void* workspace, int64_t workspace_bytes, na_stream_t stream);
+int na_sdf_input_bwd_sorted(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
+                            float vmax, const void* dy, int32_t dy_dtype, int64_t ld,
+                            float* grad_table, int32_t active_levels, void* workspace,
+                            int64_t workspace_bytes, na_stream_t stream);
+int na_sdf_taps_bwd_sorted(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
+                           float vmax, float eps, const void* dy_centre, const void* dy_taps,
+                           const void* dy_shared, int32_t dy_dtype, int64_t ld,
+                           float* grad_table, int32_t active_levels, int32_t increments,
+                           void* workspace, int64_t workspace_bytes, na_stream_t stream);
 
 /* SDF network inputs of a numerical-gradient point set for the increment
- * formulation (bf16): x0_centre [n, ld] and x0_taps [4n, ld] (tap-major) as
- * na_sdf_input_fwd writes them for x and x + eps*k_i (k_i as na_sdf_taps_bwd),
- * and d_taps [4n, ld] = x0_taps - x0_centre evaluated in fp32 before rounding
- * (the point columns hold the exact offset).  Replaces the encoder calls of
- * modules.py:84-86 and :157-175 for one point set; all outputs bf16. */
-int na_sdf_taps_input_fwd(const na_grid_desc* desc, const float* points, int64_t n, float vmin,
-                          float vmax, float eps, const void* table, int32_t table_dtype,
-                          int32_t active_levels, void* x0_centre, void* x0_taps, void* d_taps,
-                          int64_t ld, na_stream_t stream);
+ * formulation (bf16; replaces the encoder calls of modules.py:84-86 and
+ * :157-175 for one point set): x0_centre [n, ld] as na_sdf_input_fwd writes it
+ * for x, and d_taps [4n, ld_d] with point-major row 4i + t holding the
+ * increment X(x + eps k_t) - X(x) (k_t as na_sdf_taps_bwd), evaluated in fp32
+ * and stored as hi = bf16(D) in columns 0..L*F-1 and lo = bf16(D - hi) in
+ * columns ld..ld+L*F-1; columns L*F..L*F+8 hold the exact point offset in the
+ * split [hi | lo | hi] layout, all other columns zero (ld_d >= ld + L*F).
+ * d_sum [n, ld] receives sum_t D_t (fp32 sum, one bf16 rounding) in x0_centre's
+ * layout. */
+int na_sdf_taps_increment_input(const na_grid_desc* desc, const float* points, int64_t n,
+                                float vmin, float vmax, float eps, const void* table,
+                                int32_t table_dtype, int32_t active_levels, void* x0_centre,
+                                int64_t ld, void* d_taps, int64_t ld_d, void* d_sum,
+                                na_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Per-ray operators.  R rays, N samples per ray, row-major [R, N(, K)].
@@ -479,11 +554,15 @@ int na_sdf_losses_bwd(const float* grad, const float* hess, const uint8_t* outsi
  * Same update as torch's: p *= 1 - lr*wd; m = lerp(m, g, 1-beta1);
  * v = beta2 v + (1-beta2) g^2; p -= lr/bc1 * m / (sqrt(v)/sqrt(bc2) + eps),
  * bc_i = 1 - beta_i^step.  If lowp != NULL the bf16 copy of the updated
- * parameter is written in the same pass (the hash table's gather copy).
+ * parameter is written in the same pass (the hash table's gather copy).  With
+ * clear_grad != 0 the gradient is zeroed in the same pass, after it is read
+ * (the optimizer.step + zero_grad pair of imaginaire/trainers/base.py:447-457,
+ * with the zeroed buffer kept for the next backward instead of a new fill).
  * ------------------------------------------------------------------------- */
-int na_adamw_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+int na_adamw_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
                   float lr, float beta1, float beta2, float eps, float weight_decay,
-                  int64_t step, void* lowp, int32_t lowp_dtype, na_stream_t stream);
+                  int64_t step, void* lowp, int32_t lowp_dtype, int32_t clear_grad,
+                  na_stream_t stream);
 
 /* The same update for up to NA_ADAMW_MULTI_MAX tensors that share the
  * hyper-parameters and the step count, in one launch (the optimizer's loop

@@ -94,6 +94,11 @@ _SIGNATURES = {
     "na_gemm": (ctypes.c_int, [_PGEMM, _P]),
     "na_gemm_wgrad_workspace": (ctypes.c_int64, [_PGEMM]),
     "na_gemm_wgrad_static": (ctypes.c_int, [_PGEMM, _P]),
+    "na_taps_increment_fwd": (ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _P,
+                                             _F32, _P, _P]),
+    "na_taps_increment_bwd": (ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _P,
+                                             _F32, _P, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P,
+                                             _P]),
     "na_colsum": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P]),
     "na_head_sum": (ctypes.c_int, [_P, _I32, _I64, _P, _P, _P]),
     "na_linear_narrow_fwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P, _I32, _I32, _P, _P]),
@@ -103,8 +108,12 @@ _SIGNATURES = {
     "na_weight_norm_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P, _P]),
     "na_sdf_input_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _P, _I32, _I32, _P, _I32, _I64, _P]),
     "na_sdf_input_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _P, _I32, _I64, _P, _I32, _P]),
-    "na_sdf_taps_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _P, _I32, _I64, _P, _I32, _P]),
-    "na_sdf_taps_input_fwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _I32, _I32, _P, _P, _P, _I64, _P]),
+    "na_sdf_taps_bwd": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _P, _P, _I32, _I64, _P, _I32, _I32, _P]),
+    "na_hashgrid_sorted_workspace": (ctypes.c_int64, [_PGRID, _I64, _I32]),
+    "na_hashgrid_bwd_sorted": (ctypes.c_int, [_PGRID, _P, _I64, _P, _I32, _P, _I32, _P, _I64, _P]),
+    "na_sdf_input_bwd_sorted": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _P, _I32, _I64, _P, _I32, _P, _I64, _P]),
+    "na_sdf_taps_bwd_sorted": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _P, _P, _I32, _I64, _P, _I32, _I32, _P, _I64, _P]),
+    "na_sdf_taps_increment_input": (ctypes.c_int, [_PGRID, _P, _I64, _F32, _F32, _F32, _P, _I32, _I32, _P, _I64, _P, _I64, _P, _P]),
     "na_ray_bounds_sample": (ctypes.c_int, [_P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P]),
     "na_sphere_crossings": (ctypes.c_int, [_P, _P, _I64, _F32, _P, _P, _P]),
     "na_camera_rays": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _P]),
@@ -138,7 +147,7 @@ _SIGNATURES = {
     "na_rgb_input_bwd": (ctypes.c_int, [_P, _I32, _I64, _I32, _P, _I32, _I64, _P, _I32, _P]),
     "na_sdf_losses_fwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P]),
     "na_sdf_losses_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P, _I32, _P, _P]),
-    "na_adamw_step": (ctypes.c_int, [_P, _P, _P, _P, _I64, _F32, _F32, _F32, _F32, _F32, _I64, _P, _I32, _P]),
+    "na_adamw_step": (ctypes.c_int, [_P, _P, _P, _P, _I64, _F32, _F32, _F32, _F32, _F32, _I64, _P, _I32, _I32, _P]),
     "na_adamw_step_multi": (ctypes.c_int, [ctypes.POINTER(AdamwTensor), _I32, _F32, _F32, _F32,
                                             _F32, _F32, _I64, _P]),
     "na_mc_workspace_size": (_I64, [_I64, _I64, _I64, _I64]),

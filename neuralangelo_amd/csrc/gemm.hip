@@ -1206,6 +1206,9 @@ enum PersistKind : int {
   PK_BIAS_SOFTPLUS = 2, // C = softplus(acc + b) [z_out = acc + b] [head]
   PK_DACT = 3,          // C = (acc [+ r1_a r1_b]) * act'(y), relu or softplus  [colsum]
   PK_DELTA = 4,         // C = softplus(zc + acc), head over softplus increments
+  // Tap increments of the 4-tap numerical gradient, rows point-major (row m is tap
+  // m % 4 of point m / 4, whose stored centre pre-activation is zc row m / 4):
+  PK_TAPS_FWD = 5,      // head over softplus(z + dz) - softplus(z) only (no C)
 };
 
 // softplus_beta with a select for the linear region (no branch); ib = 1 / beta
@@ -1246,6 +1249,27 @@ __device__ inline void softplus_step(float z0, float dz, float beta, float ib, f
     y1 = fmaf(__builtin_amdgcn_logf(1.0f + t1), s, fmaxf(z1, 0.0f));
     h = y1 - fmaf(l0, s, fmaxf(z0, 0.0f));
   }
+}
+
+// 1 - exp(-u) without cancellation: a Taylor series below |u| = 1/16 (relative
+// error under 1e-9), the exponential above (relative error a few ulps of 1/16).
+__device__ inline float one_minus_exp_neg(float u) {
+  if (fabsf(u) < 0.0625f)
+    return u * (1.0f - 0.5f * u * (1.0f - (1.0f / 3.0f) * u * (1.0f - 0.25f * u * (1.0f - 0.2f * u))));
+  return 1.0f - __expf(-u);
+}
+
+// Derivatives of softplus_beta: s1 = sp'(z0 + dz) = sigmoid(b (z0 + dz)), s0 = sp'(z0)
+// and the increment ds = sp'(z0 + dz) - sp'(z0) = sigmoid(a + u) sigmoid(-a) (1 - e^-u)
+// (a = b z0, u = b dz): a product of factors, so it keeps float32 precision relative
+// to its own size however small dz is.
+__device__ inline void softplus_deriv_step(float z0, float dz, float beta, float& s1, float& ds,
+                                           float& s0) {
+  const float a = z0 * beta, u = dz * beta;
+  s1 = 1.0f / (1.0f + __expf(-(a + u)));
+  const float sna = 1.0f / (1.0f + __expf(a));
+  ds = s1 * sna * one_minus_exp_neg(u);
+  s0 = 1.0f - sna;  // sp'(z0)
 }
 
 // One 32 x 32 block of the wave, staged in eb: lane (rl, cg) handles columns
@@ -1742,6 +1766,29 @@ __device__ inline void ws_issue(__amdgpu_buffer_rsrc_t r, uint32_t off, bf16_t* 
   __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t*)dst, 16, off, 0, 0, 0);
 }
 
+// Column sums of a weights-stationary wave: sums over the 32 rows of the lanes that
+// share h, then one atomic per column (dst + 16 g + c, the lane's columns).
+__device__ inline void ws_flush_colsum(float* dst, float (&cs)[2][8], int lane) {
+#pragma unroll
+  for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      float s = cs[gq][c];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      s += __shfl_xor(s, 4);
+      s += __shfl_xor(s, 8);
+      s += __shfl_xor(s, 16);
+      cs[gq][c] = s;
+    }
+  if ((lane & 31) == 0) {
+#pragma unroll
+    for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) atomic_add_f32(dst + 16 * gq + c, cs[gq][c]);
+  }
+}
+
 template <int NWV, int NK, int KIND>
 __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kernel(GemmParams p) {
   constexpr int kWsNS = WsCfg<NWV>::NS;
@@ -1755,7 +1802,7 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
   const uint32_t n_mt = (uint32_t)(p.M / GBM);
   const int64_t my_tiles = blockIdx.x < n_mt ? (n_mt - 1 - blockIdx.x) / gridDim.x + 1 : 0;
   const int64_t total = my_tiles * NK;
-  const int nw = wave * 32;
+  const int nw = wave * 32;  // first column of this wave
 
   // weights of row nw + lane % 32 of B: 8 k-values (offset 8 h) of each 16-wide step
   bf16x8_t bw[2 * NK];
@@ -1823,7 +1870,7 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
 
   const float ib = 1.0f / e.beta;
   // derivative kind: rank-1 coefficients of the lane's columns and column sums
-  constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA;
+  constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA || KIND == PK_TAPS_FWD;
   float rb[2][8], cs[2][8], hw[2][8];
 #pragma unroll
   for (int gq = 0; gq < 2; ++gq)
@@ -1851,7 +1898,7 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
     float r1v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     // increment kind: the stored pre-activation rows (m mod zc_rows; zc_rows is a
     // multiple of 128, so a tile's rows do not wrap)
-    float zv[KIND == PK_DELTA ? 4 : 1][2][8];
+    float zv[KIND == PK_DELTA || KIND == PK_TAPS_FWD ? 4 : 1][2][8];
     if constexpr (KIND == PK_DELTA) {
       const int64_t z0 = m0 % e.zc_rows;
 #pragma unroll
@@ -1859,6 +1906,16 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
 #pragma unroll
         for (int gq = 0; gq < 2; ++gq)
           vload8(e.zc + (z0 + i * 32 + (lane & 31)) * e.ld_zc + nw + 16 * gq + 8 * h, zv[i][gq]);
+    }
+    // point-major taps: row m belongs to point m / 4
+    auto load_zc = [&](int i, float (&z)[2][8]) {
+      const int64_t zr = (m0 + i * 32 + (lane & 31)) >> 2;
+#pragma unroll
+      for (int gq = 0; gq < 2; ++gq) vload8(e.zc + zr * e.ld_zc + nw + 16 * gq + 8 * h, z[gq]);
+    };
+    if constexpr (KIND == PK_TAPS_FWD) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) load_zc(i, zv[i]);
     }
     if constexpr (KIND == PK_DACT) {
 #pragma unroll
@@ -1931,7 +1988,7 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
             cs[gq][c] += x;
             o[gq][c] = x;
           }
-        } else if constexpr (KIND == PK_DELTA) {
+        } else if constexpr (KIND == PK_DELTA || KIND == PK_TAPS_FWD) {
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
             float hv, y1;
@@ -1958,8 +2015,10 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
           }
         }
         const int64_t col = nw + 16 * gq + 8 * h;
-        if (e.c_dtype == NA_BF16) vstore8((bf16_t*)e.C + m * e.ldc + col, o[gq]);
-        else vstore8((float*)e.C + m * e.ldc + col, o[gq]);
+        if constexpr (KIND != PK_TAPS_FWD) {
+          if (e.c_dtype == NA_BF16) vstore8((bf16_t*)e.C + m * e.ldc + col, o[gq]);
+          else vstore8((float*)e.C + m * e.ldc + col, o[gq]);
+        }
       }
       if constexpr (kHead) {
         // row m over the wave's 32 columns: this lane's 16 and its partner's 16
@@ -1971,29 +2030,124 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
     }
   }
   if constexpr (KIND == PK_DACT) {
-    if (e.colsum) {
-      // sums over the 32 rows of the lanes that share h, then one atomic per column
-#pragma unroll
-      for (int gq = 0; gq < 2; ++gq)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          float s = cs[gq][c];
-          s += __shfl_xor(s, 1);
-          s += __shfl_xor(s, 2);
-          s += __shfl_xor(s, 4);
-          s += __shfl_xor(s, 8);
-          s += __shfl_xor(s, 16);
-          cs[gq][c] = s;
-        }
-      if ((lane & 31) == 0) {
-#pragma unroll
-        for (int gq = 0; gq < 2; ++gq)
-#pragma unroll
-          for (int c = 0; c < 8; ++c) atomic_add_f32(e.colsum + nw + 16 * gq + 8 * h + c, cs[gq][c]);
-      }
-    }
+    if (e.colsum) ws_flush_colsum(e.colsum + nw + 8 * h, cs, lane);
   }
   __builtin_amdgcn_s_waitcnt(0xF70);  // no buffer-to-LDS load outlives the workgroup
+}
+
+// ---------------------------------------------------------------------------
+// Backward of the tap-increment layer (na_taps_increment_bwd), elementwise from
+// the recomputed increments dz = D W^T (fp32 [M = 4 P, N], point-major rows
+// 4 i + t).  With dds_t = a + b_t (a = the four taps' mean, the Hessian-trace
+// coefficient shared by every tap; sum_t b_t = 0 since sum_t k_t = 0), s1_t =
+// sp'(z + dz_t), s0 = sp'(z) and dsp_t = s1_t - s0 (formed as a product, relative
+// precision of its own size):
+//   tap row   dpre[4 i + t] = w (b_t s1_t + a dsp_t)      (pairs with D_t)
+//   point row q[i]          = a w s0                      (pairs with sum_t D_t)
+//   centre[i]              += sum_t dds_t w dsp_t         (and db0 += its column sums)
+//   dws                    += sum_t dds_t (sp(z + dz_t) - sp(z))
+// Each bf16 row holds a term of its natural size: no difference of nearly equal
+// terms is rounded to bf16.  A thread owns 4 columns of a point and walks points
+// with a grid stride; the workgroup's column sums are combined through LDS and
+// added with one atomic per column.
+// ---------------------------------------------------------------------------
+constexpr int kTibBlock = 256;
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4_t;
+
+__device__ inline void vload4(const float* p, float (&v)[4]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ inline void vload4(const bf16_t* p, float (&v)[4]) {
+  const uint2 a = *reinterpret_cast<const uint2*>(p);
+  v[0] = __uint_as_float(a.x << 16);
+  v[1] = __uint_as_float(a.x & 0xffff0000u);
+  v[2] = __uint_as_float(a.y << 16);
+  v[3] = __uint_as_float(a.y & 0xffff0000u);
+}
+__device__ inline void vstore4(bf16_t* p, const float (&v)[4]) {
+  const f32x4_t x = {v[0], v[1], v[2], v[3]};
+  *reinterpret_cast<bf16x4_t*>(p) = __builtin_convertvector(x, bf16x4_t);
+}
+
+__global__ void __launch_bounds__(kTibBlock)
+taps_increment_bwd_kernel(const float* __restrict__ dz, int64_t P, int N,
+                          const float* __restrict__ zc, int64_t ld_zc,
+                          const float* __restrict__ ws, float beta,
+                          const float* __restrict__ dds, bf16_t* __restrict__ dpre,
+                          int64_t ld_dpre, bf16_t* __restrict__ centre, int64_t ld_centre,
+                          bf16_t* __restrict__ q, int64_t ld_q, float* __restrict__ db0,
+                          float* __restrict__ dws) {
+  __shared__ float red[2][kTibBlock][4];
+  const int cpp = N / 4;               // 4-column chunks per point (64 or 32)
+  const int ppb = kTibBlock / cpp;     // points per workgroup pass (4 or 8)
+  const int chunk = threadIdx.x % cpp, slot = threadIdx.x / cpp;
+  const int c0 = chunk * 4;
+  const float ib = 1.0f / beta;
+  float w[4], cs[4], hs[4];
+  vload4(ws + c0, w);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) cs[c] = hs[c] = 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * ppb + slot; i < P; i += (int64_t)gridDim.x * ppb) {
+    float z0[4], x[4][4];
+    vload4(zc + i * ld_zc + c0, z0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) vload4(dz + (4 * i + t) * N + c0, x[t]);
+    const float4 d4 = *reinterpret_cast<const float4*>(dds + 4 * i);
+    const float dv[4] = {d4.x, d4.y, d4.z, d4.w};
+    const float a = 0.25f * ((dv[0] + dv[1]) + (dv[2] + dv[3]));
+    float ev[4], s0v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ev[c] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float b = dv[t] - a;
+      float o[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        float hv, y1, s1, dsp, s0;
+        softplus_step(z0[c], x[t][c], beta, ib, hv, y1);
+        softplus_deriv_step(z0[c], x[t][c], beta, s1, dsp, s0);
+        o[c] = w[c] * fmaf(b, s1, a * dsp);
+        ev[c] = fmaf(dv[t] * w[c], dsp, ev[c]);
+        hs[c] = fmaf(dv[t], hv, hs[c]);
+        s0v[c] = s0;
+      }
+      vstore4(dpre + (4 * i + t) * ld_dpre + c0, o);
+    }
+    float qv[4], cv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) qv[c] = (a * w[c]) * s0v[c];
+    vstore4(q + i * ld_q + c0, qv);
+    vload4(centre + i * ld_centre + c0, cv);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      cv[c] += ev[c];
+      cs[c] += ev[c];
+    }
+    vstore4(centre + i * ld_centre + c0, cv);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    red[0][threadIdx.x][c] = cs[c];
+    red[1][threadIdx.x][c] = hs[c];
+  }
+  __syncthreads();
+  if (slot == 0) {
+    for (int k = 1; k < ppb; ++k) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        cs[c] += red[0][k * cpp + chunk][c];
+        hs[c] += red[1][k * cpp + chunk][c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      atomic_add_f32(db0 + c0 + c, cs[c]);
+      atomic_add_f32(dws + c0 + c, hs[c]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2303,14 +2457,16 @@ static int ws_grid(int64_t n_mt, int nwv) {
 }
 
 template <int NWV>
-static void launch_ws(int nk, int kind, int grid, const GemmParams& p, hipStream_t s) {
-  const dim3 g(grid), b(64 * NWV);
+static void launch_ws(int nk, int kind, int grid, const GemmParams& p, hipStream_t s,
+                      int halves = 1) {
+  const dim3 g(grid, halves), b(64 * NWV);
 #define NA_WS(NK)                                                                                             \
   switch (kind) {                                                                                             \
     case PK_PLAIN: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_PLAIN>), g, b, 0, s, p); break;         \
     case PK_BIAS_RELU: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_RELU>), g, b, 0, s, p); break; \
     case PK_DACT: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_DACT>), g, b, 0, s, p); break;           \
     case PK_DELTA: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_DELTA>), g, b, 0, s, p); break;         \
+    case PK_TAPS_FWD: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_TAPS_FWD>), g, b, 0, s, p); break;   \
     default: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;       \
   }
   if (nk == 5) { NA_WS(5) } else if (nk == 8) { NA_WS(8) } else { NA_WS(9) }
@@ -2465,6 +2621,74 @@ act_grad_vec_kernel(const TD* __restrict__ dy, int64_t ld_dy, const TY* __restri
 
 using namespace na;
 
+// Tap-increment products of the BF16 4-tap gradient on the weights-stationary
+// kernel: rows point-major (row m = tap m % 4 of point m / 4), inputs the increments
+// D of the first layer's input, weights W [N, K], the centre pre-activations zc
+// (fp32, point rows) and the SDF head ws [N].
+static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M, int64_t K,
+                          const void* W, int64_t ldw, int64_t N, const float* zc, int64_t ld_zc,
+                          const float* ws, float beta, float* head_out, const float* dds,
+                          void* dpre, int64_t ld_dpre, void* centre, int64_t ld_centre,
+                          void* q, int64_t ld_q, float* db0, float* dws, float* work,
+                          hipStream_t s) {
+  NA_CHECK_ARG(M >= 0 && M % GBM == 0, "rows must be a multiple of 128 (pad the points to 32)");
+  NA_CHECK_ARG(N == 256 || N == 128, "layer width must be 128 or 256");
+  const int64_t nk = ceil_div(K, (int64_t)kNNBK);
+  NA_CHECK_ARG(K >= 1 && K % 8 == 0 && (nk == 5 || nk == 8 || nk == 9),
+               "K must be a multiple of 8 in 129..160, 225..288 (5, 8 or 9 stages of 32)");
+  NA_CHECK_ARG(ld_d >= K && ld_d % 8 == 0 && ldw >= K && ldw % 8 == 0 && ld_zc >= N &&
+                   ld_zc % 4 == 0,
+               "bad leading dimensions");
+  NA_CHECK_ARG(aligned16(D) && aligned16(W) && aligned16(zc) && aligned16(ws),
+               "operands must be 16-byte aligned");
+  NA_CHECK_ARG(M / GBM < 0x7fffffffLL, "too many rows");
+  if (!backward) {
+    NA_CHECK_ARG(head_out != nullptr, "head_out is null");
+  } else {
+    NA_CHECK_ARG(dds && dpre && centre && q && work && db0 && dws && aligned16(dpre) &&
+                     aligned16(centre) && aligned16(q) && aligned16(work) && aligned16(dds) &&
+                     ld_dpre >= N && ld_dpre % 8 == 0 &&
+                     ld_centre >= N && ld_centre % 8 == 0 && ld_q >= N && ld_q % 8 == 0,
+                 "bad backward operands");
+  }
+  if (M == 0) return NA_OK;
+  GemmParams p{};
+  p.M = M; p.N = N; p.K = K;
+  p.A = D; p.lda = ld_d; p.B = W; p.ldb = ldw;
+  p.k_chunk = nk * kNNBK;
+  p.e.epilogue = NA_EPI_STORE;
+  p.e.beta = beta;
+  const int nwv = N == 256 ? 8 : 4;
+  const int wg = ws_grid(M / GBM, nwv);
+  NA_CHECK_ARG(wg > 0, "could not size the launch");
+  if (!backward) {
+    p.e.act = NA_ACT_SOFTPLUS;
+    p.e.zc = zc; p.e.ld_zc = ld_zc; p.e.zc_rows = M / 4;
+    p.e.head_w = ws; p.e.head_out = head_out;
+    if (nwv == 8) launch_ws<8>((int)nk, PK_TAPS_FWD, wg, p, s);
+    else launch_ws<4>((int)nk, PK_TAPS_FWD, wg, p, s);
+    NA_CHECK_LAUNCH();
+    return NA_OK;
+  }
+  // backward: recompute dz = D W^T into the fp32 workspace, then the elementwise pass
+  p.e.C = work; p.e.ldc = N; p.e.c_dtype = NA_F32;
+  if (nwv == 8) launch_ws<8>((int)nk, PK_PLAIN, wg, p, s);
+  else launch_ws<4>((int)nk, PK_PLAIN, wg, p, s);
+  NA_CHECK_LAUNCH();
+  const int64_t P = M / 4;
+  const int ppb = kTibBlock / (int)(N / 4);
+  const int64_t groups = ceil_div(P, (int64_t)ppb);
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)ncu * 8));
+  hipLaunchKernelGGL(taps_increment_bwd_kernel, dim3((unsigned)grid), dim3(kTibBlock), 0, s,
+                     (const float*)work, P, (int)N, zc, ld_zc, ws, beta, dds, (bf16_t*)dpre,
+                     ld_dpre, (bf16_t*)centre, ld_centre, (bf16_t*)q, ld_q, db0, dws);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
 // Partial products of na_gemm_wgrad_static: a file-scope device array (one copy
 // per device, held until the process ends).  Its name is a host-side stand-in;
 // kernels receive the device address from hipGetSymbolAddress, looked up once per
@@ -2507,6 +2731,26 @@ extern "C" int na_gemm_wgrad_static(const na_gemm_args* a, na_stream_t stream) {
   b.wg_partials = ptr;
   b.wg_partials_len = NA_WGRAD_STATIC_MAX_FLOATS;
   return na_gemm(&b, stream);
+}
+
+extern "C" int na_taps_increment_fwd(const void* D, int64_t ld_d, int64_t M, int64_t K,
+                                     const void* W, int64_t ldw, int64_t N, const float* zc,
+                                     int64_t ld_zc, const float* ws, float beta, float* head_out,
+                                     na_stream_t stream) {
+  return taps_increment(false, D, ld_d, M, K, W, ldw, N, zc, ld_zc, ws, beta, head_out,
+                        nullptr, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr, nullptr,
+                        (hipStream_t)stream);
+}
+
+extern "C" int na_taps_increment_bwd(const void* D, int64_t ld_d, int64_t M, int64_t K,
+                                     const void* W, int64_t ldw, int64_t N, const float* zc,
+                                     int64_t ld_zc, const float* ws, float beta, const float* dds,
+                                     void* dpre, int64_t ld_dpre, void* centre, int64_t ld_centre,
+                                     void* q, int64_t ld_q, float* db0, float* dws, float* work,
+                                     na_stream_t stream) {
+  return taps_increment(true, D, ld_d, M, K, W, ldw, N, zc, ld_zc, ws, beta, nullptr, dds,
+                        dpre, ld_dpre, centre, ld_centre, q, ld_q, db0, dws, work,
+                        (hipStream_t)stream);
 }
 
 extern "C" int64_t na_gemm_wgrad_workspace(const na_gemm_args* a) {

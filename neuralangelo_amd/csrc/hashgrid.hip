@@ -355,12 +355,20 @@ hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
 
 constexpr float kTapDir[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
 
-// Centre + tap inputs of the SDF network for the increment formulation of the
-// 4-tap gradient (bf16 compute).  Per point: X0c (centre row), X0t (tap rows,
-// tap-major [4n]) and D = X0t - X0c evaluated in fp32 before the bf16 cast,
-// so the increments keep bf16 precision relative to their own (small) size.
-// Point columns use the split layout [hi | lo | hi]; for D they carry the
-// exact fp32 offset x_t - x_c.  One thread per (point, level).
+// Centre input and tap increments of the SDF network for the increment
+// formulation of the 4-tap gradient (bf16 compute).  Per point: X0c (the centre
+// row) and, for each tap t, the point-major row 4 i + t of D = X(x + eps k_t) - X(x),
+// evaluated in fp32 and stored as two bf16 parts (hi = bf16(D), lo = bf16(D - hi)),
+// so the increments keep about 16 mantissa bits relative to their own (small)
+// size and the four taps' second-order sum survives the rounding.  Row layouts:
+//   X0c [ld]:   enc (L*F) | x as [hi | lo | hi] (9) | zeros
+//   D   [ld_d]: enc hi (L*F) | (x_t - x) as [hi | lo | hi] (9) | zeros to ld |
+//               enc lo (L*F) | zeros
+// so the first ld columns of D match X0c's layout (the first-layer weights apply
+// to both, and the weight gradient reads them) and the lo block pairs with a
+// second copy of the encoder weights.  S [ld] holds the taps' sum of D (fp32 sum,
+// one bf16 rounding: second order, so it keeps bf16 precision of its own size).
+// One thread per (point, level).
 __device__ inline void split_store(bf16_t* e, const float v[3]) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -375,8 +383,8 @@ template <int F, typename TT>
 __global__ void __launch_bounds__(kBlock)
 sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
                       float vrange, float eps, const TT* __restrict__ table,
-                      bf16_t* __restrict__ X0c, bf16_t* __restrict__ X0t,
-                      bf16_t* __restrict__ D, int64_t ld, int active_levels) {
+                      bf16_t* __restrict__ X0c, int64_t ld, bf16_t* __restrict__ D, int64_t ld_d,
+                      bf16_t* __restrict__ S, int active_levels) {
   // level-minor mapping: the L lanes of one point write its rows' level segments
   // together, so each output row is written whole by one wave instruction
   const int L = d.n_levels;
@@ -384,6 +392,7 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
   const int64_t i = tid / L;
   const int l = (int)(tid % L);
   if (i >= n) return;
+  const int64_t LF = (int64_t)L * F;
   float xc[3], xt[4][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -392,33 +401,36 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
     for (int t = 0; t < 4; ++t) xt[t][k] = xc[k] + kTapDir[t][k] * eps;
   }
   if (l == 0) {
-    const int64_t c0 = (int64_t)L * F;
-    split_store(X0c + i * ld + c0, xc);
-    for (int64_t c = c0 + 9; c < ld; ++c) X0c[i * ld + c] = 0;
+    split_store(X0c + i * ld + LF, xc);
+    for (int64_t c = LF + 9; c < ld; ++c) X0c[i * ld + c] = 0;
+    float dsum[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int64_t r = (int64_t)t * n + i;
+      bf16_t* row = D + (4 * i + t) * ld_d;
       float dx[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) dx[k] = xt[t][k] - xc[k];  // exact (Sterbenz)
-      split_store(X0t + r * ld + c0, xt[t]);
-      split_store(D + r * ld + c0, dx);
-      for (int64_t c = c0 + 9; c < ld; ++c) {
-        X0t[r * ld + c] = 0;
-        D[r * ld + c] = 0;
+      for (int k = 0; k < 3; ++k) {
+        dx[k] = xt[t][k] - xc[k];  // exact (Sterbenz)
+        dsum[k] += dx[k];
       }
+      split_store(row + LF, dx);
+      for (int64_t c = LF + 9; c < ld; ++c) row[c] = 0;
+      for (int64_t c = ld + LF; c < ld_d; ++c) row[c] = 0;
     }
+    split_store(S + i * ld + LF, dsum);
+    for (int64_t c = LF + 9; c < ld; ++c) S[i * ld + c] = 0;
   }
   if (l >= active_levels) {  // coarse-to-fine: inactive levels are zero in all outputs
     float z[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) z[f] = 0.0f;
     RowStore<F, bf16_t>::store(X0c + i * ld + l * F, z);
+    RowStore<F, bf16_t>::store(S + i * ld + l * F, z);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int64_t r = (int64_t)t * n + i;
-      RowStore<F, bf16_t>::store(X0t + r * ld + l * F, z);
-      RowStore<F, bf16_t>::store(D + r * ld + l * F, z);
+      bf16_t* row = D + (4 * i + t) * ld_d;
+      RowStore<F, bf16_t>::store(row + l * F, z);
+      RowStore<F, bf16_t>::store(row + ld + l * F, z);
     }
     return;
   }
@@ -445,6 +457,9 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
     }
     RowStore<F, bf16_t>::store(X0c + i * ld + l * F, ac);
   }
+  float ssum[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) ssum[f] = 0.0f;
 #pragma unroll 1
   for (int t = 0; t < 4; ++t) {
     uint32_t cell[3];
@@ -465,13 +480,19 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
     } else {
       gather_cell<F, TT>(tab, p, cell, fr, at);
     }
-    const int64_t r = (int64_t)t * n + i;
-    float dv[F];
+    float hi[F], lo[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) dv[f] = at[f] - ac[f];
-    RowStore<F, bf16_t>::store(X0t + r * ld + l * F, at);
-    RowStore<F, bf16_t>::store(D + r * ld + l * F, dv);
+    for (int f = 0; f < F; ++f) {
+      const float dv = at[f] - ac[f];
+      ssum[f] += dv;
+      hi[f] = bf16_to_f32(f32_to_bf16(dv));
+      lo[f] = dv - hi[f];  // exact
+    }
+    bf16_t* row = D + (4 * i + t) * ld_d;
+    RowStore<F, bf16_t>::store(row + l * F, hi);
+    RowStore<F, bf16_t>::store(row + ld + l * F, lo);
   }
+  RowStore<F, bf16_t>::store(S + i * ld + l * F, ssum);
 }
 
 // Backward of the five encoder evaluations of one numerical-gradient point
@@ -511,7 +532,8 @@ template <int F, typename TD, bool P2>
 __global__ void __launch_bounds__(kBlock)
 hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
                          float vrange, int vshift, float eps, const TD* __restrict__ dyc,
-                         const TD* __restrict__ dyt, int64_t ld, float* __restrict__ grad) {
+                         const TD* __restrict__ dyt, const TD* __restrict__ dyq, int64_t ld,
+                         float* __restrict__ grad, int inc) {
   auto unit = [&](float v) -> float {
     return P2 ? ldexpf(v - vmin, vshift) : normalize_coord(v, vmin, vrange);
   };
@@ -531,9 +553,21 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
   const int64_t iend = i0 + kTapRun < n ? i0 + kTapRun : n;
   for (int64_t i = i0; i < iend; ++i) {
     float g[5];
-    g[0] = scalar_io<TD>::load(dyc + i * ld + col);
+    if (inc) {
+      // increments: tap rows point-major, gradients of the inputs D_t = X(x + eps k_t)
+      // - X(x) (dyt, plus the point row dyq, the gradient of sum_t D_t, shared by the
+      // four taps); the centre row holds the gradient of the centre input alone, so
+      // the centre's corners receive g_c - sum_t g_t (formed here in fp32)
+      const float q = dyq ? scalar_io<TD>::load(dyq + i * ld + col) : 0.0f;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) g[1 + t] = scalar_io<TD>::load(dyt + ((int64_t)t * n + i) * ld + col);
+      for (int t = 0; t < 4; ++t) g[1 + t] = scalar_io<TD>::load(dyt + (4 * i + t) * ld + col) + q;
+      g[0] = scalar_io<TD>::load(dyc + i * ld + col) - ((g[1] + g[2]) + (g[3] + g[4]));
+    } else {
+      g[0] = scalar_io<TD>::load(dyc + i * ld + col);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        g[1 + t] = scalar_io<TD>::load(dyt + ((int64_t)t * n + i) * ld + col);
+    }
     if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && g[3] == 0.0f && g[4] == 0.0f) continue;
     int cell[5][3];
     float fr[5][3];
@@ -632,6 +666,145 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
 }
 
 // ---------------------------------------------------------------------------
+// Deterministic (sorted) table-gradient reduction.
+//
+// The atomic kernels above add every contribution into the gradient as it
+// arrives, so the float sum order of a table entry follows the scheduling and
+// the last bits vary between runs.  The sorted variant fixes the order: per
+// level, contribution id = (i * E + e) * 8 + c (point i, evaluation e of E,
+// corner c) gets the key "entry index within the level"; a stable radix sort
+// by key lists each entry's contributions in id order, and the first thread of
+// each run of equal keys sums the run in that order (fp32 fma, weight times
+// gradient) and adds the sum into the entry's F gradient values.  The result
+// depends only on the inputs.  Evaluations: E = 1 (one input row per point,
+// na_hashgrid_bwd / na_sdf_input_bwd) or E = 5 (the centre and its four taps,
+// gradients formed as hashgrid_taps_bwd_kernel forms them).  A contribution
+// whose F gradient values are all zero takes key = level size: it sorts after
+// every entry and is skipped, as the atomic kernels skip it.
+// ---------------------------------------------------------------------------
+struct SortedSrc {
+  const void* dyc;  // centre (or only) rows [n, ld]
+  const void* dyt;  // tap rows: point-major [4n, ld] (inc) or tap-major [4][n, ld]
+  const void* dyq;  // shared row of the increment layout [n, ld], or null
+  int64_t ld;
+  int evals;  // 1 or 5
+  int inc;
+  float eps;
+};
+
+// Gradient of evaluation e of point i at columns col .. col + F - 1.
+template <int F, typename TD>
+__device__ inline void sorted_eval_grad(const SortedSrc& s, int64_t n, int64_t i, int e,
+                                        int64_t col, float (&g)[F]) {
+  const TD* dyc = (const TD*)s.dyc;
+  const TD* dyt = (const TD*)s.dyt;
+  const TD* dyq = (const TD*)s.dyq;
+  if (s.evals == 1 || (e == 0 && !s.inc)) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) g[f] = scalar_io<TD>::load(dyc + i * s.ld + col + f);
+    return;
+  }
+  if (!s.inc) {
+#pragma unroll
+    for (int f = 0; f < F; ++f)
+      g[f] = scalar_io<TD>::load(dyt + ((int64_t)(e - 1) * n + i) * s.ld + col + f);
+    return;
+  }
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    const float q = dyq ? scalar_io<TD>::load(dyq + i * s.ld + col + f) : 0.0f;
+    float gt[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) gt[t] = scalar_io<TD>::load(dyt + (4 * i + t) * s.ld + col + f) + q;
+    g[f] = e == 0 ? scalar_io<TD>::load(dyc + i * s.ld + col + f) - ((gt[0] + gt[1]) + (gt[2] + gt[3]))
+                  : gt[e - 1];
+  }
+}
+
+// Unit-cube cell and fractions of evaluation e of point i at level p.
+__device__ inline void sorted_eval_cell(const float* __restrict__ x, int64_t i, int e, float eps,
+                                        float vmin, float vrange, const LevelParams& p,
+                                        uint32_t (&cell)[3], float (&fr)[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float xc = x[i * 3 + k];
+    const float xe = e == 0 ? xc : xc + kTapDir[e - 1][k] * eps;  // as the forward built it
+    pos_fract(normalize_coord_gather(xe, vmin, vrange), p.scale, cell[k], fr[k]);
+  }
+}
+
+// One thread per (point, evaluation): the eight keys and ids of its corners.
+template <int F, typename TD>
+__global__ void __launch_bounds__(kBlock)
+sorted_keys_kernel(na_grid_desc d, int l, const float* __restrict__ x, int64_t n, float vmin,
+                   float vrange, SortedSrc s, uint32_t* __restrict__ keys,
+                   uint32_t* __restrict__ ids) {
+  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (tid >= n * s.evals) return;
+  const int64_t i = tid / s.evals;
+  const int e = (int)(tid % s.evals);
+  const LevelParams p = level_params(d, l);
+  float g[F];
+  sorted_eval_grad<F, TD>(s, n, i, e, (int64_t)l * F, g);
+  bool any = false;
+#pragma unroll
+  for (int f = 0; f < F; ++f) any = any || g[f] != 0.0f;
+  uint32_t id[8];
+  if (any) {
+    uint32_t cell[3];
+    float fr[3];
+    sorted_eval_cell(x, i, e, s.eps, vmin, vrange, p, cell, fr);
+    cell_corners(cell, p, id);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) id[c] = p.size;
+  }
+  const uint32_t base = (uint32_t)(tid * 8);
+  uint4* kp = reinterpret_cast<uint4*>(keys + tid * 8);
+  uint4* ip = reinterpret_cast<uint4*>(ids + tid * 8);
+  kp[0] = make_uint4(id[0], id[1], id[2], id[3]);
+  kp[1] = make_uint4(id[4], id[5], id[6], id[7]);
+  ip[0] = make_uint4(base, base + 1, base + 2, base + 3);
+  ip[1] = make_uint4(base + 4, base + 5, base + 6, base + 7);
+}
+
+// One thread per sorted position; the first position of each run of equal keys
+// sums the run in id order and adds the sums into the entry's gradient row.
+template <int F, typename TD>
+__global__ void __launch_bounds__(kBlock)
+sorted_reduce_kernel(na_grid_desc d, int l, const float* __restrict__ x, int64_t n, float vmin,
+                     float vrange, SortedSrc s, const uint32_t* __restrict__ keys,
+                     const uint32_t* __restrict__ ids, int64_t count, float* __restrict__ grad) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= count) return;
+  const LevelParams p = level_params(d, l);
+  const uint32_t key = keys[j];
+  if (key >= p.size) return;                // zero contributions sort last
+  if (j > 0 && keys[j - 1] == key) return;  // not the first of its run
+  float acc[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) acc[f] = 0.0f;
+  for (int64_t m = j; m < count && keys[m] == key; ++m) {
+    const uint32_t id = ids[m];
+    const int c = (int)(id & 7u);
+    const int64_t ie = (int64_t)(id >> 3);
+    const int64_t i = ie / s.evals;
+    const int e = (int)(ie % s.evals);
+    float g[F];
+    sorted_eval_grad<F, TD>(s, n, i, e, (int64_t)l * F, g);
+    uint32_t cell[3];
+    float fr[3];
+    sorted_eval_cell(x, i, e, s.eps, vmin, vrange, p, cell, fr);
+    const float w = corner_weight(c, fr);
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = fmaf(w, g[f], acc[f]);
+  }
+  float* gp = grad + ((size_t)p.off + key) * F;
+#pragma unroll
+  for (int f = 0; f < F; ++f) gp[f] += acc[f];
+}
+
+// ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
 static int check_desc(const na_grid_desc* d) {
@@ -726,32 +899,114 @@ static int run_bwd(const BwdLaunch& a, int ddt) {
 
 template <int F, typename TD>
 static int launch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, float vmin,
-                           float vrange, float eps, const void* dyc, const void* dyt, int64_t ld,
-                           float* grad, int active, hipStream_t s) {
+                           float vrange, float eps, const void* dyc, const void* dyt,
+                           const void* dyq, int64_t ld, float* grad, int active, int inc,
+                           hipStream_t s) {
   dim3 grid((unsigned)ceil_div(ceil_div(n, (int64_t)kTapRun) * F, kBlock), active);
   uint32_t vb;
   memcpy(&vb, &vrange, 4);
   if ((vb & 0x807fffffu) == 0u && (vb >> 23) != 0u)  // positive normal power of two
     hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, true>), grid, dim3(kBlock), 0, s, *d, x, n,
                        vmin, vrange, 127 - (int)(vb >> 23), eps, (const TD*)dyc, (const TD*)dyt,
-                       ld, grad);
+                       (const TD*)dyq, ld, grad, inc);
   else
     hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, false>), grid, dim3(kBlock), 0, s, *d, x, n,
-                       vmin, vrange, 0, eps, (const TD*)dyc, (const TD*)dyt, ld, grad);
+                       vmin, vrange, 0, eps, (const TD*)dyc, (const TD*)dyt, (const TD*)dyq, ld,
+                       grad, inc);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
 
 template <int F>
 static int dispatch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, float vmin,
-                             float vrange, float eps, const void* dyc, const void* dyt, int ddt,
-                             int64_t ld, float* grad, int active, hipStream_t s) {
+                             float vrange, float eps, const void* dyc, const void* dyt,
+                             const void* dyq, int ddt, int64_t ld, float* grad, int active,
+                             int inc, hipStream_t s) {
   if (ddt == NA_F32)
-    return launch_taps_bwd<F, float>(d, x, n, vmin, vrange, eps, dyc, dyt, ld, grad, active, s);
+    return launch_taps_bwd<F, float>(d, x, n, vmin, vrange, eps, dyc, dyt, dyq, ld, grad, active,
+                                     inc, s);
   if (ddt == NA_BF16)
-    return launch_taps_bwd<F, bf16_t>(d, x, n, vmin, vrange, eps, dyc, dyt, ld, grad, active, s);
+    return launch_taps_bwd<F, bf16_t>(d, x, n, vmin, vrange, eps, dyc, dyt, dyq, ld, grad, active,
+                                      inc, s);
   set_error("hash-grid taps backward: unsupported dy dtype");
   return NA_ERR_UNSUPPORTED;
+}
+
+// sorted reduction (sort.hip): stable radix sort of uint32 pairs on bits [0, end_bit)
+int radix_sort_temp_bytes(int64_t count, size_t* bytes);
+int radix_sort_pairs_u32(void* temp, size_t temp_bytes, const uint32_t* keys_in,
+                         uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                         int64_t count, int end_bit, hipStream_t s);
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Workspace of the sorted reduction: keys and ids twice (sort input and
+// output) for n * evals * 8 contributions of one level, then the sort's own
+// temporary storage.
+static int sorted_workspace(int64_t n, int evals, size_t* total, size_t* arr, size_t* temp) {
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  NA_CHECK_ARG(evals == 1 || evals == 5, "evals must be 1 or 5");
+  const int64_t count = n * evals * 8;
+  NA_CHECK_ARG(count <= 0x7fffffff, "n too large for the sorted reduction (n * evals * 8 >= 2^31)");
+  int st = radix_sort_temp_bytes(count, temp);
+  if (st) return st;
+  *arr = align256((size_t)count * 4);
+  *total = 4 * *arr + align256(*temp);
+  return NA_OK;
+}
+
+template <int F, typename TD>
+static int launch_sorted(const na_grid_desc* d, const float* x, int64_t n, float vmin,
+                         float vrange, const SortedSrc& src, float* grad, int active, void* ws,
+                         int64_t ws_bytes, hipStream_t s) {
+  NA_CHECK_ARG(ws != nullptr, "workspace is null");
+  NA_CHECK_ARG(((uintptr_t)ws & 255u) == 0, "workspace must be 256-byte aligned");
+  size_t total = 0, arr = 0, temp = 0;
+  int st = sorted_workspace(n, src.evals, &total, &arr, &temp);
+  if (st) return st;
+  NA_CHECK_ARG(ws_bytes >= (int64_t)total, "workspace smaller than na_hashgrid_sorted_workspace");
+  const int64_t count = n * src.evals * 8;
+  char* b = (char*)ws;
+  uint32_t* k0 = (uint32_t*)b;
+  uint32_t* v0 = (uint32_t*)(b + arr);
+  uint32_t* k1 = (uint32_t*)(b + 2 * arr);
+  uint32_t* v1 = (uint32_t*)(b + 3 * arr);
+  void* tmp = b + 4 * arr;
+  const dim3 gk((unsigned)ceil_div(n * src.evals, (int64_t)kBlock));
+  const dim3 gr((unsigned)ceil_div(count, (int64_t)kBlock));
+  for (int l = 0; l < active; ++l) {
+    const uint32_t size = d->offset[l + 1] - d->offset[l];
+    const int end_bit = 32 - __builtin_clz(size);  // keys run to size (the skip key)
+    hipLaunchKernelGGL((sorted_keys_kernel<F, TD>), gk, dim3(kBlock), 0, s, *d, l, x, n, vmin,
+                       vrange, src, k0, v0);
+    NA_CHECK_LAUNCH();
+    st = radix_sort_pairs_u32(tmp, temp, k0, k1, v0, v1, count, end_bit, s);
+    if (st) return st;
+    hipLaunchKernelGGL((sorted_reduce_kernel<F, TD>), gr, dim3(kBlock), 0, s, *d, l, x, n, vmin,
+                       vrange, src, k1, v1, count, grad);
+    NA_CHECK_LAUNCH();
+  }
+  return NA_OK;
+}
+
+static int run_sorted(const na_grid_desc* d, const float* x, int64_t n, float vmin, float vrange,
+                      const SortedSrc& src, int ddt, float* grad, int active, void* ws,
+                      int64_t ws_bytes, hipStream_t s) {
+  if (ddt != NA_F32 && ddt != NA_BF16) {
+    set_error("sorted table gradient: unsupported dy dtype");
+    return NA_ERR_UNSUPPORTED;
+  }
+  const bool bf = ddt == NA_BF16;
+#define NA_SORTED(FF)                                                                          \
+  return bf ? launch_sorted<FF, bf16_t>(d, x, n, vmin, vrange, src, grad, active, ws, ws_bytes, s) \
+            : launch_sorted<FF, float>(d, x, n, vmin, vrange, src, grad, active, ws, ws_bytes, s)
+  switch (d->n_features) {
+    case 1: NA_SORTED(1);
+    case 2: NA_SORTED(2);
+    case 4: NA_SORTED(4);
+    default: NA_SORTED(8);
+  }
+#undef NA_SORTED
 }
 
 }  // namespace na
@@ -841,13 +1096,15 @@ extern "C" int na_sdf_input_bwd(const na_grid_desc* desc, const float* points, i
 
 extern "C" int na_sdf_taps_bwd(const na_grid_desc* desc, const float* points, int64_t n,
                                float vmin, float vmax, float eps, const void* dy_centre,
-                               const void* dy_taps, int32_t dy_dtype, int64_t ld,
-                               float* grad_table, int32_t active_levels, na_stream_t stream) {
+                               const void* dy_taps, const void* dy_shared, int32_t dy_dtype,
+                               int64_t ld, float* grad_table, int32_t active_levels,
+                               int32_t increments, na_stream_t stream) {
   int st = check_desc(desc);
   if (st) return st;
   NA_CHECK_ARG(n >= 0, "n < 0");
   if (n == 0) return NA_OK;
   NA_CHECK_ARG(points && dy_centre && dy_taps && grad_table, "null pointer");
+  NA_CHECK_ARG(increments || !dy_shared, "dy_shared belongs to the increment layout");
   NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
   NA_CHECK_ARG(ld >= (int64_t)desc->n_levels * desc->n_features, "ld must be >= L*F");
   NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
@@ -857,29 +1114,35 @@ extern "C" int na_sdf_taps_bwd(const na_grid_desc* desc, const float* points, in
   const float vr = vmax - vmin;
   switch (desc->n_features) {
     case 1: return dispatch_taps_bwd<1>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
-                                        dy_dtype, ld, grad_table, active, s);
+                                        dy_shared, dy_dtype, ld, grad_table, active,
+                                        increments != 0, s);
     case 2: return dispatch_taps_bwd<2>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
-                                        dy_dtype, ld, grad_table, active, s);
+                                        dy_shared, dy_dtype, ld, grad_table, active,
+                                        increments != 0, s);
     case 4: return dispatch_taps_bwd<4>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
-                                        dy_dtype, ld, grad_table, active, s);
+                                        dy_shared, dy_dtype, ld, grad_table, active,
+                                        increments != 0, s);
     default: return dispatch_taps_bwd<8>(desc, points, n, vmin, vr, eps, dy_centre, dy_taps,
-                                         dy_dtype, ld, grad_table, active, s);
+                                         dy_shared, dy_dtype, ld, grad_table, active,
+                                         increments != 0, s);
   }
 }
 
-extern "C" int na_sdf_taps_input_fwd(const na_grid_desc* desc, const float* points, int64_t n,
-                                     float vmin, float vmax, float eps, const void* table,
-                                     int32_t table_dtype, int32_t active_levels, void* x0_centre,
-                                     void* x0_taps, void* d_taps, int64_t ld,
-                                     na_stream_t stream) {
+extern "C" int na_sdf_taps_increment_input(const na_grid_desc* desc, const float* points,
+                                           int64_t n, float vmin, float vmax, float eps,
+                                           const void* table, int32_t table_dtype,
+                                           int32_t active_levels, void* x0_centre, int64_t ld,
+                                           void* d_taps, int64_t ld_d, void* d_sum,
+                                           na_stream_t stream) {
   int st = check_desc(desc);
   if (st) return st;
   NA_CHECK_ARG(n >= 0, "n < 0");
   if (n == 0) return NA_OK;
-  NA_CHECK_ARG(points && table && x0_centre && x0_taps && d_taps, "null pointer");
+  NA_CHECK_ARG(points && table && x0_centre && d_taps && d_sum, "null pointer");
   NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
   const int64_t LF = (int64_t)desc->n_levels * desc->n_features;
   NA_CHECK_ARG(ld >= LF + 9 && ld % 8 == 0, "ld must be >= L*F + 9 and a multiple of 8");
+  NA_CHECK_ARG(ld_d >= ld + LF && ld_d % 8 == 0, "ld_d must be >= ld + L*F and a multiple of 8");
   NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / 8, "n too large");
   NA_CHECK_ARG(n <= (int64_t)0x7fffffff * kBlock / desc->n_levels, "n too large");
   dim3 grid((unsigned)ceil_div(n * desc->n_levels, (int64_t)kBlock));
@@ -887,8 +1150,8 @@ extern "C" int na_sdf_taps_input_fwd(const na_grid_desc* desc, const float* poin
   const float vr = vmax - vmin;
 #define NA_TAPS_IN(FF, TT)                                                                   \
   hipLaunchKernelGGL((sdf_taps_input_kernel<FF, TT>), grid, dim3(kBlock), 0, s, *desc, points, \
-                     n, vmin, vr, eps, (const TT*)table, (bf16_t*)x0_centre, (bf16_t*)x0_taps,  \
-                     (bf16_t*)d_taps, ld, active_levels)
+                     n, vmin, vr, eps, (const TT*)table, (bf16_t*)x0_centre, ld,               \
+                     (bf16_t*)d_taps, ld_d, (bf16_t*)d_sum, active_levels)
   if (table_dtype != NA_F32 && table_dtype != NA_BF16) {
     set_error("taps input: table must be f32 or bf16");
     return NA_ERR_UNSUPPORTED;
@@ -903,4 +1166,70 @@ extern "C" int na_sdf_taps_input_fwd(const na_grid_desc* desc, const float* poin
 #undef NA_TAPS_IN
   NA_CHECK_LAUNCH();
   return NA_OK;
+}
+
+extern "C" int64_t na_hashgrid_sorted_workspace(const na_grid_desc* desc, int64_t n,
+                                                int32_t evals) {
+  int st = check_desc(desc);
+  if (st) return st;
+  size_t total = 0, arr = 0, temp = 0;
+  st = sorted_workspace(n, evals, &total, &arr, &temp);
+  if (st) return st;
+  return (int64_t)total;
+}
+
+extern "C" int na_hashgrid_bwd_sorted(const na_grid_desc* desc, const float* x, int64_t n,
+                                      const void* dy, int32_t dy_dtype, float* grad_table,
+                                      int32_t active_levels, void* workspace,
+                                      int64_t workspace_bytes, na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(x && dy && grad_table, "null pointer");
+  const int active = active_levels < desc->n_levels ? active_levels : desc->n_levels;
+  if (active <= 0) return NA_OK;
+  SortedSrc src{dy, nullptr, nullptr, (int64_t)desc->n_levels * desc->n_features, 1, 0, 0.0f};
+  return run_sorted(desc, x, n, 0.0f, 1.0f, src, dy_dtype, grad_table, active, workspace,
+                    workspace_bytes, (hipStream_t)stream);
+}
+
+extern "C" int na_sdf_input_bwd_sorted(const na_grid_desc* desc, const float* points, int64_t n,
+                                       float vmin, float vmax, const void* dy, int32_t dy_dtype,
+                                       int64_t ld, float* grad_table, int32_t active_levels,
+                                       void* workspace, int64_t workspace_bytes,
+                                       na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(points && dy && grad_table, "null pointer");
+  NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
+  NA_CHECK_ARG(ld >= (int64_t)desc->n_levels * desc->n_features, "ld too small");
+  const int active = active_levels < desc->n_levels ? active_levels : desc->n_levels;
+  if (active <= 0) return NA_OK;
+  SortedSrc src{dy, nullptr, nullptr, ld, 1, 0, 0.0f};
+  return run_sorted(desc, points, n, vmin, vmax - vmin, src, dy_dtype, grad_table, active,
+                    workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+extern "C" int na_sdf_taps_bwd_sorted(const na_grid_desc* desc, const float* points, int64_t n,
+                                      float vmin, float vmax, float eps, const void* dy_centre,
+                                      const void* dy_taps, const void* dy_shared,
+                                      int32_t dy_dtype, int64_t ld, float* grad_table,
+                                      int32_t active_levels, int32_t increments, void* workspace,
+                                      int64_t workspace_bytes, na_stream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  NA_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return NA_OK;
+  NA_CHECK_ARG(points && dy_centre && dy_taps && grad_table, "null pointer");
+  NA_CHECK_ARG(increments || !dy_shared, "dy_shared belongs to the increment layout");
+  NA_CHECK_ARG(vmax > vmin, "vmax must exceed vmin");
+  NA_CHECK_ARG(ld >= (int64_t)desc->n_levels * desc->n_features, "ld must be >= L*F");
+  const int active = active_levels < desc->n_levels ? active_levels : desc->n_levels;
+  if (active <= 0) return NA_OK;
+  SortedSrc src{dy_centre, dy_taps, dy_shared, ld, 5, increments != 0, eps};
+  return run_sorted(desc, points, n, vmin, vmax - vmin, src, dy_dtype, grad_table, active,
+                    workspace, workspace_bytes, (hipStream_t)stream);
 }

@@ -11,13 +11,17 @@
 // 0,1,2,3,4,7,8,9,11 own one only on the block boundary (0: j=k=0; 1,2: k=0;
 // 3: i=k=0; 4: j=0; 7: i=0; 8: i=j=0; 9: j=0; 11: i=0), in that order.
 //
-// Work unit: a tile of 1024 consecutive cells of one block (blocks never share
-// a tile).  classify reads the volume (the only HBM-heavy pass: 4 B per
-// lattice point), counts owned vertices and triangles per tile, and writes the
-// per-cell cube index and in-tile vertex prefix only for tiles that touch the
-// surface.  A shared edge's owner cell owns a vertex, so its tile is always
-// one of those.  The tile scan turns tile counts into global offsets; emit
-// revisits only surface tiles.
+// Work unit: a tile of at most 1024 consecutive cells of one block (blocks never
+// share a tile), cut along whole z-rows: with cz <= 1024 cells per row a tile is
+// R = 1024 / cz rows of one x-plane (the last tile of a plane may hold fewer),
+// with cz > 1024 a 1024-cell segment of one row.  Tiles are numbered plane by
+// plane, so their concatenation is the block's i-major cell order, and a tile's
+// cell c sits at workspace slot tile * 1024 + c.  classify reads the volume (the
+// only HBM-heavy pass: 4 B per lattice point), counts owned vertices and
+// triangles per tile, and writes the per-cell cube index and in-tile vertex
+// prefix only for tiles that touch the surface.  A shared edge's owner cell owns
+// a vertex, so its tile is always one of those.  The tile scan turns tile counts
+// into global offsets; emit revisits only surface tiles.
 #include <algorithm>
 
 #include "common.h"
@@ -80,7 +84,44 @@ struct McBlock {
   int64_t ox, oy, oz;
   int64_t cx, cy, cz;
   int64_t ncell;
+  // tiles: rpt rows of one plane (cz <= 1024; seg = 1) or one 1024-cell segment
+  // of a row (cz > 1024; rpt = 1, seg segments per row); tpp tiles per x-plane
+  int64_t rpt, seg, tpp;
 };
+
+// The cells of tile q of an x-plane: first row j0, rows, first column k0, columns.
+struct McTileRows {
+  int64_t j0, rows, k0, cols;
+};
+
+__host__ __device__ inline void tile_geometry(int64_t cy, int64_t cz, int64_t& rpt, int64_t& seg,
+                                              int64_t& tpp) {
+  if (cz <= 1024) {
+    rpt = cz > 0 ? 1024 / cz : 1;
+    seg = 1;
+    tpp = (cy + rpt - 1) / rpt;
+  } else {
+    rpt = 1;
+    seg = (cz + 1023) / 1024;
+    tpp = cy * seg;
+  }
+}
+
+__device__ inline McTileRows tile_rows(const McBlock& k, int64_t q) {
+  McTileRows t;
+  if (k.seg == 1) {
+    t.j0 = q * k.rpt;
+    t.rows = min(k.rpt, k.cy - t.j0);
+    t.k0 = 0;
+    t.cols = k.cz;
+  } else {
+    t.j0 = q / k.seg;
+    t.rows = 1;
+    t.k0 = (q - t.j0 * k.seg) * 1024;
+    t.cols = min((int64_t)1024, k.cz - t.k0);
+  }
+  return t;
+}
 
 __device__ inline McBlock block_of(const McLattice& g, uint32_t b) {
   // block counts per axis are < 2^31 (the host checks the tile count)
@@ -97,46 +138,23 @@ __device__ inline McBlock block_of(const McLattice& g, uint32_t b) {
   k.cy = min(g.br[1], g.ny - 1 - k.oy);
   k.cz = min(g.br[2], g.nz - 1 - k.oz);
   k.ncell = k.cx * k.cy * k.cz;
+  tile_geometry(k.cy, k.cz, k.rpt, k.seg, k.tpp);
   return k;
 }
 
-// Block-local cell coordinates of this thread's cells, base + it*256 + thread,
-// advanced in mixed radix (cz, cy) without per-cell division.  I is the
-// index type of block-local cell numbers (uint32_t when they fit).
-template <typename I>
-struct CellWalk {
-  I i, j, k, lc;
-  I cy, cz, di, dj, dk;
-  __device__ CellWalk(const McBlock& b, int64_t base) {
-    cy = (I)b.cy;
-    cz = (I)b.cz;
-    lc = (I)base + (I)threadIdx.x;
-    k = lc % cz;
-    const I t = lc / cz;
-    j = t % cy;
-    i = t / cy;
-    dk = (I)kMcBlock % cz;
-    const I s = (I)kMcBlock / cz;
-    dj = s % cy;
-    di = s / cy;
+// Tile (within the block) and in-tile cell of block-local cell (i, j, k).
+__device__ inline void tile_of_cell(const McBlock& b, int64_t i, int64_t j, int64_t k,
+                                    int64_t& tile, int64_t& c) {
+  if (b.seg == 1) {
+    const int64_t jt = j / b.rpt;
+    tile = i * b.tpp + jt;
+    c = (j - jt * b.rpt) * b.cz + k;
+  } else {
+    const int64_t ks = k / 1024;
+    tile = i * b.tpp + j * b.seg + ks;
+    c = k - ks * 1024;
   }
-  __device__ void next() {
-    lc += (I)kMcBlock;
-    k += dk;
-    I cj = 0;
-    if (k >= cz) {
-      k -= cz;
-      cj = 1;
-    }
-    j += dj + cj;
-    I ci = 0;
-    if (j >= cy) {
-      j -= cy;
-      ci = 1;
-    }
-    i += di + ci;
-  }
-};
+}
 
 // bit e set if edge e is owned by block-local cell (i,j,k)
 __device__ inline int owned_mask(bool i0, bool j0, bool k0) {
@@ -213,122 +231,138 @@ __device__ inline int64_t block_excl_scan64(int64_t x, int64_t* sh, int64_t* tot
 }
 
 // ---------------------------------------------------------------------------
-// 1. classify: one workgroup per tile.  Cells of a tile are visited as local
-//    index base + it*256 + thread (coalesced volume reads).  Tiles without a
-//    crossed edge write only their zero counts.
+// 1. classify, marching along x.  Workgroup (block b, tile column q) visits the
+//    block's x-planes in order and classifies tile (i, q) of each.  The lattice
+//    points under the tile column in two consecutive planes (rows j0 .. j0 +
+//    rows, columns k0 .. k0 + cols) sit in an LDS ring of three plane buffers;
+//    the loads of plane i + 2 are issued before the cells of plane i are
+//    classified and land in the free buffer afterwards, so they are in flight
+//    meanwhile.  Each lattice point is read from memory once per tile column
+//    (the row after a tile's last row once more).  Tiles without a crossed edge
+//    write only their counts (the rest of the count arrays is zero-filled first).
 // ---------------------------------------------------------------------------
-template <typename I>
+constexpr int kMcPref = 9;  // prefetch registers per thread: planes up to 2304 points
+
+__global__ void __launch_bounds__(256)
+mc_zero_counts_kernel(int64_t ntiles, int* __restrict__ tile_v, int* __restrict__ tile_f) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) {
+    tile_v[t] = 0;
+    tile_f[t] = 0;
+  }
+}
+
 __global__ void __launch_bounds__(kMcBlock)
-mc_classify_kernel(McLattice g, uint8_t* __restrict__ cidx, uint16_t* __restrict__ vpre,
-                   int* __restrict__ tile_v, int* __restrict__ tile_f) {
+mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __restrict__ cidx,
+                   uint16_t* __restrict__ vpre, int* __restrict__ tile_v,
+                   int* __restrict__ tile_f) {
+  extern __shared__ float ring[];  // 3 planes of np_max points
   __shared__ int wv[kMcItems][kMcWaves];
   __shared__ int wf[kMcWaves];
-  __shared__ float stage[2 * (kMcTile + 1) * 2];  // 2 planes x (R+1) rows x (cz+1), R*cz = 1024
-  const uint32_t gt = blockIdx.x;
-  const uint32_t tpb = (uint32_t)g.tpb;
-  const uint32_t b = gt / tpb;
-  const int64_t base = (int64_t)(gt - b * tpb) * kMcTile;
-  const McBlock k = block_of(g, b);
-  if (base >= k.ncell) {
-    if (threadIdx.x == 0) {
-      tile_v[gt] = 0;
-      tile_f[gt] = 0;
-    }
-    return;
-  }
+  const int64_t b = (int64_t)blockIdx.x / tpp_max;
+  const int64_t q = (int64_t)blockIdx.x - b * tpp_max;
+  const McBlock k = block_of(g, (uint32_t)b);
+  if (q >= k.tpp || k.ncell == 0) return;
+  const McTileRows tr = tile_rows(k, q);
+  const int64_t pw = tr.cols + 1;            // points per staged row
+  const int64_t np = (tr.rows + 1) * pw;     // points per staged plane
+  const int64_t ncells = tr.rows * tr.cols;  // cells per tile
   const int64_t sy = g.nz, sx = g.ny * g.nz;
-  CellWalk<I> cw(k, base);
-  int cube[kMcItems], nv[kMcItems], inc[kMcItems];
-  int nf = 0;
+  const float* pl0 = g.v + (k.ox * g.ny + (k.oy + tr.j0)) * g.nz + (k.oz + tr.k0);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // Staged path: when z-rows divide the tile (cz | 1024 and 1024 | cy*cz) the tile
-  // is R = 1024/cz whole rows of one plane; its 2 x (R+1) x (cz+1) lattice points
-  // are loaded once, row by row with coalesced loads, and every corner is read
-  // from LDS.  Other blocks (ragged edges) load corners from global memory.
-  const bool staged = (kMcTile % k.cz == 0) && ((k.cy * k.cz) % kMcTile == 0);
-  const int64_t zp = k.cz + 1;  // LDS row pitch (points per staged row)
-  int64_t jrow0 = 0;
-  if (staged) {
-    const int64_t R = kMcTile / k.cz;
-    const int64_t i0 = base / (k.cy * k.cz);
-    jrow0 = (base % (k.cy * k.cz)) / k.cz;
-    for (int64_t row = w; row < 2 * (R + 1); row += kMcWaves) {
-      const int64_t a = row / (R + 1), rr = row % (R + 1);
-      const float* src = g.v + ((k.ox + i0 + a) * g.ny + (k.oy + jrow0 + rr)) * g.nz + k.oz;
-      for (int64_t kk = lane; kk < zp; kk += 64) stage[row * zp + kk] = src[kk];
-    }
-    __syncthreads();
+  // this thread's share of a staged plane: element e = thread + 256 u at row e / pw
+  int64_t goff[kMcPref];
+#pragma unroll
+  for (int u = 0; u < kMcPref; ++u) {
+    const int64_t e = threadIdx.x + (int64_t)u * kMcBlock;
+    const int64_t r = e / pw;
+    goff[u] = e < np ? r * sy + (e - r * pw) : -1;
   }
+  // this thread's cells c = it * 256 + thread: staged offset of corner (0,0,0)
+  // and the block-boundary flags of their owned edges
+  int64_t coff[kMcItems];
+  int bmask[kMcItems];
+  bool cvalid[kMcItems];
 #pragma unroll
   for (int it = 0; it < kMcItems; ++it) {
-    const bool valid = (int64_t)cw.lc < k.ncell;
-    float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f;
-    float b00, b01, b10, b11;
-    if (staged) {
-      const int64_t R1 = kMcTile / k.cz + 1;
-      const int64_t jj = (int64_t)cw.j - jrow0;
-      const float* s0 = stage + jj * zp + (int64_t)cw.k;  // plane i0, row jj
-      const float* s1 = s0 + R1 * zp;                      // plane i0 + 1
-      a00 = s0[0];  a01 = s0[zp];  a10 = s1[0];  a11 = s1[zp];
-      b00 = s0[1];  b01 = s0[zp + 1];  b10 = s1[1];  b11 = s1[zp + 1];
-    } else {
-      const float* p = valid ? g.v + ((k.ox + (int64_t)cw.i) * g.ny + (k.oy + (int64_t)cw.j)) *
-                                         g.nz + (k.oz + (int64_t)cw.k)
-                             : g.v;
-      if (valid) {
-        a00 = p[0];
-        a01 = p[sy];
-        a10 = p[sx];
-        a11 = p[sx + sy];
-      }
-      // values at k+1 come from the next lane when it holds cell k+1 of the same row
-      b00 = __shfl_down(a00, 1);
-      b01 = __shfl_down(a01, 1);
-      b10 = __shfl_down(a10, 1);
-      b11 = __shfl_down(a11, 1);
-      const bool next_ok = lane < 63 && (cw.k + 1 < cw.cz) && ((int64_t)cw.lc + 1 < k.ncell);
-      if (valid && !next_ok) {
-        b00 = p[1];
-        b01 = p[sy + 1];
-        b10 = p[sx + 1];
-        b11 = p[sx + sy + 1];
-      }
-    }
-    int c = 0, n = 0;
-    if (valid) {
-      const double iso = g.iso;
-      c = ((double)a00 <= iso) | (((double)a10 <= iso) << 1) | (((double)a11 <= iso) << 2) |
-          (((double)a01 <= iso) << 3) | (((double)b00 <= iso) << 4) | (((double)b10 <= iso) << 5) |
-          (((double)b11 <= iso) << 6) | (((double)b01 <= iso) << 7);
-      const int word = dMcCase.w[c];
-      n = __popc(word & 0xfff & owned_mask(cw.i == 0, cw.j == 0, cw.k == 0));
-      nf += word >> 12;
-    }
-    cube[it] = c;
-    nv[it] = n;
-    inc[it] = wave_incl_scan(n);
-    if (lane == 63) wv[it][w] = inc[it];
-    cw.next();
+    const int64_t c = (int64_t)it * kMcBlock + threadIdx.x;
+    cvalid[it] = c < ncells;
+    const int64_t jj = cvalid[it] ? c / tr.cols : 0;
+    const int64_t kk = cvalid[it] ? c - jj * tr.cols : 0;
+    coff[it] = jj * pw + kk;
+    bmask[it] = ((tr.j0 + jj) == 0 ? 1 : 0) | ((tr.k0 + kk) == 0 ? 2 : 0);
   }
-  nf = wave_sum(nf);
-  if (lane == 0) wf[w] = nf;
-  __syncthreads();
-  int pre[kMcItems];
-  const int tv = tile_prefix(inc, nv, wv, pre);
-  int tf = 0;
+  float r[kMcPref];
+  auto load_plane = [&](int64_t i) {
+    const float* src = pl0 + i * sx;
 #pragma unroll
-  for (int q = 0; q < kMcWaves; ++q) tf += wf[q];
-  if (tv != 0 || tf != 0) {
-    const int64_t slot0 = (int64_t)gt * kMcTile + threadIdx.x;
+    for (int u = 0; u < kMcPref; ++u)
+      if (goff[u] >= 0) r[u] = src[goff[u]];
+  };
+  auto store_plane = [&](int64_t slot) {
+    float* dst = ring + slot * np_max;
+#pragma unroll
+    for (int u = 0; u < kMcPref; ++u)
+      if (goff[u] >= 0) dst[threadIdx.x + u * kMcBlock] = r[u];
+  };
+  load_plane(0);
+  store_plane(0);
+  load_plane(1);
+  store_plane(1);
+  __syncthreads();
+  const double iso = g.iso;
+  const int64_t tile_base = b * g.tpb + q;
+  for (int64_t i = 0; i < k.cx; ++i) {
+    const bool pre = i + 2 <= k.cx;  // plane i + 2 exists (points 0 .. cx)
+    if (pre) load_plane(i + 2);
+    const float* s0 = ring + (i % 3) * np_max;
+    const float* s1 = ring + ((i + 1) % 3) * np_max;
+    int cube[kMcItems], nv[kMcItems], inc[kMcItems];
+    int nf = 0;
 #pragma unroll
     for (int it = 0; it < kMcItems; ++it) {
-      cidx[slot0 + it * kMcBlock] = (uint8_t)cube[it];
-      vpre[slot0 + it * kMcBlock] = (uint16_t)pre[it];
+      int c = 0, n = 0;
+      if (cvalid[it]) {
+        const float* p0 = s0 + coff[it];
+        const float* p1 = s1 + coff[it];
+        const float a00 = p0[0], a01 = p0[pw], a10 = p1[0], a11 = p1[pw];
+        const float b00 = p0[1], b01 = p0[pw + 1], b10 = p1[1], b11 = p1[pw + 1];
+        c = ((double)a00 <= iso) | (((double)a10 <= iso) << 1) | (((double)a11 <= iso) << 2) |
+            (((double)a01 <= iso) << 3) | (((double)b00 <= iso) << 4) |
+            (((double)b10 <= iso) << 5) | (((double)b11 <= iso) << 6) |
+            (((double)b01 <= iso) << 7);
+        const int word = dMcCase.w[c];
+        n = __popc(word & 0xfff & owned_mask(i == 0, bmask[it] & 1, bmask[it] & 2));
+        nf += word >> 12;
+      }
+      cube[it] = c;
+      nv[it] = n;
+      inc[it] = wave_incl_scan(n);
+      if (lane == 63) wv[it][w] = inc[it];
     }
-  }
-  if (threadIdx.x == 0) {
-    tile_v[gt] = tv;
-    tile_f[gt] = tf;
+    nf = wave_sum(nf);
+    if (lane == 0) wf[w] = nf;
+    __syncthreads();
+    int pre_v[kMcItems];
+    const int tv = tile_prefix(inc, nv, wv, pre_v);
+    int tf = 0;
+#pragma unroll
+    for (int qq = 0; qq < kMcWaves; ++qq) tf += wf[qq];
+    const int64_t gt = tile_base + i * k.tpp;
+    if (tv != 0 || tf != 0) {
+      const int64_t slot0 = gt * kMcTile + threadIdx.x;
+#pragma unroll
+      for (int it = 0; it < kMcItems; ++it) {
+        cidx[slot0 + it * kMcBlock] = (uint8_t)cube[it];
+        vpre[slot0 + it * kMcBlock] = (uint16_t)pre_v[it];
+      }
+      if (threadIdx.x == 0) {
+        tile_v[gt] = tv;
+        tile_f[gt] = tf;
+      }
+    }
+    if (pre) store_plane((i + 2) % 3);
+    __syncthreads();
   }
 }
 
@@ -425,16 +459,16 @@ struct McOffsets {
 // Global index of the vertex on edge e of cell (i,j,k), which that cell does
 // not own: found in the owner cell (same block) from its cube index, the
 // owner tile's offset and the owner's in-tile prefix.
-template <int E, typename I>
-__device__ inline int64_t shared_vertex(const McOffsets& o, int64_t tile0, I i, I j, I k, I cy,
-                                        I cz) {
+template <int E>
+__device__ inline int64_t shared_vertex(const McOffsets& o, int64_t tile0, int64_t i, int64_t j,
+                                        int64_t k, const McBlock& blk) {
   constexpr int a = kEa[E], b = kEb[E];
   constexpr int lx = kCx[a] < kCx[b] ? kCx[a] : kCx[b];
   constexpr int ly = kCy[a] < kCy[b] ? kCy[a] : kCy[b];
   constexpr int lz = kCz[a] < kCz[b] ? kCz[a] : kCz[b];
   constexpr int axis = E < 8 ? (E & 1) : 2;
-  const I px = i + lx, py = j + ly, pz = k + lz;  // low end of the lattice edge
-  I oi, oj, ok;
+  const int64_t px = i + lx, py = j + ly, pz = k + lz;  // low end of the lattice edge
+  int64_t oi, oj, ok;
   int oe;
   if (axis == 0) {
     oi = px;
@@ -455,9 +489,10 @@ __device__ inline int64_t shared_vertex(const McOffsets& o, int64_t tile0, I i, 
     else if (py == 0) { oi = px - 1; oj = 0; oe = 9; }
     else { oi = 0; oj = py - 1; oe = 11; }
   }
-  const int64_t olc = ((int64_t)oi * cy + oj) * cz + ok;
-  const int64_t ot = tile0 + olc / kMcTile;
-  const int64_t oslot = tile0 * kMcTile + olc;
+  int64_t otl, oc;
+  tile_of_cell(blk, oi, oj, ok, otl, oc);
+  const int64_t ot = tile0 + otl;
+  const int64_t oslot = ot * kMcTile + oc;
   const int oown = dMcCase.w[o.cidx[oslot]] & owned_mask(oi == 0, oj == 0, ok == 0);
   int bm = 0;
   switch (oe) {  // edges before oe in kOrder
@@ -477,11 +512,12 @@ __device__ inline int64_t shared_vertex(const McOffsets& o, int64_t tile0, I i, 
   return o.toff_v[ot] + o.gsum_v[ot / kScanGroup] + o.vpre[oslot] + __popc(oown & bm);
 }
 
-template <int Q, typename I>
+template <int Q>
 __device__ inline void emit_edge(const McLattice& g, const McWorld& wt, const McOffsets& o,
                                  const double (&org)[3], const float* p, int64_t sx, int64_t sy,
-                                 int64_t tile0, I i, I j, I k, I cy, I cz, int edges, int own,
-                                 int64_t vb, double* __restrict__ V, int64_t* sidx) {
+                                 int64_t tile0, int64_t i, int64_t j, int64_t k, const McBlock& blk,
+                                 int edges, int own, int64_t vb, double* __restrict__ V,
+                                 int64_t* sidx) {
   constexpr int e = kOrder[Q];
   if (!(edges & (1 << e))) return;
   if (own & (1 << e)) {
@@ -508,11 +544,10 @@ __device__ inline void emit_edge(const McLattice& g, const McWorld& wt, const Mc
     out[2] = q[2];
     sidx[e * kMcBlock] = vi;
   } else {
-    sidx[e * kMcBlock] = shared_vertex<e>(o, tile0, i, j, k, cy, cz);
+    sidx[e * kMcBlock] = shared_vertex<e>(o, tile0, i, j, k, blk);
   }
 }
 
-template <typename I>
 __global__ void __launch_bounds__(kMcBlock)
 mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, int64_t ntiles,
                const int* __restrict__ tile_v, const int* __restrict__ tile_f,
@@ -539,10 +574,14 @@ mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, int64_t ntiles,
     org[2] = (double)wt.ax[2][k.oz];
   }
   const int64_t slot0 = (int64_t)gt * kMcTile + threadIdx.x;
+  // the tile's place in the block: x-plane ti and tile column tq
+  const int64_t ti = base / kMcTile / k.tpp;
+  const McTileRows tr = tile_rows(k, base / kMcTile - ti * k.tpp);
+  const int64_t ncells = tr.rows * tr.cols;
   int cube[kMcItems], nf[kMcItems], inc[kMcItems];
 #pragma unroll
   for (int it = 0; it < kMcItems; ++it) {
-    const bool in = base + it * kMcBlock + (int64_t)threadIdx.x < k.ncell;
+    const bool in = it * kMcBlock + (int64_t)threadIdx.x < ncells;
     cube[it] = in ? o.cidx[slot0 + it * kMcBlock] : 0;
     nf[it] = dMcCase.w[cube[it]] >> 12;
     inc[it] = wave_incl_scan(nf[it]);
@@ -551,18 +590,19 @@ mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, int64_t ntiles,
   __syncthreads();
   int fpre[kMcItems];
   tile_prefix(inc, nf, wf, fpre);
-  CellWalk<I> cw(k, base);
 #pragma unroll 1
-  for (int it = 0; it < kMcItems; ++it, cw.next()) {
+  for (int it = 0; it < kMcItems; ++it) {
     const int edges = dMcCase.w[cube[it]] & 0xfff;
     if (!edges) continue;
-    const I i = cw.i, j = cw.j, kk = cw.k;
+    const int64_t c = it * kMcBlock + (int64_t)threadIdx.x;
+    const int64_t jj = c / tr.cols;
+    const int64_t i = ti, j = tr.j0 + jj, kk = tr.k0 + (c - jj * tr.cols);
     const int own = edges & owned_mask(i == 0, j == 0, kk == 0);
     const float* p = g.v + ((k.ox + (int64_t)i) * g.ny + (k.oy + (int64_t)j)) * g.nz +
                      (k.oz + (int64_t)kk);
     const int64_t vb = vbase + o.vpre[slot0 + it * kMcBlock];
 #define NA_MC_EDGE(Q) \
-  emit_edge<Q>(g, wt, o, org, p, sx, sy, tile0, i, j, kk, cw.cy, cw.cz, edges, own, vb, V, sidx)
+  emit_edge<Q>(g, wt, o, org, p, sx, sy, tile0, i, j, kk, k, edges, own, vb, V, sidx)
     NA_MC_EDGE(0); NA_MC_EDGE(1); NA_MC_EDGE(2); NA_MC_EDGE(3);
     NA_MC_EDGE(4); NA_MC_EDGE(5); NA_MC_EDGE(6); NA_MC_EDGE(7);
     NA_MC_EDGE(8); NA_MC_EDGE(9); NA_MC_EDGE(10); NA_MC_EDGE(11);
@@ -611,8 +651,9 @@ mc_block_offsets_kernel(int64_t nblocks, int64_t tpb, int64_t ntiles,
 struct McPlan {
   McLattice g;
   int64_t ntiles, ngroups;
+  int64_t tpp;     // tiles per x-plane of a full block (the classify grid's tile columns)
+  int64_t np_max;  // lattice points of the largest staged plane over the block shapes
   bool empty;
-  bool small;  // block-local cell numbers (plus one tile of slack) fit in uint32
 };
 
 static McPlan mc_plan(const float* v, int64_t nx, int64_t ny, int64_t nz, int64_t block_res,
@@ -630,10 +671,24 @@ static McPlan mc_plan(const float* v, int64_t nx, int64_t ny, int64_t nz, int64_
     p.g.br[a] = block_res > 0 ? std::min<int64_t>(block_res, n[a] - 1) : n[a] - 1;
     p.g.nb[a] = ceil_div(n[a] - 1, p.g.br[a]);
   }
-  p.g.tpb = ceil_div(p.g.br[0] * p.g.br[1] * p.g.br[2], (int64_t)kMcTile);
+  // a smaller (last) block along y or z has no more tiles per plane than a full
+  // one (fewer rows, and at least as many rows per tile), so the full block's
+  // count bounds every block's
+  int64_t rpt, seg;
+  tile_geometry(p.g.br[1], p.g.br[2], rpt, seg, p.tpp);
+  p.g.tpb = p.g.br[0] * p.tpp;
   p.ntiles = p.g.nb[0] * p.g.nb[1] * p.g.nb[2] * p.g.tpb;
   p.ngroups = ceil_div(p.ntiles, (int64_t)kScanGroup);
-  p.small = p.g.tpb * kMcTile + kMcBlock < (int64_t)UINT32_MAX;
+  p.np_max = 0;
+  const int64_t ys[2] = {p.g.br[1], (ny - 1) - (p.g.nb[1] - 1) * p.g.br[1]};
+  const int64_t zs[2] = {p.g.br[2], (nz - 1) - (p.g.nb[2] - 1) * p.g.br[2]};
+  for (int64_t cy : ys)
+    for (int64_t cz : zs) {
+      int64_t r, sg, tp;
+      tile_geometry(cy, cz, r, sg, tp);
+      const int64_t rows = std::min(r, cy), cols = std::min(cz, (int64_t)1024);
+      p.np_max = std::max(p.np_max, (rows + 1) * (cols + 1));
+    }
   return p;
 }
 
@@ -701,12 +756,14 @@ extern "C" int na_mc_count(const float* volume, int64_t nx, int64_t ny, int64_t 
   NA_CHECK_ARG((int64_t)mc_ws_layout(p, nullptr, nullptr) <= ws_bytes, "workspace too small");
   McWorkspace w;
   mc_ws_layout(p, workspace, &w);
-  if (p.small)
-    hipLaunchKernelGGL(mc_classify_kernel<uint32_t>, dim3((unsigned)p.ntiles), dim3(kMcBlock), 0,
-                       s, p.g, w.cidx, w.vpre, w.tile_v, w.tile_f);
-  else
-    hipLaunchKernelGGL(mc_classify_kernel<uint64_t>, dim3((unsigned)p.ntiles), dim3(kMcBlock), 0,
-                       s, p.g, w.cidx, w.vpre, w.tile_v, w.tile_f);
+  NA_CHECK_ARG(p.np_max <= (int64_t)kMcPref * kMcBlock, "staged plane too large");
+  hipLaunchKernelGGL(mc_zero_counts_kernel, dim3((unsigned)ceil_div(p.ntiles, (int64_t)256)),
+                     dim3(256), 0, s, p.ntiles, w.tile_v, w.tile_f);
+  NA_CHECK_LAUNCH();
+  const int64_t nblocks = p.g.nb[0] * p.g.nb[1] * p.g.nb[2];
+  const size_t lds = (size_t)(3 * p.np_max) * sizeof(float);
+  hipLaunchKernelGGL(mc_classify_kernel, dim3((unsigned)(nblocks * p.tpp)), dim3(kMcBlock), lds, s,
+                     p.g, p.tpp, p.np_max, w.cidx, w.vpre, w.tile_v, w.tile_f);
   NA_CHECK_LAUNCH();
   hipLaunchKernelGGL(mc_scan_tiles_kernel, dim3((unsigned)p.ngroups), dim3(kMcBlock), 0, s,
                      p.ntiles, w.tile_v, w.tile_f, w.toff_v, w.toff_f, w.gsum_v, w.gsum_f);
@@ -734,12 +791,8 @@ extern "C" int na_mc_emit(const float* volume, int64_t nx, int64_t ny, int64_t n
   hipStream_t s = (hipStream_t)stream;
   McOffsets o{w.cidx, w.vpre, w.toff_v, w.gsum_v};
   const dim3 grid((unsigned)p.ntiles);
-  if (p.small)
-    hipLaunchKernelGGL(mc_emit_kernel<uint32_t>, grid, dim3(kMcBlock), 0, s, p.g, wt, o, p.ntiles,
-                       w.tile_v, w.tile_f, w.toff_f, w.gsum_f, vertices, faces);
-  else
-    hipLaunchKernelGGL(mc_emit_kernel<uint64_t>, grid, dim3(kMcBlock), 0, s, p.g, wt, o, p.ntiles,
-                       w.tile_v, w.tile_f, w.toff_f, w.gsum_f, vertices, faces);
+  hipLaunchKernelGGL(mc_emit_kernel, grid, dim3(kMcBlock), 0, s, p.g, wt, o, p.ntiles, w.tile_v,
+                     w.tile_f, w.toff_f, w.gsum_f, vertices, faces);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -30,7 +30,7 @@ __device__ inline void st_nt(float* p, const float4& v) {
 }
 
 // The update of the 4-parameter chunk at i (element-wise tail past n - 4).
-template <bool LOWP>
+template <bool LOWP, bool CLEAR = false>
 __device__ inline void adam_chunk(float* __restrict__ p, const float* __restrict__ g,
                                   float* __restrict__ m, float* __restrict__ v, int64_t n,
                                   int64_t i, const AdamArgs& a, bf16_t* __restrict__ lowp) {
@@ -38,6 +38,7 @@ __device__ inline void adam_chunk(float* __restrict__ p, const float* __restrict
   if (i + 4 <= n) {
     float4 pp = ld_nt(p + i);
     float4 gg = ld_nt(g + i);
+    if (CLEAR) st_nt(const_cast<float*>(g) + i, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     float4 mm = ld_nt(m + i);
     float4 vv = ld_nt(v + i);
     float* P = &pp.x; const float* G = &gg.x; float* Mv = &mm.x; float* V = &vv.x;
@@ -60,9 +61,11 @@ __device__ inline void adam_chunk(float* __restrict__ p, const float* __restrict
     }
   } else {
     for (int64_t j = i; j < n; ++j) {
+      const float gj = g[j];
+      if (CLEAR) const_cast<float*>(g)[j] = 0.0f;
       float pe = p[j] * a.decay;
-      float me = lerp_torch(m[j], g[j], 1.0f - a.beta1);
-      float ve = v[j] * a.beta2 + (1.0f - a.beta2) * g[j] * g[j];
+      float me = lerp_torch(m[j], gj, 1.0f - a.beta1);
+      float ve = v[j] * a.beta2 + (1.0f - a.beta2) * gj * gj;
       float den = sqrtf(ve) * a.inv_bc2_sqrt + a.eps;
       pe = pe - a.step_size * me / den;
       p[j] = pe; m[j] = me; v[j] = ve;
@@ -73,11 +76,12 @@ __device__ inline void adam_chunk(float* __restrict__ p, const float* __restrict
 
 // One chunk of 4 parameters per thread (no grid-stride loop: the launch holds
 // n / 4 threads, which keeps the most loads in flight).
-template <bool LOWP>
+template <bool LOWP, bool CLEAR>
 __global__ void __launch_bounds__(256)
-adamw_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+adamw_kernel(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m,
              float* __restrict__ v, int64_t n, AdamArgs a, bf16_t* __restrict__ lowp) {
-  adam_chunk<LOWP>(p, g, m, v, n, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4, a, lowp);
+  adam_chunk<LOWP, CLEAR>(p, g, m, v, n, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4, a,
+                          lowp);
 }
 
 // Several tensors in one launch: the list travels in the kernel arguments;
@@ -116,10 +120,10 @@ static AdamArgs adam_args(float lr, float beta1, float beta2, float eps, float w
   return a;
 }
 
-extern "C" int na_adamw_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+extern "C" int na_adamw_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq,
                              int64_t n, float lr, float beta1, float beta2, float eps,
                              float weight_decay, int64_t step, void* lowp, int32_t lowp_dtype,
-                             na_stream_t stream) {
+                             int32_t clear_grad, na_stream_t stream) {
   NA_CHECK_ARG(n >= 0 && step >= 1, "bad sizes / step");
   if (n == 0) return NA_OK;
   NA_CHECK_ARG(param && grad && exp_avg && exp_avg_sq, "null pointer");
@@ -132,12 +136,15 @@ extern "C" int na_adamw_step(float* param, const float* grad, float* exp_avg, fl
   const int64_t blocks = ceil_div(threads, 256);
   NA_CHECK_ARG(blocks <= 0x7fffffffLL, "too many parameters for one launch");
   hipStream_t s = (hipStream_t)stream;
-  if (lowp)
-    hipLaunchKernelGGL(adamw_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, param, grad,
-                       exp_avg, exp_avg_sq, n, a, (bf16_t*)lowp);
-  else
-    hipLaunchKernelGGL(adamw_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, param, grad,
-                       exp_avg, exp_avg_sq, n, a, (bf16_t*)nullptr);
+#define NA_ADAM(L, C)                                                                          \
+  hipLaunchKernelGGL((adamw_kernel<L, C>), dim3((unsigned)blocks), dim3(256), 0, s, param, grad, \
+                     exp_avg, exp_avg_sq, n, a, (bf16_t*)lowp)
+  if (lowp) {
+    if (clear_grad) NA_ADAM(true, true); else NA_ADAM(true, false);
+  } else {
+    if (clear_grad) NA_ADAM(false, true); else NA_ADAM(false, false);
+  }
+#undef NA_ADAM
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

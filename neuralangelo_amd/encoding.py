@@ -67,7 +67,7 @@ class _HashGridFunction(torch.autograd.Function):
                 "implemented; use numerical gradients")
         grad_params = None
         if ctx.needs_input_grad[1]:
-            grad_params = torch.zeros(enc.n_params, dtype=torch.float32, device=x.device)
+            grad_params = enc.table_grad_buffer()
             enc.backward_into(x, dy, grad_params)
         return None, grad_params, None
 
@@ -99,6 +99,9 @@ class Encoding(torch.nn.Module):
         init = torch.rand(self.n_params, generator=gen, dtype=torch.float32) * 2e-4 - 1e-4
         self.params = torch.nn.Parameter(init)
         self.active_levels = self.n_levels
+        # table gradient by the sorted reduction (bitwise repeatable) instead of
+        # float atomics; torch.use_deterministic_algorithms(True) selects it too
+        self.deterministic = False
         self._lowp_cache = None  # (data_ptr, version, tensor)
 
     # -- level table accessors -------------------------------------------------
@@ -139,6 +142,40 @@ class Encoding(torch.nn.Module):
             self._lowp_cache = ((p.data_ptr(), p._version), p.detach().to(self.dtype))
         return self._lowp_cache[1]
 
+    def table_grad_buffer(self):
+        """A zeroed fp32 gradient buffer of the table for a backward pass.  When the
+        optimizer cleared the previous gradient in its update pass (na_adamw_step with
+        clear_grad) and handed it back, that buffer is reused -- it is zero already, so
+        no fill runs; else a new zero-filled buffer.  The spare is taken (not kept), so
+        autograd can adopt it as params.grad without a copy; it is only reused while
+        params.grad is None (a gradient still attached to the parameter is never
+        aliased)."""
+        spare, self._spare_grad = getattr(self, "_spare_grad", None), None
+        p = self.params
+        if (spare is not None and p.grad is None and spare.device == p.device
+                and spare.numel() == self.n_params):
+            return spare
+        return torch.zeros(self.n_params, dtype=torch.float32, device=p.device)
+
+    def sorted_grad(self):
+        """True when the table gradient takes the sorted reduction: per table entry the
+        contributions are summed in a fixed (point, evaluation, corner) order, so the
+        gradient is bitwise repeatable (``deterministic = True`` on this encoder, or
+        torch.use_deterministic_algorithms(True)).  The default scatter uses float
+        atomics, as tcnn's backward does."""
+        return bool(self.deterministic) or torch.are_deterministic_algorithms_enabled()
+
+    def sorted_workspace(self, n, evals, device):
+        """Workspace (uint8) of the sorted reduction for n points of `evals` (1 or 5)
+        evaluations each."""
+        nbytes = _lib.call("na_hashgrid_sorted_workspace", ctypes.byref(self.desc), int(n),
+                           int(evals))
+        return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+    def take_cleared_grad(self, grad):
+        """Keep `grad` (zeroed by the optimizer's update pass) for the next backward."""
+        self._spare_grad = grad
+
     def mark_shadow_fresh(self):
         """Declare the bf16 copy up to date with the master (written by the optimizer)."""
         if self._lowp_cache is not None:
@@ -164,6 +201,12 @@ class Encoding(torch.nn.Module):
         dy = dy.contiguous()
         if dy.dtype not in (torch.float32, torch.bfloat16):
             dy = dy.float()
+        if self.sorted_grad():
+            ws = self.sorted_workspace(x.shape[0], 1, x.device)
+            _lib.call("na_hashgrid_bwd_sorted", ctypes.byref(self.desc), _lib.ptr(x), x.shape[0],
+                      _lib.ptr(dy), _lib.dtype_code(dy.dtype), _lib.ptr(grad_params),
+                      int(self.active_levels), _lib.ptr(ws), ws.numel(), _lib.stream_of(x))
+            return
         _lib.call("na_hashgrid_bwd", ctypes.byref(self.desc), _lib.ptr(x), x.shape[0],
                   _lib.ptr(dy), _lib.dtype_code(dy.dtype), _lib.ptr(grad_params),
                   int(self.active_levels), _lib.stream_of(x))

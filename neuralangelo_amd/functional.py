@@ -32,6 +32,10 @@ RGB_LD = 280  # 256 features + 3 + 16 + 3, padded to a multiple of 8
 # Optional HIP-event timing of named launches on the launching stream: a dict
 # name -> list that collects (start, end) torch.cuda.Event pairs while set.
 LAUNCH_EVENTS = None
+# Optional record of the table-gradient scatter's inputs: while a list, each launch
+# of the centre + taps scatter appends (points, eps) (used to count the table
+# entries a launch touches).
+SCATTER_CAPTURE = None
 
 
 class _timed:
@@ -242,6 +246,18 @@ class SDFWeights:
         self.ws_shape, self.bs_shape = ws.shape, bs.shape
         self.hidden = W0.shape[0]
 
+    def increment_weights(self, ld_d):
+        """First-layer weights against the tap-increment rows [hi part and point | lo
+        part of the encoder columns]: [W0p | W_enc | 0] of width ld_d (bf16)."""
+        key = (ld_d, self.W0p.data_ptr(), self.W0p._version)
+        if getattr(self, "_inc_key", None) != key:
+            n, k0p = self.n_enc, self.k0p
+            W = torch.zeros(self.W0p.shape[0], ld_d, dtype=self.W0p.dtype, device=self.W0p.device)
+            W[:, :k0p] = self.W0p
+            W[:, k0p:k0p + n] = self.W0p[:, :n]
+            self._inc, self._inc_key = W, key
+        return self._inc
+
     def unpermute_dW0(self, dW0p):
         """[hidden, K0p] kernel-layout gradient -> reference layout [hidden, 3+LF].
         With the split layout d/dW_x = sum dpre (x_hi + x_lo): columns hi + lo."""
@@ -326,14 +342,17 @@ class _SDFGrads:
         self.dW1, self.db1 = (z[4], z[5]) if with_feat else (None, None)
 
 
-def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc):
+def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc, dpre0=None, wgrad=True):
     """Backward through the SDF MLP; weight gradients accumulate into `acc`.
-    Returns the first layer's pre-activation gradient dpre0 [P, hid] (compute dtype)."""
+    Returns the first layer's pre-activation gradient dpre0 [P, hid] (compute dtype,
+    written into `dpre0` when given).  wgrad=False leaves the first layer's weight
+    gradient to the caller (which may still add to dpre0)."""
     P, hid, cdt, dev = X0.shape[0], w.hidden, w.cdt, X0.device
     dsdf = (torch.zeros(P, 1, device=dev) if dsdf is None else dsdf).reshape(P, 1)
     dsdf = dsdf.contiguous().float()
     has_feat = FB is not None and dfeat is not None
-    dpre0 = torch.empty(P, hid, dtype=cdt, device=dev)
+    if dpre0 is None:
+        dpre0 = torch.empty(P, hid, dtype=cdt, device=dev)
     if has_feat:
         dpre1 = act_grad(dfeat, FB, NA_ACT_SOFTPLUS, cdt, colsum=acc.db1, n_cols=hid)
         linear_weight_grad(dpre1, H0, hid, hid, acc.dW1)
@@ -350,7 +369,8 @@ def _sdf_mlp_backward(X0, H0, FB, w, dsdf, dfeat, acc):
                   _lib.ptr(dsdf), _lib.ptr(acc.dws), _lib.ptr(acc.dbs), _lib.ptr(dpre0), hid,
                   _code(dpre0), 0, _lib.ptr(H0), hid, _code(H0), NA_ACT_SOFTPLUS,
                   SOFTPLUS_BETA, _lib.ptr(acc.db0), _s(X0))
-    linear_weight_grad(dpre0, X0, w.k0p, w.k0p, acc.dW0p)
+    if wgrad:
+        linear_weight_grad(dpre0, X0, w.k0p, w.k0p, acc.dW0p)
     return dpre0
 
 
@@ -406,11 +426,18 @@ class SDFNetFunction(torch.autograd.Function):
         dtable = dinp = None
         if enc is not None and ctx.needs_input_grad[1]:
             dX = _encoder_grad(dpre0, w)
-            dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
+            dtable = enc.table_grad_buffer()
             vmin, vmax = enc.vol_range
-            _lib.call("na_sdf_input_bwd", ctypes.byref(enc.desc), _lib.ptr(points), P, float(vmin),
-                      float(vmax), _lib.ptr(dX), _code(dX), w.n_enc, _lib.ptr(dtable),
-                      int(enc.active_levels), _s(X0))
+            if enc.sorted_grad():
+                ws = enc.sorted_workspace(P, 1, dev)
+                _lib.call("na_sdf_input_bwd_sorted", ctypes.byref(enc.desc), _lib.ptr(points), P,
+                          float(vmin), float(vmax), _lib.ptr(dX), _code(dX), w.n_enc,
+                          _lib.ptr(dtable), int(enc.active_levels), _lib.ptr(ws), ws.numel(),
+                          _s(X0))
+            else:
+                _lib.call("na_sdf_input_bwd", ctypes.byref(enc.desc), _lib.ptr(points), P,
+                          float(vmin), float(vmax), _lib.ptr(dX), _code(dX), w.n_enc,
+                          _lib.ptr(dtable), int(enc.active_levels), _s(X0))
         if enc is None and ctx.needs_input_grad[0]:
             dinp = torch.empty(P, w.k0p, dtype=torch.float32, device=dev)
             gemm(dpre0, w.hidden, 0, _t(w.W0p), w.hidden, 0, P, w.k0p, w.hidden, dinp, w.k0p)
@@ -450,61 +477,138 @@ class SDFCentreTapsFunction(torch.autograd.Function):
                                      ds_taps) + (None,)
 
 
+def taps_increment_width(k0p, n_enc):
+    """Row width of the tap-increment buffer D: the centre layout's k0p columns (hi
+    part and point offset) plus the lo part of the encoder columns, padded to 5, 8 or
+    9 stages of 32 columns (the weights-stationary kernel's K range)."""
+    w = (k0p + n_enc + 31) // 32 * 32
+    for stages in (5, 8, 9):
+        if w <= 32 * stages:
+            return 32 * stages
+    raise NotImplementedError(f"increment input of {k0p + n_enc} columns exceeds 288")
+
+
 class SDFCentreTapsDeltaFunction(torch.autograd.Function):
     """BF16 form of SDFCentreTapsFunction with the tap SDFs as increments.
 
-    Forward: sdf [P,1], feat [P,256] and ds [4,P] = sdf(x + eps k_i) - sdf(x).
-    The encoder writes the tap inputs' differences from the centre input in
-    fp32 before rounding; the tap GEMM multiplies those increments and its
-    epilogue forms softplus(z_c + dz) - softplus(z_c) from the centre's fp32
-    pre-activation z_c, so each increment keeps bf16 precision relative to its
-    own size.  The SDF head runs in fp32 inside the GEMM epilogues.
-    Backward: the derivative of ds_i is that of sdf(x + eps k_i) minus that of
-    sdf(x); the tap passes use the ordinary backward with the stored tap inputs."""
+    Forward: sdf [P,1], feat [P,256] and ds [4,P] = sdf(x + eps k_i) - sdf(x).  The
+    encoder writes each tap input's difference from the centre input,
+    D = X(x + eps k_i) - X(x), in fp32 and stores it as two bf16 parts (rows
+    point-major, 4 per point); the tap product multiplies both parts with the
+    first-layer weights, so dz keeps about 16 mantissa bits relative to its own
+    size, and its epilogue forms softplus(z_c + dz) - softplus(z_c) from the
+    centre's fp32 pre-activation z_c.  The four increments' second-order sum (the
+    Hessian trace) thus keeps float32-like precision.
+    Backward in the same increment form (na_taps_increment_bwd recomputes the fp32
+    dz_i): with dds_i = a + b_i (a the taps' mean, the Hessian-trace coefficient),
+    the tap rows carry ws (b_i sp'(z_c + dz_i) + a dsp'_i) paired with D_i, a shared
+    row a ws sp'(z_c) pairs with sum_i D_i, and the centre row receives its own
+    gradient plus sum_i dds_i ws dsp'_i (dsp'_i = sp'(z_c + dz_i) - sp'(z_c)); the
+    encoder scatter gives the taps' corners g_i + g_shared and the centre's
+    g_c - sum_i (g_i + g_shared), in fp32.  Every bf16 row holds a term of its own
+    natural size: no difference of nearly equal terms is ever rounded to bf16."""
 
     @staticmethod
     def forward(ctx, points, table_params, W0, b0, W1, b1, ws, bs, enc, n_enc, eps):
         _lib.require_cuda(points)
         cdt = torch.bfloat16
         w = SDFWeights(W0, b0, W1, b1, ws, bs, n_enc, cdt)
+        if w.hidden not in (128, 256):
+            raise NotImplementedError("the tap-increment product needs a hidden width of 128 or 256")
         pts = points.contiguous().float()
         P, dev, hid = pts.shape[0], pts.device, w.hidden
+        P32 = (P + 31) // 32 * 32  # 4 * P32 rows: whole 128-row tiles
+        ld_d = taps_increment_width(w.k0p, n_enc)
         X0c = torch.empty(P, w.k0p, dtype=cdt, device=dev)
-        X0t = torch.empty(4 * P, w.k0p, dtype=cdt, device=dev)
-        D = torch.empty(4 * P, w.k0p, dtype=cdt, device=dev)
+        D = torch.empty(4 * P32, ld_d, dtype=cdt, device=dev)
+        if P32 > P:
+            D[4 * P:].zero_()
+        Dsum = torch.empty(P, w.k0p, dtype=cdt, device=dev)
         table = enc.table_for_forward()
         vmin, vmax = enc.vol_range
-        _lib.call("na_sdf_taps_input_fwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
-                  float(vmax), float(eps), _lib.ptr(table), _code(table), int(enc.active_levels),
-                  _lib.ptr(X0c), _lib.ptr(X0t), _lib.ptr(D), w.k0p, _s(pts))
+        _lib.call("na_sdf_taps_increment_input", ctypes.byref(enc.desc), _lib.ptr(pts), P,
+                  float(vmin), float(vmax), float(eps), _lib.ptr(table), _code(table),
+                  int(enc.active_levels), _lib.ptr(X0c), w.k0p, _lib.ptr(D), ld_d, _lib.ptr(Dsum),
+                  _s(pts))
         nparts = head_parts(hid)
         H0c = torch.empty(P, hid, dtype=cdt, device=dev)
-        Zc = torch.empty(P, hid, dtype=torch.float32, device=dev)
+        Zc = torch.empty(P32, hid, dtype=torch.float32, device=dev)
+        if P32 > P:
+            Zc[P:].zero_()
         parts_c = torch.empty(nparts, P, device=dev)
         gemm(X0c, w.k0p, 0, w.W0p, w.k0p, 0, P, hid, w.k0p, H0c, hid, bias=w.b0,
              act=NA_ACT_SOFTPLUS, z_out=Zc, head_w=w.ws.view(-1), head_out=parts_c)
         sdf = head_sum(parts_c, w.bs).view(P, 1)
         FB = torch.empty(P, RGB_LD, dtype=cdt, device=dev)
         gemm(H0c, hid, 0, w.W1, hid, 0, P, hid, hid, FB, RGB_LD, bias=w.b1, act=NA_ACT_SOFTPLUS)
-        H0t = torch.empty(4 * P, hid, dtype=cdt, device=dev)
-        parts_t = torch.empty(nparts, 4 * P, device=dev)
-        gemm(D, w.k0p, 0, w.W0p, w.k0p, 0, 4 * P, hid, w.k0p, H0t, hid, act=NA_ACT_SOFTPLUS,
-             zc=Zc, zc_rows=P, head_w=w.ws.view(-1), head_out=parts_t)
-        ds = head_sum(parts_t).view(4, P)
-        del D, Zc
-        ctx.save_for_backward(pts, X0c, H0c, FB, X0t, H0t)
-        ctx.w, ctx.enc, ctx.eps = w, enc, float(eps)
+        W0d = w.increment_weights(ld_d)
+        parts_t = torch.empty(hid // 32, 4 * P32, device=dev)
+        _lib.call("na_taps_increment_fwd", _lib.ptr(D), ld_d, 4 * P32, ld_d, _lib.ptr(W0d), ld_d,
+                  hid, _lib.ptr(Zc), hid, _lib.ptr(w.ws), float(SOFTPLUS_BETA), _lib.ptr(parts_t),
+                  _s(pts))
+        ds = head_sum(parts_t)[:4 * P].view(P, 4).t().contiguous()
+        ctx.save_for_backward(pts, X0c, H0c, FB, D, Dsum, Zc, W0d)
+        ctx.w, ctx.enc, ctx.eps, ctx.ld_d = w, enc, float(eps), ld_d
         return sdf, FB[:, :hid], ds
 
     @staticmethod
     def backward(ctx, dsdf, dfeat, dds):
-        pts, X0c, H0c, FB, X0t, H0t = ctx.saved_tensors
-        P = pts.shape[0]
+        pts, X0c, H0c, FB, D, Dsum, Zc, W0d = ctx.saved_tensors
+        w, enc, ld_d = ctx.w, ctx.enc, ctx.ld_d
+        P, dev, hid = pts.shape[0], pts.device, w.hidden
+        P32 = Zc.shape[0]
+        acc = _SDFGrads(w, dev, True)
+        # the centre's own pre-activation gradient and the taps' shared rows, in
+        # P32-row buffers (the tap product covers whole 128-row tiles)
+        dpre_c = torch.empty(P32, hid, dtype=w.cdt, device=dev)
+        q = torch.empty(P32, hid, dtype=w.cdt, device=dev)
+        if P32 > P:
+            dpre_c[P:].zero_()
+        _sdf_mlp_backward(X0c, H0c, FB, w, dsdf, dfeat, acc, dpre0=dpre_c[:P], wgrad=False)
+        dpre_t = torch.empty(4 * P32, hid, dtype=w.cdt, device=dev)
+        dds_pm = torch.zeros(P32, 4, device=dev)
         if dds is not None:
-            dds = dds.contiguous().float()
-            dsdf = (torch.zeros(P, 1, device=pts.device) if dsdf is None else dsdf.float())
-            dsdf = dsdf.reshape(P, 1) - dds.sum(dim=0).view(P, 1)
-        return _centre_taps_backward(ctx, pts, X0c, H0c, FB, X0t, H0t, dsdf, dfeat, dds)
+            dds_pm[:P] = dds.t()
+        # the tap increments dz = D W0d^T are recomputed (fp32) into a transient buffer
+        dz = torch.empty(4 * P32, hid, dtype=torch.float32, device=dev)
+        _lib.call("na_taps_increment_bwd", _lib.ptr(D), ld_d, 4 * P32, ld_d, _lib.ptr(W0d), ld_d,
+                  hid, _lib.ptr(Zc), hid, _lib.ptr(w.ws), float(SOFTPLUS_BETA), _lib.ptr(dds_pm),
+                  _lib.ptr(dpre_t), hid, _lib.ptr(dpre_c), hid, _lib.ptr(q), hid,
+                  _lib.ptr(acc.db0), _lib.ptr(acc.dws), _lib.ptr(dz), _s(pts))
+        del dz
+        # first-layer weights: centre rows x X0c, tap rows x D (hi part and point
+        # offset), shared rows x sum_t D_t
+        linear_weight_grad(dpre_c[:P], X0c, w.k0p, w.k0p, acc.dW0p)
+        linear_weight_grad(dpre_t, D, ld_d, w.k0p, acc.dW0p)
+        linear_weight_grad(q[:P], Dsum, w.k0p, w.k0p, acc.dW0p)
+        dtable = None
+        if ctx.needs_input_grad[1]:
+            dXc = _encoder_grad(dpre_c[:P], w)
+            dXt = _encoder_grad(dpre_t, w)
+            dXq = _encoder_grad(q[:P], w)
+            dtable = enc.table_grad_buffer()
+            _taps_scatter(enc, pts, ctx.eps, dXc, dXt, dXq, w.n_enc, dtable, 1)
+        dW0 = w.unpermute_dW0(acc.dW0p)
+        return (None, dtable, dW0, acc.db0, acc.dW1, acc.db1, acc.dws.view(w.ws_shape),
+                acc.dbs.view(w.bs_shape), None, None, None)
+
+
+def _taps_scatter(enc, pts, eps, dXc, dXt, dXq, ld, dtable, increments):
+    """Table gradient of one centre + four-tap point set (na_sdf_taps_bwd), or its
+    sorted, bitwise-repeatable variant when the encoder asks for it."""
+    P = pts.shape[0]
+    vmin, vmax = enc.vol_range
+    args = (ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin), float(vmax), eps,
+            _lib.ptr(dXc), _lib.ptr(dXt), _lib.ptr(dXq) if dXq is not None else None, _code(dXc),
+            ld, _lib.ptr(dtable), int(enc.active_levels), increments)
+    if SCATTER_CAPTURE is not None:
+        SCATTER_CAPTURE.append((pts, eps))
+    if enc.sorted_grad():
+        ws = enc.sorted_workspace(P, 5, pts.device)
+        _lib.call("na_sdf_taps_bwd_sorted", *args, _lib.ptr(ws), ws.numel(), _s(pts))
+        return
+    with _timed("na_sdf_taps_bwd"):
+        _lib.call("na_sdf_taps_bwd", *args, _s(pts))
 
 
 def _centre_taps_backward(ctx, pts, X0c, H0c, FB, X0t, H0t, dsdf, dfeat, ds_taps):
@@ -522,12 +626,8 @@ def _centre_taps_backward(ctx, pts, X0c, H0c, FB, X0t, H0t, dsdf, dfeat, ds_taps
     if ctx.needs_input_grad[1]:
         dXc = _encoder_grad(dpre_c, w)
         dXt = _encoder_grad(dpre_t, w)
-        dtable = torch.zeros(enc.n_params, dtype=torch.float32, device=dev)
-        vmin, vmax = enc.vol_range
-        with _timed("na_sdf_taps_bwd"):
-            _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), P, float(vmin),
-                      float(vmax), ctx.eps, _lib.ptr(dXc), _lib.ptr(dXt), _code(dXc), w.n_enc,
-                      _lib.ptr(dtable), int(enc.active_levels), _s(pts))
+        dtable = enc.table_grad_buffer()
+        _taps_scatter(enc, pts, ctx.eps, dXc, dXt, None, w.n_enc, dtable, 0)
     dW0 = w.unpermute_dW0(acc.dW0p)
     return (None, dtable, dW0, acc.db0, acc.dW1, acc.db1, acc.dws.view(w.ws_shape),
             acc.dbs.view(w.bs_shape), None, None, None)
@@ -585,6 +685,18 @@ class RGBWeights:
         self.W_last = Ws[-1].contiguous().float()
         self.b = [b.contiguous().float() for b in bs]
         self.hidden = W0.shape[0]
+
+    def increment_weights(self, ld_d):
+        """First-layer weights against the tap-increment rows [hi part and point | lo
+        part of the encoder columns]: [W0p | W_enc | 0] of width ld_d (bf16)."""
+        key = (ld_d, self.W0p.data_ptr(), self.W0p._version)
+        if getattr(self, "_inc_key", None) != key:
+            n, k0p = self.n_enc, self.k0p
+            W = torch.zeros(self.W0p.shape[0], ld_d, dtype=self.W0p.dtype, device=self.W0p.device)
+            W[:, :k0p] = self.W0p
+            W[:, k0p:k0p + n] = self.W0p[:, :n]
+            self._inc, self._inc_key = W, key
+        return self._inc
 
     def unpermute_dW0(self, dW0p):
         n = self.n_feat

@@ -3,8 +3,11 @@
 Replaces the optimizer the reference builds at
 imaginaire/trainers/utils/get_trainer.py:106-150 (AdamW, lr 1e-3, wd 1e-2,
 projects/neuralangelo/configs/base.yaml:117-121).  Parameters registered with
-``set_lowp_shadow`` (the hash table when it is gathered in bf16) get their
-bf16 copy refreshed in the same pass.
+``set_lowp_shadow`` (the hash table) get their bf16 copy refreshed in the same
+pass, and their gradient zeroed in it too: the holder keeps the zeroed buffer for
+the next backward, so the step's zero_grad (imaginaire/trainers/base.py:447-457,
+with set_to_none) costs no fill of the 1.46 GB table gradient.  The parameter's
+.grad still holds that buffer until zero_grad; it reads as zero after step().
 """
 import ctypes
 
@@ -51,9 +54,14 @@ class AdamW(torch.optim.Optimizer):
                     batch.setdefault((int(st["step"]), p.device), []).append((p, g, st))
                     continue
                 lowp = holder.lowp_shadow()
+                # the gradient is zeroed in the same pass when it is the parameter's own
+                # buffer; the holder keeps it for the next backward (no separate fill)
+                clear = g is p.grad and hasattr(holder, "take_cleared_grad")
                 _lib.call("na_adamw_step", _lib.ptr(p), _lib.ptr(g), _lib.ptr(st["exp_avg"]),
                           _lib.ptr(st["exp_avg_sq"]), p.numel(), *hyper, int(st["step"]),
-                          _lib.ptr(lowp), _lib.NA_BF16, _lib.stream_of(p))
+                          _lib.ptr(lowp), _lib.NA_BF16, int(clear), _lib.stream_of(p))
+                if clear:
+                    holder.take_cleared_grad(g)
                 # the kernel wrote through a raw pointer: bump the version counter so
                 # caches keyed on it (the encoder's bf16 copy) see the update
                 torch.autograd.graph.increment_version(p)

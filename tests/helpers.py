@@ -65,3 +65,16 @@ def sphere_cameras(n, H=32, W=32, fov_deg=50.0, radius=3.0, seed=0):
         intrs.append(torch.tensor([[f, 0.0, W / 2], [0.0, f, H / 2], [0.0, 0.0, 1.0]],
                                   dtype=torch.float64))
     return torch.stack(poses).float(), torch.stack(intrs).float()
+
+
+def hessian_floor(cdt, table_scale, err_ref32):
+    """Absolute part of the tr H error bound (the relative part is 3 % of |tr H|).
+    BF16 at the configuration's own table scale (1e-4): 0.2, from the increments'
+    precision -- hi/lo parts keep 2^-17 of |D| ~ eps |grad s|, which divided by
+    eps^2 (7.9e-8) is ~0.05 per tap.  Otherwise twice the 99th-percentile error of
+    the reference's own float32 arithmetic at the same points (the FP32 oracle):
+    FP32 is that arithmetic in another order, and on a rough field (table 0.05)
+    the bf16 rounding of the table and weights (2^-9 relative) is of that size."""
+    if cdt == torch.bfloat16 and table_scale <= 1e-3:
+        return 0.2
+    return 2.0 * err_ref32.quantile(0.99).item()

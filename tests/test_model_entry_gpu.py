@@ -103,8 +103,12 @@ def test_inference_full_image_matches_oracle(cuda, seed):
     with torch.no_grad():
         o = orr.render_rays(fld, P, c, v, m.s_var.detach().cpu().double(), m.progress,
                             m.neural_sdf.normal_eps, False, dists=dists)
-        near, far, _ = orr.get_dist_bounds(c, v)
-        own = orr.sample_dists_all(c, v, near, far, fld.sdf, 64, 16, 2)[..., 0]
+        # the sampling itself against the oracle's float32 field (the arithmetic the
+        # sampling runs in), as the pdf inversion amplifies tiny SDF differences
+        f32, _ = oracle_field(m, oracle_params(m, torch.float32), dtype=torch.float32)
+        near, far, _ = orr.get_dist_bounds(c.float(), v.float())
+        own = orr.sample_dists_all(c.float(), v.float(), near, far, f32.sdf, 64, 16,
+                                   2)[..., 0].double()
     e = dict(rgb=(out["rgb"][0].cpu().double() - o["rgb"]).abs().max().item(),
              opacity=(out["opacity"][0].cpu().double() - o["opacity"]).abs().max().item(),
              gradient=(out["gradient"][0].cpu().double() - o["gradient"]).abs().max().item(),

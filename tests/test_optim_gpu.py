@@ -64,8 +64,56 @@ def test_adamw_multi_tensor_equals_single(cuda):
         opt.step()
         for p, gr, m, v in zip(single, grads, m1, v1):
             _lib.call("na_adamw_step", _lib.ptr(p), _lib.ptr(gr), _lib.ptr(m), _lib.ptr(v), p.numel(),
-                      1e-3, 0.9, 0.999, 1e-8, 1e-2, step, None, _lib.NA_BF16, _lib.stream_of(p))
+                      1e-3, 0.9, 0.999, 1e-8, 1e-2, step, None, _lib.NA_BF16, 0,
+                      _lib.stream_of(p))
     for p, s, m, v in zip(params, single, m1, v1):
         st = opt.state[p]
         assert torch.equal(p.detach(), s)
         assert torch.equal(st["exp_avg"], m) and torch.equal(st["exp_avg_sq"], v)
+
+
+def test_table_gradient_cleared_in_the_update_pass(cuda):
+    """The table gradient is zeroed by the AdamW pass and handed back to the encoder,
+    which gives the same (zero) buffer to the next backward: no fill, same results as
+    fresh zero-filled gradients (bit for bit, with the sorted table-gradient
+    reduction), and no aliasing while .grad is still attached."""
+    from neuralangelo_amd.encoding import Encoding
+    cfg = dict(otype="HashGrid", n_levels=8, n_features_per_level=4, log2_hashmap_size=14,
+               base_resolution=8, per_level_scale=1.5)
+    torch.manual_seed(0)
+    runs = []
+    for fused in (True, False):
+        enc = Encoding(3, cfg, dtype=torch.bfloat16).to(cuda)
+        enc.deterministic = True
+        with torch.no_grad():
+            enc.params.copy_(torch.linspace(-1e-2, 1e-2, enc.n_params, device=cuda))
+        opt = AdamW(enc.parameters(), lr=1e-2)
+        opt.set_lowp_shadow(enc.params, enc)
+        g = torch.Generator().manual_seed(1)
+        ptrs = []
+        for step in range(3):
+            x = torch.rand(4000, 3, generator=g).to(cuda)
+            opt.zero_grad(set_to_none=True)
+            if not fused:
+                enc._spare_grad = None  # the reference run: a fresh zero fill every step
+            enc(x).float().square().sum().backward()
+            ptrs.append(enc.params.grad.data_ptr())
+            opt.step()
+            if fused:
+                assert enc.params.grad.abs().max().item() == 0.0  # cleared in the pass
+        runs.append((enc.params.detach().clone(), ptrs))
+    (p_fused, ptr_fused), (p_ref, _) = runs
+    assert torch.equal(p_fused, p_ref)
+    assert ptr_fused[1] == ptr_fused[0] and ptr_fused[2] == ptr_fused[0]  # the buffer is reused
+    # with the gradient still attached (no zero_grad), a fresh buffer is used and the
+    # attached one accumulates as autograd defines it
+    enc = Encoding(3, cfg).to(cuda)
+    opt = AdamW(enc.parameters(), lr=1e-2)
+    opt.set_lowp_shadow(enc.params, enc)
+    x = torch.rand(1000, 3, device=cuda)
+    enc(x).square().sum().backward()
+    opt.step()
+    enc(x).square().sum().backward()
+    once = torch.zeros_like(enc.params)
+    enc.backward_into(x, 2 * enc(x).detach(), once)
+    assert torch.allclose(enc.params.grad, once, rtol=1e-5, atol=1e-7)

@@ -8,7 +8,7 @@ import torch.nn.functional as torch_F
 
 from neuralangelo_amd.model import Model, default_config
 from oracle import field as ofield
-from tests.helpers import oracle_field, oracle_params, randomize_model
+from tests.helpers import hessian_floor, oracle_field, oracle_params, randomize_model
 
 pytestmark = pytest.mark.gpu
 
@@ -64,3 +64,74 @@ def test_error_against_fp64_oracle(setup, cuda, cdt):
           f"max |rgb err| {e_rgb:.3e}")
     t = TOL[cdt]
     assert e_sdf <= t["sdf"] and e_grad <= t["grad"] and e_rgb <= t["rgb"], (e_sdf, e_grad, e_rgb)
+
+
+def _field_model(cdt, table_scale, cuda):
+    torch.manual_seed(0)
+    m = Model(default_config(log2_T=22, F=8), compute_dtype=torch.float32)
+    randomize_model(m, 3, table_scale=table_scale)
+    m.neural_sdf.active_levels = m.neural_sdf.anneal_levels = 16
+    m.neural_sdf.set_normal_epsilon()
+    if cdt == torch.float32:
+        return m.to(cuda).train(), m
+    m16 = Model(default_config(log2_T=22, F=8), compute_dtype=cdt)
+    m16.load_state_dict(m.state_dict())
+    m16.neural_sdf.active_levels = m16.neural_sdf.anneal_levels = 16
+    m16.neural_sdf.set_normal_epsilon()
+    return m16.to(cuda).train(), m
+
+
+@pytest.mark.parametrize("table_scale", [1e-4, 0.05])
+@pytest.mark.parametrize("cdt", [torch.float32, torch.bfloat16])
+def test_hessian_and_curvature_gradient_against_fp64_oracle(cuda, cdt, table_scale):
+    """The 4-tap Hessian trace (modules.py:168-175) and the gradient of the curvature
+    loss mean|tr H| (misc.py:83-89) against the FP64 oracle, on 16,384 random points of
+    the 2^22 x 16 x 8 model: the configuration's own init scale of the table (1e-4) and
+    a rough random field (0.05)."""
+    m, m32 = _field_model(cdt, table_scale, cuda)
+    P = oracle_params(m32)
+    fld, table = oracle_field(m32, P)
+    g = torch.Generator().manual_seed(12)
+    x = torch.rand(16384, 3, generator=g) * 1.6 - 0.8
+    sm = m.neural_sdf
+    sdf, feat, taps, delta = sm.forward_with_taps(x.to(cuda))
+    _, hess = sm.gradients_from_taps(taps, sdf, sdf, True, delta)
+    h = hess[:, 0].detach().cpu().double() * 3
+    curv = hess.sum(-1).abs().mean()
+    params = [sm.tcnn_encoding.params] + list(sm.mlp.parameters())
+    got = torch.autograd.grad(curv, params, allow_unused=True)
+    s_ref = fld.sdf(x.double())
+    _, h_ref3 = fld.compute_gradients(x.double(), sm.normal_eps, training=True, sdf=s_ref)
+    h_ref = h_ref3[:, 0].detach() * 3
+    names = ["W0v", "W0g", "b0", "W1v", "W1g", "b1", "ws", "bs"]
+    ref = torch.autograd.grad(h_ref3.sum(-1).abs().mean(), [table] + [P[k] for k in names],
+                              allow_unused=True)
+    # the reference's own arithmetic: the same field evaluated in float32 (FP32 oracle)
+    P32 = oracle_params(m32, torch.float32)
+    fld32, _ = oracle_field(m32, P32, dtype=torch.float32)
+    with torch.no_grad():
+        s32 = fld32.sdf(x.float())
+        _, h32 = fld32.compute_gradients(x.float(), sm.normal_eps, training=True, sdf=s32)
+    err_ref32 = (h32[:, 0].double() * 3 - h_ref).abs()
+    err = (h - h_ref).abs()
+    rel = err / h_ref.abs().clamp_min(1.0)
+    coss = {}
+    for name, a, b in zip(["table"] + names, got, ref):
+        if a is None or b is None or b.abs().max() == 0:
+            continue
+        a, b = a.detach().reshape(-1).cpu().double(), b.reshape(-1).double()
+        coss[name] = (a @ b / (a.norm() * b.norm() + 1e-300)).item()
+    print(f"\n{cdt} table {table_scale}: tr H median |ref| {h_ref.abs().median().item():.3f} "
+          f"|err| median {err.median().item():.4f} p99 {err.quantile(0.99).item():.4f} max "
+          f"{err.max().item():.4f}; rel median {rel.median().item():.2e} p99 "
+          f"{rel.quantile(0.99).item():.2e}; float32 oracle |err| p99 "
+          f"{err_ref32.quantile(0.99).item():.4f}", flush=True)
+    print("  curvature grad cosines: " + " ".join(f"{k} {v:.5f}" for k, v in coss.items()),
+          flush=True)
+    assert torch.isfinite(h).all()
+    over = (err / (0.03 * h_ref.abs() + hessian_floor(cdt, table_scale, err_ref32))).quantile(0.99)
+    assert over.item() <= 1.0, over.item()
+    if cdt == torch.bfloat16:
+        for name, cs in coss.items():
+            # the table's curvature gradient carries the bf16 table copy's rounding
+            assert cs > (0.995 if name == "table" else 0.999), ("curvature gradient", name, cs)

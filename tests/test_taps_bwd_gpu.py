@@ -60,7 +60,8 @@ def test_merged_scatter_equals_five_scatters(cuda, eps, active, dt, F, layout):
                   _lib.ptr(dy), _lib.dtype_code(dy.dtype), LF, _lib.ptr(ref), active, s)
     got = torch.zeros(enc.n_params, device=cuda)
     _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), n, -2.0, 2.0, float(eps),
-              _lib.ptr(dyc), _lib.ptr(dyt), _lib.dtype_code(dt), LF, _lib.ptr(got), active, s)
+              _lib.ptr(dyc), _lib.ptr(dyt), None, _lib.dtype_code(dt), LF, _lib.ptr(got), active,
+              0, s)
     torch.cuda.synchronize()
     scale = ref.abs().max().item()
     assert (got - ref).abs().max().item() <= 2e-5 * scale
@@ -151,3 +152,32 @@ def test_fused_node_matches_unfused_path(cuda):
         assert cos > 0.999999, cos
         # the hessian term divides by eps^2: summation-order differences reach ~1e-2
         assert (a - b).abs().max() <= 2e-2 * b.abs().max() + 1e-12
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.bfloat16])
+def test_increment_layout_scatter(cuda, dt):
+    """The increment layout (point-major tap rows, a shared row per point, the centre's
+    own row) scatters what the plain layout does with the taps' gradients g_t = dyt +
+    dyq (tap-major) and the centre's dyc - sum_t g_t."""
+    enc = _enc(16, 8, cuda)
+    n, LF, eps = 3001, 128, 2e-3
+    g = torch.Generator().manual_seed(5)
+    pts = _ray_points(n, g, 2e-3).to(cuda)
+    dyc = torch.randn(n, LF, generator=g).to(dt)
+    dyt = torch.randn(n, 4, LF, generator=g).to(dt)
+    dyq = torch.randn(n, LF, generator=g).to(dt)
+    gt = dyt.float() + dyq.float()[:, None, :]
+    plain_c = (dyc.float() - gt.sum(1)).to(cuda)
+    plain_t = gt.transpose(0, 1).reshape(4 * n, LF).contiguous().to(cuda)
+    s = _lib.stream_of(pts)
+    ref = torch.zeros(enc.n_params, device=cuda)
+    _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), n, -2.0, 2.0, float(eps),
+              _lib.ptr(plain_c), _lib.ptr(plain_t), None, _lib.NA_F32, LF, _lib.ptr(ref), 16, 0, s)
+    got = torch.zeros(enc.n_params, device=cuda)
+    dyt_pm = dyt.reshape(4 * n, LF).contiguous().to(cuda)
+    _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), n, -2.0, 2.0, float(eps),
+              _lib.ptr(dyc.to(cuda)), _lib.ptr(dyt_pm), _lib.ptr(dyq.to(cuda)),
+              _lib.dtype_code(dt), LF, _lib.ptr(got), 16, 1, s)
+    torch.cuda.synchronize()
+    scale = ref.abs().max().item()
+    assert (got - ref).abs().max().item() <= 2e-5 * scale
