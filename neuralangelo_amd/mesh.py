@@ -184,22 +184,74 @@ def write_ply(path, V, F, C=None):
         f.write(faces.tobytes())
 
 
+MERGE_DIGITS = 8  # trimesh's tol.merge = 1e-8, as decimal digits
+
+
+@torch.no_grad()
+def weld_blocks(V, F, block_v, digits=MERGE_DIGITS):
+    """The processing trimesh.Trimesh(V, F) applies at construction (process=True,
+    mesh.py:124-127): merge_vertices -- vertices whose coordinates agree after rounding
+    to `digits` decimals become one -- and removal of vertices no face references;
+    done for every block on its own (the blocks are joined afterwards without merging,
+    trimesh.util.concatenate at mesh.py:46).  V [nv,3] and F [nf,3] on the GPU, block b
+    owning vertices block_v[b] .. block_v[b+1]-1.  Kept vertices stay in the order of
+    their first occurrence and faces are renumbered (faces are neither dropped nor
+    reordered).  Returns (V, F, keep, block_v) with keep the input index of every kept
+    vertex (to carry per-vertex data) and the blocks' new vertex offsets.
+    In MC output a vertex repeats only where the field is exactly zero at a lattice
+    point (the edges meeting there all place their vertex on it)."""
+    nv = V.shape[0]
+    dev = V.device
+    block_v = torch.as_tensor(block_v, dtype=torch.int64, device=dev)
+    if nv == 0 or F.shape[0] == 0:
+        empty = torch.zeros(0, dtype=torch.int64, device=dev)
+        return V[:0], F[:0], empty, torch.zeros_like(block_v)
+    blk = torch.repeat_interleave(torch.arange(block_v.numel() - 1, device=dev),
+                                  block_v[1:] - block_v[:-1])
+    q = torch.round(V.double() * 10.0 ** digits).to(torch.int64)
+    ref = torch.zeros(nv, dtype=torch.bool, device=dev)
+    ref[F.reshape(-1)] = True
+    idx = torch.nonzero(ref)[:, 0]
+    key = torch.cat([blk[idx, None], q[idx]], dim=1)
+    _, inv = torch.unique(key, dim=0, return_inverse=True)
+    ngroups = int(inv.max().item()) + 1
+    first = torch.full((ngroups,), nv, dtype=torch.int64, device=dev)
+    first.scatter_reduce_(0, inv, idx, reduce="amin")
+    order = torch.argsort(first)
+    rank = torch.empty_like(order)
+    rank[order] = torch.arange(ngroups, device=dev)
+    new_id = torch.full((nv,), -1, dtype=torch.int64, device=dev)
+    new_id[idx] = rank[inv]
+    keep = first[order]
+    counts = torch.bincount(blk[keep], minlength=block_v.numel() - 1)
+    new_block_v = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(counts, 0)])
+    return V[keep], new_id[F], keep, new_block_v
+
+
 @torch.no_grad()
 def extract_mesh(sdf_func, bounds, intv, block_res=64, texture_func=None, filter_lcc=False,
-                 device=None, max_points=1 << 24):
+                 device=None, max_points=1 << 24, process=True):
     """sdf_func(xyz [...,3] on the GPU) -> [...,1]; the caller passes the negated SDF
     (projects/neuralangelo/scripts/extract_mesh.py:85).  texture_func(V [N,3]) -> uint8
     colours [N,3] (e.g. functools.partial(extract_texture, neural_rgb=..., neural_sdf=...)).
+    process: trimesh's construction-time processing of each block mesh, as the
+    reference builds every block with trimesh.Trimesh (weld_blocks, then the sphere
+    filter, whose new Trimesh drops the vertices left without a face); process=False
+    returns the marching-cubes output with only the sphere filter's own masking.
     Returns (V, F) or, with texture_func, (V, F, C)."""
     device = device or torch.device("cuda", torch.cuda.current_device())
     axes = lattice_axes(bounds, intv)
     vol = lattice_sdf(sdf_func, axes, device, max_points)
-    V, F, _, block_f = lattice_marching_cubes(vol, block_res, 0.0, axes=axes, intv=intv,
-                                              block_ranges=True)
+    V, F, block_v, block_f = lattice_marching_cubes(vol, block_res, 0.0, axes=axes, intv=intv,
+                                                    block_ranges=True)
     del vol
     C = None
     if texture_func is not None and V.shape[0] > 0:
         C = np.asarray(texture_func(V.float()), dtype=np.uint8)
+    if process and V.shape[0] > 0:
+        V, F, keep, block_v = weld_blocks(V, F, block_v)
+        if C is not None:
+            C = C[keep.cpu().numpy()]
     empty = (np.zeros((0, 3)), np.zeros((0, 3), np.int64)) + \
         ((np.zeros((0, 3), np.uint8),) if texture_func is not None else ())
     if V.shape[0] == 0:
@@ -210,10 +262,22 @@ def extract_mesh(sdf_func, bounds, intv, block_res=64, texture_func=None, filter
     fmask = mask[F[:, 0]] & mask[F[:, 1]] & mask[F[:, 2]]
     kept_before = torch.cumsum(fmask.to(torch.int64), 0)
     block_f = torch.cat([torch.zeros(1, dtype=torch.int64, device=F.device), kept_before])[block_f]
+    nb = block_f.numel() - 1
+    vblock = torch.repeat_interleave(torch.arange(nb, device=V.device),
+                                     block_v[1:] - block_v[:-1])[mask]
     V, F = filter_outside_sphere(V, F)
-    Vn, Fn = V.cpu().numpy(), F.cpu().numpy()
     if C is not None:
         C = C[mask.cpu().numpy()]
+    if process:
+        # the filtered block is a new Trimesh: a block with faces keeps only the
+        # vertices they reference; a block left without faces keeps its vertices
+        used = (block_f[1:] - block_f[:-1])[vblock] == 0
+        used[F.reshape(-1)] = True
+        if not bool(used.all()):
+            V, F = V[used], (torch.cumsum(used.to(torch.int64), 0) - 1)[F]
+            if C is not None:
+                C = C[used.cpu().numpy()]
+    Vn, Fn = V.cpu().numpy(), F.cpu().numpy()
     if filter_lcc:
         res = largest_component_per_block(Vn, Fn, block_f.cpu().numpy(), C)
         Vn, Fn = res[0], res[1]

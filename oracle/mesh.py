@@ -4,8 +4,9 @@ marching_cubes  -> the C restatement in oracle/mc_oracle.c (PyMCubes algorithm,
                    projects/neuralangelo/utils/mesh.py:121)
 extract_mesh    -> projects/neuralangelo/utils/mesh.py:24-49 (LatticeGrid blocks
                    of block_res+1 points, :68-100), marching_cubes per block
-                   (:119-133), sphere filter (:136-148), concatenation without
-                   vertex welding (:44-46)
+                   (:119-133) with trimesh's construction-time vertex merge
+                   (trimesh_process), sphere filter (:136-148), concatenation
+                   without vertex welding (:44-46)
 largest_component -> filter_largest_cc (:151-158): trimesh's edge-connected split and
                    the component of largest area (trimesh is absent; its split is
                    restated from its documented behaviour)
@@ -155,6 +156,34 @@ def largest_component(V, F, C=None):
     return (V[used], remap[Fk]) + ((C[used],) if C is not None else ())
 
 
+def trimesh_process(V, F, C=None, digits=8):
+    """trimesh.Trimesh(V, F) construction processing (process=True, validate=False),
+    restated from trimesh's documented merge_vertices: vertices whose coordinates agree
+    after rounding to `digits` decimals (tol.merge = 1e-8) are merged, and only
+    vertices referenced by a face take part -- the rest are dropped (a mesh without
+    faces keeps all its vertices).  Kept vertices in order of first occurrence (the
+    order trimesh's order-preserving merge gives; trimesh is absent and unpinned,
+    so that order is unpinned), faces renumbered, neither dropped nor reordered.
+    Sequential: one dictionary lookup per vertex."""
+    if len(F) == 0:
+        return (V, F) + ((C,) if C is not None else ())
+    referenced = np.zeros(len(V), bool)
+    referenced[F.reshape(-1)] = True
+    scale = 10.0 ** digits
+    group, keep = {}, []
+    new_id = np.full(len(V), -1, np.int64)
+    for i in range(len(V)):
+        if not referenced[i]:
+            continue
+        k = tuple(int(np.round(c * scale)) for c in V[i])
+        if k not in group:
+            group[k] = len(keep)
+            keep.append(i)
+        new_id[i] = group[k]
+    keep = np.array(keep, np.int64)
+    return (V[keep], new_id[F]) + ((C[keep],) if C is not None else ())
+
+
 def lattice_blocks(bounds, intv, block_res):
     """LatticeGrid (mesh.py:68-100): yields (block index, xyz [bx,by,bz,3] float32)."""
     (x0, x1), (y0, y1), (z0, z1) = bounds
@@ -187,11 +216,14 @@ def filter_points_outside_bounding_sphere(V, F, C=None):
     return out + ((C[mask],) if C is not None else ())
 
 
-def extract_mesh(sdf_func, bounds, intv, block_res=64, texture_func=None, filter_lcc=False):
+def extract_mesh(sdf_func, bounds, intv, block_res=64, texture_func=None, filter_lcc=False,
+                 process=True):
     """mesh.py:24-49 with marching_cubes (:119-133): per block, MC on sdf_func(xyz)
     (already negated by the caller, extract_mesh.py:85), V*intv + xyz[0,0,0], colours
-    texture_func(V), sphere filter, the block's largest component when filter_lcc;
-    the blocks concatenated.  Returns (V, F) or, with texture_func, (V, F, C)."""
+    texture_func(V), the Trimesh construction (trimesh_process when process), sphere
+    filter (a new Trimesh: processed again), the block's largest component when
+    filter_lcc; the blocks concatenated without merging.  Returns (V, F) or, with
+    texture_func, (V, F, C)."""
     Vs, Fs, Cs, off = [], [], [], 0
     for _, xyz in lattice_blocks(bounds, intv, block_res):
         sdf = sdf_func(xyz)[..., 0]
@@ -200,7 +232,12 @@ def extract_mesh(sdf_func, bounds, intv, block_res=64, texture_func=None, filter
             continue
         V = V * intv + xyz[0, 0, 0].numpy()
         C = np.asarray(texture_func(V), np.uint8) if texture_func is not None else None
+        if process:
+            V, F, *rest = trimesh_process(V, F, C)
+            C = rest[0] if rest else None
         res = filter_points_outside_bounding_sphere(V, F, C)
+        if process:
+            res = trimesh_process(*res)
         if filter_lcc and res[0].shape[0] > 0:
             res = largest_component(*res)
         V, F = res[0], res[1]

@@ -111,6 +111,48 @@ def test_extract_mesh_blocks_match_oracle(block_res):
     assert np.all(np.linalg.norm(V, axis=-1) < 1.0)
 
 
+def _zero_sphere_sdf(xyz):
+    """Exact zeros at lattice points: with intv = 1/32 every coordinate is k/32, whose
+    square and the sum of three squares are exact in float32, so the sphere of radius^2
+    854/1024 passes exactly through the 528 lattice points with a^2 + b^2 + c^2 = 854
+    (marching cubes repeats vertices there)."""
+    return -((xyz * xyz).sum(-1, keepdim=True) - 854.0 / 1024.0)
+
+
+@pytest.mark.parametrize("block_res", [16, 64])
+def test_weld_blocks_and_processed_extraction_match_oracle(block_res):
+    """trimesh's construction-time merge of every block (weld_blocks) against the
+    oracle's sequential restatement block by block, and the processed extract_mesh
+    (welding, sphere filter, dropping vertices left without faces) against the
+    oracle's per-block loop, on a field with exact zeros at lattice points."""
+    dev = torch.device("cuda", 0)
+    res, intv = 64, 2.0 / 64
+    bounds = ((-1.0, 1.0),) * 3
+    axes = gmesh.lattice_axes(bounds, intv)
+    vol = gmesh.lattice_sdf(_zero_sphere_sdf, axes, dev)
+    assert int((vol == 0).sum()) > 20
+    V, F, bv, bf = mcubes.lattice_marching_cubes(vol, block_res, 0.0, axes=[a.to(dev) for a in axes],
+                                                 intv=intv, block_ranges=True)
+    W, G, keep, bw = gmesh.weld_blocks(V, F, bv)
+    Vn, Fn, bvn, bfn = V.cpu().numpy(), F.cpu().numpy(), bv.cpu().numpy(), bf.cpu().numpy()
+    Ws, Gs, off = [], [], 0
+    for b in range(len(bvn) - 1):
+        Vb = Vn[bvn[b]:bvn[b + 1]]
+        Fb = Fn[bfn[b]:bfn[b + 1]] - bvn[b]
+        Vp, Fp = omesh.trimesh_process(Vb, Fb)
+        Ws.append(Vp)
+        Gs.append(Fp + off)
+        off += len(Vp)
+    Wr, Gr = np.concatenate(Ws), np.concatenate(Gs)
+    assert len(Wr) < len(Vn)  # vertices were merged
+    assert np.array_equal(W.cpu().numpy(), Wr) and np.array_equal(G.cpu().numpy(), Gr)
+    assert np.array_equal(Vn[keep.cpu().numpy()], Wr)
+    assert bw[-1].item() == len(Wr)
+    Vg, Fg = gmesh.extract_mesh(_zero_sphere_sdf, bounds, intv, block_res=block_res, device=dev)
+    Vr, Fr = omesh.extract_mesh(_zero_sphere_sdf, bounds, intv, block_res=block_res)
+    assert np.array_equal(Fg, Fr) and np.array_equal(Vg, Vr)
+
+
 def _spheres_sdf(xyz):
     """Negated implicit function of a union of small spheres (several components per
     block): min over spheres of |x - c|^2 - r^2, written with products and sums only so

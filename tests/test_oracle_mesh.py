@@ -142,7 +142,8 @@ def test_empty_and_degenerate_volumes():
 
 
 def test_lattice_loop_equals_blockwise_extraction():
-    """Stored-lattice block loop + global sphere filter == the per-block extract_mesh."""
+    """Stored-lattice block loop + global sphere filter == the per-block extract_mesh
+    (without trimesh's processing, which the stored-lattice loop does not apply)."""
     import torch
 
     def sdf(xyz):
@@ -150,7 +151,7 @@ def test_lattice_loop_equals_blockwise_extraction():
         return -(x * x + y * y + z * z - 0.5)[..., None]
     bounds = ((-1.0, 1.0), (-1.0, 1.0), (-1.0, 1.0))
     intv = 2.0 / 40
-    V1, F1 = omesh.extract_mesh(sdf, bounds, intv, block_res=12)
+    V1, F1 = omesh.extract_mesh(sdf, bounds, intv, block_res=12, process=False)
     axes = [torch.arange(lo, hi, intv) for lo, hi in bounds]
     x, y, z = torch.meshgrid(*axes, indexing="ij")
     vol = sdf(torch.stack([x, y, z], -1))[..., 0].numpy()
@@ -173,3 +174,83 @@ def test_block_loop_threads_keep_the_sequential_order():
     V4, F4, c4, f4 = omesh.lattice_marching_cubes(vol, 8, counts=True, threads=4)
     assert np.array_equal(V1, V4) and np.array_equal(F1, F4)
     assert np.array_equal(c1, c4) and np.array_equal(f1, f4)
+
+
+def _zero_sphere(n=15, r2=25):
+    """x^2 + y^2 + z^2 - r2 on an integer lattice: exact zeros at the lattice points of
+    the sphere, where marching cubes puts the vertices of several edges on one point."""
+    g = np.arange(n, dtype=np.float32) - n // 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    return (X * X + Y * Y + Z * Z - r2).astype(np.float32)
+
+
+def _edge_counts(F):
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    e.sort(axis=1)
+    return np.unique(e, axis=0, return_counts=True)
+
+
+def test_trimesh_process_merges_coincident_vertices():
+    """The processed sphere has no repeated vertex, the same face geometry, the first
+    occurrence order, and its non-degenerate faces form a closed surface of Euler
+    characteristic 2 (the merged points collapse faces to zero area, which stay)."""
+    V, F = omesh.marching_cubes(_zero_sphere(), 0.0)
+    assert len(V) - len(np.unique(V, axis=0)) > 50  # the exact zeros repeat vertices
+    V2, F2 = omesh.trimesh_process(V, F)
+    assert len(np.unique(V2, axis=0)) == len(V2)
+    assert F2.shape == F.shape and np.array_equal(V2[F2], V[F])
+    first = np.array([np.flatnonzero((V == v).all(axis=1))[0] for v in V2])
+    assert np.all(np.diff(first) > 0)
+    nd = (F2[:, 0] != F2[:, 1]) & (F2[:, 1] != F2[:, 2]) & (F2[:, 0] != F2[:, 2])
+    assert (~nd).sum() > 0
+    edges, counts = _edge_counts(F2[nd])
+    assert np.all(counts == 2)
+    assert len(np.unique(F2[nd])) - len(edges) + nd.sum() == 2
+
+
+def test_trimesh_process_against_unique_rows():
+    """Against a vectorised restatement (rows rounded to 8 decimals, np.unique,
+    first-occurrence order) on random vertices with exact repeats, repeats within the
+    rounding (3e-9 apart), near misses (3e-8 apart) and unreferenced vertices."""
+    rng = np.random.default_rng(4)
+    base = rng.uniform(-1, 1, (300, 3))
+    V = np.concatenate([base, base[:40], base[40:80] + 3e-9, base[80:120] + 3e-8,
+                        rng.uniform(-1, 1, (30, 3))])
+    F = rng.integers(0, 420, (500, 3))  # vertices 420.. are never referenced
+    V2, F2 = omesh.trimesh_process(V, F)
+    ref = np.zeros(len(V), bool)
+    ref[F.reshape(-1)] = True
+    idx = np.flatnonzero(ref)
+    q = np.round(V[idx] * 1e8).astype(np.int64)
+    _, first, inv = np.unique(q, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    new_id = np.full(len(V), -1)
+    new_id[idx] = rank[inv.reshape(-1)]
+    assert np.array_equal(V2, V[idx[first[order]]])
+    assert np.array_equal(F2, new_id[F])
+    assert len(V2) == len(np.unique(np.round(V[idx] * 1e8), axis=0))
+    # a mesh without faces keeps every vertex
+    V3, F3 = omesh.trimesh_process(V, F[:0])
+    assert np.array_equal(V3, V) and F3.shape == (0, 3)
+
+
+def test_extract_mesh_processes_each_block():
+    """Blocks are processed on their own: welding never joins two blocks (their seam
+    vertices stay duplicated, as trimesh.util.concatenate leaves them), and the
+    processed output has no repeated vertex inside a block."""
+    import torch
+    res, block_res = 32, 16
+    intv = 2.0 / res
+
+    def sdf(xyz):  # exact zeros at lattice points of the radius-0.5 sphere
+        return (xyz * xyz).sum(-1, keepdim=True) - 0.25
+
+    V, F = omesh.extract_mesh(sdf, ((-1.0, 1.0),) * 3, intv, block_res=block_res)
+    V0, F0 = omesh.extract_mesh(sdf, ((-1.0, 1.0),) * 3, intv, block_res=block_res, process=False)
+    assert len(V) < len(V0) and len(F) == len(F0)
+    assert np.array_equal(V[F], V0[F0])
+    # seam vertices (on block faces) appear once per block that holds them
+    assert len(np.unique(V, axis=0)) < len(V)
+    del torch
