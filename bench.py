@@ -181,6 +181,23 @@ def measured_traffic():
     return None
 
 
+TRAFFIC_TABLE = os.path.join(REPO, "profiles", "train_step_hbm_traffic_pmc.md")
+
+
+def table_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter table (the
+    copy-calibrated `total GB` column), or None."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.split("|")]
+                if len(cells) > 5 and cells[1] == f"`{kernel}`":
+                    return float(cells[5]) * 1e9
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 TAP_DIRS = ((1, -1, -1), (-1, -1, 1), (-1, 1, -1), (1, 1, 1))
 
 
@@ -191,7 +208,7 @@ def scatter_distinct_entries(model, captured):
     the captured launch, over all levels."""
     if not captured:
         return None
-    pts, eps = captured[-1]
+    pts, eps = captured[-1][:2]
     enc = model.neural_sdf.tcnn_encoding
     vmin, vmax = enc.vol_range
     total = 0
@@ -270,10 +287,13 @@ def encoder_roofline(model, dev, n_points=786432, iters=20):
     per_point = L * 8 * F * bt + 12 + k0p * bo
     achieved = per_point * n_points / (ms * 1e-3) / 1e9
     del g
+    traffic = table_traffic("hashgrid_fwd_kernel")
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=round(traffic) if traffic is not None else None,
                 kernel="hashgrid_fwd_kernel (SDF input)", bytes_per_point=per_point,
-                points_per_launch=n_points, avg_launch_ms=round(ms, 4))
+                points_per_launch=n_points, avg_launch_ms=round(ms, 4),
+                traffic_source=os.path.relpath(TRAFFIC_TABLE, REPO) if traffic else None)
 
 
 def mc_leg(model, dev, res=1024, block_res=128, iters=5, cpu=True):
@@ -333,7 +353,9 @@ def mc_leg(model, dev, res=1024, block_res=128, iters=5, cpu=True):
         faces=int(F.shape[0]), sdf_eval_s=round(t_sdf, 3),
         sdf_points_per_s=round(nx * ny * nz / t_sdf, 1),
         roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                      frac=round(achieved / HBM_PEAK_GBS, 4),
+                      traffic=(round(table_traffic("mc_classify"))
+                               if table_traffic("mc_classify") else None),
                       kernel="na_mc_count (classify + tile scan), 4 B per lattice point",
                       avg_launch_ms=round(t_count * 1e3, 4)))
     if cpu:

@@ -56,6 +56,25 @@ typedef struct na_grid_desc {
 /* library */
 const char* na_last_error(void);
 int na_version(void);
+
+/* Launch tunables (no counterpart in the reference, which has no launches of its
+ * own to shape; the project's autotuner, neuralangelo_amd/autotune.py, picks them):
+ *   "taps_bwd.run"                   samples per thread of the tap scatter (1..32, 4)
+ *   "taps_increment_bwd.wg_per_cu"   workgroups per CU of the tap-increment
+ *                                    backward (1..32, 8)
+ *   "packed.rays_per_wg"             rays per workgroup of the packed compositing
+ *                                    kernels (64..256, multiple of 64; 64)
+ *   "taps_increment_bwd.fused"       tap-increment backward fused with the
+ *                                    recomputation of its product (1) or split in
+ *                                    two passes (0); 0
+ * na_tune_set returns the previous value (or an error code for an unknown name or
+ * a value out of range); the launchers read the current value at every launch.
+ * Set them between launches: a change is not ordered with launches in flight on
+ * other host threads.  na_tune_count / na_tune_name enumerate the names. */
+int64_t na_tune_set(const char* name, int64_t value);
+int64_t na_tune_get(const char* name);
+int na_tune_count(void);
+const char* na_tune_name(int32_t index);
 int na_device_synchronize(void);
 
 /* ---------------------------------------------------------------------------
@@ -191,9 +210,11 @@ int na_gemm(const na_gemm_args* args, na_stream_t stream);
  *        terms of their natural size.  Increments and their derivatives are formed
  *        in fp32 from z and dz, so the second-order sums (the Hessian trace and its
  *        gradient) keep fp32 precision relative to their own size.  The backward
- *        recomputes dz = D W^T (fp32) into work [M, N], a caller-owned 16-byte
- *        aligned workspace of M * N floats, and then forms the terms elementwise;
- *        dds [M] is 16-byte aligned (four taps per point). */
+ *        recomputes dz = D W^T: into work [M, N], a caller-owned 16-byte aligned
+ *        workspace of M * N floats, followed by an elementwise pass (the launch
+ *        tunable "taps_increment_bwd.fused" = 0, the default), or inside one
+ *        fused kernel that needs no workspace (= 1; work may be NULL); dds [M] is
+ *        16-byte aligned (four taps per point). */
 int na_taps_increment_fwd(const void* D, int64_t ld_d, int64_t M, int64_t K, const void* W,
                           int64_t ldw, int64_t N, const float* zc, int64_t ld_zc,
                           const float* ws, float beta, float* head_out, na_stream_t stream);

@@ -86,6 +86,10 @@ _PGEMM = ctypes.POINTER(GemmArgs)
 _SIGNATURES = {
     "na_last_error": (ctypes.c_char_p, []),
     "na_version": (ctypes.c_int, []),
+    "na_tune_set": (ctypes.c_int64, [ctypes.c_char_p, _I64]),
+    "na_tune_get": (ctypes.c_int64, [ctypes.c_char_p]),
+    "na_tune_count": (ctypes.c_int, []),
+    "na_tune_name": (ctypes.c_char_p, [_I32]),
     "na_device_synchronize": (ctypes.c_int, []),
     "na_hashgrid_layout": (_I64, [_I32, _I32, _I32, _I32, _F32, _PGRID]),
     "na_hashgrid_corners": (ctypes.c_int, [_PGRID, _P, _I64, _P, _P, _P]),
@@ -179,6 +183,9 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    # launch shapes chosen by the autotuner for the architecture the library targets
+    from .autotune import apply_cache
+    apply_cache(lib)
     _lib = lib
     return lib
 

@@ -5,10 +5,56 @@
 
 #include "common.h"
 
+#include <atomic>
+#include <cstring>
+
 namespace na {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+struct Tunable {
+  const char* name;
+  int64_t lo, hi, step;  // allowed values: lo..hi, multiples of step
+  std::atomic<int64_t> value;
+};
+static Tunable g_tune[TUNE_COUNT] = {
+    {"taps_bwd.run", 1, 32, 1, {4}},
+    {"taps_increment_bwd.wg_per_cu", 1, 32, 1, {8}},
+    {"packed.rays_per_wg", 64, 256, 64, {64}},
+    {"taps_increment_bwd.fused", 0, 1, 1, {0}},
+};
+
+int64_t tune_value(TuneKey key) { return g_tune[key].value.load(std::memory_order_relaxed); }
+
+static int tune_index(const char* name) {
+  if (!name) return -1;
+  for (int k = 0; k < TUNE_COUNT; ++k)
+    if (std::strcmp(g_tune[k].name, name) == 0) return k;
+  return -1;
+}
 }  // namespace na
+
+extern "C" int64_t na_tune_set(const char* name, int64_t value) {
+  const int k = na::tune_index(name);
+  NA_CHECK_ARG(k >= 0, std::string("unknown tunable ") + (name ? name : "(null)"));
+  const na::Tunable& t = na::g_tune[k];
+  NA_CHECK_ARG(value >= t.lo && value <= t.hi && value % t.step == 0,
+               std::string(t.name) + ": value out of range [" + std::to_string(t.lo) + ", " +
+                   std::to_string(t.hi) + "] or not a multiple of " + std::to_string(t.step));
+  return na::g_tune[k].value.exchange(value);
+}
+
+extern "C" int64_t na_tune_get(const char* name) {
+  const int k = na::tune_index(name);
+  NA_CHECK_ARG(k >= 0, std::string("unknown tunable ") + (name ? name : "(null)"));
+  return na::g_tune[k].value.load();
+}
+
+extern "C" int na_tune_count(void) { return na::TUNE_COUNT; }
+
+extern "C" const char* na_tune_name(int32_t index) {
+  return index >= 0 && index < na::TUNE_COUNT ? na::g_tune[index].name : nullptr;
+}
 
 extern "C" const char* na_last_error(void) { return na::g_last_error.c_str(); }
 

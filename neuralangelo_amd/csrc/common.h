@@ -18,6 +18,18 @@ namespace na {
 // ---------------------------------------------------------------------------
 void set_error(const std::string& msg);
 
+// Launch tunables (na_tune_set / na_tune_get): shapes of launches whose best
+// value depends on the device and the workload, chosen by the autotuner and
+// read by the host launchers.  Values are atomics: set rarely, read per launch.
+enum TuneKey : int {
+  TUNE_TAPS_BWD_RUN = 0,   // consecutive samples per thread in the tap scatter
+  TUNE_TIB_WG_PER_CU = 1,  // workgroups per CU of the tap-increment backward
+  TUNE_PACKED_RAYS = 2,    // rays (threads) per workgroup of the packed compositing
+  TUNE_TIB_FUSED = 3,      // tap-increment backward fused with its product (1) or split (0)
+  TUNE_COUNT = 4
+};
+int64_t tune_value(TuneKey key);
+
 #define NA_CHECK_ARG(cond, msg)                                              \
   do {                                                                       \
     if (!(cond)) {                                                           \

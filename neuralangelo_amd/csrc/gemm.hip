@@ -2151,6 +2151,166 @@ taps_increment_bwd_kernel(const float* __restrict__ dz, int64_t P, int N,
 }
 
 // ---------------------------------------------------------------------------
+// The same backward fused with the recomputation of dz: a weights-stationary
+// product on 16 x 16 x 32 MFMAs whose accumulators hand each lane the four taps
+// of one point in one column, so the per-point sums stay in the lane.
+// NW waves of 16 columns each (the encoder weights of a wave's columns in 36
+// VGPRs per lane; blockIdx.y selects the 16 NW-column slice), one workgroup per
+// CU walking 128-row tiles (the slices of a tile run side by side); tap rows
+// stream through LDS stages of 128 x 32 bf16 loaded by eight of the waves (16
+// rows each, buffer-to-LDS, the chunk swizzle of the weights-stationary
+// kernel).  MFMA operands: A = 16 tap rows x 32 k from the stage (lane l: row
+// l % 16, k 8 (l / 16) .. + 7), B = 32 k x 16 columns from registers (lane l:
+// column l % 16, same k), so D lane l holds rows 4 (l / 16) .. + 3 of the
+// 16-row group in column l % 16: the four taps of one point.  The epilogue forms
+// the terms of taps_increment_bwd_kernel per (point, column) and never writes dz.
+// ---------------------------------------------------------------------------
+typedef __attribute__((ext_vector_type(4))) float f32x4v_t;
+constexpr int kTbStages = 12;
+
+struct TapsBwdArgs {
+  const bf16_t* D;
+  int64_t ld_d, M, K;
+  const bf16_t* W;
+  int64_t ldw;
+  const float* zc;
+  int64_t ld_zc;
+  const float* ws;
+  float beta;
+  const float* dds;
+  bf16_t* dpre;
+  int64_t ld_dpre;
+  bf16_t* centre;
+  int64_t ld_centre;
+  bf16_t* q;
+  int64_t ld_q;
+  float* db0;
+  float* dws;
+};
+
+template <int NW, int NK>
+__global__ void __launch_bounds__(64 * NW, 1) taps_bwd_fused_kernel(TapsBwdArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16_t st0[kTbStages * kWsStage];
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = (int)blockIdx.y * 16 * NW + wave * 16 + (lane & 15);  // output column
+  const int rq = lane >> 4;               // tap rows 4 rq .. 4 rq + 3 of each 16-row group
+  const uint32_t n_mt = (uint32_t)(a.M / GBM);
+  const int64_t my_tiles = blockIdx.x < n_mt ? (n_mt - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+
+  // weights of column n: k = 32 s + 8 rq .. + 7 of every stage s
+  bf16x8_t bw[NK];
+  {
+    const bf16_t* brow = a.W + (int64_t)n * a.ldw;
+#pragma unroll
+    for (int s = 0; s < NK; ++s) {
+      const int64_t k = 32 * s + 8 * rq;
+      uint4 u = *reinterpret_cast<const uint4*>(brow + (k < a.K ? k : a.K - 8));
+      if (k >= a.K) u = make_uint4(0u, 0u, 0u, 0u);
+      bw[s] = __builtin_bit_cast(bf16x8_t, u);
+    }
+  }
+  const float w = a.ws[n];
+  const float ib = 1.0f / a.beta;
+
+  // stage loads: waves 0..7 fill the 16-row blocks of a stage (one 1 KB load each)
+  const bool loader = wave < 8;
+  const int arow = wave * 16 + (lane >> 2);
+  const int achunk = (lane & 3) ^ ((arow >> 2) & 3);
+  const uint32_t avoff = (uint32_t)(arow * a.ld_d * 2 + achunk * 16);
+  const int akofs = achunk * 8;
+  int64_t issued = 0;
+  uint32_t is_tile = blockIdx.x;
+  int is_t = 0, is_slot = 0;
+  const char* is_a = reinterpret_cast<const char*>(a.D + (int64_t)is_tile * GBM * a.ld_d);
+  int64_t is_ae = (a.M - (int64_t)is_tile * GBM) * a.ld_d * 2;
+  auto issue_next = [&]() {
+    if (loader) {
+      const int64_t k0 = (int64_t)is_t * kNNBK;
+      const __amdgpu_buffer_rsrc_t rs = stage_rsrc(is_a, is_ae, k0 * 2);
+      ws_issue(rs, akofs < a.K - k0 ? avoff : kOobOffset,
+               st0 + is_slot * kWsStage + wave * 16 * kNNBK);
+    }
+    ++issued;
+    is_slot = is_slot + 1 == kTbStages ? 0 : is_slot + 1;
+    if (++is_t == NK) {
+      is_t = 0;
+      is_tile += gridDim.x;
+      is_a = reinterpret_cast<const char*>(a.D + (int64_t)is_tile * GBM * a.ld_d);
+      is_ae = (a.M - (int64_t)is_tile * GBM) * a.ld_d * 2;
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < kTbStages - 1; ++s) issue_next();
+
+  float cs = 0.0f, hs = 0.0f;  // column sums of the centre term and of dds * dh
+  int slot = 0;
+  int64_t g = 0;
+  for (int64_t jt = 0; jt < my_tiles; ++jt) {
+    const int64_t m0 = (int64_t)(blockIdx.x + (uint32_t)jt * gridDim.x) * GBM;
+    f32x4v_t acc[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[r] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < NK; ++t, ++g) {
+      if (loader) ws_wait((int)(issued - g - 1));
+      __builtin_amdgcn_s_barrier();  // stage g landed; stage g - 1 consumed by every wave
+      issue_next();
+      const bf16_t* sa = st0 + slot * kWsStage;
+      slot = slot + 1 == kTbStages ? 0 : slot + 1;
+      // two row groups' operands in flight at a time (the register budget of
+      // four waves per SIMD does not hold all eight)
+#pragma unroll
+      for (int r = 0; r < 8; r += 2) {
+        const int ra0 = r * 16 + (lane & 15), ra1 = ra0 + 16;
+        const bf16x8_t av0 =
+            *reinterpret_cast<const bf16x8_t*>(sa + ra0 * kNNBK + ((rq ^ ((ra0 >> 2) & 3)) << 3));
+        const bf16x8_t av1 =
+            *reinterpret_cast<const bf16x8_t*>(sa + ra1 * kNNBK + ((rq ^ ((ra1 >> 2) & 3)) << 3));
+        acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av0, bw[t], acc[r], 0, 0, 0);
+        acc[r + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av1, bw[t], acc[r + 1], 0, 0, 0);
+        asm volatile("" ::: "memory");
+      }
+    }
+    // epilogue: row group r holds point m0 / 4 + 4 r + rq, taps 0..3, column n
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int64_t pt = (m0 >> 2) + 4 * r + rq;
+      const float z0 = a.zc[pt * a.ld_zc + n];
+      const float4 d4 = *reinterpret_cast<const float4*>(a.dds + 4 * pt);
+      const float dv[4] = {d4.x, d4.y, d4.z, d4.w};
+      const float am = 0.25f * ((dv[0] + dv[1]) + (dv[2] + dv[3]));
+      float ev = 0.0f, s0 = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float hv, y1, s1, dsp;
+        softplus_step(z0, acc[r][i], a.beta, ib, hv, y1);
+        softplus_deriv_step(z0, acc[r][i], a.beta, s1, dsp, s0);
+        a.dpre[(4 * pt + i) * a.ld_dpre + n] = f32_to_bf16(w * fmaf(dv[i] - am, s1, am * dsp));
+        ev = fmaf(dv[i] * w, dsp, ev);
+        hs = fmaf(dv[i], hv, hs);
+      }
+      a.q[pt * a.ld_q + n] = f32_to_bf16((am * w) * s0);
+      bf16_t* cp = a.centre + pt * a.ld_centre + n;
+      *cp = f32_to_bf16(bf16_to_f32(*cp) + ev);
+      cs += ev;
+      // one row group at a time: its loads are not hoisted above the previous
+      // group's stores, so only one group's operands are live (register budget)
+      asm volatile("" ::: "memory");
+    }
+  }
+  // column sums: the four lanes of column n (lane bits 4 and 5)
+  cs += __shfl_xor(cs, 16);
+  cs += __shfl_xor(cs, 32);
+  hs += __shfl_xor(hs, 16);
+  hs += __shfl_xor(hs, 32);
+  if (rq == 0 && my_tiles > 0) {
+    atomic_add_f32(a.db0 + n, cs);
+    atomic_add_f32(a.dws + n, hs);
+  }
+  __builtin_amdgcn_s_waitcnt(0xF70);  // no buffer-to-LDS load outlives the workgroup
+}
+
+// ---------------------------------------------------------------------------
 // Weight-gradient GEMM: C[n, k] += sum_m A[m, n] B[m, k] (bf16 in, fp32 out),
 // A = [M, lda] and B = [M, ldb] row-major, M (the points) much larger than N
 // and K (the layer widths).  Each workgroup owns a C tile of 256 x 64*KB in
@@ -2645,8 +2805,10 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
   if (!backward) {
     NA_CHECK_ARG(head_out != nullptr, "head_out is null");
   } else {
-    NA_CHECK_ARG(dds && dpre && centre && q && work && db0 && dws && aligned16(dpre) &&
-                     aligned16(centre) && aligned16(q) && aligned16(work) && aligned16(dds) &&
+    const bool fused = tune_value(TUNE_TIB_FUSED) != 0;
+    NA_CHECK_ARG(fused || (work && aligned16(work)), "work is null (the split backward needs it)");
+    NA_CHECK_ARG(dds && dpre && centre && q && db0 && dws && aligned16(dpre) &&
+                     aligned16(centre) && aligned16(q) && aligned16(dds) &&
                      ld_dpre >= N && ld_dpre % 8 == 0 &&
                      ld_centre >= N && ld_centre % 8 == 0 && ld_q >= N && ld_q % 8 == 0,
                  "bad backward operands");
@@ -2670,6 +2832,25 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
     NA_CHECK_LAUNCH();
     return NA_OK;
   }
+  if (tune_value(TUNE_TIB_FUSED) != 0) {
+    // backward fused with the recomputation of dz (no workspace)
+    TapsBwdArgs t{(const bf16_t*)D, ld_d, M, K, (const bf16_t*)W, ldw, zc, ld_zc, ws, beta, dds,
+                  (bf16_t*)dpre, ld_dpre, (bf16_t*)centre, ld_centre, (bf16_t*)q, ld_q, db0, dws};
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    // eight waves (128 columns) per workgroup, one per CU; the N / 128 column
+    // slices of a tile are handled by workgroups launched together
+    const int64_t tiles = M / GBM, slices = N / 128;
+    const dim3 gf((unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, ncu / slices)),
+                  (unsigned)slices);
+    const dim3 bf(512);
+#define NA_TBF(NKK) hipLaunchKernelGGL((taps_bwd_fused_kernel<8, NKK>), gf, bf, 0, s, t)
+    if (nk == 5) NA_TBF(5); else if (nk == 8) NA_TBF(8); else NA_TBF(9);
+#undef NA_TBF
+    NA_CHECK_LAUNCH();
+    return NA_OK;
+  }
   // backward: recompute dz = D W^T into the fp32 workspace, then the elementwise pass
   p.e.C = work; p.e.ldc = N; p.e.c_dtype = NA_F32;
   if (nwv == 8) launch_ws<8>((int)nk, PK_PLAIN, wg, p, s);
@@ -2681,7 +2862,8 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)ncu * 8));
+  const int64_t grid =
+      std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)ncu * tune_value(TUNE_TIB_WG_PER_CU)));
   hipLaunchKernelGGL(taps_increment_bwd_kernel, dim3((unsigned)grid), dim3(kTibBlock), 0, s,
                      (const float*)work, P, (int)N, zc, ld_zc, ws, beta, dds, (bf16_t*)dpre,
                      ld_dpre, (bf16_t*)centre, ld_centre, (bf16_t*)q, ld_q, db0, dws);

@@ -500,15 +500,15 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
 // in one pass.  The taps lie less than one cell from the centre at every
 // active level, so the union of the five points' corners fits a 3x3x3 box of
 // lattice points; the five contributions to each corner are summed in
-// registers.  A thread owns one feature of one level for kTapRun consecutive
-// points (consecutive samples of a ray): while their boxes share one anchor
+// registers.  A thread owns one feature of one level for `run` consecutive
+// points (consecutive samples of a ray; the launch tunable taps_bwd.run,
+// default 4): while their boxes share one anchor
 // the sums keep accumulating, and each touched corner gets one atomic when the
 // anchor changes or the run ends.  At coarse levels, where neighbouring
 // samples share cells and atomics to one address would otherwise queue, this
 // removes most of the atomics.  When a level's cells spread wider than one cell
 // (an epsilon larger than the finest active cell), that point falls back to
 // one atomic per point and corner.
-constexpr int kTapRun = 4;
 
 template <int F>
 __device__ __attribute__((always_inline)) inline void taps_flush(float* gtab, const LevelParams& p,
@@ -533,13 +533,13 @@ __global__ void __launch_bounds__(kBlock)
 hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
                          float vrange, int vshift, float eps, const TD* __restrict__ dyc,
                          const TD* __restrict__ dyt, const TD* __restrict__ dyq, int64_t ld,
-                         float* __restrict__ grad, int inc) {
+                         float* __restrict__ grad, int inc, int run) {
   auto unit = [&](float v) -> float {
     return P2 ? ldexpf(v - vmin, vshift) : normalize_coord(v, vmin, vrange);
   };
   const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int l = blockIdx.y;
-  const int64_t i0 = (tid / F) * kTapRun;
+  const int64_t i0 = (tid / F) * run;
   const int f = (int)(tid % F);
   if (i0 >= n) return;
   const int64_t col = (int64_t)l * F + f;
@@ -550,7 +550,7 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
   for (int o = 0; o < 27; ++o) acc[o] = 0.0f;
   uint32_t mask = 0;
   int blo[3] = {0, 0, 0};
-  const int64_t iend = i0 + kTapRun < n ? i0 + kTapRun : n;
+  const int64_t iend = i0 + run < n ? i0 + run : n;
   for (int64_t i = i0; i < iend; ++i) {
     float g[5];
     if (inc) {
@@ -902,17 +902,18 @@ static int launch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, flo
                            float vrange, float eps, const void* dyc, const void* dyt,
                            const void* dyq, int64_t ld, float* grad, int active, int inc,
                            hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(ceil_div(n, (int64_t)kTapRun) * F, kBlock), active);
+  const int run = (int)tune_value(TUNE_TAPS_BWD_RUN);
+  dim3 grid((unsigned)ceil_div(ceil_div(n, (int64_t)run) * F, kBlock), active);
   uint32_t vb;
   memcpy(&vb, &vrange, 4);
   if ((vb & 0x807fffffu) == 0u && (vb >> 23) != 0u)  // positive normal power of two
     hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, true>), grid, dim3(kBlock), 0, s, *d, x, n,
                        vmin, vrange, 127 - (int)(vb >> 23), eps, (const TD*)dyc, (const TD*)dyt,
-                       (const TD*)dyq, ld, grad, inc);
+                       (const TD*)dyq, ld, grad, inc, run);
   else
     hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, false>), grid, dim3(kBlock), 0, s, *d, x, n,
                        vmin, vrange, 0, eps, (const TD*)dyc, (const TD*)dyt, (const TD*)dyq, ld,
-                       grad, inc);
+                       grad, inc, run);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

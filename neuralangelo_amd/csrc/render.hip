@@ -498,11 +498,11 @@ composite_bwd_kernel(const float* __restrict__ alpha, const float* __restrict__ 
 // every per-sample array.  Same arithmetic and order as the dense kernels; the
 // backward keeps the transmittance of the forward pass in dalpha until the
 // reverse pass overwrites it, so a ray may hold any number of samples.
-__global__ void __launch_bounds__(kRayBlock)
+__global__ void __launch_bounds__(256)
 composite_packed_fwd_kernel(const float* __restrict__ alpha, const float* __restrict__ q,
                             const int32_t* __restrict__ off, int64_t R, int K,
                             float* __restrict__ weights, float* __restrict__ out) {
-  const int64_t r = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   const int64_t b = off[r], e = off[r + 1];
   float logT = 0.0f;
@@ -533,13 +533,13 @@ composite_packed_fwd_kernel(const float* __restrict__ alpha, const float* __rest
   for (int k = 0; k < K; ++k) out[r * K + k] = acc[k];
 }
 
-__global__ void __launch_bounds__(kRayBlock)
+__global__ void __launch_bounds__(256)
 composite_packed_bwd_kernel(const float* __restrict__ alpha, const float* __restrict__ q,
                             const float* __restrict__ weights, const int32_t* __restrict__ off,
                             int64_t R, int K, const float* __restrict__ dout,
                             const float* __restrict__ dweights, float* __restrict__ dalpha,
                             float* __restrict__ dq) {
-  const int64_t r = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   const int64_t b = off[r], e = off[r + 1];
   float logT = 0.0f;
@@ -1328,7 +1328,8 @@ extern "C" int na_composite_packed_fwd(const float* alpha, const float* q, const
   NA_CHECK_ARG(R >= 0 && K >= 0 && K <= 4, "bad sizes (K <= 4)");
   if (R == 0) return NA_OK;
   NA_CHECK_ARG(offsets && alpha && weights && (K == 0 || (q && out)), "null pointer");
-  hipLaunchKernelGGL(composite_packed_fwd_kernel, grid1d(R, kRayBlock), dim3(kRayBlock), 0,
+  const int rb = (int)tune_value(TUNE_PACKED_RAYS);
+  hipLaunchKernelGGL(composite_packed_fwd_kernel, grid1d(R, rb), dim3(rb), 0,
                      (hipStream_t)stream, alpha, q, offsets, R, K, weights, out);
   NA_CHECK_LAUNCH();
   return NA_OK;
@@ -1342,7 +1343,8 @@ extern "C" int na_composite_packed_bwd(const float* alpha, const float* q, const
   if (R == 0) return NA_OK;
   NA_CHECK_ARG(offsets && alpha && weights && dalpha, "null pointer");
   NA_CHECK_ARG(K == 0 || q, "q needed when K > 0");
-  hipLaunchKernelGGL(composite_packed_bwd_kernel, grid1d(R, kRayBlock), dim3(kRayBlock), 0,
+  const int rb = (int)tune_value(TUNE_PACKED_RAYS);
+  hipLaunchKernelGGL(composite_packed_bwd_kernel, grid1d(R, rb), dim3(rb), 0,
                      (hipStream_t)stream, alpha, q, weights, offsets, R, K, dout, dweights, dalpha,
                      dq);
   NA_CHECK_LAUNCH();

@@ -33,8 +33,9 @@ RGB_LD = 280  # 256 features + 3 + 16 + 3, padded to a multiple of 8
 # name -> list that collects (start, end) torch.cuda.Event pairs while set.
 LAUNCH_EVENTS = None
 # Optional record of the table-gradient scatter's inputs: while a list, each launch
-# of the centre + taps scatter appends (points, eps) (used to count the table
-# entries a launch touches).
+# of the centre + taps scatter appends (points, eps, dy_centre, dy_taps, dy_shared,
+# ld, increments) (used to count the table entries a launch touches and to tune
+# its launch on the step's own inputs).
 SCATTER_CAPTURE = None
 
 
@@ -569,12 +570,15 @@ class SDFCentreTapsDeltaFunction(torch.autograd.Function):
         dds_pm = torch.zeros(P32, 4, device=dev)
         if dds is not None:
             dds_pm[:P] = dds.t()
-        # the tap increments dz = D W0d^T are recomputed (fp32) into a transient buffer
-        dz = torch.empty(4 * P32, hid, dtype=torch.float32, device=dev)
+        # the tap increments dz = D W0d^T are recomputed inside the fused kernel, or
+        # (split launch shape) into a transient fp32 buffer
+        fused = _lib.call("na_tune_get", b"taps_increment_bwd.fused") != 0
+        dz = None if fused else torch.empty(4 * P32, hid, dtype=torch.float32, device=dev)
         _lib.call("na_taps_increment_bwd", _lib.ptr(D), ld_d, 4 * P32, ld_d, _lib.ptr(W0d), ld_d,
                   hid, _lib.ptr(Zc), hid, _lib.ptr(w.ws), float(SOFTPLUS_BETA), _lib.ptr(dds_pm),
                   _lib.ptr(dpre_t), hid, _lib.ptr(dpre_c), hid, _lib.ptr(q), hid,
-                  _lib.ptr(acc.db0), _lib.ptr(acc.dws), _lib.ptr(dz), _s(pts))
+                  _lib.ptr(acc.db0), _lib.ptr(acc.dws), _lib.ptr(dz) if dz is not None else None,
+                  _s(pts))
         del dz
         # first-layer weights: centre rows x X0c, tap rows x D (hi part and point
         # offset), shared rows x sum_t D_t
@@ -602,7 +606,7 @@ def _taps_scatter(enc, pts, eps, dXc, dXt, dXq, ld, dtable, increments):
             _lib.ptr(dXc), _lib.ptr(dXt), _lib.ptr(dXq) if dXq is not None else None, _code(dXc),
             ld, _lib.ptr(dtable), int(enc.active_levels), increments)
     if SCATTER_CAPTURE is not None:
-        SCATTER_CAPTURE.append((pts, eps))
+        SCATTER_CAPTURE.append((pts, eps, dXc, dXt, dXq, ld, increments))
     if enc.sorted_grad():
         ws = enc.sorted_workspace(P, 5, pts.device)
         _lib.call("na_sdf_taps_bwd_sorted", *args, _lib.ptr(ws), ws.numel(), _s(pts))
