@@ -1,9 +1,9 @@
 """Launch-shape autotuner.
 
 The library's launch tunables (na_tune_set / na_tune_get: the tap scatter's run
-of samples per thread, the tap-increment backward's workgroups per CU, the packed
-compositing kernels' rays per workgroup) have a best value that depends on the
-device and on the shape of the work.  `tune()` times every candidate of each on
+of samples per thread, the tap-increment backward's launch form and workgroups
+per CU, the packed compositing kernels' rays per workgroup) have a best value
+that depends on the device and on the shape of the work.  `tune()` times every candidate of each on
 the workload it serves -- the train step's samples (8192 rays x 96 samples,
 ray-ordered, as the renderer hands them over) and the ragged packed batch
 (65,536 rays, 0..192 samples) -- with HIP events on the launching stream, and

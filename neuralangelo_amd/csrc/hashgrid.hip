@@ -191,6 +191,28 @@ template <int F> struct RowStore<F, bf16_t> {
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
 
+// Backward launches cover (chunk of points, level) pairs.  Each level reads its
+// own 4F or 2F bytes of every dL/dy row, so the rows' cache lines are shared by
+// all levels: the levels of one chunk run back to back on one XCD (workgroups
+// are dispatched to the eight XCDs round-robin by id), and the rows come from
+// HBM about once instead of once per level.  Block b: XCD b mod 8, slot b / 8;
+// level slot mod levels, chunk (slot / levels) * 8 + XCD.  The grid holds
+// ceil(chunks / 8) * 8 * levels blocks; chunks past the end return at once.
+constexpr int kXcds = 8;
+
+__device__ inline void chunk_level(int levels, int64_t& chunk, int& level) {
+  const uint32_t b = blockIdx.x, slot = b / kXcds;
+  level = (int)(slot % (uint32_t)levels);
+  chunk = (int64_t)(slot / (uint32_t)levels) * kXcds + (b % kXcds);
+}
+
+static inline int chunk_level_grid(int64_t chunks, int levels, dim3* grid) {
+  const int64_t blocks = ceil_div(chunks, (int64_t)kXcds) * kXcds * levels;
+  NA_CHECK_ARG(blocks * kBlock <= (int64_t)0xffffffffu, "n too large for one launch");
+  *grid = dim3((unsigned)blocks);
+  return NA_OK;
+}
+
 __global__ void __launch_bounds__(kBlock)
 hashgrid_corners_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
                         int32_t* __restrict__ idx_out, float* __restrict__ w_out) {
@@ -328,9 +350,11 @@ template <int F, typename TD>
 __global__ void __launch_bounds__(kBlock)
 hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
                     float vrange, const TD* __restrict__ dy, int64_t ld,
-                    float* __restrict__ grad) {
-  int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  int l = blockIdx.y;
+                    float* __restrict__ grad, int active) {
+  int64_t chunk;
+  int l;
+  chunk_level(active, chunk, l);
+  int64_t tid = chunk * kBlock + threadIdx.x;
   int64_t i = tid / F;
   int f = (int)(tid % F);
   if (i >= n) return;
@@ -533,12 +557,14 @@ __global__ void __launch_bounds__(kBlock)
 hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, float vmin,
                          float vrange, int vshift, float eps, const TD* __restrict__ dyc,
                          const TD* __restrict__ dyt, const TD* __restrict__ dyq, int64_t ld,
-                         float* __restrict__ grad, int inc, int run) {
+                         float* __restrict__ grad, int inc, int run, int active) {
   auto unit = [&](float v) -> float {
     return P2 ? ldexpf(v - vmin, vshift) : normalize_coord(v, vmin, vrange);
   };
-  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int l = blockIdx.y;
+  int64_t chunk;
+  int l;
+  chunk_level(active, chunk, l);
+  const int64_t tid = chunk * kBlock + threadIdx.x;
   const int64_t i0 = (tid / F) * run;
   const int f = (int)(tid % F);
   if (i0 >= n) return;
@@ -551,29 +577,49 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
   uint32_t mask = 0;
   int blo[3] = {0, 0, 0};
   const int64_t iend = i0 + run < n ? i0 + run : n;
+  // the rows and coordinates of point i + 1 are loaded while point i is processed
+  struct Rows {
+    float c, t[4], q, x[3];
+  };
+  auto fetch = [&](int64_t i, Rows& r) {
+    if (inc) {
+      r.q = dyq ? scalar_io<TD>::load(dyq + i * ld + col) : 0.0f;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) r.t[t] = scalar_io<TD>::load(dyt + (4 * i + t) * ld + col);
+    } else {
+      r.q = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) r.t[t] = scalar_io<TD>::load(dyt + ((int64_t)t * n + i) * ld + col);
+    }
+    r.c = scalar_io<TD>::load(dyc + i * ld + col);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r.x[k] = x[i * 3 + k];
+  };
+  Rows nxt;
+  fetch(i0, nxt);
   for (int64_t i = i0; i < iend; ++i) {
+    const Rows cur = nxt;
+    if (i + 1 < iend) fetch(i + 1, nxt);
     float g[5];
     if (inc) {
       // increments: tap rows point-major, gradients of the inputs D_t = X(x + eps k_t)
       // - X(x) (dyt, plus the point row dyq, the gradient of sum_t D_t, shared by the
       // four taps); the centre row holds the gradient of the centre input alone, so
       // the centre's corners receive g_c - sum_t g_t (formed here in fp32)
-      const float q = dyq ? scalar_io<TD>::load(dyq + i * ld + col) : 0.0f;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) g[1 + t] = scalar_io<TD>::load(dyt + (4 * i + t) * ld + col) + q;
-      g[0] = scalar_io<TD>::load(dyc + i * ld + col) - ((g[1] + g[2]) + (g[3] + g[4]));
+      for (int t = 0; t < 4; ++t) g[1 + t] = cur.t[t] + cur.q;
+      g[0] = cur.c - ((g[1] + g[2]) + (g[3] + g[4]));
     } else {
-      g[0] = scalar_io<TD>::load(dyc + i * ld + col);
+      g[0] = cur.c;
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-        g[1 + t] = scalar_io<TD>::load(dyt + ((int64_t)t * n + i) * ld + col);
+      for (int t = 0; t < 4; ++t) g[1 + t] = cur.t[t];
     }
     if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && g[3] == 0.0f && g[4] == 0.0f) continue;
     int cell[5][3];
     float fr[5][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const float xc = x[i * 3 + k];
+      const float xc = cur.x[k];
       uint32_t c;
       pos_fract(unit(xc), p.scale, c, fr[0][k]);
       cell[0][k] = (int)c;
@@ -873,9 +919,10 @@ struct BwdLaunch {
 template <int F, typename TD>
 static int launch_bwd(const BwdLaunch& a) {
   if (a.active <= 0) return NA_OK;
-  dim3 grid((unsigned)ceil_div(a.n * F, kBlock), a.active);
+  dim3 grid;
+  if (int st = chunk_level_grid(ceil_div(a.n * F, kBlock), a.active, &grid)) return st;
   hipLaunchKernelGGL((hashgrid_bwd_kernel<F, TD>), grid, dim3(kBlock), 0, a.s, *a.d, a.x, a.n,
-                     a.vmin, a.vrange, (const TD*)a.dy, a.ld, a.grad);
+                     a.vmin, a.vrange, (const TD*)a.dy, a.ld, a.grad, a.active);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
@@ -903,17 +950,21 @@ static int launch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, flo
                            const void* dyq, int64_t ld, float* grad, int active, int inc,
                            hipStream_t s) {
   const int run = (int)tune_value(TUNE_TAPS_BWD_RUN);
-  dim3 grid((unsigned)ceil_div(ceil_div(n, (int64_t)run) * F, kBlock), active);
+  dim3 grid;
+  if (int st = chunk_level_grid(ceil_div(ceil_div(n, (int64_t)run) * F, kBlock), active, &grid))
+    return st;
   uint32_t vb;
   memcpy(&vb, &vrange, 4);
-  if ((vb & 0x807fffffu) == 0u && (vb >> 23) != 0u)  // positive normal power of two
+  const bool p2 = (vb & 0x807fffffu) == 0u && (vb >> 23) != 0u;  // positive normal power of two
+  const int vshift = p2 ? 127 - (int)(vb >> 23) : 0;
+  if (p2)
     hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, true>), grid, dim3(kBlock), 0, s, *d, x, n,
-                       vmin, vrange, 127 - (int)(vb >> 23), eps, (const TD*)dyc, (const TD*)dyt,
-                       (const TD*)dyq, ld, grad, inc, run);
+                       vmin, vrange, vshift, eps, (const TD*)dyc, (const TD*)dyt, (const TD*)dyq,
+                       ld, grad, inc, run, active);
   else
     hipLaunchKernelGGL((hashgrid_taps_bwd_kernel<F, TD, false>), grid, dim3(kBlock), 0, s, *d, x, n,
-                       vmin, vrange, 0, eps, (const TD*)dyc, (const TD*)dyt, (const TD*)dyq, ld,
-                       grad, inc, run);
+                       vmin, vrange, vshift, eps, (const TD*)dyc, (const TD*)dyt, (const TD*)dyq,
+                       ld, grad, inc, run, active);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

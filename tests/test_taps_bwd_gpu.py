@@ -175,8 +175,9 @@ def test_increment_layout_scatter(cuda, dt):
               _lib.ptr(plain_c), _lib.ptr(plain_t), None, _lib.NA_F32, LF, _lib.ptr(ref), 16, 0, s)
     got = torch.zeros(enc.n_params, device=cuda)
     dyt_pm = dyt.reshape(4 * n, LF).contiguous().to(cuda)
+    dyc_d, dyq_d = dyc.to(cuda), dyq.to(cuda)  # held until the launch has run
     _lib.call("na_sdf_taps_bwd", ctypes.byref(enc.desc), _lib.ptr(pts), n, -2.0, 2.0, float(eps),
-              _lib.ptr(dyc.to(cuda)), _lib.ptr(dyt_pm), _lib.ptr(dyq.to(cuda)),
+              _lib.ptr(dyc_d), _lib.ptr(dyt_pm), _lib.ptr(dyq_d),
               _lib.dtype_code(dt), LF, _lib.ptr(got), 16, 1, s)
     torch.cuda.synchronize()
     scale = ref.abs().max().item()
