@@ -64,9 +64,9 @@ int na_version(void);
  *                                    backward (1..32, 8)
  *   "packed.rays_per_wg"             rays per workgroup of the packed compositing
  *                                    kernels (64..256, multiple of 64; 64)
- *   "taps_increment_bwd.fused"       tap-increment backward fused with the
- *                                    recomputation of its product (1) or split in
- *                                    two passes (0); 0
+ *   "taps_increment_bwd.fused"       tap-increment backward in the epilogue of
+ *                                    its recomputed product (1, no workspace) or
+ *                                    split in two passes (0); 0
  * na_tune_set returns the previous value (or an error code for an unknown name or
  * a value out of range); the launchers read the current value at every launch.
  * Set them between launches: a change is not ordered with launches in flight on
@@ -212,9 +212,10 @@ int na_gemm(const na_gemm_args* args, na_stream_t stream);
  *        gradient) keep fp32 precision relative to their own size.  The backward
  *        recomputes dz = D W^T: into work [M, N], a caller-owned 16-byte aligned
  *        workspace of M * N floats, followed by an elementwise pass (the launch
- *        tunable "taps_increment_bwd.fused" = 0, the default), or inside one
- *        fused kernel that needs no workspace (= 1; work may be NULL); dds [M] is
- *        16-byte aligned (four taps per point). */
+ *        tunable "taps_increment_bwd.fused" = 0, the default), or in the epilogue
+ *        of that product, where dz never leaves the registers and no workspace is
+ *        needed (= 1; work may be NULL); dds [M] is 16-byte aligned (four taps per
+ *        point). */
 int na_taps_increment_fwd(const void* D, int64_t ld_d, int64_t M, int64_t K, const void* W,
                           int64_t ldw, int64_t N, const float* zc, int64_t ld_zc,
                           const float* ws, float beta, float* head_out, na_stream_t stream);

@@ -25,7 +25,7 @@ enum TuneKey : int {
   TUNE_TAPS_BWD_RUN = 0,   // consecutive samples per thread in the tap scatter
   TUNE_TIB_WG_PER_CU = 1,  // workgroups per CU of the tap-increment backward
   TUNE_PACKED_RAYS = 2,    // rays (threads) per workgroup of the packed compositing
-  TUNE_TIB_FUSED = 3,      // tap-increment backward fused with its product (1) or split (0)
+  TUNE_TIB_FUSED = 3,      // tap-increment backward: split (0) or in the product's epilogue (1)
   TUNE_COUNT = 4
 };
 int64_t tune_value(TuneKey key);

@@ -46,6 +46,14 @@ struct EpiParams {
   int64_t zc_rows;
   const float* head_w;
   float* head_out;
+  // tap-increment backward (PK_TAPS_BWD): dL/d(tap SDF increment) per row, the
+  // centre rows (read-modify-write), the shared rows and the head-weight sums
+  const float* tap_dds;
+  void* tap_centre;
+  int64_t tap_ld_centre;
+  void* tap_q;
+  int64_t tap_ld_q;
+  float* colsum2;
 };
 
 struct GemmParams {
@@ -1209,6 +1217,7 @@ enum PersistKind : int {
   // Tap increments of the 4-tap numerical gradient, rows point-major (row m is tap
   // m % 4 of point m / 4, whose stored centre pre-activation is zc row m / 4):
   PK_TAPS_FWD = 5,      // head over softplus(z + dz) - softplus(z) only (no C)
+  PK_TAPS_BWD = 6,      // backward of the increments: C = tap rows, centre +=, shared rows, sums
 };
 
 // softplus_beta with a select for the linear region (no branch); ib = 1 / beta
@@ -1270,6 +1279,88 @@ __device__ inline void softplus_deriv_step(float z0, float dz, float beta, float
   const float sna = 1.0f / (1.0f + __expf(a));
   ds = s1 * sna * one_minus_exp_neg(u);
   s0 = 1.0f - sna;  // sp'(z0)
+}
+
+// Branch-free forms for the tap backward epilogue (sixteen elements per lane in
+// flight; a data-dependent branch per element splits its schedule).
+// Increment h = softplus(z0 + dz) - softplus(z0) as softplus_step forms it, with
+// ln P = ln(1 + e^{-(|a0| + |u|)}) (same signs) or -m + ln(1 + e^{-(M - m)})
+// (opposite signs; m, M the smaller and larger of |a0|, |u|): the larger
+// exponent is taken out first, so P cannot underflow and no slow path is needed.
+__device__ inline float softplus_incr_nb(float z0, float dz, float beta, float ib) {
+  constexpr float kLog2e = 1.4426950408889634f, kLn2 = 0.6931471805599453f;
+  const float a0 = fabsf(z0 * beta), u = fabsf(dz * beta);
+  const bool same = (z0 >= 0.0f) == (dz >= 0.0f);
+  const float m = fminf(a0, u);
+  const float sp = same ? a0 + u : fmaxf(a0, u) - m;
+  const float off = same ? 0.0f : m * kLog2e;
+  const float lP = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-sp * kLog2e)) - off;
+  const float l0 = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-a0 * kLog2e));
+  return fmaf(lP - l0, kLn2 * ib, fmaxf(dz, 0.0f));
+}
+
+// 1 - exp(-u) as one_minus_exp_neg, both branches evaluated and selected.
+__device__ inline float one_minus_exp_neg_nb(float u) {
+  const float t = u * (1.0f - 0.5f * u * (1.0f - (1.0f / 3.0f) * u * (1.0f - 0.25f * u * (1.0f - 0.2f * u))));
+  const float x = 1.0f - __expf(-u);
+  return fabsf(u) < 0.0625f ? t : x;
+}
+
+__device__ inline void softplus_deriv_step_nb(float z0, float dz, float beta, float& s1, float& ds,
+                                              float& s0) {
+  const float a = z0 * beta, u = dz * beta;
+  s1 = 1.0f / (1.0f + __expf(-(a + u)));
+  const float sna = 1.0f / (1.0f + __expf(a));
+  ds = s1 * sna * one_minus_exp_neg_nb(u);
+  s0 = 1.0f - sna;
+}
+
+// Backward terms of one (point, column) of the tap-increment layer, from the
+// centre pre-activation z and the four taps' increments dz_t, with the terms of z
+// alone formed once for the four taps.  With a = b z, u_t = b dz_t (b = beta),
+// t0 = e^{-|a|}, e_t = e^{-|u_t|} and p_t = e^{-|a + u_t|}:
+//   sp'(z) = sigmoid(a), sigmoid(-a)            one reciprocal of 1 + t0
+//   sp'(z + dz_t) = sigmoid(a + u_t)            one reciprocal of 1 + p_t
+//   b h_t = max(u_t, 0) + ln(1 + p_t) - [opposite signs] min(|a|, |u_t|)
+//           - ln(1 + t0)                        (softplus_step's P, with the
+//                                               larger exponent taken out: no
+//                                               underflow, no slow path)
+//   dsp'_t = sigmoid(a + u_t) sigmoid(-a) (1 - e^{-u_t}), where 1 - e^{-u} is
+//           1 - e_t (a series below |u| = 1/16) for u >= 0 and -(1 - e_t) / e_t
+//           below zero: a product of factors, precise relative to its own size.
+// Outputs: tap row values o_t = w (b_t sp'(z + dz_t) + am dsp'_t) (b_t = dv_t -
+// am), ev = sum_t dv_t w dsp'_t, hs += sum_t dv_t h_t (in tap order), and sp'(z).
+__device__ inline void tap_bwd_column(float z, const float (&dz)[4], const float (&dv)[4], float am,
+                                      float w, float beta, float ib, float (&o)[4], float& ev,
+                                      float& hs, float& s0) {
+  constexpr float kLog2e = 1.4426950408889634f, kLn2 = 0.6931471805599453f;
+  const float za = z * beta, aa = fabsf(za);
+  const float t0 = __builtin_amdgcn_exp2f(-aa * kLog2e);
+  const float l0 = __builtin_amdgcn_logf(1.0f + t0);
+  const float r0 = __builtin_amdgcn_rcpf(1.0f + t0);
+  s0 = za >= 0.0f ? r0 : t0 * r0;
+  const float sna = za >= 0.0f ? t0 * r0 : r0;
+  const float sc = kLn2 * ib;
+  ev = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const float u = dz[t] * beta, au = fabsf(u), v = za + u;
+    const float e = __builtin_amdgcn_exp2f(-au * kLog2e);
+    const float pp = __builtin_amdgcn_exp2f(-fabsf(v) * kLog2e);
+    const bool same = (z >= 0.0f) == (dz[t] >= 0.0f);
+    const float off = same ? 0.0f : fminf(aa, au) * kLog2e;
+    const float hv = fmaf(__builtin_amdgcn_logf(1.0f + pp) - off - l0, sc, fmaxf(dz[t], 0.0f));
+    const float r1 = __builtin_amdgcn_rcpf(1.0f + pp);
+    const float s1 = v >= 0.0f ? r1 : pp * r1;
+    const float ser = au * (1.0f - 0.5f * au * (1.0f - (1.0f / 3.0f) * au *
+                                                  (1.0f - 0.25f * au * (1.0f - 0.2f * au))));
+    const float om = au < 0.0625f ? ser : 1.0f - e;
+    const float ome = u >= 0.0f ? om : -om * __builtin_amdgcn_rcpf(e);
+    const float dsp = s1 * sna * ome;
+    o[t] = w * fmaf(dv[t] - am, s1, am * dsp);
+    ev = fmaf(dv[t] * w, dsp, ev);
+    hs = fmaf(dv[t], hv, hs);
+  }
 }
 
 // One 32 x 32 block of the wave, staged in eb: lane (rl, cg) handles columns
@@ -1766,6 +1857,54 @@ __device__ inline void ws_issue(__amdgpu_buffer_rsrc_t r, uint32_t off, bf16_t* 
   __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t*)dst, 16, off, 0, 0, 0);
 }
 
+// Four-element row chunks (16 B of fp32, 8 B of bf16).
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4_t;
+
+__device__ inline void vload4(const float* p, float (&v)[4]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ inline void vload4(const bf16_t* p, float (&v)[4]) {
+  const uint2 a = *reinterpret_cast<const uint2*>(p);
+  v[0] = __uint_as_float(a.x << 16);
+  v[1] = __uint_as_float(a.x & 0xffff0000u);
+  v[2] = __uint_as_float(a.y << 16);
+  v[3] = __uint_as_float(a.y & 0xffff0000u);
+}
+__device__ inline void vstore4(bf16_t* p, const float (&v)[4]) {
+  const f32x4_t x = {v[0], v[1], v[2], v[3]};
+  *reinterpret_cast<bf16x4_t*>(p) = __builtin_convertvector(x, bf16x4_t);
+}
+
+// Lane K of this lane's quad, and the quad's sum (the same value in all four
+// lanes: (v0 + v1) + (v2 + v3)), through DPP quad permutes.
+template <int K>
+__device__ inline float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xf, 0xf, false));
+}
+__device__ inline float quad_sum(float v) {
+  const float s = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xf, 0xf, false));
+  return s + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x4E, 0xf, 0xf, false));
+}
+
+// Sums over the 32 rows held by the lanes of each wave half (the lanes that share
+// h); every lane of the half ends with the sums of its 16 columns.
+__device__ inline void ws_reduce_rows(float (&cs)[2][8]) {
+#pragma unroll
+  for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      float s = cs[gq][c];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      s += __shfl_xor(s, 4);
+      s += __shfl_xor(s, 8);
+      s += __shfl_xor(s, 16);
+      cs[gq][c] = s;
+    }
+}
+
 // Column sums of a weights-stationary wave: sums over the 32 rows of the lanes that
 // share h, then one atomic per column (dst + 16 g + c, the lane's columns).
 __device__ inline void ws_flush_colsum(float* dst, float (&cs)[2][8], int lane) {
@@ -1870,13 +2009,15 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
 
   const float ib = 1.0f / e.beta;
   // derivative kind: rank-1 coefficients of the lane's columns and column sums
-  constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA || KIND == PK_TAPS_FWD;
+  constexpr bool kHead = KIND == PK_BIAS_SOFTPLUS || KIND == PK_DELTA || KIND == PK_TAPS_FWD ||
+                        KIND == PK_TAPS_BWD;
+  constexpr bool kTB = KIND == PK_TAPS_BWD;
   float rb[2][8], cs[2][8], hw[2][8];
 #pragma unroll
   for (int gq = 0; gq < 2; ++gq)
 #pragma unroll
     for (int c = 0; c < 8; ++c) rb[gq][c] = cs[gq][c] = hw[gq][c] = 0.0f;
-  if constexpr (kHead) {
+  if constexpr (kHead && !kTB) {
     if (e.head_w) {
 #pragma unroll
       for (int gq = 0; gq < 2; ++gq) vload8(e.head_w + nw + 16 * gq + 8 * h, hw[gq]);
@@ -1888,6 +2029,16 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
       for (int gq = 0; gq < 2; ++gq) vload8(e.r1_b + nw + 16 * gq + 8 * h, rb[gq]);
     }
   }
+  // tap backward: lane t of each lane quad (the four taps of one point, rows
+  // 4 p .. 4 p + 3) owns columns 4 t .. 4 t + 3 of the quad's 16 in the centre and
+  // shared rows; hs2 collects the head-weight gradient of the lane's columns
+  // shared rows; the column sums of a tile are reduced over the wave's rows and
+  // kept in LDS (one region per wave: [sum][h][16 columns]), so no sums stay in
+  // registers across the product loop
+  const int tq = lane & 3;
+  const int64_t tb_col = nw + 16 * (tq >> 1) + 8 * h + 4 * (tq & 1);
+  __shared__ float tb_sums[kTB ? NWV * 64 : 1];
+  if constexpr (kTB) tb_sums[wave * 64 + lane] = 0.0f;
   int slot = 0;
   int64_t g = 0;
   for (int64_t jt = 0; jt < my_tiles; ++jt) {
@@ -1916,6 +2067,16 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
     if constexpr (KIND == PK_TAPS_FWD) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) load_zc(i, zv[i]);
+    }
+    // tap backward: the first row group's centre pre-activations now, the others'
+    // at the start of their group
+    if constexpr (kTB) load_zc(0, zv[0]);
+    // tap backward: each row's dL/d(increment) (dds is point-major [M / 4, 4], so
+    // row m's entry is dds[m]) and the centre chunk this lane updates
+    float dvr[kTB ? 4 : 1];
+    if constexpr (kTB) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dvr[i] = e.tap_dds[m0 + i * 32 + (lane & 31)];
     }
     if constexpr (KIND == PK_DACT) {
 #pragma unroll
@@ -1970,6 +2131,83 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
           o[gq][q] = __uint_as_float(r[0]);
           o[gq][4 + q] = __uint_as_float(r[1]);
         }
+      if constexpr (kTB) {
+        if (i == 0) {
+#pragma unroll
+          for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) cs[gq][c] = rb[gq][c] = 0.0f;  // this tile's sums
+        }
+        // row m is tap t = m % 4 of point m / 4; o holds its dz for 16 columns.
+        // With dds_t = a + b_t (a the quad's mean): tap row w (b_t sp'(z + dz) +
+        // a dsp'), centre += sum_t dds_t w dsp' (a quad sum), shared row a w sp'(z),
+        // and the column sums of the centre increments (b0) and of dds_t (sp(z + dz)
+        // - sp(z)) (head weights)
+        float (&z)[2][8] = zv[0];
+        if (i > 0) load_zc(i, z);
+        const int64_t pt = m >> 2;
+        const uint2 cen = *reinterpret_cast<const uint2*>((const bf16_t*)e.tap_centre +
+                                                          pt * e.tap_ld_centre + tb_col);
+        float w8[2][8];  // head weights of the lane's columns (cache hits)
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq) vload8(e.head_w + nw + 16 * gq + 8 * h, w8[gq]);
+        const float dv = dvr[i];
+        const float a = 0.25f * ((quad_bcast<0>(dv) + quad_bcast<1>(dv)) +
+                                 (quad_bcast<2>(dv) + quad_bcast<3>(dv)));
+        const float b = dv - a;
+        // the lane keeps the quad sums and shared-row values of its own 4 columns
+        // (chunk tq = 2 gq + c / 4 of its 16)
+        float ce[4], cq[4];
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            float s1, dsp, s0;
+            const float hv = softplus_incr_nb(z[gq][c], o[gq][c], e.beta, ib);
+            softplus_deriv_step_nb(z[gq][c], o[gq][c], e.beta, s1, dsp, s0);
+            const float w = w8[gq][c];
+            const float ev = (dv * w) * dsp;
+            cs[gq][c] += ev;
+            rb[gq][c] = fmaf(dv, hv, rb[gq][c]);
+            const float es = quad_sum(ev), qv = (a * w) * s0;
+            const bool mine = tq == 2 * gq + c / 4;
+            if (gq == 0 && c < 4) {
+              ce[c & 3] = es;
+              cq[c & 3] = qv;
+            } else {
+              ce[c & 3] = mine ? es : ce[c & 3];
+              cq[c & 3] = mine ? qv : cq[c & 3];
+            }
+            o[gq][c] = w * fmaf(b, s1, a * dsp);
+          }
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq)
+          vstore8((bf16_t*)e.C + m * e.ldc + nw + 16 * gq + 8 * h, o[gq]);
+        const uint32_t cw[2] = {cen.x, cen.y};
+        float cf[4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          cf[2 * k] = __uint_as_float(cw[k] << 16) + ce[2 * k];
+          cf[2 * k + 1] = __uint_as_float(cw[k] & 0xffff0000u) + ce[2 * k + 1];
+        }
+        vstore4((bf16_t*)e.tap_centre + pt * e.tap_ld_centre + tb_col, cf);
+        vstore4((bf16_t*)e.tap_q + pt * e.tap_ld_q + tb_col, cq);
+        if (i == 3) {
+          // the tile's column sums over the 32 rows of each lane half, into LDS
+          ws_reduce_rows(cs);
+          ws_reduce_rows(rb);
+          if ((lane & 31) == 0) {
+#pragma unroll
+            for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+              for (int c = 0; c < 8; ++c) {
+                tb_sums[wave * 64 + 16 * h + 8 * gq + c] += cs[gq][c];
+                tb_sums[wave * 64 + 32 + 16 * h + 8 * gq + c] += rb[gq][c];
+              }
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int gq = 0; gq < 2; ++gq) {
         if constexpr (KIND == PK_DACT) {
@@ -2032,6 +2270,12 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
   if constexpr (KIND == PK_DACT) {
     if (e.colsum) ws_flush_colsum(e.colsum + nw + 8 * h, cs, lane);
   }
+  if constexpr (kTB) {
+    // lane j adds entry j of the wave's sums: (sum j / 32, h, group, column)
+    const int j = lane & 31, hh = j >> 4, gq = (j >> 3) & 1, c = j & 7;
+    float* dst = lane < 32 ? e.colsum : e.colsum2;
+    atomic_add_f32(dst + nw + 16 * gq + 8 * hh + c, tb_sums[wave * 64 + lane]);
+  }
   __builtin_amdgcn_s_waitcnt(0xF70);  // no buffer-to-LDS load outlives the workgroup
 }
 
@@ -2052,25 +2296,6 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
 // added with one atomic per column.
 // ---------------------------------------------------------------------------
 constexpr int kTibBlock = 256;
-typedef __attribute__((ext_vector_type(4))) float f32x4_t;
-typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4_t;
-
-__device__ inline void vload4(const float* p, float (&v)[4]) {
-  const float4 a = *reinterpret_cast<const float4*>(p);
-  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-}
-__device__ inline void vload4(const bf16_t* p, float (&v)[4]) {
-  const uint2 a = *reinterpret_cast<const uint2*>(p);
-  v[0] = __uint_as_float(a.x << 16);
-  v[1] = __uint_as_float(a.x & 0xffff0000u);
-  v[2] = __uint_as_float(a.y << 16);
-  v[3] = __uint_as_float(a.y & 0xffff0000u);
-}
-__device__ inline void vstore4(bf16_t* p, const float (&v)[4]) {
-  const f32x4_t x = {v[0], v[1], v[2], v[3]};
-  *reinterpret_cast<bf16x4_t*>(p) = __builtin_convertvector(x, bf16x4_t);
-}
-
 __global__ void __launch_bounds__(kTibBlock)
 taps_increment_bwd_kernel(const float* __restrict__ dz, int64_t P, int N,
                           const float* __restrict__ zc, int64_t ld_zc,
@@ -2097,25 +2322,17 @@ taps_increment_bwd_kernel(const float* __restrict__ dz, int64_t P, int N,
     const float4 d4 = *reinterpret_cast<const float4*>(dds + 4 * i);
     const float dv[4] = {d4.x, d4.y, d4.z, d4.w};
     const float a = 0.25f * ((dv[0] + dv[1]) + (dv[2] + dv[3]));
-    float ev[4], s0v[4];
+    float ev[4], s0v[4], o[4][4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) ev[c] = 0.0f;
+    for (int c = 0; c < 4; ++c) {
+      const float dzc[4] = {x[0][c], x[1][c], x[2][c], x[3][c]};
+      float oc[4];
+      tap_bwd_column(z0[c], dzc, dv, a, w[c], beta, ib, oc, ev[c], hs[c], s0v[c]);
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const float b = dv[t] - a;
-      float o[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        float hv, y1, s1, dsp, s0;
-        softplus_step(z0[c], x[t][c], beta, ib, hv, y1);
-        softplus_deriv_step(z0[c], x[t][c], beta, s1, dsp, s0);
-        o[c] = w[c] * fmaf(b, s1, a * dsp);
-        ev[c] = fmaf(dv[t] * w[c], dsp, ev[c]);
-        hs[c] = fmaf(dv[t], hv, hs[c]);
-        s0v[c] = s0;
-      }
-      vstore4(dpre + (4 * i + t) * ld_dpre + c0, o);
+      for (int t = 0; t < 4; ++t) o[t][c] = oc[t];
     }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) vstore4(dpre + (4 * i + t) * ld_dpre + c0, o[t]);
     float qv[4], cv[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) qv[c] = (a * w[c]) * s0v[c];
@@ -2148,166 +2365,6 @@ taps_increment_bwd_kernel(const float* __restrict__ dz, int64_t P, int N,
       atomic_add_f32(dws + c0 + c, hs[c]);
     }
   }
-}
-
-// ---------------------------------------------------------------------------
-// The same backward fused with the recomputation of dz: a weights-stationary
-// product on 16 x 16 x 32 MFMAs whose accumulators hand each lane the four taps
-// of one point in one column, so the per-point sums stay in the lane.
-// NW waves of 16 columns each (the encoder weights of a wave's columns in 36
-// VGPRs per lane; blockIdx.y selects the 16 NW-column slice), one workgroup per
-// CU walking 128-row tiles (the slices of a tile run side by side); tap rows
-// stream through LDS stages of 128 x 32 bf16 loaded by eight of the waves (16
-// rows each, buffer-to-LDS, the chunk swizzle of the weights-stationary
-// kernel).  MFMA operands: A = 16 tap rows x 32 k from the stage (lane l: row
-// l % 16, k 8 (l / 16) .. + 7), B = 32 k x 16 columns from registers (lane l:
-// column l % 16, same k), so D lane l holds rows 4 (l / 16) .. + 3 of the
-// 16-row group in column l % 16: the four taps of one point.  The epilogue forms
-// the terms of taps_increment_bwd_kernel per (point, column) and never writes dz.
-// ---------------------------------------------------------------------------
-typedef __attribute__((ext_vector_type(4))) float f32x4v_t;
-constexpr int kTbStages = 12;
-
-struct TapsBwdArgs {
-  const bf16_t* D;
-  int64_t ld_d, M, K;
-  const bf16_t* W;
-  int64_t ldw;
-  const float* zc;
-  int64_t ld_zc;
-  const float* ws;
-  float beta;
-  const float* dds;
-  bf16_t* dpre;
-  int64_t ld_dpre;
-  bf16_t* centre;
-  int64_t ld_centre;
-  bf16_t* q;
-  int64_t ld_q;
-  float* db0;
-  float* dws;
-};
-
-template <int NW, int NK>
-__global__ void __launch_bounds__(64 * NW, 1) taps_bwd_fused_kernel(TapsBwdArgs a) {
-  __shared__ __attribute__((aligned(16))) bf16_t st0[kTbStages * kWsStage];
-  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n = (int)blockIdx.y * 16 * NW + wave * 16 + (lane & 15);  // output column
-  const int rq = lane >> 4;               // tap rows 4 rq .. 4 rq + 3 of each 16-row group
-  const uint32_t n_mt = (uint32_t)(a.M / GBM);
-  const int64_t my_tiles = blockIdx.x < n_mt ? (n_mt - 1 - blockIdx.x) / gridDim.x + 1 : 0;
-
-  // weights of column n: k = 32 s + 8 rq .. + 7 of every stage s
-  bf16x8_t bw[NK];
-  {
-    const bf16_t* brow = a.W + (int64_t)n * a.ldw;
-#pragma unroll
-    for (int s = 0; s < NK; ++s) {
-      const int64_t k = 32 * s + 8 * rq;
-      uint4 u = *reinterpret_cast<const uint4*>(brow + (k < a.K ? k : a.K - 8));
-      if (k >= a.K) u = make_uint4(0u, 0u, 0u, 0u);
-      bw[s] = __builtin_bit_cast(bf16x8_t, u);
-    }
-  }
-  const float w = a.ws[n];
-  const float ib = 1.0f / a.beta;
-
-  // stage loads: waves 0..7 fill the 16-row blocks of a stage (one 1 KB load each)
-  const bool loader = wave < 8;
-  const int arow = wave * 16 + (lane >> 2);
-  const int achunk = (lane & 3) ^ ((arow >> 2) & 3);
-  const uint32_t avoff = (uint32_t)(arow * a.ld_d * 2 + achunk * 16);
-  const int akofs = achunk * 8;
-  int64_t issued = 0;
-  uint32_t is_tile = blockIdx.x;
-  int is_t = 0, is_slot = 0;
-  const char* is_a = reinterpret_cast<const char*>(a.D + (int64_t)is_tile * GBM * a.ld_d);
-  int64_t is_ae = (a.M - (int64_t)is_tile * GBM) * a.ld_d * 2;
-  auto issue_next = [&]() {
-    if (loader) {
-      const int64_t k0 = (int64_t)is_t * kNNBK;
-      const __amdgpu_buffer_rsrc_t rs = stage_rsrc(is_a, is_ae, k0 * 2);
-      ws_issue(rs, akofs < a.K - k0 ? avoff : kOobOffset,
-               st0 + is_slot * kWsStage + wave * 16 * kNNBK);
-    }
-    ++issued;
-    is_slot = is_slot + 1 == kTbStages ? 0 : is_slot + 1;
-    if (++is_t == NK) {
-      is_t = 0;
-      is_tile += gridDim.x;
-      is_a = reinterpret_cast<const char*>(a.D + (int64_t)is_tile * GBM * a.ld_d);
-      is_ae = (a.M - (int64_t)is_tile * GBM) * a.ld_d * 2;
-    }
-  };
-#pragma unroll
-  for (int s = 0; s < kTbStages - 1; ++s) issue_next();
-
-  float cs = 0.0f, hs = 0.0f;  // column sums of the centre term and of dds * dh
-  int slot = 0;
-  int64_t g = 0;
-  for (int64_t jt = 0; jt < my_tiles; ++jt) {
-    const int64_t m0 = (int64_t)(blockIdx.x + (uint32_t)jt * gridDim.x) * GBM;
-    f32x4v_t acc[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) acc[r] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int t = 0; t < NK; ++t, ++g) {
-      if (loader) ws_wait((int)(issued - g - 1));
-      __builtin_amdgcn_s_barrier();  // stage g landed; stage g - 1 consumed by every wave
-      issue_next();
-      const bf16_t* sa = st0 + slot * kWsStage;
-      slot = slot + 1 == kTbStages ? 0 : slot + 1;
-      // two row groups' operands in flight at a time (the register budget of
-      // four waves per SIMD does not hold all eight)
-#pragma unroll
-      for (int r = 0; r < 8; r += 2) {
-        const int ra0 = r * 16 + (lane & 15), ra1 = ra0 + 16;
-        const bf16x8_t av0 =
-            *reinterpret_cast<const bf16x8_t*>(sa + ra0 * kNNBK + ((rq ^ ((ra0 >> 2) & 3)) << 3));
-        const bf16x8_t av1 =
-            *reinterpret_cast<const bf16x8_t*>(sa + ra1 * kNNBK + ((rq ^ ((ra1 >> 2) & 3)) << 3));
-        acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av0, bw[t], acc[r], 0, 0, 0);
-        acc[r + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av1, bw[t], acc[r + 1], 0, 0, 0);
-        asm volatile("" ::: "memory");
-      }
-    }
-    // epilogue: row group r holds point m0 / 4 + 4 r + rq, taps 0..3, column n
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int64_t pt = (m0 >> 2) + 4 * r + rq;
-      const float z0 = a.zc[pt * a.ld_zc + n];
-      const float4 d4 = *reinterpret_cast<const float4*>(a.dds + 4 * pt);
-      const float dv[4] = {d4.x, d4.y, d4.z, d4.w};
-      const float am = 0.25f * ((dv[0] + dv[1]) + (dv[2] + dv[3]));
-      float ev = 0.0f, s0 = 0.0f;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float hv, y1, s1, dsp;
-        softplus_step(z0, acc[r][i], a.beta, ib, hv, y1);
-        softplus_deriv_step(z0, acc[r][i], a.beta, s1, dsp, s0);
-        a.dpre[(4 * pt + i) * a.ld_dpre + n] = f32_to_bf16(w * fmaf(dv[i] - am, s1, am * dsp));
-        ev = fmaf(dv[i] * w, dsp, ev);
-        hs = fmaf(dv[i], hv, hs);
-      }
-      a.q[pt * a.ld_q + n] = f32_to_bf16((am * w) * s0);
-      bf16_t* cp = a.centre + pt * a.ld_centre + n;
-      *cp = f32_to_bf16(bf16_to_f32(*cp) + ev);
-      cs += ev;
-      // one row group at a time: its loads are not hoisted above the previous
-      // group's stores, so only one group's operands are live (register budget)
-      asm volatile("" ::: "memory");
-    }
-  }
-  // column sums: the four lanes of column n (lane bits 4 and 5)
-  cs += __shfl_xor(cs, 16);
-  cs += __shfl_xor(cs, 32);
-  hs += __shfl_xor(hs, 16);
-  hs += __shfl_xor(hs, 32);
-  if (rq == 0 && my_tiles > 0) {
-    atomic_add_f32(a.db0 + n, cs);
-    atomic_add_f32(a.dws + n, hs);
-  }
-  __builtin_amdgcn_s_waitcnt(0xF70);  // no buffer-to-LDS load outlives the workgroup
 }
 
 // ---------------------------------------------------------------------------
@@ -2627,6 +2684,7 @@ static void launch_ws(int nk, int kind, int grid, const GemmParams& p, hipStream
     case PK_DACT: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_DACT>), g, b, 0, s, p); break;           \
     case PK_DELTA: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_DELTA>), g, b, 0, s, p); break;         \
     case PK_TAPS_FWD: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_TAPS_FWD>), g, b, 0, s, p); break;   \
+    case PK_TAPS_BWD: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_TAPS_BWD>), g, b, 0, s, p); break;   \
     default: hipLaunchKernelGGL((gemm_bf16_ws_kernel<NWV, NK, PK_BIAS_SOFTPLUS>), g, b, 0, s, p); break;       \
   }
   if (nk == 5) { NA_WS(5) } else if (nk == 8) { NA_WS(8) } else { NA_WS(9) }
@@ -2833,21 +2891,17 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
     return NA_OK;
   }
   if (tune_value(TUNE_TIB_FUSED) != 0) {
-    // backward fused with the recomputation of dz (no workspace)
-    TapsBwdArgs t{(const bf16_t*)D, ld_d, M, K, (const bf16_t*)W, ldw, zc, ld_zc, ws, beta, dds,
-                  (bf16_t*)dpre, ld_dpre, (bf16_t*)centre, ld_centre, (bf16_t*)q, ld_q, db0, dws};
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    // eight waves (128 columns) per workgroup, one per CU; the N / 128 column
-    // slices of a tile are handled by workgroups launched together
-    const int64_t tiles = M / GBM, slices = N / 128;
-    const dim3 gf((unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, ncu / slices)),
-                  (unsigned)slices);
-    const dim3 bf(512);
-#define NA_TBF(NKK) hipLaunchKernelGGL((taps_bwd_fused_kernel<8, NKK>), gf, bf, 0, s, t)
-    if (nk == 5) NA_TBF(5); else if (nk == 8) NA_TBF(8); else NA_TBF(9);
-#undef NA_TBF
+    // backward in the weights-stationary product's epilogue: dz never leaves
+    // the registers (no workspace)
+    p.e.zc = zc; p.e.ld_zc = ld_zc; p.e.zc_rows = M / 4;
+    p.e.head_w = ws;
+    p.e.C = dpre; p.e.ldc = ld_dpre; p.e.c_dtype = NA_BF16;
+    p.e.tap_dds = dds;
+    p.e.tap_centre = centre; p.e.tap_ld_centre = ld_centre;
+    p.e.tap_q = q; p.e.tap_ld_q = ld_q;
+    p.e.colsum = db0; p.e.colsum2 = dws;
+    if (nwv == 8) launch_ws<8>((int)nk, PK_TAPS_BWD, wg, p, s);
+    else launch_ws<4>((int)nk, PK_TAPS_BWD, wg, p, s);
     NA_CHECK_LAUNCH();
     return NA_OK;
   }
@@ -2860,8 +2914,9 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
   const int ppb = kTibBlock / (int)(N / 4);
   const int64_t groups = ceil_div(P, (int64_t)ppb);
   int dev = 0, ncu = 256;
-  if (hipGetDevice(&dev) == hipSuccess)
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 256;
   const int64_t grid =
       std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)ncu * tune_value(TUNE_TIB_WG_PER_CU)));
   hipLaunchKernelGGL(taps_increment_bwd_kernel, dim3((unsigned)grid), dim3(kTibBlock), 0, s,
