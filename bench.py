@@ -697,8 +697,8 @@ def error_vs_fp64(dev, n_rays=128, n_points=65536):
     """Output error of both precisions against the FP64 oracle, for the step's model at
     its seeded initialisation.  Field: SDF, 4-tap gradient and Hessian trace at
     n_points random points in [-1,1]^3.  Render: n_rays rays of the workload's cameras
-    rendered in train mode at the samples the GPU drew (stratification off), the
-    oracle rendering the same samples.  sdf, colour: max abs; gradient: max |g - g_ref|
+    rendered in train mode at the samples the FP32 GPU path drew (stratification
+    off), both precisions and the oracle rendering those samples.  sdf, colour: max abs; gradient: max |g - g_ref|
     / mean |g_ref|; tr H: median and 99th-percentile abs error."""
     from oracle import render as orr
     c, v = camera_rays(1)
@@ -707,12 +707,16 @@ def error_vs_fp64(dev, n_rays=128, n_points=65536):
     out = {"workload": f"field: {n_points} random points in [-1,1]^3; render: {n_rays} rays "
                        "of one camera, 96 samples each; the 2^22 x 16 x 8 model at its seeded "
                        "initialisation, train mode"}
-    ref = field_ref = None
+    ref = field_ref = dists = None
     for dtype in (torch.float32, torch.bfloat16):
         m = build_model(dtype, dev)
         cd, vd = c.to(dev), v.to(dev)
         with torch.no_grad():
-            dists = m.render_rays(cd, vd)["dists"][..., 0]
+            # both precisions render the samples the FP32 path drew, where the
+            # oracle renders them too (a precision's own hierarchical samples
+            # would move with its SDF and compare different points)
+            if dists is None:
+                dists = m.render_rays(cd, vd)["dists"][..., 0]
             o = m.render_rays(cd, vd, dists=dists)
         if ref is None:
             fld, P = oracle_of(m, torch.float64)

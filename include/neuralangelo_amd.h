@@ -215,16 +215,19 @@ int na_gemm(const na_gemm_args* args, na_stream_t stream);
  *        tunable "taps_increment_bwd.fused" = 0, the default), or in the epilogue
  *        of that product, where dz never leaves the registers and no workspace is
  *        needed (= 1; work may be NULL); dds [M] is 16-byte aligned (four taps per
- *        point). */
+ *        point).  The forward can keep dz: dz_out [M, N] fp32 (16-byte aligned, or
+ *        NULL); a backward given that buffer as work with work_holds_dz = 1 skips
+ *        the recomputation (split form; the fused form ignores it). */
 int na_taps_increment_fwd(const void* D, int64_t ld_d, int64_t M, int64_t K, const void* W,
                           int64_t ldw, int64_t N, const float* zc, int64_t ld_zc,
-                          const float* ws, float beta, float* head_out, na_stream_t stream);
+                          const float* ws, float beta, float* head_out, float* dz_out,
+                          na_stream_t stream);
 int na_taps_increment_bwd(const void* D, int64_t ld_d, int64_t M, int64_t K, const void* W,
                           int64_t ldw, int64_t N, const float* zc, int64_t ld_zc,
                           const float* ws, float beta, const float* dds, void* dpre,
                           int64_t ld_dpre, void* centre, int64_t ld_centre, void* q,
                           int64_t ld_q, float* db0, float* dws, float* work,
-                          na_stream_t stream);
+                          int32_t work_holds_dz, na_stream_t stream);
 
 /* Maximum partial-product floats of na_gemm_wgrad_static (80 MiB per device):
  * every weight gradient of the train step fits (the largest, the 280-wide colour

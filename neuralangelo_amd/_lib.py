@@ -99,10 +99,10 @@ _SIGNATURES = {
     "na_gemm_wgrad_workspace": (ctypes.c_int64, [_PGEMM]),
     "na_gemm_wgrad_static": (ctypes.c_int, [_PGEMM, _P]),
     "na_taps_increment_fwd": (ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _P,
-                                             _F32, _P, _P]),
+                                             _F32, _P, _P, _P]),
     "na_taps_increment_bwd": (ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _P,
                                              _F32, _P, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P,
-                                             _P]),
+                                             _I32, _P]),
     "na_colsum": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P]),
     "na_head_sum": (ctypes.c_int, [_P, _I32, _I64, _P, _P, _P]),
     "na_linear_narrow_fwd": (ctypes.c_int, [_P, _I64, _I32, _I64, _I64, _P, _P, _I32, _I32, _P, _P]),

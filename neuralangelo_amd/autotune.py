@@ -192,7 +192,7 @@ def tune(dev=None, write=True, path=CACHE_PATH):
     def taps_bwd():
         _lib.call("na_taps_increment_bwd", _lib.ptr(D), ld_d, 4 * P32, ld_d, _lib.ptr(W), ld_d,
                   hid, _lib.ptr(Zc), hid, _lib.ptr(ws), 100.0, _lib.ptr(dds), _lib.ptr(dpre), hid, _lib.ptr(centre), hid, _lib.ptr(q), hid, _lib.ptr(db0),
-                  _lib.ptr(dws), _lib.ptr(work), _lib.stream_of(D))
+                  _lib.ptr(dws), _lib.ptr(work), 0, _lib.stream_of(D))
     rec = _sweep("taps_increment_bwd.fused", taps_bwd)
     rec["workload"] = f"na_taps_increment_bwd, {4 * P32} tap rows x 256 columns"
     out["taps_increment_bwd.fused"] = rec

@@ -2227,6 +2227,10 @@ __global__ void __launch_bounds__(64 * NWV, NWV == 4 ? 2 : 1) gemm_bf16_ws_kerne
             o[gq][c] = x;
           }
         } else if constexpr (KIND == PK_DELTA || KIND == PK_TAPS_FWD) {
+          if constexpr (KIND == PK_TAPS_FWD) {
+            // the increments themselves, kept for the backward when asked for
+            if (e.z_out) vstore8(e.z_out + m * e.ld_z + nw + 16 * gq + 8 * h, o[gq]);
+          }
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
             float hv, y1;
@@ -2848,7 +2852,7 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
                           const float* ws, float beta, float* head_out, const float* dds,
                           void* dpre, int64_t ld_dpre, void* centre, int64_t ld_centre,
                           void* q, int64_t ld_q, float* db0, float* dws, float* work,
-                          hipStream_t s) {
+                          int work_holds_dz, hipStream_t s) {
   NA_CHECK_ARG(M >= 0 && M % GBM == 0, "rows must be a multiple of 128 (pad the points to 32)");
   NA_CHECK_ARG(N == 256 || N == 128, "layer width must be 128 or 256");
   const int64_t nk = ceil_div(K, (int64_t)kNNBK);
@@ -2862,6 +2866,7 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
   NA_CHECK_ARG(M / GBM < 0x7fffffffLL, "too many rows");
   if (!backward) {
     NA_CHECK_ARG(head_out != nullptr, "head_out is null");
+    NA_CHECK_ARG(aligned16(work), "dz_out must be 16-byte aligned");
   } else {
     const bool fused = tune_value(TUNE_TIB_FUSED) != 0;
     NA_CHECK_ARG(fused || (work && aligned16(work)), "work is null (the split backward needs it)");
@@ -2885,6 +2890,7 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
     p.e.act = NA_ACT_SOFTPLUS;
     p.e.zc = zc; p.e.ld_zc = ld_zc; p.e.zc_rows = M / 4;
     p.e.head_w = ws; p.e.head_out = head_out;
+    p.e.z_out = work; p.e.ld_z = N;  // dz_out (optional)
     if (nwv == 8) launch_ws<8>((int)nk, PK_TAPS_FWD, wg, p, s);
     else launch_ws<4>((int)nk, PK_TAPS_FWD, wg, p, s);
     NA_CHECK_LAUNCH();
@@ -2905,11 +2911,14 @@ static int taps_increment(bool backward, const void* D, int64_t ld_d, int64_t M,
     NA_CHECK_LAUNCH();
     return NA_OK;
   }
-  // backward: recompute dz = D W^T into the fp32 workspace, then the elementwise pass
-  p.e.C = work; p.e.ldc = N; p.e.c_dtype = NA_F32;
-  if (nwv == 8) launch_ws<8>((int)nk, PK_PLAIN, wg, p, s);
-  else launch_ws<4>((int)nk, PK_PLAIN, wg, p, s);
-  NA_CHECK_LAUNCH();
+  // backward: recompute dz = D W^T into the fp32 workspace (unless the forward
+  // kept it there), then the elementwise pass
+  if (!work_holds_dz) {
+    p.e.C = work; p.e.ldc = N; p.e.c_dtype = NA_F32;
+    if (nwv == 8) launch_ws<8>((int)nk, PK_PLAIN, wg, p, s);
+    else launch_ws<4>((int)nk, PK_PLAIN, wg, p, s);
+    NA_CHECK_LAUNCH();
+  }
   const int64_t P = M / 4;
   const int ppb = kTibBlock / (int)(N / 4);
   const int64_t groups = ceil_div(P, (int64_t)ppb);
@@ -2973,9 +2982,9 @@ extern "C" int na_gemm_wgrad_static(const na_gemm_args* a, na_stream_t stream) {
 extern "C" int na_taps_increment_fwd(const void* D, int64_t ld_d, int64_t M, int64_t K,
                                      const void* W, int64_t ldw, int64_t N, const float* zc,
                                      int64_t ld_zc, const float* ws, float beta, float* head_out,
-                                     na_stream_t stream) {
+                                     float* dz_out, na_stream_t stream) {
   return taps_increment(false, D, ld_d, M, K, W, ldw, N, zc, ld_zc, ws, beta, head_out,
-                        nullptr, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr, dz_out, 0,
                         (hipStream_t)stream);
 }
 
@@ -2984,10 +2993,10 @@ extern "C" int na_taps_increment_bwd(const void* D, int64_t ld_d, int64_t M, int
                                      int64_t ld_zc, const float* ws, float beta, const float* dds,
                                      void* dpre, int64_t ld_dpre, void* centre, int64_t ld_centre,
                                      void* q, int64_t ld_q, float* db0, float* dws, float* work,
-                                     na_stream_t stream) {
+                                     int32_t work_holds_dz, na_stream_t stream) {
   return taps_increment(true, D, ld_d, M, K, W, ldw, N, zc, ld_zc, ws, beta, nullptr, dds,
                         dpre, ld_dpre, centre, ld_centre, q, ld_q, db0, dws, work,
-                        (hipStream_t)stream);
+                        work_holds_dz, (hipStream_t)stream);
 }
 
 extern "C" int64_t na_gemm_wgrad_workspace(const na_gemm_args* a) {

@@ -500,8 +500,8 @@ class SDFCentreTapsDeltaFunction(torch.autograd.Function):
     size, and its epilogue forms softplus(z_c + dz) - softplus(z_c) from the
     centre's fp32 pre-activation z_c.  The four increments' second-order sum (the
     Hessian trace) thus keeps float32-like precision.
-    Backward in the same increment form (na_taps_increment_bwd recomputes the fp32
-    dz_i): with dds_i = a + b_i (a the taps' mean, the Hessian-trace coefficient),
+    Backward in the same increment form (na_taps_increment_bwd reads the fp32 dz_i the
+    forward kept, or recomputes them): with dds_i = a + b_i (a the taps' mean, the Hessian-trace coefficient),
     the tap rows carry ws (b_i sp'(z_c + dz_i) + a dsp'_i) paired with D_i, a shared
     row a ws sp'(z_c) pairs with sum_i D_i, and the centre row receives its own
     gradient plus sum_i dds_i ws dsp'_i (dsp'_i = sp'(z_c + dz_i) - sp'(z_c)); the
@@ -544,11 +544,17 @@ class SDFCentreTapsDeltaFunction(torch.autograd.Function):
         gemm(H0c, hid, 0, w.W1, hid, 0, P, hid, hid, FB, RGB_LD, bias=w.b1, act=NA_ACT_SOFTPLUS)
         W0d = w.increment_weights(ld_d)
         parts_t = torch.empty(hid // 32, 4 * P32, device=dev)
+        # the split backward reads the increments dz = D W0d^T: kept from here when a
+        # backward will run, instead of recomputed there
+        keep = any(ctx.needs_input_grad) and _lib.call(
+            "na_tune_get", b"taps_increment_bwd.fused") == 0
+        dz = torch.empty(4 * P32, hid, dtype=torch.float32, device=dev) if keep else None
         _lib.call("na_taps_increment_fwd", _lib.ptr(D), ld_d, 4 * P32, ld_d, _lib.ptr(W0d), ld_d,
                   hid, _lib.ptr(Zc), hid, _lib.ptr(w.ws), float(SOFTPLUS_BETA), _lib.ptr(parts_t),
-                  _s(pts))
+                  _lib.ptr(dz), _s(pts))
         ds = head_sum(parts_t)[:4 * P].view(P, 4).t().contiguous()
         ctx.save_for_backward(pts, X0c, H0c, FB, D, Dsum, Zc, W0d)
+        ctx.dz = dz
         ctx.w, ctx.enc, ctx.eps, ctx.ld_d = w, enc, float(eps), ld_d
         return sdf, FB[:, :hid], ds
 
@@ -570,15 +576,19 @@ class SDFCentreTapsDeltaFunction(torch.autograd.Function):
         dds_pm = torch.zeros(P32, 4, device=dev)
         if dds is not None:
             dds_pm[:P] = dds.t()
-        # the tap increments dz = D W0d^T are recomputed inside the fused kernel, or
-        # (split launch shape) into a transient fp32 buffer
+        # the tap increments dz = D W0d^T: kept by the forward, or recomputed in the
+        # fused form's epilogue, or (split form without a kept copy) recomputed into a
+        # transient fp32 buffer
         fused = _lib.call("na_tune_get", b"taps_increment_bwd.fused") != 0
-        dz = None if fused else torch.empty(4 * P32, hid, dtype=torch.float32, device=dev)
+        dz, held = ctx.dz, 1
+        ctx.dz = None
+        if dz is None and not fused:
+            dz, held = torch.empty(4 * P32, hid, dtype=torch.float32, device=dev), 0
         _lib.call("na_taps_increment_bwd", _lib.ptr(D), ld_d, 4 * P32, ld_d, _lib.ptr(W0d), ld_d,
                   hid, _lib.ptr(Zc), hid, _lib.ptr(w.ws), float(SOFTPLUS_BETA), _lib.ptr(dds_pm),
                   _lib.ptr(dpre_t), hid, _lib.ptr(dpre_c), hid, _lib.ptr(q), hid,
                   _lib.ptr(acc.db0), _lib.ptr(acc.dws), _lib.ptr(dz) if dz is not None else None,
-                  _s(pts))
+                  held if dz is not None else 0, _s(pts))
         del dz
         # first-layer weights: centre rows x X0c, tap rows x D (hi part and point
         # offset), shared rows x sum_t D_t

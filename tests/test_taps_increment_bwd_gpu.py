@@ -1,6 +1,6 @@
 """Backward of the tap-increment layer (na_taps_increment_bwd) in each launch form
-(split: the recomputed dz in a work buffer and an elementwise pass; fused: the
-weights-stationary product's epilogue, no workspace) against a float64
+(split: the recomputed dz in a work buffer and an elementwise pass; split on the dz
+the forward kept; fused: the weights-stationary product's epilogue, no workspace) against a float64
 restatement of its formulas on the same bf16 operands:
   dz = D W^T, z = the point's centre pre-activation, dds_t = a + b_t (a the mean of
   the point's four rows), sp = softplus_beta,
@@ -57,13 +57,13 @@ def _reference(D, W, zc, ws, dds, centre):
             (d * hv).sum((0, 1)))
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, "kept"])
 @pytest.mark.parametrize("P,hid,K", [(2000, 256, 288), (1000, 128, 160)])
 def test_taps_increment_bwd_forms(cuda, mode, P, hid, K):
     print("taps increment bwd", mode, P, hid, K, flush=True)
-    D, W, zc, ws, dds, centre, P32 = _inputs(P, hid, K, 5 + mode)
+    D, W, zc, ws, dds, centre, P32 = _inputs(P, hid, K, 5 + (mode if mode != "kept" else 2))
     ref = _reference(D, W, zc, ws, dds, centre)
-    old = _lib.call("na_tune_set", b"taps_increment_bwd.fused", mode)
+    old = _lib.call("na_tune_set", b"taps_increment_bwd.fused", 1 if mode == 1 else 0)
     try:
         Dd, Wd, zd, wsd, ddsd = D.to(cuda), W.to(cuda), zc.to(cuda), ws.to(cuda), dds.to(cuda)
         cen = centre.to(cuda)
@@ -71,11 +71,23 @@ def test_taps_increment_bwd_forms(cuda, mode, P, hid, K):
         q = torch.full((P32, hid), 3.0, dtype=torch.bfloat16, device=cuda)
         db0 = torch.zeros(hid, device=cuda)
         dws = torch.zeros(hid, device=cuda)
-        work = torch.empty(4 * P32, hid, device=cuda) if mode == 0 else None
+        work = torch.empty(4 * P32, hid, device=cuda) if mode != 1 else None
+        held = 0
+        if mode == "kept":
+            # the forward keeps dz = D W^T in work; the backward then reads it
+            parts = torch.empty(hid // 32, 4 * P32, device=cuda)
+            _lib.call("na_taps_increment_fwd", _lib.ptr(Dd), K, 4 * P32, K, _lib.ptr(Wd), K, hid,
+                      _lib.ptr(zd), hid, _lib.ptr(wsd), BETA, _lib.ptr(parts), _lib.ptr(work),
+                      _lib.stream_of(Dd))
+            torch.cuda.synchronize()
+            dz_ref = D.double() @ W.double().t()
+            scale = (D.double().abs() @ W.double().abs().t()).max().item()
+            assert (work.cpu().double() - dz_ref).abs().max().item() <= 1e-5 * scale
+            held = 1
         _lib.call("na_taps_increment_bwd", _lib.ptr(Dd), K, 4 * P32, K, _lib.ptr(Wd), K, hid,
                   _lib.ptr(zd), hid, _lib.ptr(wsd), BETA, _lib.ptr(ddsd), _lib.ptr(dpre), hid,
                   _lib.ptr(cen), hid, _lib.ptr(q), hid, _lib.ptr(db0), _lib.ptr(dws),
-                  _lib.ptr(work), _lib.stream_of(Dd))
+                  _lib.ptr(work), held, _lib.stream_of(Dd))
         torch.cuda.synchronize()
     finally:
         _lib.call("na_tune_set", b"taps_increment_bwd.fused", old)
