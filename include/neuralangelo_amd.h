@@ -67,6 +67,8 @@ int na_version(void);
  *   "taps_increment_bwd.fused"       tap-increment backward in the epilogue of
  *                                    its recomputed product (1, no workspace) or
  *                                    split in two passes (0); 0
+ *   "packed.wg_per_cu"               workgroups per CU of the persistent packed
+ *                                    launches (1..16; 4)
  * na_tune_set returns the previous value (or an error code for an unknown name or
  * a value out of range); the launchers read the current value at every launch.
  * Set them between launches: a change is not ordered with launches in flight on
@@ -500,6 +502,33 @@ int na_neus_alpha_packed_bwd(const float* ray_unit, const float* sdfs, const flo
                              float anneal_end, const int32_t* offsets, const int32_t* ray_idx,
                              int64_t S, const float* dalpha, float* dsdf, float* dgrad,
                              float* ds_var, na_stream_t stream);
+
+/* The packed NeuS alpha and compositing of all rays in one persistent launch each
+ * way (the separate packed entries above, fused; same arithmetic and order per
+ * sample).  A grid of a few workgroups per CU whose waves take chunks of 64
+ * consecutive rays from *queue (an int32 the caller zeroes before the call; the
+ * launch leaves it non-zero) until all rays are done, so uneven sample counts
+ * balance across the chip.  Tile shape: the launch tunables "packed.rays_per_wg"
+ * (workgroup size) and "packed.wg_per_cu".
+ *   fwd: alpha [S] (model.py:275-298), weights [S] and out [R, K] = sum of q [S, K]
+ *        times weights (render.py:87-112), K <= 4;
+ *   bwd: from dout [R, K] (or NULL), dweights [S] (or NULL) and a direct dalpha [S]
+ *        (or NULL): dsdf [S], dgrad [S, 3], dq [S, K] (or NULL), ds_var += (one
+ *        atomic per wave, so the last bits follow the schedule). */
+int na_neus_composite_packed_fwd(const float* ray_unit, const float* sdfs, const float* grads,
+                                 const float* dists, const float* far, const uint8_t* outside,
+                                 float outside_val, const float* s_var, float progress,
+                                 float anneal_end, const int32_t* offsets, int64_t R,
+                                 const float* q, int32_t K, float* alpha, float* weights,
+                                 float* out, int32_t* queue, na_stream_t stream);
+int na_neus_composite_packed_bwd(const float* ray_unit, const float* sdfs, const float* grads,
+                                 const float* dists, const float* far, const uint8_t* outside,
+                                 float outside_val, const float* s_var, float progress,
+                                 float anneal_end, const int32_t* offsets, int64_t R,
+                                 const float* q, int32_t K, const float* alpha,
+                                 const float* weights, const float* dout, const float* dweights,
+                                 const float* dalpha, float* dsdf, float* dgrad, float* dq,
+                                 float* ds_var, int32_t* queue, na_stream_t stream);
 
 /* Background NeRF (projects/neuralangelo/utils/modules.py:232-311).
  * na_bg_encode: points [n,3] -> rows [n, ld] of [q | Fourier(q)] with

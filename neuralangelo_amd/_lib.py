@@ -139,6 +139,11 @@ _SIGNATURES = {
     "na_weighted_sum_packed_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _I32, _P, _P, _P, _P]),
     "na_neus_alpha_packed_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F32, _P, _F32, _F32, _P, _P, _I64, _P, _P]),
     "na_neus_alpha_packed_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F32, _P, _F32, _F32, _P, _P, _I64, _P, _P, _P, _P, _P]),
+    "na_neus_composite_packed_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F32, _P, _F32, _F32,
+                                                    _P, _I64, _P, _I32, _P, _P, _P, _P, _P]),
+    "na_neus_composite_packed_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F32, _P, _F32, _F32,
+                                                    _P, _I64, _P, _I32, _P, _P, _P, _P, _P, _P,
+                                                    _P, _P, _P, _P, _P]),
     "na_bg_encode": (ctypes.c_int, [_P, _I64, _I32, _P, _I32, _I64, _P]),
     "na_bg_alpha_fwd": (ctypes.c_int, [_P, _I64, _P, _I64, _I32, _P, _P, _P]),
     "na_bg_alpha_bwd": (ctypes.c_int, [_P, _I64, _P, _I64, _I32, _P, _P, _P, _I64, _P]),

@@ -2,8 +2,8 @@
 
 The library's launch tunables (na_tune_set / na_tune_get: the tap scatter's run
 of samples per thread, the tap-increment backward's launch form and workgroups
-per CU, the packed compositing kernels' rays per workgroup) have a best value
-that depends on the device and on the shape of the work.  `tune()` times every candidate of each on
+per CU, the persistent packed launches' rays per workgroup and workgroups per CU)
+have a best value that depends on the device and on the shape of the work.  `tune()` times every candidate of each on
 the workload it serves -- the train step's samples (8192 rays x 96 samples,
 ray-ordered, as the renderer hands them over) and the ragged packed batch
 (65,536 rays, 0..192 samples) -- with HIP events on the launching stream, and
@@ -28,6 +28,7 @@ CANDIDATES = {
     "taps_bwd.run": [1, 2, 4, 6, 8, 12, 16],
     "taps_increment_bwd.wg_per_cu": [2, 4, 8, 16],
     "packed.rays_per_wg": [64, 128, 256],
+    "packed.wg_per_cu": [1, 2, 4, 8, 16],
     "taps_increment_bwd.fused": [0, 1],
 }
 
@@ -204,28 +205,43 @@ def tune(dev=None, write=True, path=CACHE_PATH):
     out["taps_increment_bwd.wg_per_cu"] = rec
     del D, W, Zc, dds, dpre, centre, q, work
 
-    # packed compositing on the ragged batch
+    # the persistent packed NeuS + compositing launches on the ragged batch: samples
+    # along each ray 0.01 apart, the SDF crossing zero along the ray
     off = _ragged(65536, dev)
     R, S = off.numel() - 1, int(off[-1].item())
-    alpha = torch.rand(S, device=dev, generator=g) * 0.2
+    ray_idx = torch.empty(S, dtype=torch.int32, device=dev)
+    _lib.call("na_packed_ray_index", _lib.ptr(off), R, _lib.ptr(ray_idx), _lib.stream_of(off))
+    pos = (torch.arange(S, device=dev) - off.long()[ray_idx.long()]).float()
+    dists = 1.0 + 0.01 * pos
+    sdfs = 0.3 - 0.007 * pos
+    grads = torch.nn.functional.normalize(torch.randn(S, 3, device=dev, generator=g), dim=-1)
+    rays = torch.nn.functional.normalize(torch.randn(R, 3, device=dev, generator=g), dim=-1)
+    s_var = torch.full((1,), 3.0, device=dev)
     rgbs = torch.rand(S, 3, device=dev, generator=g)
-    wts = torch.empty(S, device=dev)
+    alpha, wts = torch.empty(S, device=dev), torch.empty(S, device=dev)
     rgb = torch.empty(R, 3, device=dev)
     dout = torch.randn(R, 3, device=dev, generator=g)
-    dw = torch.randn(S, device=dev, generator=g)
-    dalpha = torch.empty(S, device=dev)
-    drgbs = torch.empty(S, 3, device=dev)
+    dsdf, dgrad = torch.empty(S, device=dev), torch.empty(S, 3, device=dev)
+    drgbs, ds_var = torch.empty(S, 3, device=dev), torch.zeros(1, device=dev)
+    queues = torch.zeros(2, dtype=torch.int32, device=dev)
 
     def packed():
         s = _lib.stream_of(alpha)
-        _lib.call("na_composite_packed_fwd", _lib.ptr(alpha), _lib.ptr(rgbs), _lib.ptr(off), R, 3,
-                  _lib.ptr(wts), _lib.ptr(rgb), s)
-        _lib.call("na_composite_packed_bwd", _lib.ptr(alpha), _lib.ptr(rgbs), _lib.ptr(wts),
-                  _lib.ptr(off), R, 3, _lib.ptr(dout), _lib.ptr(dw), _lib.ptr(dalpha),
-                  _lib.ptr(drgbs), s)
-    rec = _sweep("packed.rays_per_wg", packed)
-    rec["workload"] = f"na_composite_packed_fwd + _bwd, {R} rays, {S} samples (0..192 per ray)"
-    out["packed.rays_per_wg"] = rec
+        queues.zero_()
+        _lib.call("na_neus_composite_packed_fwd", _lib.ptr(rays), _lib.ptr(sdfs), _lib.ptr(grads),
+                  _lib.ptr(dists), None, None, 1000.0, _lib.ptr(s_var), 1.0, 0.1, _lib.ptr(off), R,
+                  _lib.ptr(rgbs), 3, _lib.ptr(alpha), _lib.ptr(wts), _lib.ptr(rgb),
+                  _lib.ptr(queues[:1]), s)
+        _lib.call("na_neus_composite_packed_bwd", _lib.ptr(rays), _lib.ptr(sdfs), _lib.ptr(grads),
+                  _lib.ptr(dists), None, None, 1000.0, _lib.ptr(s_var), 1.0, 0.1, _lib.ptr(off), R,
+                  _lib.ptr(rgbs), 3, _lib.ptr(alpha), _lib.ptr(wts), _lib.ptr(dout), None, None,
+                  _lib.ptr(dsdf), _lib.ptr(dgrad), _lib.ptr(drgbs), _lib.ptr(ds_var),
+                  _lib.ptr(queues[1:]), s)
+    for name in ("packed.rays_per_wg", "packed.wg_per_cu"):
+        rec = _sweep(name, packed)
+        rec["workload"] = (f"na_neus_composite_packed_fwd + _bwd (persistent), {R} rays, {S} "
+                           "samples (0..192 per ray)")
+        out[name] = rec
     del m
     torch.cuda.empty_cache()
     if write:

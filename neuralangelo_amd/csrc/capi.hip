@@ -22,6 +22,7 @@ static Tunable g_tune[TUNE_COUNT] = {
     {"taps_increment_bwd.wg_per_cu", 1, 32, 1, {8}},
     {"packed.rays_per_wg", 64, 256, 64, {64}},
     {"taps_increment_bwd.fused", 0, 1, 1, {0}},
+    {"packed.wg_per_cu", 1, 16, 1, {4}},
 };
 
 int64_t tune_value(TuneKey key) { return g_tune[key].value.load(std::memory_order_relaxed); }

@@ -26,7 +26,8 @@ enum TuneKey : int {
   TUNE_TIB_WG_PER_CU = 1,  // workgroups per CU of the tap-increment backward
   TUNE_PACKED_RAYS = 2,    // rays (threads) per workgroup of the packed compositing
   TUNE_TIB_FUSED = 3,      // tap-increment backward: split (0) or in the product's epilogue (1)
-  TUNE_COUNT = 4
+  TUNE_PACKED_WG_PER_CU = 4,  // workgroups per CU of the persistent packed launches
+  TUNE_COUNT = 5
 };
 int64_t tune_value(TuneKey key);
 

@@ -35,6 +35,7 @@ constexpr int kMcBlock = 256;
 constexpr int kMcItems = 4;
 constexpr int kMcWaves = kMcBlock / 64;
 constexpr int kMcTile = kMcBlock * kMcItems;  // cells per tile
+static_assert(kMcItems == 4, "the classify's wave vote names four items");
 constexpr int kScanGroup = kMcBlock * kMcItems;  // tiles per scan group
 
 // Bourke corner offsets and edge endpoints; kOrder = per-cell vertex order.
@@ -74,6 +75,8 @@ struct McLattice {
   const float* v;
   int64_t nx, ny, nz;   // lattice points per axis
   double iso;
+  float isof;           // iso as a float, when that is exact (iso_is_f): then
+  int iso_is_f;         // v <= iso compares in float with the same result
   int64_t br[3];        // cells per full block per axis
   int64_t nb[3];        // blocks per axis
   int64_t tpb;          // tiles per block
@@ -233,15 +236,62 @@ __device__ inline int64_t block_excl_scan64(int64_t x, int64_t* sh, int64_t* tot
 // ---------------------------------------------------------------------------
 // 1. classify, marching along x.  Workgroup (block b, tile column q) visits the
 //    block's x-planes in order and classifies tile (i, q) of each.  The lattice
-//    points under the tile column in two consecutive planes (rows j0 .. j0 +
-//    rows, columns k0 .. k0 + cols) sit in an LDS ring of three plane buffers;
-//    the loads of plane i + 2 are issued before the cells of plane i are
-//    classified and land in the free buffer afterwards, so they are in flight
-//    meanwhile.  Each lattice point is read from memory once per tile column
-//    (the row after a tile's last row once more).  Tiles without a crossed edge
-//    write only their counts (the rest of the count arrays is zero-filled first).
+//    points under the tile column in one plane (rows j0 .. j0 + rows, columns
+//    k0 .. k0 + cols) are staged in LDS by buffer-to-LDS loads (no registers),
+//    into a ring of lead + 2 plane slots: planes i + 2 .. i + 1 + lead are in
+//    flight while the cells between planes i and i + 1 are classified, so each
+//    workgroup keeps several planes of requests outstanding.  Wave w's u-th
+//    load of a plane fills staged elements (w U + u) 64 .. + 63 (U loads per
+//    wave per plane; element e is row e / pw, column e % pw).  Each lattice
+//    point is read from memory once per tile column (the row after a tile's
+//    last row once more).  Tiles without a crossed edge write only their counts
+//    (the rest of the count arrays is zero-filled first).
 // ---------------------------------------------------------------------------
-constexpr int kMcPref = 9;  // prefetch registers per thread: planes up to 2304 points
+constexpr int kMcPref = 9;  // loads per wave per plane at most: planes up to 2304 points
+constexpr uint32_t kMcOob = 0x80000000u;  // a buffer offset past every plane (reads zero)
+typedef __attribute__((address_space(3))) void mc_lds_t;
+
+// vmcnt(n) for n = 0 .. 15
+__device__ inline void mc_wait(int n) {
+  switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    case 1: __builtin_amdgcn_s_waitcnt(0xF71); break;
+    case 2: __builtin_amdgcn_s_waitcnt(0xF72); break;
+    case 3: __builtin_amdgcn_s_waitcnt(0xF73); break;
+    case 4: __builtin_amdgcn_s_waitcnt(0xF74); break;
+    case 5: __builtin_amdgcn_s_waitcnt(0xF75); break;
+    case 6: __builtin_amdgcn_s_waitcnt(0xF76); break;
+    case 7: __builtin_amdgcn_s_waitcnt(0xF77); break;
+    case 8: __builtin_amdgcn_s_waitcnt(0xF78); break;
+    case 9: __builtin_amdgcn_s_waitcnt(0xF79); break;
+    case 10: __builtin_amdgcn_s_waitcnt(0xF7A); break;
+    case 11: __builtin_amdgcn_s_waitcnt(0xF7B); break;
+    case 12: __builtin_amdgcn_s_waitcnt(0xF7C); break;
+    case 13: __builtin_amdgcn_s_waitcnt(0xF7D); break;
+    case 14: __builtin_amdgcn_s_waitcnt(0xF7E); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF7F); break;
+  }
+}
+
+// Workgroup barrier for the plane pipeline: this wave's plane loads but the
+// newest `pending` are done (pending < 0: no wait on them), its LDS accesses are
+// done, then s_barrier.  (A __syncthreads would drain every plane load in
+// flight: its release fence counts the buffer-to-LDS loads as outstanding
+// stores.)
+__device__ inline void mc_sync(int pending) {
+  if (pending >= 0) mc_wait(pending);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Loads per wave per staged plane (U) and planes in flight beyond the two being
+// classified (lead), so that lead * U <= 15 loads are outstanding at the wait.
+__host__ __device__ inline void mc_pipeline(int64_t np_max, int& U, int& lead) {
+  U = (int)((np_max + 4 * 64 - 1) / (4 * 64));
+  lead = U > 0 ? (15 / U < 3 ? 15 / U : 3) : 1;
+  if (lead < 1) lead = 1;
+}
 
 __global__ void __launch_bounds__(256)
 mc_zero_counts_kernel(int64_t ntiles, int* __restrict__ tile_v, int* __restrict__ tile_f) {
@@ -256,9 +306,12 @@ __global__ void __launch_bounds__(kMcBlock)
 mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __restrict__ cidx,
                    uint16_t* __restrict__ vpre, int* __restrict__ tile_v,
                    int* __restrict__ tile_f) {
-  extern __shared__ float ring[];  // 3 planes of np_max points
+  extern __shared__ float ring[];  // lead + 2 slots of 256 U floats (a plane each)
   __shared__ int wv[kMcItems][kMcWaves];
   __shared__ int wf[kMcWaves];
+  // the case words in LDS: a vector-memory lookup in the loop would wait for
+  // every plane load issued before it (the load counter drains in order)
+  __shared__ uint16_t case_w[256];
   const int64_t b = (int64_t)blockIdx.x / tpp_max;
   const int64_t q = (int64_t)blockIdx.x - b * tpp_max;
   const McBlock k = block_of(g, (uint32_t)b);
@@ -269,15 +322,33 @@ mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __rest
   const int64_t ncells = tr.rows * tr.cols;  // cells per tile
   const int64_t sy = g.nz, sx = g.ny * g.nz;
   const float* pl0 = g.v + (k.ox * g.ny + (k.oy + tr.j0)) * g.nz + (k.oz + tr.k0);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // this thread's share of a staged plane: element e = thread + 256 u at row e / pw
-  int64_t goff[kMcPref];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int U, lead;
+  mc_pipeline(np_max, U, lead);
+  const int slots = lead + 2;
+  // a slot holds every element the wave loads write, np .. 256 U - 1 included
+  // (those lanes read zeros through the out-of-range offset but still write)
+  const int64_t stride = 256 * (int64_t)U;
+  // this lane's byte offsets within a plane for the wave's U loads (the host
+  // checks (rows + 1) * nz * 4 < 2^31)
+  uint32_t voff[kMcPref];
 #pragma unroll
   for (int u = 0; u < kMcPref; ++u) {
-    const int64_t e = threadIdx.x + (int64_t)u * kMcBlock;
+    const int64_t e = ((int64_t)w * U + u) * 64 + lane;
     const int64_t r = e / pw;
-    goff[u] = e < np ? r * sy + (e - r * pw) : -1;
+    voff[u] = (u < U && e < np) ? (uint32_t)((r * sy + (e - r * pw)) * 4) : kMcOob;
   }
+  const int plane_bytes = (int)((tr.rows + 1) * sy * 4);
+  auto issue_plane = [&](int64_t p) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(pl0 + p * sx), 0, plane_bytes, 0x00020000);
+    float* dst = ring + (p % slots) * stride + (int64_t)w * U * 64;
+#pragma unroll
+    for (int u = 0; u < kMcPref; ++u)
+      if (u < U) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (mc_lds_t*)(dst + u * 64), 4, voff[u],
+                                                         0, 0, 0);
+  };
   // this thread's cells c = it * 256 + thread: staged offset of corner (0,0,0)
   // and the block-boundary flags of their owned edges
   int64_t coff[kMcItems];
@@ -292,31 +363,21 @@ mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __rest
     coff[it] = jj * pw + kk;
     bmask[it] = ((tr.j0 + jj) == 0 ? 1 : 0) | ((tr.k0 + kk) == 0 ? 2 : 0);
   }
-  float r[kMcPref];
-  auto load_plane = [&](int64_t i) {
-    const float* src = pl0 + i * sx;
-#pragma unroll
-    for (int u = 0; u < kMcPref; ++u)
-      if (goff[u] >= 0) r[u] = src[goff[u]];
-  };
-  auto store_plane = [&](int64_t slot) {
-    float* dst = ring + slot * np_max;
-#pragma unroll
-    for (int u = 0; u < kMcPref; ++u)
-      if (goff[u] >= 0) dst[threadIdx.x + u * kMcBlock] = r[u];
-  };
-  load_plane(0);
-  store_plane(0);
-  load_plane(1);
-  store_plane(1);
-  __syncthreads();
+  case_w[threadIdx.x] = dMcCase.w[threadIdx.x];  // (published by the loop's first barrier)
+  // planes 0 .. lead, then one more at the top of each iteration (points 0 .. cx)
+  for (int64_t p = 0; p <= lead && p <= k.cx; ++p) issue_plane(p);
   const double iso = g.iso;
+  const float isof = g.isof;
+  const bool iso_f = g.iso_is_f != 0;
   const int64_t tile_base = b * g.tpb + q;
   for (int64_t i = 0; i < k.cx; ++i) {
-    const bool pre = i + 2 <= k.cx;  // plane i + 2 exists (points 0 .. cx)
-    if (pre) load_plane(i + 2);
-    const float* s0 = ring + (i % 3) * np_max;
-    const float* s1 = ring + ((i + 1) % 3) * np_max;
+    if (i + 1 + lead <= k.cx) issue_plane(i + 1 + lead);
+    // planes i and i + 1 have landed once at most the loads of the planes
+    // issued after i + 1 (i + 2 .. i + 1 + lead, those that exist) are pending
+    const int64_t after = (i + 1 + lead <= k.cx ? i + 1 + lead : k.cx) - (i + 1);
+    mc_sync((int)(after > 0 ? after : 0) * U);
+    const float* s0 = ring + (i % slots) * stride;
+    const float* s1 = ring + ((i + 1) % slots) * stride;
     int cube[kMcItems], nv[kMcItems], inc[kMcItems];
     int nf = 0;
 #pragma unroll
@@ -327,22 +388,41 @@ mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __rest
         const float* p1 = s1 + coff[it];
         const float a00 = p0[0], a01 = p0[pw], a10 = p1[0], a11 = p1[pw];
         const float b00 = p0[1], b01 = p0[pw + 1], b10 = p1[1], b11 = p1[pw + 1];
-        c = ((double)a00 <= iso) | (((double)a10 <= iso) << 1) | (((double)a11 <= iso) << 2) |
-            (((double)a01 <= iso) << 3) | (((double)b00 <= iso) << 4) |
-            (((double)b10 <= iso) << 5) | (((double)b11 <= iso) << 6) |
-            (((double)b01 <= iso) << 7);
-        const int word = dMcCase.w[c];
+        if (iso_f) {
+          c = (a00 <= isof) | ((a10 <= isof) << 1) | ((a11 <= isof) << 2) | ((a01 <= isof) << 3) |
+              ((b00 <= isof) << 4) | ((b10 <= isof) << 5) | ((b11 <= isof) << 6) |
+              ((b01 <= isof) << 7);
+        } else {
+          c = ((double)a00 <= iso) | (((double)a10 <= iso) << 1) |
+              (((double)a11 <= iso) << 2) | (((double)a01 <= iso) << 3) |
+              (((double)b00 <= iso) << 4) | (((double)b10 <= iso) << 5) |
+              (((double)b11 <= iso) << 6) | (((double)b01 <= iso) << 7);
+        }
+        const int word = case_w[c];
         n = __popc(word & 0xfff & owned_mask(i == 0, bmask[it] & 1, bmask[it] & 2));
         nf += word >> 12;
       }
       cube[it] = c;
       nv[it] = n;
-      inc[it] = wave_incl_scan(n);
-      if (lane == 63) wv[it][w] = inc[it];
     }
-    nf = wave_sum(nf);
+    // the scans only for waves with a crossed edge (most of a lattice has none)
+    if (__any((nv[0] | nv[1] | nv[2] | nv[3] | nf) != 0)) {
+#pragma unroll
+      for (int it = 0; it < kMcItems; ++it) {
+        inc[it] = wave_incl_scan(nv[it]);
+        if (lane == 63) wv[it][w] = inc[it];
+      }
+      nf = wave_sum(nf);
+    } else {
+#pragma unroll
+      for (int it = 0; it < kMcItems; ++it) {
+        inc[it] = 0;
+        if (lane == 63) wv[it][w] = 0;
+      }
+      nf = 0;
+    }
     if (lane == 0) wf[w] = nf;
-    __syncthreads();
+    mc_sync(-1);  // the wave totals (LDS); plane loads stay in flight
     int pre_v[kMcItems];
     const int tv = tile_prefix(inc, nv, wv, pre_v);
     int tf = 0;
@@ -361,9 +441,11 @@ mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __rest
         tile_f[gt] = tf;
       }
     }
-    if (pre) store_plane((i + 2) % 3);
-    __syncthreads();
+    // every wave has read planes i and i + 1 (and the counts) before a slot or
+    // the counts are written again
+    mc_sync(-1);
   }
+  mc_wait(0);  // no buffer-to-LDS load outlives the workgroup
 }
 
 // ---------------------------------------------------------------------------
@@ -667,6 +749,8 @@ static McPlan mc_plan(const float* v, int64_t nx, int64_t ny, int64_t nz, int64_
   p.g.ny = ny;
   p.g.nz = nz;
   p.g.iso = iso;
+  p.g.isof = (float)iso;
+  p.g.iso_is_f = (double)p.g.isof == iso;
   for (int a = 0; a < 3; ++a) {
     p.g.br[a] = block_res > 0 ? std::min<int64_t>(block_res, n[a] - 1) : n[a] - 1;
     p.g.nb[a] = ceil_div(n[a] - 1, p.g.br[a]);
@@ -757,11 +841,14 @@ extern "C" int na_mc_count(const float* volume, int64_t nx, int64_t ny, int64_t 
   McWorkspace w;
   mc_ws_layout(p, workspace, &w);
   NA_CHECK_ARG(p.np_max <= (int64_t)kMcPref * kMcBlock, "staged plane too large");
+  NA_CHECK_ARG(1025 * nz * 4 < (int64_t)INT32_MAX, "lattice rows too long (nz)");
   hipLaunchKernelGGL(mc_zero_counts_kernel, dim3((unsigned)ceil_div(p.ntiles, (int64_t)256)),
                      dim3(256), 0, s, p.ntiles, w.tile_v, w.tile_f);
   NA_CHECK_LAUNCH();
   const int64_t nblocks = p.g.nb[0] * p.g.nb[1] * p.g.nb[2];
-  const size_t lds = (size_t)(3 * p.np_max) * sizeof(float);
+  int U, lead;
+  mc_pipeline(p.np_max, U, lead);
+  const size_t lds = (size_t)(lead + 2) * 256 * U * sizeof(float);  // slots of 256 U floats
   hipLaunchKernelGGL(mc_classify_kernel, dim3((unsigned)(nblocks * p.tpp)), dim3(kMcBlock), lds, s,
                      p.g, p.tpp, p.np_max, w.cidx, w.vpre, w.tile_v, w.tile_f);
   NA_CHECK_LAUNCH();

@@ -314,17 +314,9 @@ struct NeusTerms {
   int64_t r;
 };
 
-__device__ inline NeusTerms neus_terms(const NeusIn& p, int64_t t) {
+// Terms of sample t of ray r (has_next: t is not the ray's last sample).
+__device__ inline NeusTerms neus_terms_of(const NeusIn& p, int64_t t, int64_t r, bool has_next) {
   NeusTerms q;
-  int64_t r;
-  bool has_next;
-  if (p.off) {
-    r = p.ray_idx[t];
-    has_next = t + 1 < (int64_t)p.off[r + 1];
-  } else {
-    r = t / p.N;
-    has_next = (int)(t % p.N) + 1 < p.N;
-  }
   q.r = r;
   q.outside = p.outside && p.outside[r];
   q.sdf = q.outside ? p.outside_val : p.sdfs[t];
@@ -348,6 +340,46 @@ __device__ inline NeusTerms neus_terms(const NeusIn& p, int64_t t) {
   return q;
 }
 
+__device__ inline NeusTerms neus_terms(const NeusIn& p, int64_t t) {
+  int64_t r;
+  bool has_next;
+  if (p.off) {
+    r = p.ray_idx[t];
+    has_next = t + 1 < (int64_t)p.off[r + 1];
+  } else {
+    r = t / p.N;
+    has_next = (int)(t % p.N) + 1 < p.N;
+  }
+  return neus_terms_of(p, t, r, has_next);
+}
+
+// Backward of one sample's alpha: dsdf, dgrad (3) and the sample's share of
+// d(s_var) from dalpha (the clamp passes the gradient where 0 <= ratio <= 1).
+__device__ inline float neus_sample_bwd(const NeusIn& p, const NeusTerms& q, float da, int64_t t,
+                                        float* __restrict__ dsdf, float* __restrict__ dgrad) {
+  float dr = (q.ratio >= 0.0f && q.ratio <= 1.0f) ? da : 0.0f;
+  float num = q.pc - q.nc, den = q.pc + 1e-5f;
+  float dnum = dr / den;
+  float dden = -dr * num / (den * den);
+  float dpc = dnum + dden;
+  float dnc = -dnum;
+  float dzp = dpc * q.pc * (1.0f - q.pc);  // d(est_prev * inv_s)
+  float dzn = dnc * q.nc * (1.0f - q.nc);
+  float dep = dzp * q.inv_s, den_ = dzn * q.inv_s;
+  float dinv = dzp * q.est_prev + dzn * q.est_next;
+  dsdf[t] = q.outside ? 0.0f : (dep + den_);
+  float diter = (-dep + den_) * q.intv * 0.5f;
+  float a = p.anneal_ratio;
+  float u1 = -q.cosv * 0.5f + 0.5f;
+  float u2 = -q.cosv;
+  float dcos = diter * (0.5f * (1.0f - a) * (u1 > 0.0f ? 1.0f : 0.0f) + a * (u2 > 0.0f ? 1.0f : 0.0f));
+  const int64_t r = q.r;
+  dgrad[t * 3] = dcos * p.ray[r * 3];
+  dgrad[t * 3 + 1] = dcos * p.ray[r * 3 + 1];
+  dgrad[t * 3 + 2] = dcos * p.ray[r * 3 + 2];
+  return dinv * q.inv_s;
+}
+
 __global__ void __launch_bounds__(256)
 neus_alpha_fwd_kernel(NeusIn p, float* __restrict__ alpha) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -367,29 +399,7 @@ neus_alpha_bwd_kernel(NeusIn p, const float* __restrict__ dalpha, float* __restr
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p.R * p.N;
        t += (int64_t)gridDim.x * blockDim.x) {
     NeusTerms q = neus_terms(p, t);
-    float da = dalpha[t];
-    // clamp backward: pass-through where 0 <= ratio <= 1
-    float dr = (q.ratio >= 0.0f && q.ratio <= 1.0f) ? da : 0.0f;
-    float num = q.pc - q.nc, den = q.pc + 1e-5f;
-    float dnum = dr / den;
-    float dden = -dr * num / (den * den);
-    float dpc = dnum + dden;
-    float dnc = -dnum;
-    float dzp = dpc * q.pc * (1.0f - q.pc);  // d(est_prev * inv_s)
-    float dzn = dnc * q.nc * (1.0f - q.nc);
-    float dep = dzp * q.inv_s, den_ = dzn * q.inv_s;
-    float dinv = dzp * q.est_prev + dzn * q.est_next;
-    dsv += dinv * q.inv_s;
-    dsdf[t] = q.outside ? 0.0f : (dep + den_);
-    float diter = (-dep + den_) * q.intv * 0.5f;
-    float a = p.anneal_ratio;
-    float u1 = -q.cosv * 0.5f + 0.5f;
-    float u2 = -q.cosv;
-    float dcos = diter * (0.5f * (1.0f - a) * (u1 > 0.0f ? 1.0f : 0.0f) + a * (u2 > 0.0f ? 1.0f : 0.0f));
-    const int64_t r = q.r;
-    dgrad[t * 3] = dcos * p.ray[r * 3];
-    dgrad[t * 3 + 1] = dcos * p.ray[r * 3 + 1];
-    dgrad[t * 3 + 2] = dcos * p.ray[r * 3 + 2];
+    dsv += neus_sample_bwd(p, q, dalpha[t], t, dsdf, dgrad);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) dsv += __shfl_xor(dsv, o);
@@ -620,6 +630,100 @@ weighted_sum_packed_bwd_kernel(const float* __restrict__ q, const float* __restr
 }
 
 // ray of every packed sample from the offsets (one thread per ray)
+// ---------------------------------------------------------------------------
+// Packed rays in one persistent launch each way: NeuS alpha + compositing.
+// A persistent grid (a few workgroups per CU) whose waves take chunks of 64
+// consecutive rays from a work counter (queue, zero at launch) until every
+// chunk is done, so rays of uneven sample counts balance across the chip; a
+// lane walks one ray's samples in order.  Forward: alpha of each sample
+// (neus_alpha_fwd_kernel's terms, the ray's last sample against its far
+// distance), the compositing weights and the composited q (K <= 4 channels),
+// with the arithmetic and order of the separate alpha and composite kernels.
+// Backward: the transmittance pass (kept in dsdf until overwritten), then one
+// reverse pass forming dalpha (composite_packed_bwd_kernel's recurrence, plus an
+// optional direct dalpha) and from it dsdf, dgrad and the s_var gradient
+// (reduced per wave, one atomic per wave).
+// ---------------------------------------------------------------------------
+__device__ inline int64_t packed_next_chunk(int* queue) {
+  int c = 0;
+  if ((threadIdx.x & 63) == 0) c = atomicAdd(queue, 1);
+  return (int64_t)__shfl(c, 0);
+}
+
+__global__ void __launch_bounds__(256)
+neus_composite_packed_fwd_kernel(NeusIn p, const float* __restrict__ q, int K,
+                                 float* __restrict__ alpha, float* __restrict__ weights,
+                                 float* __restrict__ out, int* __restrict__ queue) {
+  const int lane = threadIdx.x & 63;
+  const int64_t R = p.R;
+  for (int64_t ch = packed_next_chunk(queue); ch * 64 < R; ch = packed_next_chunk(queue)) {
+    const int64_t r = ch * 64 + lane;
+    if (r >= R) continue;
+    const int64_t b = p.off[r], e = p.off[r + 1];
+    float logT = 0.0f;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t i = b; i < e; ++i) {
+      const NeusTerms t = neus_terms_of(p, i, r, i + 1 < e);
+      const float ra = t.ratio;
+      const float a = ra < 0.0f ? 0.0f : (ra > 1.0f ? 1.0f : ra);  // clip_, NaN kept
+      alpha[i] = a;
+      const float T = expf(logT);
+      const float w = a * T;
+      weights[i] = w;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < K) acc[k] = acc[k] + q[i * K + k] * w;
+      logT = logT + log1pf(-a);
+    }
+    for (int k = 0; k < K; ++k) out[r * K + k] = acc[k];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+neus_composite_packed_bwd_kernel(NeusIn p, const float* __restrict__ q, int K,
+                                 const float* __restrict__ alpha,
+                                 const float* __restrict__ weights,
+                                 const float* __restrict__ dout,
+                                 const float* __restrict__ dweights,
+                                 const float* __restrict__ dalpha_in, float* __restrict__ dsdf,
+                                 float* __restrict__ dgrad, float* __restrict__ dq,
+                                 float* __restrict__ ds_var, int* __restrict__ queue) {
+  const int lane = threadIdx.x & 63;
+  const int64_t R = p.R;
+  float dsv = 0.0f;
+  for (int64_t ch = packed_next_chunk(queue); ch * 64 < R; ch = packed_next_chunk(queue)) {
+    const int64_t r = ch * 64 + lane;
+    if (r >= R) continue;
+    const int64_t b = p.off[r], e = p.off[r + 1];
+    float logT = 0.0f;
+    for (int64_t i = b; i < e; ++i) {
+      dsdf[i] = expf(logT);  // T_i, replaced by the gradient below
+      logT = logT + log1pf(-alpha[i]);
+    }
+    float d[4] = {0.f, 0.f, 0.f, 0.f};
+    if (dout)
+      for (int k = 0; k < K; ++k) d[k] = dout[r * K + k];
+    float U = 0.0f;
+    for (int64_t j = e - 1; j >= b; --j) {
+      float g = dweights ? dweights[j] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < K) g = g + d[k] * q[j * K + k];
+      if (dq)
+        for (int k = 0; k < K; ++k) dq[j * K + k] = weights[j] * d[k];
+      const float a = alpha[j];
+      float da = dsdf[j] * (g - U);
+      U = g * a + (1.0f - a) * U;
+      if (dalpha_in) da = da + dalpha_in[j];
+      const NeusTerms t = neus_terms_of(p, j, r, j + 1 < e);
+      dsv += neus_sample_bwd(p, t, da, j, dsdf, dgrad);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) dsv += __shfl_xor(dsv, o);
+  if (lane == 0 && dsv != 0.0f) atomic_add_f32(ds_var, dsv);
+}
+
 __global__ void __launch_bounds__(256)
 packed_ray_index_kernel(const int32_t* __restrict__ off, int64_t R, int32_t* __restrict__ ray_idx) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1411,6 +1515,73 @@ extern "C" int na_neus_alpha_packed_bwd(const float* ray_unit, const float* sdfs
   p.off = offsets; p.ray_idx = ray_idx;
   hipLaunchKernelGGL(neus_alpha_bwd_kernel, grid_reduce(S), dim3(256), 0, (hipStream_t)stream, p,
                      dalpha, dsdf, dgrad, ds_var);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+// Persistent grid of the fused packed kernels: rays_per_wg-thread workgroups,
+// at most wg_per_cu of them per CU, never more than the ray chunks need.
+static dim3 packed_persistent_grid(int64_t R, int rb) {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 256;
+  const int64_t chunks = (R + 63) / 64, waves = rb / 64;
+  const int64_t want = (chunks + waves - 1) / waves;
+  const int64_t cap = (int64_t)ncu * tune_value(TUNE_PACKED_WG_PER_CU);
+  return dim3((unsigned)std::max<int64_t>(1, std::min(want, cap)));
+}
+
+extern "C" int na_neus_composite_packed_fwd(const float* ray_unit, const float* sdfs,
+                                            const float* grads, const float* dists,
+                                            const float* far, const uint8_t* outside,
+                                            float outside_val, const float* s_var, float progress,
+                                            float anneal_end, const int32_t* offsets, int64_t R,
+                                            const float* q, int32_t K, float* alpha,
+                                            float* weights, float* out, int32_t* queue,
+                                            na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && K >= 0 && K <= 4, "bad sizes (K <= 4)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(ray_unit && sdfs && grads && dists && s_var && offsets && alpha && weights && queue,
+               "null pointer");
+  NA_CHECK_ARG(K == 0 || (q && out), "q and out are needed when K > 0");
+  NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
+  NA_CHECK_ARG(R <= (int64_t)0x7fffffff * 64, "too many rays");
+  NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
+                       anneal_end, R, 1);
+  p.off = offsets;
+  const int rb = (int)tune_value(TUNE_PACKED_RAYS);
+  hipLaunchKernelGGL(neus_composite_packed_fwd_kernel, packed_persistent_grid(R, rb), dim3(rb), 0,
+                     (hipStream_t)stream, p, q, (int)K, alpha, weights, out, (int*)queue);
+  NA_CHECK_LAUNCH();
+  return NA_OK;
+}
+
+extern "C" int na_neus_composite_packed_bwd(const float* ray_unit, const float* sdfs,
+                                            const float* grads, const float* dists,
+                                            const float* far, const uint8_t* outside,
+                                            float outside_val, const float* s_var, float progress,
+                                            float anneal_end, const int32_t* offsets, int64_t R,
+                                            const float* q, int32_t K, const float* alpha,
+                                            const float* weights, const float* dout,
+                                            const float* dweights, const float* dalpha,
+                                            float* dsdf, float* dgrad, float* dq, float* ds_var,
+                                            int32_t* queue, na_stream_t stream) {
+  NA_CHECK_ARG(R >= 0 && K >= 0 && K <= 4, "bad sizes (K <= 4)");
+  if (R == 0) return NA_OK;
+  NA_CHECK_ARG(ray_unit && sdfs && grads && dists && s_var && offsets && alpha && weights && dsdf &&
+                   dgrad && ds_var && queue,
+               "null pointer");
+  NA_CHECK_ARG(K == 0 || q, "q is needed when K > 0");
+  NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
+  NA_CHECK_ARG(R <= (int64_t)0x7fffffff * 64, "too many rays");
+  NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
+                       anneal_end, R, 1);
+  p.off = offsets;
+  const int rb = (int)tune_value(TUNE_PACKED_RAYS);
+  hipLaunchKernelGGL(neus_composite_packed_bwd_kernel, packed_persistent_grid(R, rb), dim3(rb), 0,
+                     (hipStream_t)stream, p, q, (int)K, alpha, weights, dout, dweights, dalpha,
+                     dsdf, dgrad, dq, ds_var, (int*)queue);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }

@@ -1197,6 +1197,73 @@ class NeusAlphaPacked(torch.autograd.Function):
                 None, None)
 
 
+class NeusCompositePacked(torch.autograd.Function):
+    """Packed NeuS alphas and compositing in one persistent launch each way
+    (na_neus_composite_packed_fwd/bwd): ray_unit [R,3], sdfs [S], gradients [S,3],
+    dists [S], far [R] (or None), outside [R] (or None), offsets [R+1], s_var, q [S,K]
+    (K <= 4) -> alphas [S], weights [S], out [R,K].  The same values as
+    NeusAlphaPacked followed by CompositePacked."""
+
+    @staticmethod
+    def forward(ctx, ray_unit, sdfs, gradients, dists, far, outside, offsets, s_var, q,
+                progress, anneal_end, outside_val):
+        off = _offsets(offsets)
+        R, S = off.shape[0] - 1, sdfs.shape[0]
+        dev = sdfs.device
+        ctx.shapes = (sdfs.shape, gradients.shape, q.shape)
+        ray_unit = ray_unit.contiguous().float()
+        sdfs = sdfs.reshape(S).contiguous().float()
+        gradients = gradients.reshape(S, 3).contiguous().float()
+        dists = dists.reshape(S).contiguous().float()
+        far = far.reshape(-1).contiguous().float() if far is not None else None
+        out_u8 = outside.contiguous().to(torch.uint8) if outside is not None else None
+        qc = q.reshape(S, -1).contiguous().float()
+        K = qc.shape[1]
+        alpha = torch.empty(S, device=dev)
+        weights = torch.empty(S, device=dev)
+        out = torch.empty(R, K, device=dev)
+        queue = torch.zeros(1, dtype=torch.int32, device=dev)
+        _lib.call("na_neus_composite_packed_fwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
+                  _lib.ptr(gradients), _lib.ptr(dists), _lib.ptr(far), _lib.ptr(out_u8),
+                  float(outside_val), _lib.ptr(s_var), float(progress), float(anneal_end),
+                  _lib.ptr(off), R, _lib.ptr(qc), K, _lib.ptr(alpha), _lib.ptr(weights),
+                  _lib.ptr(out), _lib.ptr(queue), _s(sdfs))
+        empty = torch.empty(0, device=dev)
+        ctx.save_for_backward(ray_unit, sdfs, gradients, dists,
+                              far if far is not None else empty,
+                              out_u8 if out_u8 is not None else empty, s_var, off, qc, alpha,
+                              weights)
+        ctx.has_far, ctx.has_out = far is not None, out_u8 is not None
+        ctx.progress, ctx.anneal_end, ctx.outside_val = progress, anneal_end, outside_val
+        return alpha, weights, out
+
+    @staticmethod
+    def backward(ctx, dalpha, dweights, dout):
+        ray_unit, sdfs, gradients, dists, far, out_u8, s_var, off, qc, alpha, weights = \
+            ctx.saved_tensors
+        R, S, K = off.shape[0] - 1, sdfs.shape[0], qc.shape[1]
+        dev = sdfs.device
+        dsdf = torch.empty_like(sdfs)
+        dgrad = torch.empty_like(gradients)
+        dq = torch.empty(S, K, device=dev) if ctx.needs_input_grad[8] else None
+        ds_var = torch.zeros(1, device=dev)
+        queue = torch.zeros(1, dtype=torch.int32, device=dev)
+        dout_c = dout.contiguous().float() if dout is not None else None
+        dw_c = dweights.contiguous().float() if dweights is not None else None
+        da_c = dalpha.contiguous().float() if dalpha is not None else None
+        _lib.call("na_neus_composite_packed_bwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
+                  _lib.ptr(gradients), _lib.ptr(dists), _lib.ptr(far) if ctx.has_far else None,
+                  _lib.ptr(out_u8) if ctx.has_out else None, float(ctx.outside_val),
+                  _lib.ptr(s_var), float(ctx.progress), float(ctx.anneal_end), _lib.ptr(off), R,
+                  _lib.ptr(qc), K, _lib.ptr(alpha), _lib.ptr(weights), _lib.ptr(dout_c),
+                  _lib.ptr(dw_c), _lib.ptr(da_c), _lib.ptr(dsdf), _lib.ptr(dgrad), _lib.ptr(dq),
+                  _lib.ptr(ds_var), _lib.ptr(queue), _s(sdfs))
+        s_shape, g_shape, q_shape = ctx.shapes
+        return (None, dsdf.view(s_shape), dgrad.view(g_shape), None, None, None, None,
+                ds_var.view_as(s_var), dq.view(q_shape) if dq is not None else None, None, None,
+                None)
+
+
 # ---------------------------------------------------------------------------
 # eikonal + curvature losses
 # ---------------------------------------------------------------------------

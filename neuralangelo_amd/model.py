@@ -357,8 +357,10 @@ class Model(torch.nn.Module):
     def render_rays_packed(self, center, ray_unit, dists, offsets, validate=True):
         """render_rays_object (model.py:158-190) over packed rays with uneven sample
         counts: center, ray_unit [R,3]; dists [S] sorted within each ray; offsets int32
-        [R+1].  Every kernel of the dense path runs on the packed samples; per-ray
-        quantities are read through the sample's ray index.  Returns rgb [R,3] and the
+        [R+1].  The encoder and the MLPs run on the flat sample list (per-ray
+        quantities read through the sample's ray index); the NeuS alphas and the
+        compositing run in one persistent launch each way that balances rays of
+        uneven sample counts across the chip.  Returns rgb [R,3] and the
         per-sample tensors (gradients/hessians [S,1,3] and outside [S] for the losses).
         validate: check the offsets against the sample count first (one host read)."""
         c = center.reshape(-1, 3).contiguous().float()
@@ -380,10 +382,10 @@ class Model(torch.nn.Module):
                                                                   self.training, delta)
         rgbs = self.neural_rgb.forward(points, gradients, v_s, feats, None,
                                        normals_are_gradients=True)  # [S,3]
-        alphas = self.compute_neus_alphas_packed(v, sdfs, gradients, dists, offsets, far,
-                                                 ray_idx=ray_idx, outside=outside,
-                                                 progress=self.progress)
-        weights, rgb = Fn.CompositePacked.apply(alphas, rgbs, offsets)
+        # NeuS alphas and the compositing of every ray: one persistent launch each way
+        alphas, weights, rgb = Fn.NeusCompositePacked.apply(
+            v, sdfs, gradients, dists, far, outside, offsets, self.s_var, rgbs,
+            float(self.progress), float(self.anneal_end), float(self.outside_val))
         return dict(rgb=rgb, rgbs=rgbs, sdfs=sdfs_filled.view(S), dists=dists, alphas=alphas,
                     weights=weights, ray_idx=ray_idx, outside=out_s,
                     gradients=gradients.view(S, 1, 3),

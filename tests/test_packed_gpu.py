@@ -192,3 +192,65 @@ def test_packed_train_step_full_size(cuda, batch):
         opt.step()  # (clears the table gradient in its update pass)
     assert table_grad_max > 0
     assert torch.isfinite(enc.params).all()
+
+
+@pytest.mark.parametrize("shape", [(64, 1), (256, 16), (128, 4)])
+def test_persistent_fused_launch_matches_separate_kernels(cuda, batch, shape):
+    """NeuS alphas + compositing in one persistent launch each way (work taken in
+    64-ray chunks from a counter) give the separate kernels' values bit for bit
+    (alphas, weights, colour; dsdf, dgrad, drgb and a direct dalpha's share), s_var's
+    gradient within float summation order, at several workgroup sizes and grids; the
+    alphas also against the oracle."""
+    from neuralangelo_amd import _lib
+    rb, wpc = shape
+    c, v, off, d = batch
+    R, S = off.numel() - 1, int(off[-1])
+    print("persistent packed", rb, wpc, R, S, flush=True)
+    g = torch.Generator().manual_seed(8)
+    sdf = torch.randn(S, generator=g) * 0.05
+    grad = torch.randn(S, 3, generator=g)
+    q = torch.rand(S, 3, generator=g)
+    _, far, outside = orr.get_dist_bounds(c, v)
+    outside = outside.reshape(-1).clone()
+    outside[::97] = True  # some rays outside the sphere
+    m = Model(default_config(log2_T=15)).to(cuda)
+    offd, vd, dd = off.to(cuda), v.to(cuda), d.to(cuda)
+    fard, outd = far.reshape(-1).to(cuda), outside.to(cuda)
+    go = torch.randn(R, 3, generator=g).to(cuda)
+    gw = torch.randn(S, generator=g).to(cuda)
+    ga = torch.randn(S, generator=g).to(cuda)
+
+    def run(fused):
+        sd = sdf.to(cuda).requires_grad_(True)
+        gd = grad.to(cuda).requires_grad_(True)
+        qd = q.to(cuda).requires_grad_(True)
+        m.s_var.grad = None
+        if fused:
+            a, w, out = Fn.NeusCompositePacked.apply(vd, sd, gd, dd, fard, outd, offd, m.s_var, qd,
+                                                     0.5, 0.1, 1000.0)
+        else:
+            ridx = Fn.packed_ray_index(offd, S)
+            a = Fn.NeusAlphaPacked.apply(vd, sd, gd, dd, fard, outd, offd, ridx, m.s_var, 0.5, 0.1,
+                                         1000.0)
+            w, out = Fn.CompositePacked.apply(a, qd, offd)
+        ((out * go).sum() + (w * gw).sum() + (a * ga).sum()).backward()
+        return [t.detach().clone() for t in (a, w, out, sd.grad, gd.grad, qd.grad)], \
+            m.s_var.grad.detach().clone()
+
+    old = (_lib.call("na_tune_set", b"packed.rays_per_wg", rb),
+           _lib.call("na_tune_set", b"packed.wg_per_cu", wpc))
+    try:
+        got, gs = run(True)
+    finally:
+        _lib.call("na_tune_set", b"packed.rays_per_wg", old[0])
+        _lib.call("na_tune_set", b"packed.wg_per_cu", old[1])
+    want, ws = run(False)
+    for name, x, y in zip(("alpha", "weights", "rgb", "dsdf", "dgrad", "drgb"), got, want):
+        assert torch.equal(x, y), name
+    assert (gs - ws).abs().item() <= 1e-5 * ws.abs().item() + 1e-6
+    s64 = m.s_var.detach().cpu().double()
+    ref = orr.compute_neus_alphas_packed(v.double(), sdf.double()[:, None], grad.double(),
+                                         d.double()[:, None], far.double(), off, s64, 0.5,
+                                         m.anneal_end)
+    inside = ~outside[torch.repeat_interleave(torch.arange(R), off[1:] - off[:-1])]
+    assert (got[0].cpu().double()[inside] - ref.detach()[inside]).abs().max() < 2e-5
