@@ -632,6 +632,7 @@ def packed_leg(dev, n_rays=65536, steps=5, warmup=2):
     """Train rays/s of the ragged packed batch (BF16): packed render of every ray's own
     samples, L1 + eikonal + curvature, backward, AdamW; the batch is fixed and resident
     before the timed steps."""
+    from neuralangelo_amd import functional as Fn
     from neuralangelo_amd import misc
     from neuralangelo_amd.optim import AdamW
     model = build_model(torch.bfloat16, dev)
@@ -652,19 +653,37 @@ def packed_leg(dev, n_rays=65536, steps=5, warmup=2):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
+    names = ("na_neus_composite_packed_fwd", "na_neus_composite_packed_bwd")
+    Fn.LAUNCH_EVENTS = {k: [] for k in names}
     t0 = time.perf_counter()
     for _ in range(steps):
         loss = step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    ev, Fn.LAUNCH_EVENTS = Fn.LAUNCH_EVENTS, None
     assert torch.isfinite(loss).item(), "non-finite loss (packed batch)"
     counts = (off[1:] - off[:-1]).cpu()
+    S = int(off[-1].item())
+    # the persistent launches (NeuS alpha + compositing, one each way): algorithmic
+    # bytes 32 B read + 8 B written per sample forward (sdf, gradient, distance,
+    # colour; alpha, weight), 40 B read + 36 B written + the transmittance's 8 B
+    # round trip backward (dsdf, dgrad, dcolour), 37 B per ray each way (offsets,
+    # direction, far, outside flag, colour or its gradient)
+    ms = sum(sum(a.elapsed_time(b) for a, b in ev[k]) / max(1, len(ev[k])) for k in names)
+    nbytes = S * (40 + 84) + n_rays * 74
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     del model, opt
     torch.cuda.empty_cache()
     return dict(rays_per_s=round(n_rays / dt, 1), ms_per_step=round(dt * 1e3, 3),
-                samples=int(off[-1].item()), samples_per_s=round(int(off[-1].item()) / dt, 1),
+                samples=S, samples_per_s=round(S / dt, 1),
                 empty_rays=int((counts == 0).sum()), max_samples_per_ray=int(counts.max()),
                 steps=steps, warmup=warmup,
+                roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                              unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                              kernel="na_neus_composite_packed_fwd + _bwd (persistent, one "
+                                     "lane per ray)",
+                              bytes_per_sample=124, bytes_per_ray=74, avg_ms_fwd_plus_bwd=round(ms, 4),
+                              share_of_step=round(ms / (dt * 1e3), 4)),
                 workload=f"{n_rays} rays, 0..192 samples each (random union of spheres), "
                          "packed render, losses, backward, AdamW; BF16, 2^22 x 16 x 8")
 

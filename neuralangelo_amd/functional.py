@@ -1223,11 +1223,12 @@ class NeusCompositePacked(torch.autograd.Function):
         weights = torch.empty(S, device=dev)
         out = torch.empty(R, K, device=dev)
         queue = torch.zeros(1, dtype=torch.int32, device=dev)
-        _lib.call("na_neus_composite_packed_fwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
-                  _lib.ptr(gradients), _lib.ptr(dists), _lib.ptr(far), _lib.ptr(out_u8),
-                  float(outside_val), _lib.ptr(s_var), float(progress), float(anneal_end),
-                  _lib.ptr(off), R, _lib.ptr(qc), K, _lib.ptr(alpha), _lib.ptr(weights),
-                  _lib.ptr(out), _lib.ptr(queue), _s(sdfs))
+        with _timed("na_neus_composite_packed_fwd"):
+            _lib.call("na_neus_composite_packed_fwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
+                      _lib.ptr(gradients), _lib.ptr(dists), _lib.ptr(far), _lib.ptr(out_u8),
+                      float(outside_val), _lib.ptr(s_var), float(progress), float(anneal_end),
+                      _lib.ptr(off), R, _lib.ptr(qc), K, _lib.ptr(alpha), _lib.ptr(weights),
+                      _lib.ptr(out), _lib.ptr(queue), _s(sdfs))
         empty = torch.empty(0, device=dev)
         ctx.save_for_backward(ray_unit, sdfs, gradients, dists,
                               far if far is not None else empty,
@@ -1251,13 +1252,15 @@ class NeusCompositePacked(torch.autograd.Function):
         dout_c = dout.contiguous().float() if dout is not None else None
         dw_c = dweights.contiguous().float() if dweights is not None else None
         da_c = dalpha.contiguous().float() if dalpha is not None else None
-        _lib.call("na_neus_composite_packed_bwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
-                  _lib.ptr(gradients), _lib.ptr(dists), _lib.ptr(far) if ctx.has_far else None,
-                  _lib.ptr(out_u8) if ctx.has_out else None, float(ctx.outside_val),
-                  _lib.ptr(s_var), float(ctx.progress), float(ctx.anneal_end), _lib.ptr(off), R,
-                  _lib.ptr(qc), K, _lib.ptr(alpha), _lib.ptr(weights), _lib.ptr(dout_c),
-                  _lib.ptr(dw_c), _lib.ptr(da_c), _lib.ptr(dsdf), _lib.ptr(dgrad), _lib.ptr(dq),
-                  _lib.ptr(ds_var), _lib.ptr(queue), _s(sdfs))
+        with _timed("na_neus_composite_packed_bwd"):
+            _lib.call("na_neus_composite_packed_bwd", _lib.ptr(ray_unit), _lib.ptr(sdfs),
+                      _lib.ptr(gradients), _lib.ptr(dists),
+                      _lib.ptr(far) if ctx.has_far else None,
+                      _lib.ptr(out_u8) if ctx.has_out else None, float(ctx.outside_val),
+                      _lib.ptr(s_var), float(ctx.progress), float(ctx.anneal_end), _lib.ptr(off),
+                      R, _lib.ptr(qc), K, _lib.ptr(alpha), _lib.ptr(weights), _lib.ptr(dout_c),
+                      _lib.ptr(dw_c), _lib.ptr(da_c), _lib.ptr(dsdf), _lib.ptr(dgrad),
+                      _lib.ptr(dq), _lib.ptr(ds_var), _lib.ptr(queue), _s(sdfs))
         s_shape, g_shape, q_shape = ctx.shapes
         return (None, dsdf.view(s_shape), dgrad.view(g_shape), None, None, None, None,
                 ds_var.view_as(s_var), dq.view(q_shape) if dq is not None else None, None, None,
