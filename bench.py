@@ -680,8 +680,8 @@ def packed_leg(dev, n_rays=65536, steps=5, warmup=2):
                 steps=steps, warmup=warmup,
                 roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                               unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                              kernel="na_neus_composite_packed_fwd + _bwd (persistent, one "
-                                     "lane per ray)",
+                              kernel="na_neus_composite_packed_fwd + _bwd (persistent, a "
+                                     "wave per ray)",
                               bytes_per_sample=124, bytes_per_ray=74, avg_ms_fwd_plus_bwd=round(ms, 4),
                               share_of_step=round(ms / (dt * 1e3), 4)),
                 workload=f"{n_rays} rays, 0..192 samples each (random union of spheres), "

@@ -504,11 +504,12 @@ int na_neus_alpha_packed_bwd(const float* ray_unit, const float* sdfs, const flo
                              float* ds_var, na_stream_t stream);
 
 /* The packed NeuS alpha and compositing of all rays in one persistent launch each
- * way (the separate packed entries above, fused; same arithmetic and order per
- * sample).  A grid of a few workgroups per CU whose waves take chunks of 64
- * consecutive rays from *queue (an int32 the caller zeroes before the call; the
- * launch leaves it non-zero) until all rays are done, so uneven sample counts
- * balance across the chip.  Tile shape: the launch tunables "packed.rays_per_wg"
+ * way (the separate packed entries above, fused; the same per-sample terms, the
+ * transmittance and its backward recurrence as wave scans).  A grid of a few
+ * workgroups per CU whose waves take chunks of 8 consecutive rays from *queue (an
+ * int32 the caller zeroes before the call; the launch leaves it non-zero) until all
+ * rays are done, a wave rendering one ray at a time with its lanes over the ray's
+ * samples, so uneven sample counts balance across the chip.  Tile shape: the launch tunables "packed.rays_per_wg"
  * (workgroup size) and "packed.wg_per_cu".
  *   fwd: alpha [S] (model.py:275-298), weights [S] and out [R, K] = sum of q [S, K]
  *        times weights (render.py:87-112), K <= 4;

@@ -632,22 +632,46 @@ weighted_sum_packed_bwd_kernel(const float* __restrict__ q, const float* __restr
 // ray of every packed sample from the offsets (one thread per ray)
 // ---------------------------------------------------------------------------
 // Packed rays in one persistent launch each way: NeuS alpha + compositing.
-// A persistent grid (a few workgroups per CU) whose waves take chunks of 64
-// consecutive rays from a work counter (queue, zero at launch) until every
-// chunk is done, so rays of uneven sample counts balance across the chip; a
-// lane walks one ray's samples in order.  Forward: alpha of each sample
-// (neus_alpha_fwd_kernel's terms, the ray's last sample against its far
-// distance), the compositing weights and the composited q (K <= 4 channels),
-// with the arithmetic and order of the separate alpha and composite kernels.
-// Backward: the transmittance pass (kept in dsdf until overwritten), then one
-// reverse pass forming dalpha (composite_packed_bwd_kernel's recurrence, plus an
-// optional direct dalpha) and from it dsdf, dgrad and the s_var gradient
-// (reduced per wave, one atomic per wave).
+// A persistent grid (a few workgroups per CU) whose waves take chunks of
+// kPackedChunk consecutive rays from a work counter (queue, zero at launch)
+// until every ray is done, so rays of uneven sample counts balance across the
+// chip.  A wave renders one ray at a time, its 64 lanes over 64 consecutive
+// samples (coalesced loads and stores):
+//   forward: each sample's alpha (neus_alpha_fwd_kernel's terms, the ray's last
+//   sample against its far distance), log T as the running sum of log(1 - alpha)
+//   (a wave scan per 64 samples plus the carry of the earlier ones), the weights
+//   and the composited q (K <= 4 channels, summed over the wave);
+//   backward: the same transmittance pass (kept in dsdf until overwritten), then
+//   the 64-sample groups from the ray's end: U_j = sum_{i>j} g_i a_i prod_{j<k<i}
+//   (1 - a_k) is the composition of the affine maps x -> (1 - a_i) x + g_i a_i
+//   of the later samples applied to the carry of the later groups (a suffix scan
+//   of maps), dalpha_j = T_j (g_j - U_j) plus any direct dalpha, and from it
+//   dsdf, dgrad and the s_var gradient (one atomic per wave).
+// The sums run in wave-scan order, so values match the one-lane-per-ray kernels
+// to float rounding.
 // ---------------------------------------------------------------------------
+constexpr int kPackedChunk = 8;  // rays per work-counter request
+
 __device__ inline int64_t packed_next_chunk(int* queue) {
   int c = 0;
   if ((threadIdx.x & 63) == 0) c = atomicAdd(queue, 1);
   return (int64_t)__shfl(c, 0);
+}
+
+__device__ inline float wave_incl_sum(float x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const float y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+__device__ inline float wave_total(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
 }
 
 __global__ void __launch_bounds__(256)
@@ -656,26 +680,44 @@ neus_composite_packed_fwd_kernel(NeusIn p, const float* __restrict__ q, int K,
                                  float* __restrict__ out, int* __restrict__ queue) {
   const int lane = threadIdx.x & 63;
   const int64_t R = p.R;
-  for (int64_t ch = packed_next_chunk(queue); ch * 64 < R; ch = packed_next_chunk(queue)) {
-    const int64_t r = ch * 64 + lane;
-    if (r >= R) continue;
-    const int64_t b = p.off[r], e = p.off[r + 1];
-    float logT = 0.0f;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int64_t i = b; i < e; ++i) {
-      const NeusTerms t = neus_terms_of(p, i, r, i + 1 < e);
-      const float ra = t.ratio;
-      const float a = ra < 0.0f ? 0.0f : (ra > 1.0f ? 1.0f : ra);  // clip_, NaN kept
-      alpha[i] = a;
-      const float T = expf(logT);
-      const float w = a * T;
-      weights[i] = w;
+  for (int64_t ch = packed_next_chunk(queue); ch * kPackedChunk < R; ch = packed_next_chunk(queue)) {
+    for (int rr = 0; rr < kPackedChunk; ++rr) {
+      const int64_t r = ch * kPackedChunk + rr;
+      if (r >= R) break;
+      const int64_t b = p.off[r], e = p.off[r + 1];
+      float carry = 0.0f;  // log T before this group's first sample
+      float acc[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int64_t base = b; base < e; base += 64) {
+        const int64_t i = base + lane;
+        const bool valid = i < e;
+        float a = 0.0f;
+        if (valid) {
+          const NeusTerms t = neus_terms_of(p, i, r, i + 1 < e);
+          const float ra = t.ratio;
+          a = ra < 0.0f ? 0.0f : (ra > 1.0f ? 1.0f : ra);  // clip_, NaN kept
+          alpha[i] = a;
+        }
+        const float l = valid ? log1pf(-a) : 0.0f;
+        const float incl = wave_incl_sum(l);
+        float ex = __shfl_up(incl, 1);
+        if (lane == 0) ex = 0.0f;
+        if (valid) {
+          const float w = a * expf(carry + ex);
+          weights[i] = w;
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (k < K) acc[k] = acc[k] + q[i * K + k] * w;
-      logT = logT + log1pf(-a);
+          for (int k = 0; k < 4; ++k)
+            if (k < K) acc[k] = acc[k] + q[i * K + k] * w;
+        }
+        carry = carry + __shfl(incl, 63);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k < K) {
+          const float sk = wave_total(acc[k]);
+          if (lane == 0) out[r * K + k] = sk;
+        }
+      }
     }
-    for (int k = 0; k < K; ++k) out[r * K + k] = acc[k];
   }
 }
 
@@ -691,36 +733,73 @@ neus_composite_packed_bwd_kernel(NeusIn p, const float* __restrict__ q, int K,
   const int lane = threadIdx.x & 63;
   const int64_t R = p.R;
   float dsv = 0.0f;
-  for (int64_t ch = packed_next_chunk(queue); ch * 64 < R; ch = packed_next_chunk(queue)) {
-    const int64_t r = ch * 64 + lane;
-    if (r >= R) continue;
-    const int64_t b = p.off[r], e = p.off[r + 1];
-    float logT = 0.0f;
-    for (int64_t i = b; i < e; ++i) {
-      dsdf[i] = expf(logT);  // T_i, replaced by the gradient below
-      logT = logT + log1pf(-alpha[i]);
-    }
-    float d[4] = {0.f, 0.f, 0.f, 0.f};
-    if (dout)
-      for (int k = 0; k < K; ++k) d[k] = dout[r * K + k];
-    float U = 0.0f;
-    for (int64_t j = e - 1; j >= b; --j) {
-      float g = dweights ? dweights[j] : 0.0f;
+  for (int64_t ch = packed_next_chunk(queue); ch * kPackedChunk < R; ch = packed_next_chunk(queue)) {
+    for (int rr = 0; rr < kPackedChunk; ++rr) {
+      const int64_t r = ch * kPackedChunk + rr;
+      if (r >= R) break;
+      const int64_t b = p.off[r], e = p.off[r + 1];
+      // transmittance of every sample, kept in dsdf (as the forward formed it)
+      float carry = 0.0f;
+      for (int64_t base = b; base < e; base += 64) {
+        const int64_t i = base + lane;
+        const bool valid = i < e;
+        const float l = valid ? log1pf(-alpha[i]) : 0.0f;
+        const float incl = wave_incl_sum(l);
+        float ex = __shfl_up(incl, 1);
+        if (lane == 0) ex = 0.0f;
+        if (valid) dsdf[i] = expf(carry + ex);
+        carry = carry + __shfl(incl, 63);
+      }
+      float d[4] = {0.f, 0.f, 0.f, 0.f};
+      if (dout)
+        for (int k = 0; k < K; ++k) d[k] = dout[r * K + k];
+      float uc = 0.0f;  // U of the sample before the later groups' first one
+      const int64_t n = e - b;
+      for (int64_t base = b + ((n - 1) / 64) * 64; n > 0 && base >= b; base -= 64) {
+        const int64_t j = base + lane;
+        const bool valid = j < e;
+        float g = 0.0f, a = 0.0f, tj = 0.0f;
+        if (valid) {
+          g = dweights ? dweights[j] : 0.0f;
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (k < K) g = g + d[k] * q[j * K + k];
-      if (dq)
-        for (int k = 0; k < K; ++k) dq[j * K + k] = weights[j] * d[k];
-      const float a = alpha[j];
-      float da = dsdf[j] * (g - U);
-      U = g * a + (1.0f - a) * U;
-      if (dalpha_in) da = da + dalpha_in[j];
-      const NeusTerms t = neus_terms_of(p, j, r, j + 1 < e);
-      dsv += neus_sample_bwd(p, t, da, j, dsdf, dgrad);
+          for (int k = 0; k < 4; ++k)
+            if (k < K) g = g + d[k] * q[j * K + k];
+          if (dq) {
+            const float wj = weights[j];
+            for (int k = 0; k < K; ++k) dq[j * K + k] = wj * d[k];
+          }
+          a = alpha[j];
+          tj = dsdf[j];
+        }
+        // suffix composition of x -> m x + c over lanes j .. 63 (identity past e)
+        float m = valid ? 1.0f - a : 1.0f, c = valid ? g * a : 0.0f;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const float m2 = __shfl_down(m, o), c2 = __shfl_down(c, o);
+          if (lane + o < 64) {
+            c = m * c2 + c;
+            m = m * m2;
+          }
+        }
+        // exclusive: the maps of lanes j + 1 .. 63 applied to the carry
+        float mx = __shfl_down(m, 1), cx = __shfl_down(c, 1);
+        if (lane == 63) {
+          mx = 1.0f;
+          cx = 0.0f;
+        }
+        const float U = mx * uc + cx;
+        const float u0 = __shfl(m, 0) * uc + __shfl(c, 0);  // U before this group
+        if (valid) {
+          float da = tj * (g - U);
+          if (dalpha_in) da = da + dalpha_in[j];
+          const NeusTerms t = neus_terms_of(p, j, r, j + 1 < e);
+          dsv += neus_sample_bwd(p, t, da, j, dsdf, dgrad);
+        }
+        uc = u0;
+      }
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) dsv += __shfl_xor(dsv, o);
+  dsv = wave_total(dsv);
   if (lane == 0 && dsv != 0.0f) atomic_add_f32(ds_var, dsv);
 }
 
@@ -1526,7 +1605,7 @@ static dim3 packed_persistent_grid(int64_t R, int rb) {
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     ncu = 256;
-  const int64_t chunks = (R + 63) / 64, waves = rb / 64;
+  const int64_t chunks = (R + kPackedChunk - 1) / kPackedChunk, waves = rb / 64;
   const int64_t want = (chunks + waves - 1) / waves;
   const int64_t cap = (int64_t)ncu * tune_value(TUNE_PACKED_WG_PER_CU);
   return dim3((unsigned)std::max<int64_t>(1, std::min(want, cap)));
@@ -1546,7 +1625,7 @@ extern "C" int na_neus_composite_packed_fwd(const float* ray_unit, const float* 
                "null pointer");
   NA_CHECK_ARG(K == 0 || (q && out), "q and out are needed when K > 0");
   NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
-  NA_CHECK_ARG(R <= (int64_t)0x7fffffff * 64, "too many rays");
+  NA_CHECK_ARG(R <= (int64_t)0x7fffffff * kPackedChunk, "too many rays");
   NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
                        anneal_end, R, 1);
   p.off = offsets;
@@ -1574,7 +1653,7 @@ extern "C" int na_neus_composite_packed_bwd(const float* ray_unit, const float* 
                "null pointer");
   NA_CHECK_ARG(K == 0 || q, "q is needed when K > 0");
   NA_CHECK_ARG(anneal_end > 0.0f, "anneal_end must be > 0");
-  NA_CHECK_ARG(R <= (int64_t)0x7fffffff * 64, "too many rays");
+  NA_CHECK_ARG(R <= (int64_t)0x7fffffff * kPackedChunk, "too many rays");
   NeusIn p = make_neus(ray_unit, sdfs, grads, dists, far, outside, outside_val, s_var, progress,
                        anneal_end, R, 1);
   p.off = offsets;

@@ -196,11 +196,12 @@ def test_packed_train_step_full_size(cuda, batch):
 
 @pytest.mark.parametrize("shape", [(64, 1), (256, 16), (128, 4)])
 def test_persistent_fused_launch_matches_separate_kernels(cuda, batch, shape):
-    """NeuS alphas + compositing in one persistent launch each way (work taken in
-    64-ray chunks from a counter) give the separate kernels' values bit for bit
-    (alphas, weights, colour; dsdf, dgrad, drgb and a direct dalpha's share), s_var's
-    gradient within float summation order, at several workgroup sizes and grids; the
-    alphas also against the oracle."""
+    """NeuS alphas + compositing in one persistent launch each way (a wave per ray,
+    lanes over samples, rays taken in chunks from a counter) against the separate
+    one-lane-per-ray kernels: alphas bit for bit (the same per-sample terms); weights,
+    colour, dsdf, dgrad, drgb (with a direct dalpha's share) and s_var's gradient to
+    float rounding (the transmittance and its backward recurrence run as wave scans),
+    at several workgroup sizes and grids; the alphas also against the oracle."""
     from neuralangelo_amd import _lib
     rb, wpc = shape
     c, v, off, d = batch
@@ -245,9 +246,11 @@ def test_persistent_fused_launch_matches_separate_kernels(cuda, batch, shape):
         _lib.call("na_tune_set", b"packed.rays_per_wg", old[0])
         _lib.call("na_tune_set", b"packed.wg_per_cu", old[1])
     want, ws = run(False)
-    for name, x, y in zip(("alpha", "weights", "rgb", "dsdf", "dgrad", "drgb"), got, want):
-        assert torch.equal(x, y), name
-    assert (gs - ws).abs().item() <= 1e-5 * ws.abs().item() + 1e-6
+    assert torch.equal(got[0], want[0]), "alpha"
+    for name, x, y in zip(("weights", "rgb", "dsdf", "dgrad", "drgb"), got[1:], want[1:]):
+        err = (x - y).abs().max().item()
+        assert err <= 1e-5 * y.abs().max().item() + 1e-7, (name, err)
+    assert (gs - ws).abs().item() <= 1e-4 * ws.abs().item() + 1e-6
     s64 = m.s_var.detach().cpu().double()
     ref = orr.compute_neus_alphas_packed(v.double(), sdf.double()[:, None], grad.double(),
                                          d.double()[:, None], far.double(), off, s64, 0.5,
