@@ -69,7 +69,7 @@ constexpr McCaseTable make_case_table() {
   return t;
 }
 __constant__ McCaseTable dMcCase = make_case_table();
-__constant__ int8_t dMcTri[256][16] = NA_MC_TRI_TABLE_INIT;
+__constant__ __attribute__((aligned(16))) int8_t dMcTri[256][16] = NA_MC_TRI_TABLE_INIT;
 
 struct McLattice {
   const float* v;
@@ -145,17 +145,18 @@ __device__ inline McBlock block_of(const McLattice& g, uint32_t b) {
   return k;
 }
 
-// Tile (within the block) and in-tile cell of block-local cell (i, j, k).
-__device__ inline void tile_of_cell(const McBlock& b, int64_t i, int64_t j, int64_t k,
-                                    int64_t& tile, int64_t& c) {
+// Tile (within the block) and in-tile cell of block-local cell (i, j, k), in
+// 32 bits: tile indices are below 2^31 (the host checks the tile count).
+__device__ inline void tile_of_cell(const McBlock& b, uint32_t i, uint32_t j, uint32_t k,
+                                    uint32_t& tile, uint32_t& c) {
   if (b.seg == 1) {
-    const int64_t jt = j / b.rpt;
-    tile = i * b.tpp + jt;
-    c = (j - jt * b.rpt) * b.cz + k;
+    const uint32_t jt = j / (uint32_t)b.rpt;
+    tile = i * (uint32_t)b.tpp + jt;
+    c = (j - jt * (uint32_t)b.rpt) * (uint32_t)b.cz + k;
   } else {
-    const int64_t ks = k / 1024;
-    tile = i * b.tpp + j * b.seg + ks;
-    c = k - ks * 1024;
+    const uint32_t ks = k >> 10;
+    tile = i * (uint32_t)b.tpp + j * (uint32_t)b.seg + ks;
+    c = k & 1023u;
   }
 }
 
@@ -294,8 +295,10 @@ __host__ __device__ inline void mc_pipeline(int64_t np_max, int& U, int& lead) {
 }
 
 __global__ void __launch_bounds__(256)
-mc_zero_counts_kernel(int64_t ntiles, int* __restrict__ tile_v, int* __restrict__ tile_f) {
+mc_zero_counts_kernel(int64_t ntiles, int* __restrict__ tile_v, int* __restrict__ tile_f,
+                      int* __restrict__ nlive) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *nlive = 0;
   if (t < ntiles) {
     tile_v[t] = 0;
     tile_f[t] = 0;
@@ -305,10 +308,13 @@ mc_zero_counts_kernel(int64_t ntiles, int* __restrict__ tile_v, int* __restrict_
 __global__ void __launch_bounds__(kMcBlock)
 mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __restrict__ cidx,
                    uint16_t* __restrict__ vpre, int* __restrict__ tile_v,
-                   int* __restrict__ tile_f) {
+                   int* __restrict__ tile_f, int* __restrict__ live, int* __restrict__ nlive) {
   extern __shared__ float ring[];  // lead + 2 slots of 256 U floats (a plane each)
   __shared__ int wv[kMcItems][kMcWaves];
   __shared__ int wf[kMcWaves];
+  // per-wave flags of a staged plane: bit 0 a point at or below iso, bit 1 one
+  // above (planes p >= 1 alternate between rows 0 and 1, plane 0 uses row 2)
+  __shared__ int wfl[3][kMcWaves];
   // the case words in LDS: a vector-memory lookup in the loop would wait for
   // every plane load issued before it (the load counter drains in order)
   __shared__ uint16_t case_w[256];
@@ -370,12 +376,53 @@ mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __rest
   const float isof = g.isof;
   const bool iso_f = g.iso_is_f != 0;
   const int64_t tile_base = b * g.tpb + q;
+  // which of this lane's staged elements are lattice points (the rest read zeros)
+  uint32_t own_pts = 0;
+#pragma unroll
+  for (int u = 0; u < kMcPref; ++u) own_pts |= voff[u] != kMcOob ? 1u << u : 0u;
+  // flags of this wave's own staged points of plane p (its loads have landed)
+  auto wave_flags = [&](int64_t p) -> int {
+    const float* src = ring + (p % slots) * stride + (int64_t)w * U * 64 + lane;
+    bool below = false, above = false;
+#pragma unroll
+    for (int u = 0; u < kMcPref; ++u) {
+      if (own_pts & (1u << u)) {
+        const float v = src[u * 64];
+        const bool le = iso_f ? v <= isof : (double)v <= iso;
+        below = below || le;
+        above = above || !le;
+      }
+    }
+    return (__any(below) ? 1 : 0) | (__any(above) ? 2 : 0);
+  };
+  int fl_lo = 0;  // the workgroup's flags of plane i
   for (int64_t i = 0; i < k.cx; ++i) {
     if (i + 1 + lead <= k.cx) issue_plane(i + 1 + lead);
     // planes i and i + 1 have landed once at most the loads of the planes
     // issued after i + 1 (i + 2 .. i + 1 + lead, those that exist) are pending
     const int64_t after = (i + 1 + lead <= k.cx ? i + 1 + lead : k.cx) - (i + 1);
-    mc_sync((int)(after > 0 ? after : 0) * U);
+    mc_wait((int)(after > 0 ? after : 0) * U);
+    asm volatile("" ::: "memory");
+    const int fw1 = wave_flags(i + 1);
+    const int fw0 = i == 0 ? wave_flags(0) : 0;
+    if (lane == 0) {
+      wfl[(i + 1) & 1][w] = fw1;
+      if (i == 0) wfl[2][w] = fw0;
+    }
+    mc_sync(-1);  // both planes and every wave's flags are visible
+    int fl_hi = 0;
+#pragma unroll
+    for (int qq = 0; qq < kMcWaves; ++qq) {
+      fl_hi |= wfl[(i + 1) & 1][qq];
+      if (i == 0) fl_lo |= wfl[2][qq];
+    }
+    const bool crossed = (fl_lo | fl_hi) == 3;
+    fl_lo = fl_hi;
+    // Planes i and i + 1 all at or below iso, or all above: every cell of the
+    // slab has case 0 or 255, so no vertex, no face and nothing to write.  No
+    // end barrier either: the slab's slots were read only by their own loading
+    // waves (the flags above), before this iteration's barrier.
+    if (!crossed) continue;
     const float* s0 = ring + (i % slots) * stride;
     const float* s1 = ring + ((i + 1) % slots) * stride;
     int cube[kMcItems], nv[kMcItems], inc[kMcItems];
@@ -439,6 +486,7 @@ mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __rest
       if (threadIdx.x == 0) {
         tile_v[gt] = tv;
         tile_f[gt] = tf;
+        live[atomicAdd(nlive, 1)] = (int)gt;  // emit's work list (any order)
       }
     }
     // every wave has read planes i and i + 1 (and the counts) before a slot or
@@ -538,85 +586,97 @@ struct McOffsets {
   const int64_t* gsum_v;
 };
 
-// Global index of the vertex on edge e of cell (i,j,k), which that cell does
-// not own: found in the owner cell (same block) from its cube index, the
-// owner tile's offset and the owner's in-tile prefix.
+// A cell's crossed edges in two phases, so that the loads of all its edges are
+// in flight together: emit_load issues them (the two lattice values of an owned
+// edge; the owner's cube index, in-tile prefix and tile offsets of a shared
+// one), emit_store consumes them (writes the owned vertex, or forms the shared
+// vertex's global index) and records the cell's vertex index of each edge.
+struct McEdgeData {
+  float fa[12], fb[12];
+  int64_t otoff[12], ogsum[12];
+  int ocx[12], ovp[12], omask[12];
+};
+
+// Owner of the vertex on edge E of block-local cell (i,j,k), which that cell
+// does not own: the owner cell's workspace slot and tile, and the mask of the
+// owner's edges that precede its own edge for this vertex and that it owns.
 template <int E>
-__device__ inline int64_t shared_vertex(const McOffsets& o, int64_t tile0, int64_t i, int64_t j,
-                                        int64_t k, const McBlock& blk) {
+__device__ inline void shared_owner(uint32_t tile0, uint32_t i, uint32_t j, uint32_t k,
+                                    const McBlock& blk, int64_t& oslot, uint32_t& ot, int& omask) {
   constexpr int a = kEa[E], b = kEb[E];
   constexpr int lx = kCx[a] < kCx[b] ? kCx[a] : kCx[b];
   constexpr int ly = kCy[a] < kCy[b] ? kCy[a] : kCy[b];
   constexpr int lz = kCz[a] < kCz[b] ? kCz[a] : kCz[b];
   constexpr int axis = E < 8 ? (E & 1) : 2;
-  const int64_t px = i + lx, py = j + ly, pz = k + lz;  // low end of the lattice edge
-  int64_t oi, oj, ok;
-  int oe;
+  const uint32_t px = i + lx, py = j + ly, pz = k + lz;  // low end of the lattice edge
+  uint32_t oi, oj, ok;
+  int bm;  // edges before the owner's edge in kOrder
   if (axis == 0) {
     oi = px;
-    if (py >= 1 && pz >= 1) { oj = py - 1; ok = pz - 1; oe = 6; }
-    else if (py == 0 && pz == 0) { oj = 0; ok = 0; oe = 0; }
-    else if (pz == 0) { oj = py - 1; ok = 0; oe = 2; }
-    else { oj = 0; ok = pz - 1; oe = 4; }
+    if (py >= 1 && pz >= 1) { oj = py - 1; ok = pz - 1; bm = before_mask(6); }
+    else if (py == 0 && pz == 0) { oj = 0; ok = 0; bm = before_mask(0); }
+    else if (pz == 0) { oj = py - 1; ok = 0; bm = before_mask(2); }
+    else { oj = 0; ok = pz - 1; bm = before_mask(4); }
   } else if (axis == 1) {
     oj = py;
-    if (px >= 1 && pz >= 1) { oi = px - 1; ok = pz - 1; oe = 5; }
-    else if (px == 0 && pz == 0) { oi = 0; ok = 0; oe = 3; }
-    else if (pz == 0) { oi = px - 1; ok = 0; oe = 1; }
-    else { oi = 0; ok = pz - 1; oe = 7; }
+    if (px >= 1 && pz >= 1) { oi = px - 1; ok = pz - 1; bm = before_mask(5); }
+    else if (px == 0 && pz == 0) { oi = 0; ok = 0; bm = before_mask(3); }
+    else if (pz == 0) { oi = px - 1; ok = 0; bm = before_mask(1); }
+    else { oi = 0; ok = pz - 1; bm = before_mask(7); }
   } else {
     ok = pz;
-    if (px >= 1 && py >= 1) { oi = px - 1; oj = py - 1; oe = 10; }
-    else if (px == 0 && py == 0) { oi = 0; oj = 0; oe = 8; }
-    else if (py == 0) { oi = px - 1; oj = 0; oe = 9; }
-    else { oi = 0; oj = py - 1; oe = 11; }
+    if (px >= 1 && py >= 1) { oi = px - 1; oj = py - 1; bm = before_mask(10); }
+    else if (px == 0 && py == 0) { oi = 0; oj = 0; bm = before_mask(8); }
+    else if (py == 0) { oi = px - 1; oj = 0; bm = before_mask(9); }
+    else { oi = 0; oj = py - 1; bm = before_mask(11); }
   }
-  int64_t otl, oc;
+  uint32_t otl, oc;
   tile_of_cell(blk, oi, oj, ok, otl, oc);
-  const int64_t ot = tile0 + otl;
-  const int64_t oslot = ot * kMcTile + oc;
-  const int oown = dMcCase.w[o.cidx[oslot]] & owned_mask(oi == 0, oj == 0, ok == 0);
-  int bm = 0;
-  switch (oe) {  // edges before oe in kOrder
-    case 6: bm = before_mask(6); break;
-    case 5: bm = before_mask(5); break;
-    case 10: bm = before_mask(10); break;
-    case 0: bm = before_mask(0); break;
-    case 1: bm = before_mask(1); break;
-    case 2: bm = before_mask(2); break;
-    case 3: bm = before_mask(3); break;
-    case 4: bm = before_mask(4); break;
-    case 7: bm = before_mask(7); break;
-    case 8: bm = before_mask(8); break;
-    case 9: bm = before_mask(9); break;
-    default: bm = before_mask(11); break;
-  }
-  return o.toff_v[ot] + o.gsum_v[ot / kScanGroup] + o.vpre[oslot] + __popc(oown & bm);
+  ot = tile0 + otl;
+  oslot = (int64_t)ot * kMcTile + oc;
+  omask = owned_mask(oi == 0, oj == 0, ok == 0) & bm;
 }
 
 template <int Q>
-__device__ inline void emit_edge(const McLattice& g, const McWorld& wt, const McOffsets& o,
-                                 const double (&org)[3], const float* p, int64_t sx, int64_t sy,
-                                 int64_t tile0, int64_t i, int64_t j, int64_t k, const McBlock& blk,
-                                 int edges, int own, int64_t vb, double* __restrict__ V,
-                                 int64_t* sidx) {
+__device__ inline void emit_load(const McOffsets& o, const float* p, int64_t sx, int64_t sy,
+                                 uint32_t tile0, uint32_t i, uint32_t j, uint32_t k,
+                                 const McBlock& blk, int edges, int own, McEdgeData& d) {
+  constexpr int e = kOrder[Q];
+  if (!(edges & (1 << e))) return;
+  if (own & (1 << e)) {
+    constexpr int a = kEa[e], b = kEb[e];
+    d.fa[e] = p[kCx[a] * sx + kCy[a] * sy + kCz[a]];
+    d.fb[e] = p[kCx[b] * sx + kCy[b] * sy + kCz[b]];
+  } else {
+    int64_t oslot;
+    uint32_t ot;
+    shared_owner<e>(tile0, i, j, k, blk, oslot, ot, d.omask[e]);
+    d.ocx[e] = o.cidx[oslot];
+    d.ovp[e] = o.vpre[oslot];
+    d.otoff[e] = o.toff_v[ot];
+    d.ogsum[e] = o.gsum_v[ot / kScanGroup];
+  }
+}
+
+template <int Q>
+__device__ inline void emit_store(const McLattice& g, const McWorld& wt, const double (&org)[3],
+                                  const uint16_t* case_w, uint32_t i, uint32_t j, uint32_t k,
+                                  int edges, int own, int64_t vb, const McEdgeData& d,
+                                  double* __restrict__ V, int64_t* sidx) {
   constexpr int e = kOrder[Q];
   if (!(edges & (1 << e))) return;
   if (own & (1 << e)) {
     constexpr int a = kEa[e], b = kEb[e];
     constexpr int axis = e < 8 ? (e & 1) : 2;
-    const int64_t oa = kCx[a] * sx + kCy[a] * sy + kCz[a];
-    const int64_t ob = kCx[b] * sx + kCy[b] * sy + kCz[b];
-    const double fa = (double)p[oa], fb = (double)p[ob];
     double q[3] = {(double)(i + kCx[a]), (double)(j + kCy[a]), (double)(k + kCz[a])};
     const double xb = axis == 0 ? (double)(i + kCx[b]) : axis == 1 ? (double)(j + kCy[b])
                                                                    : (double)(k + kCz[b]);
-    q[axis] = interp(q[axis], xb, fa, fb, g.iso);
+    q[axis] = interp(q[axis], xb, (double)d.fa[e], (double)d.fb[e], g.iso);
     if (wt.ax[0]) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double s = q[d] * wt.intv;
-        q[d] = s + org[d];
+      for (int c = 0; c < 3; ++c) {
+        const double s = q[c] * wt.intv;
+        q[c] = s + org[c];
       }
     }
     const int64_t vi = vb + __popc(own & edges & before_mask(e));
@@ -626,78 +686,123 @@ __device__ inline void emit_edge(const McLattice& g, const McWorld& wt, const Mc
     out[2] = q[2];
     sidx[e * kMcBlock] = vi;
   } else {
-    sidx[e * kMcBlock] = shared_vertex<e>(o, tile0, i, j, k, blk);
+    sidx[e * kMcBlock] =
+        d.otoff[e] + d.ogsum[e] + d.ovp[e] + __popc(case_w[d.ocx[e]] & d.omask[e]);
   }
 }
 
+// Persistent: workgroup b takes entries b, b + grid, ... of the classify's list
+// of surface tiles (the only tiles with output); each tile's output positions
+// come from the tile scan, so the list's order does not matter.
 __global__ void __launch_bounds__(kMcBlock)
-mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, int64_t ntiles,
-               const int* __restrict__ tile_v, const int* __restrict__ tile_f,
-               const int64_t* __restrict__ toff_f, const int64_t* __restrict__ gsum_f,
-               double* __restrict__ V, int64_t* __restrict__ F) {
+mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, const int* __restrict__ live,
+               const int* __restrict__ nlive, const int64_t* __restrict__ toff_f,
+               const int64_t* __restrict__ gsum_f, double* __restrict__ V,
+               int64_t* __restrict__ F) {
   __shared__ int wf[kMcItems][kMcWaves];
+  __shared__ int ws[kMcWaves];
   __shared__ int64_t sidx_all[12 * kMcBlock];
+  __shared__ uint16_t case_w[256];
+  // the tile's cells with a crossed edge: cell | cube << 10 | face prefix << 18
+  __shared__ uint32_t surf[kMcTile];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t gt = blockIdx.x;
-  if (tile_v[gt] == 0 && tile_f[gt] == 0) return;
   const int64_t sy = g.nz, sx = g.ny * g.nz;
   int64_t* sidx = sidx_all + threadIdx.x;
   const uint32_t tpb = (uint32_t)g.tpb;
-  const uint32_t b = gt / tpb;
-  const int64_t base = (int64_t)(gt - b * tpb) * kMcTile;
-  const McBlock k = block_of(g, b);
-  const int64_t vbase = o.toff_v[gt] + o.gsum_v[gt / kScanGroup];
-  const int64_t fbase = toff_f[gt] + gsum_f[gt / kScanGroup];
-  const int64_t tile0 = (int64_t)b * tpb;
-  double org[3] = {0.0, 0.0, 0.0};
-  if (wt.ax[0]) {
-    org[0] = (double)wt.ax[0][k.ox];
-    org[1] = (double)wt.ax[1][k.oy];
-    org[2] = (double)wt.ax[2][k.oz];
-  }
-  const int64_t slot0 = (int64_t)gt * kMcTile + threadIdx.x;
-  // the tile's place in the block: x-plane ti and tile column tq
-  const int64_t ti = base / kMcTile / k.tpp;
-  const McTileRows tr = tile_rows(k, base / kMcTile - ti * k.tpp);
-  const int64_t ncells = tr.rows * tr.cols;
-  int cube[kMcItems], nf[kMcItems], inc[kMcItems];
-#pragma unroll
-  for (int it = 0; it < kMcItems; ++it) {
-    const bool in = it * kMcBlock + (int64_t)threadIdx.x < ncells;
-    cube[it] = in ? o.cidx[slot0 + it * kMcBlock] : 0;
-    nf[it] = dMcCase.w[cube[it]] >> 12;
-    inc[it] = wave_incl_scan(nf[it]);
-    if (lane == 63) wf[it][w] = inc[it];
-  }
+  case_w[threadIdx.x] = dMcCase.w[threadIdx.x];
   __syncthreads();
-  int fpre[kMcItems];
-  tile_prefix(inc, nf, wf, fpre);
-#pragma unroll 1
-  for (int it = 0; it < kMcItems; ++it) {
-    const int edges = dMcCase.w[cube[it]] & 0xfff;
-    if (!edges) continue;
-    const int64_t c = it * kMcBlock + (int64_t)threadIdx.x;
-    const int64_t jj = c / tr.cols;
-    const int64_t i = ti, j = tr.j0 + jj, kk = tr.k0 + (c - jj * tr.cols);
-    const int own = edges & owned_mask(i == 0, j == 0, kk == 0);
-    const float* p = g.v + ((k.ox + (int64_t)i) * g.ny + (k.oy + (int64_t)j)) * g.nz +
-                     (k.oz + (int64_t)kk);
-    const int64_t vb = vbase + o.vpre[slot0 + it * kMcBlock];
-#define NA_MC_EDGE(Q) \
-  emit_edge<Q>(g, wt, o, org, p, sx, sy, tile0, i, j, kk, k, edges, own, vb, V, sidx)
-    NA_MC_EDGE(0); NA_MC_EDGE(1); NA_MC_EDGE(2); NA_MC_EDGE(3);
-    NA_MC_EDGE(4); NA_MC_EDGE(5); NA_MC_EDGE(6); NA_MC_EDGE(7);
-    NA_MC_EDGE(8); NA_MC_EDGE(9); NA_MC_EDGE(10); NA_MC_EDGE(11);
-#undef NA_MC_EDGE
-    int64_t fo = fbase + fpre[it];
-    const int8_t* tri = dMcTri[cube[it]];
-    for (int q3 = 0; q3 < 15 && tri[q3] >= 0; q3 += 3) {
-      int64_t* out = F + 3 * fo;
-      out[0] = sidx[tri[q3] * kMcBlock];
-      out[1] = sidx[tri[q3 + 1] * kMcBlock];
-      out[2] = sidx[tri[q3 + 2] * kMcBlock];
-      ++fo;
+  const int n_live = *nlive;
+  for (int li = blockIdx.x; li < n_live; li += gridDim.x) {
+    const uint32_t gt = (uint32_t)live[li];
+    const uint32_t b = gt / tpb;
+    const uint32_t tl = gt - b * tpb;  // tile within the block
+    const McBlock k = block_of(g, b);
+    const int64_t vbase = o.toff_v[gt] + o.gsum_v[gt / kScanGroup];
+    const int64_t fbase = toff_f[gt] + gsum_f[gt / kScanGroup];
+    const uint32_t tile0 = b * tpb;
+    double org[3] = {0.0, 0.0, 0.0};
+    if (wt.ax[0]) {
+      org[0] = (double)wt.ax[0][k.ox];
+      org[1] = (double)wt.ax[1][k.oy];
+      org[2] = (double)wt.ax[2][k.oz];
     }
+    const int64_t slot0 = (int64_t)gt * kMcTile + threadIdx.x;
+    // the tile's place in the block: x-plane ti and tile column tl - ti * tpp
+    const uint32_t ti = tl / (uint32_t)k.tpp;
+    const McTileRows tr = tile_rows(k, (int64_t)(tl - ti * (uint32_t)k.tpp));
+    const int64_t ncells = tr.rows * tr.cols;
+    int cube[kMcItems], nf[kMcItems], inc[kMcItems], ns = 0;
+#pragma unroll
+    for (int it = 0; it < kMcItems; ++it) {
+      const bool in = it * kMcBlock + (int64_t)threadIdx.x < ncells;
+      cube[it] = in ? o.cidx[slot0 + it * kMcBlock] : 0;
+      nf[it] = case_w[cube[it]] >> 12;
+      ns += (case_w[cube[it]] & 0xfff) != 0 ? 1 : 0;
+      inc[it] = wave_incl_scan(nf[it]);
+      if (lane == 63) wf[it][w] = inc[it];
+    }
+    const int ns_inc = wave_incl_scan(ns);
+    if (lane == 63) ws[w] = ns_inc;
+    __syncthreads();
+    int fpre[kMcItems];
+    tile_prefix(inc, nf, wf, fpre);
+    // list the surface cells, so that every lane of the passes below has one
+    int sb = ns_inc - ns, n_surf = 0;
+#pragma unroll
+    for (int qq = 0; qq < kMcWaves; ++qq) {
+      sb += qq < w ? ws[qq] : 0;
+      n_surf += ws[qq];
+    }
+#pragma unroll
+    for (int it = 0; it < kMcItems; ++it) {
+      if ((case_w[cube[it]] & 0xfff) != 0)
+        surf[sb++] = ((uint32_t)(it * kMcBlock) + threadIdx.x) | ((uint32_t)cube[it] << 10) |
+                     ((uint32_t)fpre[it] << 18);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int si = threadIdx.x; si < n_surf; si += kMcBlock) {
+      const uint32_t e0 = surf[si];
+      const uint32_t c = e0 & 1023u;
+      const int cb = (int)((e0 >> 10) & 255u);
+      const int edges = case_w[cb] & 0xfff;
+      // triangle list of the case (16 bytes, -1 terminated), loaded with the edges
+      const uint4 trow = *reinterpret_cast<const uint4*>(&dMcTri[cb][0]);
+      const uint32_t jj = c / (uint32_t)tr.cols;
+      const uint32_t i = ti, j = (uint32_t)tr.j0 + jj, kk = (uint32_t)tr.k0 + (c - jj * (uint32_t)tr.cols);
+      const int own = edges & owned_mask(i == 0, j == 0, kk == 0);
+      const float* p = g.v + ((k.ox + (int64_t)i) * g.ny + (k.oy + (int64_t)j)) * g.nz +
+                       (k.oz + (int64_t)kk);
+      const int64_t vb = vbase + o.vpre[(int64_t)gt * kMcTile + c];
+      // two batches of six edges (the registers of twelve in flight would halve
+      // the waves per SIMD)
+      McEdgeData d;
+#define NA_MC_LOAD(Q) emit_load<Q>(o, p, sx, sy, tile0, i, j, kk, k, edges, own, d)
+#define NA_MC_STORE(Q) emit_store<Q>(g, wt, org, case_w, i, j, kk, edges, own, vb, d, V, sidx)
+      NA_MC_LOAD(0); NA_MC_LOAD(1); NA_MC_LOAD(2); NA_MC_LOAD(3); NA_MC_LOAD(4); NA_MC_LOAD(5);
+      NA_MC_STORE(0); NA_MC_STORE(1); NA_MC_STORE(2); NA_MC_STORE(3); NA_MC_STORE(4); NA_MC_STORE(5);
+      asm volatile("" ::: "memory");  // the second batch's loads stay after the first's use
+      NA_MC_LOAD(6); NA_MC_LOAD(7); NA_MC_LOAD(8); NA_MC_LOAD(9); NA_MC_LOAD(10); NA_MC_LOAD(11);
+      NA_MC_STORE(6); NA_MC_STORE(7); NA_MC_STORE(8); NA_MC_STORE(9); NA_MC_STORE(10);
+      NA_MC_STORE(11);
+#undef NA_MC_LOAD
+#undef NA_MC_STORE
+      const uint32_t tw[4] = {trow.x, trow.y, trow.z, trow.w};
+      int64_t fo = fbase + (int64_t)(e0 >> 18);
+#pragma unroll
+      for (int q3 = 0; q3 < 15; q3 += 3) {
+        const int t0 = (int8_t)(tw[q3 >> 2] >> ((q3 & 3) * 8));
+        if (t0 < 0) break;
+        const int t1 = (int8_t)(tw[(q3 + 1) >> 2] >> (((q3 + 1) & 3) * 8));
+        const int t2 = (int8_t)(tw[(q3 + 2) >> 2] >> (((q3 + 2) & 3) * 8));
+        int64_t* out = F + 3 * fo;
+        out[0] = sidx[t0 * kMcBlock];
+        out[1] = sidx[t1 * kMcBlock];
+        out[2] = sidx[t2 * kMcBlock];
+        ++fo;
+      }
+    }
+    __syncthreads();  // every wave has read wf, ws and surf before the next tile writes them
   }
 }
 
@@ -781,6 +886,8 @@ struct McWorkspace {
   uint16_t* vpre;
   int* tile_v;
   int* tile_f;
+  int* live;   // surface tiles (classify appends, emit walks)
+  int* nlive;  // their count
   int64_t* toff_v;
   int64_t* toff_f;
   int64_t* gsum_v;
@@ -803,6 +910,8 @@ static size_t mc_ws_layout(const McPlan& p, void* base, McWorkspace* ws) {
   w.vpre = (uint16_t*)take(nslots * 2);
   w.tile_v = (int*)take((size_t)p.ntiles * 4);
   w.tile_f = (int*)take((size_t)p.ntiles * 4);
+  w.live = (int*)take((size_t)p.ntiles * 4);
+  w.nlive = (int*)take(4);
   w.toff_v = (int64_t*)take((size_t)p.ntiles * 8);
   w.toff_f = (int64_t*)take((size_t)p.ntiles * 8);
   w.gsum_v = (int64_t*)take((size_t)p.ngroups * 8);
@@ -843,14 +952,14 @@ extern "C" int na_mc_count(const float* volume, int64_t nx, int64_t ny, int64_t 
   NA_CHECK_ARG(p.np_max <= (int64_t)kMcPref * kMcBlock, "staged plane too large");
   NA_CHECK_ARG(1025 * nz * 4 < (int64_t)INT32_MAX, "lattice rows too long (nz)");
   hipLaunchKernelGGL(mc_zero_counts_kernel, dim3((unsigned)ceil_div(p.ntiles, (int64_t)256)),
-                     dim3(256), 0, s, p.ntiles, w.tile_v, w.tile_f);
+                     dim3(256), 0, s, p.ntiles, w.tile_v, w.tile_f, w.nlive);
   NA_CHECK_LAUNCH();
   const int64_t nblocks = p.g.nb[0] * p.g.nb[1] * p.g.nb[2];
   int U, lead;
   mc_pipeline(p.np_max, U, lead);
   const size_t lds = (size_t)(lead + 2) * 256 * U * sizeof(float);  // slots of 256 U floats
   hipLaunchKernelGGL(mc_classify_kernel, dim3((unsigned)(nblocks * p.tpp)), dim3(kMcBlock), lds, s,
-                     p.g, p.tpp, p.np_max, w.cidx, w.vpre, w.tile_v, w.tile_f);
+                     p.g, p.tpp, p.np_max, w.cidx, w.vpre, w.tile_v, w.tile_f, w.live, w.nlive);
   NA_CHECK_LAUNCH();
   hipLaunchKernelGGL(mc_scan_tiles_kernel, dim3((unsigned)p.ngroups), dim3(kMcBlock), 0, s,
                      p.ntiles, w.tile_v, w.tile_f, w.toff_v, w.toff_f, w.gsum_v, w.gsum_f);
@@ -877,9 +986,19 @@ extern "C" int na_mc_emit(const float* volume, int64_t nx, int64_t ny, int64_t n
   McWorld wt{{axis_x, axis_y, axis_z}, intv};
   hipStream_t s = (hipStream_t)stream;
   McOffsets o{w.cidx, w.vpre, w.toff_v, w.gsum_v};
-  const dim3 grid((unsigned)p.ntiles);
-  hipLaunchKernelGGL(mc_emit_kernel, grid, dim3(kMcBlock), 0, s, p.g, wt, o, p.ntiles, w.tile_v,
-                     w.tile_f, w.toff_f, w.gsum_f, vertices, faces);
+  // persistent grid of as many workgroups as are resident at once (the surface
+  // tiles are few next to the lattice's tiles)
+  int dev = 0, ncu = 256, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_emit_kernel, kMcBlock, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const dim3 grid((unsigned)std::min<int64_t>(p.ntiles, (int64_t)ncu * per_cu));
+  hipLaunchKernelGGL(mc_emit_kernel, grid, dim3(kMcBlock), 0, s, p.g, wt, o, w.live, w.nlive,
+                     w.toff_f, w.gsum_f, vertices, faces);
   NA_CHECK_LAUNCH();
   return NA_OK;
 }
