@@ -37,6 +37,7 @@ constexpr int kMcWaves = kMcBlock / 64;
 constexpr int kMcTile = kMcBlock * kMcItems;  // cells per tile
 static_assert(kMcItems == 4, "the classify's wave vote names four items");
 constexpr int kScanGroup = kMcBlock * kMcItems;  // tiles per scan group
+constexpr int kMcEmitBlock = 64;                 // emit: one wave per workgroup
 
 // Bourke corner offsets and edge endpoints; kOrder = per-cell vertex order.
 constexpr int kCx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
@@ -684,33 +685,38 @@ __device__ inline void emit_store(const McLattice& g, const McWorld& wt, const d
     out[0] = q[0];
     out[1] = q[1];
     out[2] = q[2];
-    sidx[e * kMcBlock] = vi;
+    sidx[e * kMcEmitBlock] = vi;
   } else {
-    sidx[e * kMcBlock] =
+    sidx[e * kMcEmitBlock] =
         d.otoff[e] + d.ogsum[e] + d.ovp[e] + __popc(case_w[d.ocx[e]] & d.omask[e]);
   }
 }
 
-// Persistent: workgroup b takes entries b, b + grid, ... of the classify's list
-// of surface tiles (the only tiles with output); each tile's output positions
-// come from the tile scan, so the list's order does not matter.
-__global__ void __launch_bounds__(kMcBlock)
+// Persistent, one wave per workgroup: workgroup b takes entries b, b + grid, ...
+// of the classify's list of surface tiles (the only tiles with output); each
+// tile's output positions come from the tile scan, so the list's order does not
+// matter.  Lane l reads the cube indices of cells 16 l .. 16 l + 15 (one 16-byte
+// load), scans the face counts and lists the surface cells in LDS; the wave
+// then takes the listed cells 64 at a time.  Waves work on their own tiles, with
+// no workgroup barrier between them.
+__global__ void __launch_bounds__(kMcEmitBlock)
 mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, const int* __restrict__ live,
                const int* __restrict__ nlive, const int64_t* __restrict__ toff_f,
                const int64_t* __restrict__ gsum_f, double* __restrict__ V,
                int64_t* __restrict__ F) {
-  __shared__ int wf[kMcItems][kMcWaves];
-  __shared__ int ws[kMcWaves];
-  __shared__ int64_t sidx_all[12 * kMcBlock];
+  __shared__ int64_t sidx_all[12 * kMcEmitBlock];
   __shared__ uint16_t case_w[256];
   // the tile's cells with a crossed edge: cell | cube << 10 | face prefix << 18
   __shared__ uint32_t surf[kMcTile];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x;
   const int64_t sy = g.nz, sx = g.ny * g.nz;
-  int64_t* sidx = sidx_all + threadIdx.x;
+  int64_t* sidx = sidx_all + lane;
   const uint32_t tpb = (uint32_t)g.tpb;
-  case_w[threadIdx.x] = dMcCase.w[threadIdx.x];
+#pragma unroll
+  for (int q = 0; q < 256 / kMcEmitBlock; ++q)
+    case_w[q * kMcEmitBlock + lane] = dMcCase.w[q * kMcEmitBlock + lane];
   __syncthreads();
+  const bool cidx16 = ((uintptr_t)o.cidx & 15) == 0;
   const int n_live = *nlive;
   for (int li = blockIdx.x; li < n_live; li += gridDim.x) {
     const uint32_t gt = (uint32_t)live[li];
@@ -726,42 +732,46 @@ mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, const int* __restrict__ liv
       org[1] = (double)wt.ax[1][k.oy];
       org[2] = (double)wt.ax[2][k.oz];
     }
-    const int64_t slot0 = (int64_t)gt * kMcTile + threadIdx.x;
     // the tile's place in the block: x-plane ti and tile column tl - ti * tpp
     const uint32_t ti = tl / (uint32_t)k.tpp;
     const McTileRows tr = tile_rows(k, (int64_t)(tl - ti * (uint32_t)k.tpp));
-    const int64_t ncells = tr.rows * tr.cols;
-    int cube[kMcItems], nf[kMcItems], inc[kMcItems], ns = 0;
+    const uint32_t ncells = (uint32_t)(tr.rows * tr.cols);
+    const uint32_t c0 = (uint32_t)lane * 16;
+    const uint8_t* cp = o.cidx + (int64_t)gt * kMcTile + c0;
+    uint32_t cw[4];
+    if (cidx16) {
+      const uint4 v = *reinterpret_cast<const uint4*>(cp);
+      cw[0] = v.x; cw[1] = v.y; cw[2] = v.z; cw[3] = v.w;
+    } else {
 #pragma unroll
-    for (int it = 0; it < kMcItems; ++it) {
-      const bool in = it * kMcBlock + (int64_t)threadIdx.x < ncells;
-      cube[it] = in ? o.cidx[slot0 + it * kMcBlock] : 0;
-      nf[it] = case_w[cube[it]] >> 12;
-      ns += (case_w[cube[it]] & 0xfff) != 0 ? 1 : 0;
-      inc[it] = wave_incl_scan(nf[it]);
-      if (lane == 63) wf[it][w] = inc[it];
+      for (int q = 0; q < 4; ++q)
+        cw[q] = (uint32_t)cp[4 * q] | ((uint32_t)cp[4 * q + 1] << 8) |
+                ((uint32_t)cp[4 * q + 2] << 16) | ((uint32_t)cp[4 * q + 3] << 24);
     }
-    const int ns_inc = wave_incl_scan(ns);
-    if (lane == 63) ws[w] = ns_inc;
-    __syncthreads();
-    int fpre[kMcItems];
-    tile_prefix(inc, nf, wf, fpre);
-    // list the surface cells, so that every lane of the passes below has one
-    int sb = ns_inc - ns, n_surf = 0;
+    // cells past the tile's end hold no case (their slots were never written)
 #pragma unroll
-    for (int qq = 0; qq < kMcWaves; ++qq) {
-      sb += qq < w ? ws[qq] : 0;
-      n_surf += ws[qq];
+    for (int t = 0; t < 16; ++t)
+      if (c0 + t >= ncells) cw[t >> 2] &= ~(0xffu << ((t & 3) * 8));
+    int nf_l = 0, ns_l = 0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int wd = case_w[(cw[t >> 2] >> ((t & 3) * 8)) & 255u];
+      nf_l += wd >> 12;
+      ns_l += (wd & 0xfff) != 0 ? 1 : 0;
     }
+    const int nf_inc = wave_incl_scan(nf_l), ns_inc = wave_incl_scan(ns_l);
+    const int n_surf = __shfl(ns_inc, 63);
+    int fp = nf_inc - nf_l, sb = ns_inc - ns_l;
 #pragma unroll
-    for (int it = 0; it < kMcItems; ++it) {
-      if ((case_w[cube[it]] & 0xfff) != 0)
-        surf[sb++] = ((uint32_t)(it * kMcBlock) + threadIdx.x) | ((uint32_t)cube[it] << 10) |
-                     ((uint32_t)fpre[it] << 18);
+    for (int t = 0; t < 16; ++t) {
+      const uint32_t cb = (cw[t >> 2] >> ((t & 3) * 8)) & 255u;
+      const int wd = case_w[cb];
+      if ((wd & 0xfff) != 0) surf[sb++] = (c0 + t) | (cb << 10) | ((uint32_t)fp << 18);
+      fp += wd >> 12;
     }
     __syncthreads();
 #pragma unroll 1
-    for (int si = threadIdx.x; si < n_surf; si += kMcBlock) {
+    for (int si = lane; si < n_surf; si += kMcEmitBlock) {
       const uint32_t e0 = surf[si];
       const uint32_t c = e0 & 1023u;
       const int cb = (int)((e0 >> 10) & 255u);
@@ -796,13 +806,13 @@ mc_emit_kernel(McLattice g, McWorld wt, McOffsets o, const int* __restrict__ liv
         const int t1 = (int8_t)(tw[(q3 + 1) >> 2] >> (((q3 + 1) & 3) * 8));
         const int t2 = (int8_t)(tw[(q3 + 2) >> 2] >> (((q3 + 2) & 3) * 8));
         int64_t* out = F + 3 * fo;
-        out[0] = sidx[t0 * kMcBlock];
-        out[1] = sidx[t1 * kMcBlock];
-        out[2] = sidx[t2 * kMcBlock];
+        out[0] = sidx[t0 * kMcEmitBlock];
+        out[1] = sidx[t1 * kMcEmitBlock];
+        out[2] = sidx[t2 * kMcEmitBlock];
         ++fo;
       }
     }
-    __syncthreads();  // every wave has read wf, ws and surf before the next tile writes them
+    __syncthreads();  // surf is read before the next tile writes it
   }
 }
 
@@ -992,12 +1002,12 @@ extern "C" int na_mc_emit(const float* volume, int64_t nx, int64_t ny, int64_t n
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     ncu = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_emit_kernel, kMcBlock, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_emit_kernel, kMcEmitBlock, 0) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   const dim3 grid((unsigned)std::min<int64_t>(p.ntiles, (int64_t)ncu * per_cu));
-  hipLaunchKernelGGL(mc_emit_kernel, grid, dim3(kMcBlock), 0, s, p.g, wt, o, w.live, w.nlive,
+  hipLaunchKernelGGL(mc_emit_kernel, grid, dim3(kMcEmitBlock), 0, s, p.g, wt, o, w.live, w.nlive,
                      w.toff_f, w.gsum_f, vertices, faces);
   NA_CHECK_LAUNCH();
   return NA_OK;
