@@ -378,6 +378,12 @@ hashgrid_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, floa
 }
 
 constexpr float kTapDir[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+// Coordinate class per evaluation (0 the centre, then the four taps) and axis:
+// 0 the centre's coordinate, 1 x + eps, 2 x - eps.  The tap scatter groups the
+// taps by their x class: taps 0 and 3 on +eps, taps 1 and 2 on -eps.
+constexpr int kEvalClass[5][3] = {{0, 0, 0}, {1, 2, 2}, {2, 2, 1}, {2, 1, 2}, {1, 1, 1}};
+static_assert(kTapDir[0][0] > 0 && kTapDir[3][0] > 0 && kTapDir[1][0] < 0 && kTapDir[2][0] < 0,
+              "tap grouping by x class");
 
 // Centre input and tap increments of the SDF network for the increment
 // formulation of the 4-tap gradient (bf16 compute).  Per point: X0c (the centre
@@ -539,14 +545,33 @@ __device__ __attribute__((always_inline)) inline void taps_flush(float* gtab, co
                                                                  const int (&blo)[3],
                                                                  const float (&acc)[27],
                                                                  uint32_t mask) {
+  // corner_index of the 27 box corners from per-axis terms: a hashed level
+  // xors one term per axis; a dense level adds them, and its wrap (index %
+  // size) is the identity while the box lies inside the grid (every coordinate
+  // below res, so the index is below res^3 <= size)
+  constexpr uint32_t kPy = 2654435761u, kPz = 805459861u;
+  const uint32_t x0 = (uint32_t)blo[0], y0 = (uint32_t)blo[1], z0 = (uint32_t)blo[2];
+  uint32_t id[27];
+  if (p.hashed) {
+    const uint32_t m = p.size - 1u;
 #pragma unroll
-  for (int o = 0; o < 27; ++o) {
-    if (mask & (1u << o)) {
-      const uint32_t id = corner_index((uint32_t)(blo[0] + o % 3), (uint32_t)(blo[1] + (o / 3) % 3),
-                                       (uint32_t)(blo[2] + o / 9), p);
-      atomic_add_f32(gtab + (size_t)id * F, acc[o]);
+    for (int o = 0; o < 27; ++o)
+      id[o] = ((x0 + (uint32_t)(o % 3)) ^ ((y0 + (uint32_t)((o / 3) % 3)) * kPy) ^
+               ((z0 + (uint32_t)(o / 9)) * kPz)) & m;
+  } else {
+    const uint32_t r1 = p.res, r2 = p.res * p.res;
+    const uint32_t d0 = x0 + y0 * r1 + z0 * r2;
+#pragma unroll
+    for (int o = 0; o < 27; ++o)
+      id[o] = d0 + (uint32_t)(o % 3) + (uint32_t)((o / 3) % 3) * r1 + (uint32_t)(o / 9) * r2;
+    if (!(x0 + 2u < p.res && y0 + 2u < p.res && z0 + 2u < p.res)) {
+#pragma unroll
+      for (int o = 0; o < 27; ++o) id[o] %= p.size;
     }
   }
+#pragma unroll
+  for (int o = 0; o < 27; ++o)
+    if (mask & (1u << o)) atomic_add_f32(gtab + (size_t)id[o] * F, acc[o]);
 }
 
 // P2: the volume range is a positive normal power of two, 2^(-vshift) of the
@@ -615,44 +640,43 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
       for (int t = 0; t < 4; ++t) g[1 + t] = cur.t[t];
     }
     if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && g[3] == 0.0f && g[4] == 0.0f) continue;
-    int cell[5][3];
-    float fr[5][3];
+    // Per axis the five evaluations take three coordinates: the centre's and
+    // x +- eps (each tap moves every axis by +eps or -eps, as the forward built
+    // it: x + k_t eps).  Class 0 is the centre, 1 the +eps side, 2 the -eps side;
+    // the cells are ordered cl[2] <= cl[0] <= cl[1] (the lattice position is
+    // monotone in x).
+    int cl[3][3];
+    float fl[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float xc = cur.x[k];
       uint32_t c;
-      pos_fract(unit(xc), p.scale, c, fr[0][k]);
-      cell[0][k] = (int)c;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const float xt = xc + kTapDir[t][k] * eps;  // the tap point, as the forward built it
-        pos_fract(unit(xt), p.scale, c, fr[1 + t][k]);
-        cell[1 + t][k] = (int)c;
-      }
+      pos_fract(unit(xc), p.scale, c, fl[0][k]);
+      cl[0][k] = (int)c;
+      pos_fract(unit(xc + eps), p.scale, c, fl[1][k]);
+      cl[1][k] = (int)c;
+      pos_fract(unit(xc + -eps), p.scale, c, fl[2][k]);
+      cl[2][k] = (int)c;
     }
-    int lo[3];
     bool box = true;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      int a = cell[0][k], b = cell[0][k];
-#pragma unroll
-      for (int q = 1; q < 5; ++q) {
-        a = min(a, cell[q][k]);
-        b = max(b, cell[q][k]);
-      }
-      lo[k] = a;
-      box = box && (b - a <= 1);
-    }
+    for (int k = 0; k < 3; ++k) box = box && (cl[1][k] - cl[2][k] <= 1);
     if (!box) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         if (g[q] == 0.0f) continue;
+        int cq[3];
+        float fq[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          cq[k] = cl[kEvalClass[q][k]][k];
+          fq[k] = fl[kEvalClass[q][k]][k];
+        }
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-          const uint32_t id = corner_index((uint32_t)cell[q][0] + (c & 1),
-                                           (uint32_t)cell[q][1] + ((c >> 1) & 1),
-                                           (uint32_t)cell[q][2] + ((c >> 2) & 1), p);
-          atomic_add_f32(gtab + (size_t)id * F, corner_weight(c, fr[q]) * g[q]);
+          const uint32_t id = corner_index((uint32_t)cq[0] + (c & 1), (uint32_t)cq[1] + ((c >> 1) & 1),
+                                           (uint32_t)cq[2] + ((c >> 2) & 1), p);
+          atomic_add_f32(gtab + (size_t)id * F, corner_weight(c, fq) * g[q]);
         }
       }
       continue;
@@ -660,51 +684,62 @@ hashgrid_taps_bwd_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n,
     // the point joins the running box when all its cells lie in [blo, blo + 1]
     bool fits = mask != 0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      int a = cell[0][k], b = cell[0][k];
-#pragma unroll
-      for (int q = 1; q < 5; ++q) {
-        a = min(a, cell[q][k]);
-        b = max(b, cell[q][k]);
-      }
-      fits = fits && a >= blo[k] && b <= blo[k] + 1;
-    }
+    for (int k = 0; k < 3; ++k) fits = fits && cl[2][k] >= blo[k] && cl[1][k] <= blo[k] + 1;
     if (!fits) {
       if (mask) taps_flush<F>(gtab, p, blo, acc, mask);
 #pragma unroll
       for (int o = 0; o < 27; ++o) acc[o] = 0.0f;
       mask = 0;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) blo[k] = lo[k];
+      for (int k = 0; k < 3; ++k) blo[k] = cl[2][k];
     }
-    // The weights are separable: point q adds g_q wx[ox] wy[oy] wz[oz] to box
-    // corner (ox, oy, oz), each factor the linear weight of that coordinate in the
-    // point's cell (offset 0 or 1 in the box) and zero outside it.  Corners a
-    // point touches are marked with the eight-corner pattern shifted by its
-    // offset; a point with zero gradient adds nothing and marks nothing.
+    // The weights are separable: evaluation q adds g_q wx[ox] wy[oy] wz[oz] to
+    // box corner (ox, oy, oz), each factor the linear weight of that coordinate
+    // in the evaluation's cell (offset 0 or 1 in the box) and zero outside it.
+    // Per axis only the three classes' weight vectors exist; the evaluations are
+    // grouped by their x class (the centre; taps 0 and 3 on +eps; taps 1 and 2 on
+    // -eps) so that each group's yz products are formed once.
+    float wv[3][3][3];  // [class][axis][offset]
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      float wv[3][3];
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const bool o0 = cell[q][k] == blo[k];
-        const float f = fr[q][k], f1 = 1.0f - f;
-        wv[k][0] = o0 ? f1 : 0.0f;
-        wv[k][1] = o0 ? f : f1;
-        wv[k][2] = o0 ? 0.0f : f;
+        const bool o0 = cl[c][k] == blo[k];
+        const float f = fl[c][k], f1 = 1.0f - f;
+        wv[c][k][0] = o0 ? f1 : 0.0f;
+        wv[c][k][1] = o0 ? f : f1;
+        wv[c][k][2] = o0 ? 0.0f : f;
       }
+    float ayz[3][3][3];  // [x class][oy][oz]
 #pragma unroll
-      for (int oz = 0; oz < 3; ++oz)
+    for (int oy = 0; oy < 3; ++oy) {
+      const float c0 = g[0] * wv[0][1][oy];
+      const float t0 = g[1] * wv[kEvalClass[1][1]][1][oy], t3 = g[4] * wv[kEvalClass[4][1]][1][oy];
+      const float t1 = g[2] * wv[kEvalClass[2][1]][1][oy], t2 = g[3] * wv[kEvalClass[3][1]][1][oy];
 #pragma unroll
-        for (int oy = 0; oy < 3; ++oy) {
-          const float gyz = (g[q] * wv[1][oy]) * wv[2][oz];
+      for (int oz = 0; oz < 3; ++oz) {
+        ayz[0][oy][oz] = c0 * wv[0][2][oz];
+        ayz[1][oy][oz] = fmaf(t3, wv[kEvalClass[4][2]][2][oz], t0 * wv[kEvalClass[1][2]][2][oz]);
+        ayz[2][oy][oz] = fmaf(t2, wv[kEvalClass[3][2]][2][oz], t1 * wv[kEvalClass[2][2]][2][oz]);
+      }
+    }
 #pragma unroll
-          for (int ox = 0; ox < 3; ++ox) {
-            const int o = (oz * 3 + oy) * 3 + ox;
-            acc[o] = fmaf(wv[0][ox], gyz, acc[o]);
-          }
+    for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox) {
+          const int o = (oz * 3 + oy) * 3 + ox;
+          float a = fmaf(wv[0][0][ox], ayz[0][oy][oz], acc[o]);
+          a = fmaf(wv[1][0][ox], ayz[1][oy][oz], a);
+          acc[o] = fmaf(wv[2][0][ox], ayz[2][oy][oz], a);
         }
-      const int shift = (cell[q][0] - blo[0]) + 3 * (cell[q][1] - blo[1]) + 9 * (cell[q][2] - blo[2]);
+    // corners touched: the eight-corner pattern of each evaluation with a
+    // nonzero gradient, shifted by its cell's offset in the box
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const int shift = (cl[kEvalClass[q][0]][0] - blo[0]) + 3 * (cl[kEvalClass[q][1]][1] - blo[1]) +
+                        9 * (cl[kEvalClass[q][2]][2] - blo[2]);
       if (g[q] != 0.0f) mask |= 0x361Bu << shift;  // corners {0,1}^3 of a 3x3x3 box
     }
   }
@@ -949,6 +984,8 @@ static int launch_taps_bwd(const na_grid_desc* d, const float* x, int64_t n, flo
                            float vrange, float eps, const void* dyc, const void* dyt,
                            const void* dyq, int64_t ld, float* grad, int active, int inc,
                            hipStream_t s) {
+  // the scatter orders the cells of x - eps, x, x + eps along each axis
+  NA_CHECK_ARG(eps >= 0.0f, "tap epsilon must be non-negative");
   const int run = (int)tune_value(TUNE_TAPS_BWD_RUN);
   dim3 grid;
   if (int st = chunk_level_grid(ceil_div(ceil_div(n, (int64_t)run) * F, kBlock), active, &grid))
