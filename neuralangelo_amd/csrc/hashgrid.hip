@@ -430,25 +430,33 @@ sdf_taps_input_kernel(na_grid_desc d, const float* __restrict__ x, int64_t n, fl
 #pragma unroll
     for (int t = 0; t < 4; ++t) xt[t][k] = xc[k] + kTapDir[t][k] * eps;
   }
-  if (l == 0) {
-    split_store(X0c + i * ld + LF, xc);
-    for (int64_t c = LF + 9; c < ld; ++c) X0c[i * ld + c] = 0;
-    float dsum[3] = {0.0f, 0.0f, 0.0f};
+  // the six rows' coordinate columns and zero padding, a row per level lane (the
+  // point's lanes run together, so one lane writing all six would hold the rest)
+  for (int r = l; r < 6; r += L) {  // 0: X0c, 1 .. 4: tap r - 1 of D, 5: S
+    float v[3];
+    bf16_t* row;
+    if (r == 0) {
+      row = X0c + i * ld;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      bf16_t* row = D + (4 * i + t) * ld_d;
-      float dx[3];
+      for (int k = 0; k < 3; ++k) v[k] = xc[k];
+    } else if (r <= 4) {
+      row = D + (4 * i + (r - 1)) * ld_d;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[k] = xt[r - 1][k] - xc[k];  // exact (Sterbenz)
+    } else {
+      row = S + i * ld;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        dx[k] = xt[t][k] - xc[k];  // exact (Sterbenz)
-        dsum[k] += dx[k];
+        float dsum = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) dsum += xt[t][k] - xc[k];
+        v[k] = dsum;
       }
-      split_store(row + LF, dx);
-      for (int64_t c = LF + 9; c < ld; ++c) row[c] = 0;
-      for (int64_t c = ld + LF; c < ld_d; ++c) row[c] = 0;
     }
-    split_store(S + i * ld + LF, dsum);
-    for (int64_t c = LF + 9; c < ld; ++c) S[i * ld + c] = 0;
+    split_store(row + LF, v);
+    for (int64_t c = LF + 9; c < ld; ++c) row[c] = 0;
+    if (r >= 1 && r <= 4)
+      for (int64_t c = ld + LF; c < ld_d; ++c) row[c] = 0;
   }
   if (l >= active_levels) {  // coarse-to-fine: inactive levels are zero in all outputs
     float z[F];
