@@ -555,8 +555,9 @@ __device__ __attribute__((always_inline)) inline void taps_flush(float* gtab, co
                                                                  uint32_t mask) {
   // corner_index of the 27 box corners from per-axis terms: a hashed level
   // xors one term per axis; a dense level adds them, and its wrap (index %
-  // size) is the identity while the box lies inside the grid (every coordinate
-  // below res, so the index is below res^3 <= size)
+  // size) is the identity while the box lies inside the grid (0 <= blo and
+  // blo + 2 < res on every axis, so the index is below res^3 <= size; blo below
+  // zero arrives as a large unsigned value and takes the wrap)
   constexpr uint32_t kPy = 2654435761u, kPz = 805459861u;
   const uint32_t x0 = (uint32_t)blo[0], y0 = (uint32_t)blo[1], z0 = (uint32_t)blo[2];
   uint32_t id[27];
@@ -572,7 +573,9 @@ __device__ __attribute__((always_inline)) inline void taps_flush(float* gtab, co
 #pragma unroll
     for (int o = 0; o < 27; ++o)
       id[o] = d0 + (uint32_t)(o % 3) + (uint32_t)((o / 3) % 3) * r1 + (uint32_t)(o / 9) * r2;
-    if (!(x0 + 2u < p.res && y0 + 2u < p.res && z0 + 2u < p.res)) {
+    const bool inside =
+        p.res > 2u && x0 < p.res - 2u && y0 < p.res - 2u && z0 < p.res - 2u;
+    if (!inside) {
 #pragma unroll
       for (int o = 0; o < 27; ++o) id[o] %= p.size;
     }
