@@ -381,17 +381,29 @@ mc_classify_kernel(McLattice g, int64_t tpp_max, int64_t np_max, uint8_t* __rest
   uint32_t own_pts = 0;
 #pragma unroll
   for (int u = 0; u < kMcPref; ++u) own_pts |= voff[u] != kMcOob ? 1u << u : 0u;
-  // flags of this wave's own staged points of plane p (its loads have landed)
+  // flags of this wave's own staged points of plane p (its loads have landed).
+  // All reads are issued before any is used (one LDS round trip, not one per
+  // load: u >= U reads the slot's first element and is masked out), and the
+  // compare runs in one precision, chosen per launch.
   auto wave_flags = [&](int64_t p) -> int {
     const float* src = ring + (p % slots) * stride + (int64_t)w * U * 64 + lane;
-    bool below = false, above = false;
+    float v[kMcPref];
 #pragma unroll
-    for (int u = 0; u < kMcPref; ++u) {
-      if (own_pts & (1u << u)) {
-        const float v = src[u * 64];
-        const bool le = iso_f ? v <= isof : (double)v <= iso;
-        below = below || le;
-        above = above || !le;
+    for (int u = 0; u < kMcPref; ++u) v[u] = src[(u < U ? u : 0) * 64];
+    bool below = false, above = false;
+    if (iso_f) {
+#pragma unroll
+      for (int u = 0; u < kMcPref; ++u) {
+        const bool in = (own_pts >> u) & 1u, le = v[u] <= isof;
+        below = below || (in && le);
+        above = above || (in && !le);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kMcPref; ++u) {
+        const bool in = (own_pts >> u) & 1u, le = (double)v[u] <= iso;
+        below = below || (in && le);
+        above = above || (in && !le);
       }
     }
     return (__any(below) ? 1 : 0) | (__any(above) ? 2 : 0);
